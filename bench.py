@@ -1,0 +1,142 @@
+"""Benchmark: photon-traced transport build on the synthetic city block.
+
+Workload (one step): the full forward photon transport of the 8 x 8 city block
+(~51.8k triangles, vertex bins, 1296 photons per triangle = 67.1M photons
+~ 2^26): fixed-point accumulator clear, photon tracing through the on-device
+LBVH, FP32 dense operator + diagonal + exact column sums.  Geometry and the
+BVH are resident in HBM before the timed region.
+
+Prints one JSON line: value = traced ray segments per second over the whole
+step (Mrays/s); ``roofline`` for the photon kernel (algorithmic bytes per ray
+segment from the kernel's own node/triangle visit counts, HIP-event kernel
+time); ``cpu_baseline`` = the CPU oracle (C + OpenMP restatement of the
+reference tracer) on a bounded sample of the same scene, rank 0 only.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "inverse-radiative-thermal-transport_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mrays/s transport build; solves/s at ≤0.05K vs FP64; p50/p99 request latency"
+HBM_PEAK_GBS = 8000.0
+NODE_BYTES = 64      # two child boxes + two child refs
+TRI_BYTES = 40       # 3 x float3 vertices + id
+RAY_BYTES = 48       # 32 B ray in + 16 B hit out
+BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
+
+
+def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):
+    """Times the oracle tracer (all photons of a strided subset of source
+    triangles) on the host cores; returns a dict for the JSON line."""
+    import numpy as np
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    sc = oracle.OracleScene.from_irtt(sd)
+    p = oracle.make_params(threads_per_tri, rays_per_thread, batch_seed=1234, bin_mode=0)
+    stride = 512
+    # all photons of every stride-th source triangle; shrink the stride until
+    # the sample takes about target_s
+    while True:
+        t0 = time.perf_counter()
+        _, _, st = oracle.transport(sc, p, 0, sd.nt, dense=False, records=1, n_threads=cores,
+                                    tri_stride=stride)
+        dt = time.perf_counter() - t0
+        if dt >= 0.3 * target_s or stride == 1:
+            break
+        stride = max(1, int(stride * dt / target_s))
+    rays, photons = st["rays"], st["photons"]
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+            "sample": f"{photons} photons ({rays} ray segments): all "
+                      f"{threads_per_tri * rays_per_thread} photons of every {stride}-th source "
+                      f"triangle of the same city block, CPU oracle (C + OpenMP, LBVH build "
+                      f"included), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--threads-per-tri", type=int, default=162)
+    ap.add_argument("--rays-per-thread", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by source triangles across GPUs by design only)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import irtt
+    from irtt import scenes
+
+    sd = scenes.city_block(seed=args.seed)
+    s = irtt.Scene.from_data(sd, irtt.BIN_VERTEX)
+    s.build_bvh()
+    p = irtt.transport_params(args.threads_per_tri, args.rays_per_thread, batch_seed=1234)
+    for _ in range(args.warmup):
+        s.build_transport(p)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(s.build_transport(p))
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    st = stats[-1]
+    rays = st["rays"]
+    value = rays / (ms_per_step * 1e3)  # Mrays/s
+    trace_ms = sum(x["trace_ms"] for x in stats) / len(stats)
+    n_node = st["node_visits"] / rays
+    n_tri = st["tri_tests"] / rays
+    b_launch = rays * (RAY_BYTES + n_node * NODE_BYTES + n_tri * TRI_BYTES) + \
+        (st["absorbed"] + st["forced_absorb"]) * BIN_BYTES
+    achieved = b_launch / (trace_ms * 1e-3) / 1e9
+    # steady solve on the built operator (reported beside the metric)
+    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-10)
+    s.set_vertex_state(sd.kelvin, sd.fixed)
+    sol = s.solve(sp)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "city block 8x8 buildings, seed %d, %d triangles, %d vertex bins, "
+                               "%d photons (%d per triangle), forward transport build"
+                               % (args.seed, sd.nt, s.n_bins, st["photons"],
+                                  args.threads_per_tri * args.rays_per_thread),
+                   "bins": s.n_bins, "triangles": sd.nt, "photons": st["photons"],
+                   "ray_segments": rays},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_photons", "kernel_ms": round(trace_ms, 3),
+                     "bytes_per_ray": round(b_launch / rays, 1),
+                     "node_visits_per_ray": round(n_node, 2),
+                     "tri_tests_per_ray": round(n_tri, 2)},
+        "breakdown_ms": {"photon_kernel": round(trace_ms, 3),
+                         "finalize": round(sum(x["finalize_ms"] for x in stats) / len(stats), 3),
+                         "bvh_build": round(st["bvh_ms"], 3)},
+        "steady_solve": {"ms": round(sol["solve_ms"], 1), "newton_iters": sol["newton_iters"],
+                         "linear_iters": sol["linear_iters"], "converged": sol["converged"]},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread)
+    s.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,189 @@
+/*
+ * irtt.h -- C ABI of the MI355X radiative-thermal core (libirtt.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes.  Every function returns
+ * an int status: 0 = OK, negative = error; irtt_last_error() then returns a
+ * thread-local message.  The caller owns every array it passes; the library
+ * copies inputs into device memory (HBM) and writes outputs into caller
+ * buffers.  A handle may be used from one host thread at a time; different
+ * handles may be used concurrently.
+ *
+ * The reference's seams that these entry points replace (paths relative to the
+ * reference repository root):
+ *   - src/implementations/thermal_renderer/thermal_api.hpp:10-62 (external C ABI,
+ *     compatibility shims declared in thermal_api_compat.h)
+ *   - ThermalTransport::compute / computeAdjoint
+ *     (src/implementations/thermal_renderer/thermal_transport.hpp:56-57)
+ *   - ThermalSolver::residual / jacobian / solve
+ *     (src/implementations/thermal_renderer/thermal_solver.hpp:18-20)
+ */
+#ifndef IRTT_H
+#define IRTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRTT_OK 0
+#define IRTT_ERR_ARG -1
+#define IRTT_ERR_HIP -2
+#define IRTT_ERR_STATE -3
+#define IRTT_ERR_NOMEM -4
+#define IRTT_ERR_LIMIT -5
+#define IRTT_ERR_NOCONV -6
+
+#define IRTT_BIN_VERTEX 0u   /* reference binning: argmax-barycentric corner */
+#define IRTT_BIN_TRIANGLE 1u /* one bin per triangle */
+
+#define IRTT_MAT_DIFFUSE_EMISSION 1u
+#define IRTT_MAT_TRACEABLE 2u
+
+typedef struct irtt_scene_s *irtt_scene;
+
+/* Per-triangle surface description.  The reference keeps these per object
+ * (ObjectProperties, thermal_api.cpp:25-45; glTF extras, thermal_scene.cpp:76-108);
+ * per-triangle is a superset. */
+typedef struct irtt_material {
+    float diffuse_reflectance;  /* rho_d */
+    float specular_reflectance; /* rho_s; emissivity = 1 - rho_d - rho_s */
+    float thickness;            /* m */
+    float density;              /* kg/m^3 */
+    float heat_capacity;        /* J/(kg K) */
+    float heat_conductivity;    /* W/(m K) */
+    float heat_flux;            /* absorbed external flux, W/m^2 (extension) */
+    uint32_t flags;             /* IRTT_MAT_* */
+} irtt_material;
+
+typedef struct irtt_scene_info {
+    uint32_t n_vertices, n_triangles, n_bins, n_traceable;
+    uint32_t bvh_max_depth;
+    uint32_t bin_mode;
+    uint64_t device_bytes;      /* HBM held by this scene */
+} irtt_scene_info;
+
+/* ---------------------------------------------------------------- library */
+int irtt_version(void);
+const char *irtt_last_error(void);
+/* name of device 0 and its compute-unit count */
+int irtt_device_info(char *name, size_t name_len, int *cu_count);
+
+/* ------------------------------------------------------------------ scene */
+/* Replaces the scene assembly of ThermalScene::load / ThermalData::init,load
+ * (thermal_scene.cpp:9-110, thermal_data.cpp:7-173): areas, emissivities,
+ * lumped mass, cotangent conduction Laplacian.  xyz: 3*nv floats, tri: 3*nt
+ * vertex indices, mat: nt materials.  Uploads geometry to HBM. */
+int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri,
+                      uint32_t nt, const irtt_material *mat, uint32_t bin_mode,
+                      irtt_scene *out);
+int irtt_scene_destroy(irtt_scene h);
+int irtt_scene_info_get(irtt_scene h, irtt_scene_info *info);
+/* Initial / fixed temperatures per vertex (K); fixed != 0 pins the value
+ * (ThermalData::load fixedVarsVector, initialValueVector, thermal_data.cpp:101-102).
+ * In triangle bin mode a bin takes the mean of its corners and is fixed when
+ * all three corners are. */
+int irtt_scene_set_vertex_state(irtt_scene h, const float *kelvin,
+                                const uint8_t *fixed);
+/* Per-bin state directly (n_bins entries). */
+int irtt_scene_set_bin_state(irtt_scene h, const float *kelvin,
+                             const uint8_t *fixed);
+/* Replace vertex positions (same topology) and rebuild areas, mass, Laplacian
+ * and the BVH (ThermalRenderer::updateGeometryWithParameters,
+ * thermal_renderer.cpp:1327-1337). */
+int irtt_scene_update_vertices(irtt_scene h, const float *xyz);
+/* Per-bin physics as assembled: area, emissivity*area, lumped mass (n_bins each). */
+int irtt_scene_get_bin_data(irtt_scene h, double *area, double *eps_area,
+                            double *mass);
+
+/* -------------------------------------------------------------------- BVH */
+/* Replaces the BLAS/TLAS build (thermal_transport.cpp:66-139): on-device LBVH
+ * over the traceable triangles (30-bit Morton codes, LDS radix sort, Karras
+ * hierarchy, bottom-up boxes). */
+int irtt_build_bvh(irtt_scene h);
+/* Copies the LBVH out: nodes 16 words each (lo0 hi0 lo1 hi1 child0 child1 0 0),
+ * triangle records 12 words each (p0 id p1 0 p2 0), Morton codes in leaf order.
+ * Any pointer may be NULL. */
+int irtt_get_bvh(irtt_scene h, uint32_t *nodes, uint32_t *tris,
+                 uint32_t *morton, uint32_t *n_leaves, uint32_t *max_depth);
+/* Closest hit for n rays (6 floats each: origin, direction); the hook for the
+ * bit-exact traversal test (replaces traceRayEXT + transport_ray.rchit,
+ * thermal_tracing_unified.glsl:362, transport_ray.rchit:12-16).  prim = -1 on
+ * miss; b1, b2 = barycentric weights of corners 1 and 2. */
+int irtt_trace_rays(irtt_scene h, const float *rays, uint32_t n, float tmin,
+                    float tmax, int32_t *prim, float *t, float *b1, float *b2);
+
+/* -------------------------------------------------------------- transport */
+typedef struct irtt_transport_params {
+    uint32_t threads_per_tri;   /* ThermalVars_s::threadsPerTriangle (32768) */
+    uint32_t rays_per_thread;   /* ThermalVars_s::raysPerThread (8) */
+    uint32_t batch_seed;        /* ThermalVars_s::randSeed (1234) */
+    uint32_t max_depth;         /* 0 -> 512 (MAX_MAX_DEPTH) */
+    float tiny_dist;            /* ThermalVars_s::tinyDist (1e-3) */
+    uint32_t reserved[3];
+} irtt_transport_params;
+
+typedef struct irtt_transport_stats {
+    uint64_t photons, rays, absorbed, miss, too_close, backface, degenerate,
+        forced_absorb;
+    double quantum;             /* W per fixed-point unit */
+    float trace_ms;             /* photon kernel, HIP events */
+    float finalize_ms;          /* fixed point -> FP32 dense + diagonal */
+    float bvh_ms;               /* last LBVH build */
+    uint32_t reserved0;
+    uint64_t node_visits;       /* internal BVH nodes fetched over all rays */
+    uint64_t tri_tests;         /* leaf triangles tested over all rays */
+} irtt_transport_stats;
+
+/* Replaces ThermalTransport::compute (thermal_transport.cpp:599-738):
+ * photon tracing into a dense n_bins x n_bins operator, row = absorbing bin,
+ * column = emitting bin, off-diagonal -sum(photon weights), diagonal
+ * + sigma*eps*A.  Accumulation is exact fixed point, so the result is
+ * bit-reproducible. Builds the BVH first if needed. */
+int irtt_build_transport(irtt_scene h, const irtt_transport_params *p,
+                         irtt_transport_stats *st);
+int irtt_get_transport(irtt_scene h, float *dense);
+/* Raw fixed-point accumulators (n_bins^2, row-major) and the quantum. */
+int irtt_get_transport_fixed(irtt_scene h, uint64_t *acc, double *quantum);
+/* Column sums of the operator, exact from the fixed-point accumulators (FP64). */
+int irtt_get_column_sums(irtt_scene h, double *colsum);
+
+/* ------------------------------------------------------------------ solve */
+#define IRTT_JAC_EXACT 0u       /* J = T diag(4x^3) + L (+ M/dt) */
+#define IRTT_JAC_REFERENCE 1u   /* thermal_solver.cpp:109-166: diag(4x^3) T + L + I/dt */
+
+typedef struct irtt_solver_params {
+    uint32_t steady;            /* 1 steady state, 0 one implicit-Euler step */
+    uint32_t jacobian;          /* IRTT_JAC_* */
+    uint32_t max_newton;        /* reference max_iter = 100 */
+    uint32_t max_linear;        /* BiCGSTAB iterations per Newton step */
+    double dt;                  /* s; reference step_size = 300 */
+    double ambient_kelvin;      /* < 0: no ambient exchange (reference) */
+    double x_tol;               /* stop when max |dx| < x_tol (K) */
+    double f_tol;               /* stop when ||r||^2 < f_tol (reference 1e-6) */
+    double lin_rtol;            /* BiCGSTAB relative tolerance */
+} irtt_solver_params;
+
+typedef struct irtt_solve_stats {
+    uint32_t newton_iters, linear_iters, line_search_halvings, converged;
+    double residual_sq;         /* final ||r||^2 (W^2) */
+    double max_dx;              /* last Newton update, K */
+    float solve_ms;
+} irtt_solve_stats;
+
+/* Replaces ThermalSolver::solve (thermal_solver.cpp:119-229) driven by
+ * ThermalRenderer::thermalTimestep (thermal_renderer.cpp:542-660): one
+ * steady solve or one implicit-Euler step from the current state.  The
+ * result becomes the current state. */
+int irtt_solve(irtt_scene h, const irtt_solver_params *p, irtt_solve_stats *st);
+int irtt_get_temperatures(irtt_scene h, double *kelvin);   /* n_bins */
+int irtt_set_temperatures(irtt_scene h, const double *kelvin); /* current state */
+/* residual r(x) of the assembled system at x (n_bins), FP64 */
+int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x,
+                  double *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,488 @@
+// api.cpp -- C ABI entry points of libirtt.so (declared in include/irtt.h)
+// and the host-side scene assembly.
+//
+// Scene assembly restates ThermalScene::initObject (thermal_scene.cpp:76-108),
+// ThermalData::updateAreaVectorsForMesh / load / computeLumpedThermalMass /
+// computeConductiveLaplacian (thermal_data.cpp:56-81,83-173,216-313) for
+// per-triangle materials and the two bin modes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "irtt_internal.hpp"
+
+using namespace irtt;
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        return IRTT_OK;
+    } catch (const Error &e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_err = "host allocation failed";
+        return IRTT_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return IRTT_ERR_STATE;
+    }
+}
+
+Scene *S(irtt_scene h)
+{
+    if (!h) throw Error(IRTT_ERR_ARG, "null scene handle");
+    return reinterpret_cast<Scene *>(h);
+}
+
+// cotangent of the angle between a and b, thermal_data.cpp:253-258; a
+// degenerate corner contributes nothing instead of inf/NaN
+double cotan(const double a[3], const double b[3])
+{
+    double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    double bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    double den = aa * bb - ab * ab;
+    if (!(den > 0.0)) return 0.0;
+    return ab / std::sqrt(den);
+}
+
+void assemble(Scene &s)
+{
+    const uint32_t nt = s.nt, nv = s.nv;
+    s.rho_d.resize(nt);
+    s.rho_s.resize(nt);
+    s.eps.resize(nt);
+    s.tri_area.resize(nt);
+    s.flags.resize(nt);
+    std::vector<double> area_d(nt), area_raw(nt);
+    for (uint32_t t = 0; t < nt; t++) {
+        const irtt_material &m = s.mat[t];
+        float rd = m.diffuse_reflectance, rs = m.specular_reflectance;
+        // thermal_scene.cpp:87-97: normalise when total reflectance exceeds 1
+        float tot = rd + rs;
+        if (tot > 1.0f) {
+            rd /= tot;
+            rs /= tot;
+        }
+        s.rho_d[t] = rd;
+        s.rho_s[t] = rs;
+        s.eps[t] = 1.0f - rd - rs;  // thermal_tracing_unified.glsl:309
+        s.flags[t] = (uint8_t)(m.flags & 3u);
+        const float *p0 = &s.xyz[3 * s.idx[3 * t]];
+        const float *p1 = &s.xyz[3 * s.idx[3 * t + 1]];
+        const float *p2 = &s.xyz[3 * s.idx[3 * t + 2]];
+        double e1[3], e2[3];
+        for (int k = 0; k < 3; k++) {
+            e1[k] = (double)p1[k] - (double)p0[k];
+            e2[k] = (double)p2[k] - (double)p0[k];
+        }
+        double cx = e1[1] * e2[2] - e1[2] * e2[1];
+        double cy = e1[2] * e2[0] - e1[0] * e2[2];
+        double cz = e1[0] * e2[1] - e1[1] * e2[0];
+        double a = std::sqrt(cx * cx + cy * cy + cz * cz) * 0.5;
+        area_raw[t] = a;
+        // thermal_data.cpp:67-75: clamp tiny elements
+        if (a < 10.0 * FLT_EPSILON) a = 10.0 * FLT_EPSILON;
+        area_d[t] = a;
+        s.tri_area[t] = (float)a;
+    }
+    const uint32_t nb = s.bin_mode == IRTT_BIN_TRIANGLE ? nt : nv;
+    s.nb = nb;
+    s.ld = (nb + 63u) & ~63u;
+    s.bin_area.assign(nb, 0.0);
+    s.bin_eps_area.assign(nb, 0.0);
+    s.bin_mass.assign(nb, 0.0);
+    s.bin_source.assign(nb, 0.0);
+    bool any_k = false;
+    for (uint32_t t = 0; t < nt; t++) {
+        const irtt_material &m = s.mat[t];
+        double eps = 1.0 - (double)s.rho_d[t] - (double)s.rho_s[t];
+        double rct = (double)m.density * (double)m.heat_capacity * (double)m.thickness;
+        double src = (double)m.heat_flux;
+        if (m.heat_conductivity != 0.0f) any_k = true;
+        if (s.bin_mode == IRTT_BIN_TRIANGLE) {
+            s.bin_area[t] += area_d[t];
+            s.bin_eps_area[t] += eps * area_d[t];
+            s.bin_mass[t] += area_raw[t] * rct;
+            s.bin_source[t] += src * area_d[t];
+        } else {
+            for (int c = 0; c < 3; c++) {
+                uint32_t v = s.idx[3 * t + c];
+                s.bin_area[v] += area_d[t] / 3.0;
+                s.bin_eps_area[v] += eps * area_d[t] / 3.0;
+                s.bin_mass[v] += area_raw[t] / 3.0 * rct;
+                s.bin_source[v] += src * area_d[t] / 3.0;
+            }
+        }
+    }
+    s.lap = Csr{};
+    if (any_k) {
+        if (s.bin_mode == IRTT_BIN_TRIANGLE)
+            throw Error(IRTT_ERR_ARG, "heat conduction needs vertex bins (IRTT_BIN_VERTEX)");
+        // cotangent Laplacian scaled by conductivity * thickness, thermal_data.cpp:259-313
+        std::vector<std::map<uint32_t, double>> rows(nv);
+        for (uint32_t t = 0; t < nt; t++) {
+            const irtt_material &m = s.mat[t];
+            double k = (double)m.heat_conductivity * (double)m.thickness;
+            if (k == 0.0) continue;
+            uint32_t i0 = s.idx[3 * t], i1 = s.idx[3 * t + 1], i2 = s.idx[3 * t + 2];
+            double a[3], b[3], c[3];
+            for (int q = 0; q < 3; q++) {
+                a[q] = s.xyz[3 * i0 + q];
+                b[q] = s.xyz[3 * i1 + q];
+                c[q] = s.xyz[3 * i2 + q];
+            }
+            double ac[3], bc[3], ba[3], ca[3], cb[3], ab[3];
+            for (int q = 0; q < 3; q++) {
+                ac[q] = a[q] - c[q]; bc[q] = b[q] - c[q];
+                ba[q] = b[q] - a[q]; ca[q] = c[q] - a[q];
+                cb[q] = c[q] - b[q]; ab[q] = a[q] - b[q];
+            }
+            double cot1 = cotan(ac, bc), cot2 = cotan(ba, ca), cot3 = cotan(cb, ab);
+            rows[i0][i0] += k * 0.5 * (cot1 + cot3);
+            rows[i0][i1] += k * 0.5 * (-cot1);
+            rows[i0][i2] += k * 0.5 * (-cot3);
+            rows[i1][i0] += k * 0.5 * (-cot1);
+            rows[i1][i1] += k * 0.5 * (cot1 + cot2);
+            rows[i1][i2] += k * 0.5 * (-cot2);
+            rows[i2][i0] += k * 0.5 * (-cot3);
+            rows[i2][i1] += k * 0.5 * (-cot2);
+            rows[i2][i2] += k * 0.5 * (cot2 + cot3);
+        }
+        s.lap.n = nv;
+        s.lap.rowptr.assign(nv + 1, 0);
+        for (uint32_t i = 0; i < nv; i++) {
+            for (auto &kv : rows[i]) {
+                s.lap.col.push_back(kv.first);
+                s.lap.val.push_back(kv.second);
+            }
+            s.lap.rowptr[i + 1] = (uint32_t)s.lap.col.size();
+        }
+    }
+    s.d_lap_rowptr.release();
+    s.d_lap_col.release();
+    s.d_lap_val.release();
+}
+
+void upload_geometry(Scene &s)
+{
+    hipStream_t st = s.stream;
+    s.d_xyz.alloc(s.xyz.size());
+    s.d_idx.alloc(s.idx.size());
+    s.d_rho_d.alloc(s.nt);
+    s.d_rho_s.alloc(s.nt);
+    s.d_flags.alloc(s.nt);
+    IRTT_HIP(hipMemcpyAsync(s.d_xyz.p, s.xyz.data(), s.xyz.size() * 4, hipMemcpyHostToDevice, st));
+    IRTT_HIP(hipMemcpyAsync(s.d_idx.p, s.idx.data(), s.idx.size() * 4, hipMemcpyHostToDevice, st));
+    IRTT_HIP(hipMemcpyAsync(s.d_rho_d.p, s.rho_d.data(), s.nt * 4, hipMemcpyHostToDevice, st));
+    IRTT_HIP(hipMemcpyAsync(s.d_rho_s.p, s.rho_s.data(), s.nt * 4, hipMemcpyHostToDevice, st));
+    IRTT_HIP(hipMemcpyAsync(s.d_flags.p, s.flags.data(), s.nt, hipMemcpyHostToDevice, st));
+    std::vector<uint32_t> tr;
+    tr.reserve(s.nt);
+    for (uint32_t t = 0; t < s.nt; t++)
+        if (s.flags[t] & IRTT_MAT_TRACEABLE) tr.push_back(t);
+    s.d_traceable.release();
+    if (!tr.empty()) {
+        s.d_traceable.alloc(tr.size());
+        IRTT_HIP(hipMemcpyAsync(s.d_traceable.p, tr.data(), tr.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    IRTT_HIP(hipStreamSynchronize(st));
+    s.bvh.valid = false;
+    s.has_transport = false;
+}
+
+}  // namespace
+
+uint64_t Scene::device_bytes() const
+{
+    return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
+           d_traceable.bytes() + bvh.nodes.bytes() + bvh.tris.bytes() + bvh.morton.bytes() +
+           d_q.bytes() + d_acc.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
+           d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
+           d_lap_col.bytes() + d_lap_val.bytes();
+}
+
+extern "C" {
+
+int irtt_version(void) { return 1; }
+
+const char *irtt_last_error(void) { return g_err.c_str(); }
+
+int irtt_device_info(char *name, size_t name_len, int *cu_count)
+{
+    return guard([&] {
+        int n = 0;
+        IRTT_HIP(hipGetDeviceCount(&n));
+        if (n < 1) throw Error(IRTT_ERR_HIP, "no HIP device");
+        hipDeviceProp_t prop;
+        IRTT_HIP(hipGetDeviceProperties(&prop, 0));
+        if (name && name_len) {
+            std::string nm = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+            std::strncpy(name, nm.c_str(), name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (cu_count) *cu_count = prop.multiProcessorCount;
+    });
+}
+
+int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32_t nt,
+                      const irtt_material *mat, uint32_t bin_mode, irtt_scene *out)
+{
+    return guard([&] {
+        if (!out) throw Error(IRTT_ERR_ARG, "out is null");
+        *out = nullptr;
+        if (!xyz || !tri || !mat || nv == 0 || nt == 0)
+            throw Error(IRTT_ERR_ARG, "empty geometry or null pointer");
+        if (bin_mode > IRTT_BIN_TRIANGLE) throw Error(IRTT_ERR_ARG, "unknown bin mode");
+        for (uint64_t i = 0; i < 3ull * nt; i++)
+            if (tri[i] >= nv)
+                throw Error(IRTT_ERR_ARG, "triangle index " + std::to_string(tri[i]) +
+                                              " out of range (nv = " + std::to_string(nv) + ")");
+        for (uint64_t i = 0; i < 3ull * nv; i++)
+            if (!std::isfinite(xyz[i])) throw Error(IRTT_ERR_ARG, "non-finite vertex coordinate");
+        auto s = std::make_unique<Scene>();
+        s->nv = nv;
+        s->nt = nt;
+        s->bin_mode = bin_mode;
+        s->xyz.assign(xyz, xyz + 3ull * nv);
+        s->idx.assign(tri, tri + 3ull * nt);
+        s->mat.assign(mat, mat + nt);
+        assemble(*s);
+        s->bin_kelvin0.assign(s->nb, 0.0);
+        s->bin_fixed.assign(s->nb, 0);
+        s->x.assign(s->nb, 0.0);
+        IRTT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        IRTT_HIP(hipEventCreate(&s->ev0));
+        IRTT_HIP(hipEventCreate(&s->ev1));
+        upload_geometry(*s);
+        *out = reinterpret_cast<irtt_scene>(s.release());
+    });
+}
+
+int irtt_scene_destroy(irtt_scene h)
+{
+    return guard([&] {
+        if (!h) return;
+        Scene *s = S(h);
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        hipStream_t st = s->stream;
+        hipEvent_t e0 = s->ev0, e1 = s->ev1;
+        delete s;
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (st) (void)hipStreamDestroy(st);
+    });
+}
+
+int irtt_scene_info_get(irtt_scene h, irtt_scene_info *info)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!info) throw Error(IRTT_ERR_ARG, "info is null");
+        info->n_vertices = s->nv;
+        info->n_triangles = s->nt;
+        info->n_bins = s->nb;
+        info->n_traceable = (uint32_t)s->d_traceable.n;
+        info->bvh_max_depth = s->bvh.valid ? s->bvh.max_depth : 0;
+        info->bin_mode = s->bin_mode;
+        info->device_bytes = s->device_bytes();
+    });
+}
+
+int irtt_scene_set_vertex_state(irtt_scene h, const float *kelvin, const uint8_t *fixed)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!kelvin) throw Error(IRTT_ERR_ARG, "kelvin is null");
+        if (s->bin_mode == IRTT_BIN_VERTEX) {
+            for (uint32_t v = 0; v < s->nv; v++) {
+                s->bin_kelvin0[v] = kelvin[v];
+                s->bin_fixed[v] = fixed ? (fixed[v] ? 1 : 0) : 0;
+            }
+        } else {
+            for (uint32_t t = 0; t < s->nt; t++) {
+                uint32_t a = s->idx[3 * t], b = s->idx[3 * t + 1], c = s->idx[3 * t + 2];
+                s->bin_kelvin0[t] = ((double)kelvin[a] + kelvin[b] + kelvin[c]) / 3.0;
+                s->bin_fixed[t] = fixed ? ((fixed[a] && fixed[b] && fixed[c]) ? 1 : 0) : 0;
+            }
+        }
+        // ThermalData::reset, thermal_data.cpp:196-199
+        s->x = s->bin_kelvin0;
+    });
+}
+
+int irtt_scene_set_bin_state(irtt_scene h, const float *kelvin, const uint8_t *fixed)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!kelvin) throw Error(IRTT_ERR_ARG, "kelvin is null");
+        for (uint32_t i = 0; i < s->nb; i++) {
+            s->bin_kelvin0[i] = kelvin[i];
+            s->bin_fixed[i] = fixed ? (fixed[i] ? 1 : 0) : 0;
+        }
+        s->x = s->bin_kelvin0;
+    });
+}
+
+int irtt_scene_update_vertices(irtt_scene h, const float *xyz)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!xyz) throw Error(IRTT_ERR_ARG, "xyz is null");
+        for (uint64_t i = 0; i < 3ull * s->nv; i++)
+            if (!std::isfinite(xyz[i])) throw Error(IRTT_ERR_ARG, "non-finite vertex coordinate");
+        s->xyz.assign(xyz, xyz + 3ull * s->nv);
+        assemble(*s);
+        upload_geometry(*s);
+    });
+}
+
+int irtt_scene_get_bin_data(irtt_scene h, double *area, double *eps_area, double *mass)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (area) std::copy(s->bin_area.begin(), s->bin_area.end(), area);
+        if (eps_area) std::copy(s->bin_eps_area.begin(), s->bin_eps_area.end(), eps_area);
+        if (mass) std::copy(s->bin_mass.begin(), s->bin_mass.end(), mass);
+    });
+}
+
+int irtt_build_bvh(irtt_scene h)
+{
+    return guard([&] { bvh_build(*S(h)); });
+}
+
+int irtt_get_bvh(irtt_scene h, uint32_t *nodes, uint32_t *tris, uint32_t *morton,
+                 uint32_t *n_leaves, uint32_t *max_depth)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->bvh.valid) bvh_build(*s);
+        const uint32_t n = s->bvh.n_leaves;
+        if (n_leaves) *n_leaves = n;
+        if (max_depth) *max_depth = s->bvh.max_depth;
+        if (n == 0) return;
+        if (nodes && n > 1)
+            IRTT_HIP(hipMemcpy(nodes, s->bvh.nodes.p, (size_t)(n - 1) * 64, hipMemcpyDeviceToHost));
+        if (tris) IRTT_HIP(hipMemcpy(tris, s->bvh.tris.p, (size_t)n * 48, hipMemcpyDeviceToHost));
+        if (morton)
+            IRTT_HIP(hipMemcpy(morton, s->bvh.morton.p, (size_t)n * 4, hipMemcpyDeviceToHost));
+    });
+}
+
+int irtt_trace_rays(irtt_scene h, const float *rays, uint32_t n, float tmin, float tmax,
+                    int32_t *prim, float *t, float *b1, float *b2)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (n && (!rays || !prim || !t || !b1 || !b2)) throw Error(IRTT_ERR_ARG, "null pointer");
+        trace_rays(*s, rays, n, tmin, tmax, prim, t, b1, b2);
+    });
+}
+
+int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_transport_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p) throw Error(IRTT_ERR_ARG, "params is null");
+        // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B)
+        size_t need = (size_t)s->nb * s->ld * 12;
+        size_t fr = 0, tot = 0;
+        IRTT_HIP(hipMemGetInfo(&fr, &tot));
+        size_t held = s->d_acc.bytes() + s->d_T.bytes();
+        if (need > fr + held)
+            throw Error(IRTT_ERR_LIMIT, "dense operator needs " + std::to_string(need) +
+                                            " bytes, " + std::to_string(fr + held) + " available");
+        irtt_transport_stats tmp{};
+        transport_build(*s, *p, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_get_transport(irtt_scene h, float *dense)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (!dense) throw Error(IRTT_ERR_ARG, "dense is null");
+        IRTT_HIP(hipMemcpy2D(dense, (size_t)s->nb * 4, s->d_T.p, (size_t)s->ld * 4,
+                             (size_t)s->nb * 4, s->nb, hipMemcpyDeviceToHost));
+    });
+}
+
+int irtt_get_transport_fixed(irtt_scene h, uint64_t *acc, double *quantum)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (quantum) *quantum = s->quantum;
+        if (acc)
+            IRTT_HIP(hipMemcpy2D(acc, (size_t)s->nb * 8, s->d_acc.p, (size_t)s->ld * 8,
+                                 (size_t)s->nb * 8, s->nb, hipMemcpyDeviceToHost));
+    });
+}
+
+int irtt_get_column_sums(irtt_scene h, double *colsum)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (!colsum) throw Error(IRTT_ERR_ARG, "colsum is null");
+        std::copy(s->colsum.begin(), s->colsum.end(), colsum);
+    });
+}
+
+int irtt_solve(irtt_scene h, const irtt_solver_params *p, irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p) throw Error(IRTT_ERR_ARG, "params is null");
+        irtt_solve_stats tmp{};
+        solve(*s, *p, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_get_temperatures(irtt_scene h, double *kelvin)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!kelvin) throw Error(IRTT_ERR_ARG, "kelvin is null");
+        std::copy(s->x.begin(), s->x.end(), kelvin);
+    });
+}
+
+int irtt_set_temperatures(irtt_scene h, const double *kelvin)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!kelvin) throw Error(IRTT_ERR_ARG, "kelvin is null");
+        s->x.assign(kelvin, kelvin + s->nb);
+    });
+}
+
+int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x, double *r)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !x || !r) throw Error(IRTT_ERR_ARG, "null pointer");
+        residual_host(*s, *p, x, r);
+    });
+}
+
+}  // extern "C"

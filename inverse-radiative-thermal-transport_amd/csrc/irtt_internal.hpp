@@ -1,0 +1,143 @@
+// irtt_internal.hpp -- scene state shared by the host entry points and the
+// kernel launchers of libirtt.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/irtt.h"
+
+namespace irtt {
+
+// Error carrying an irtt status code; entry points translate it into the
+// return value and the thread-local message.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define IRTT_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::irtt::Error(IRTT_ERR_HIP, std::string(#call) + ": " +                \
+                                                  hipGetErrorString(e_));                \
+    } while (0)
+
+constexpr float kSigma = 5.670374419e-8f;      // thermal_common.hpp:39
+constexpr double kSigmaD = 5.670374419e-8;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMaxDepthDefault = 512;     // MAX_MAX_DEPTH, thermal_tracing_unified.glsl:7
+
+// Device buffer with RAII release.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p) return;
+        release();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error(IRTT_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                            " bytes failed: " + hipGetErrorString(e));
+        }
+        n = count;
+    }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Sparse matrix in CSR, FP64 (conduction Laplacian).
+struct Csr {
+    uint32_t n = 0;
+    std::vector<uint32_t> rowptr, col;
+    std::vector<double> val;
+};
+
+struct Bvh {
+    uint32_t n_leaves = 0, max_depth = 0;
+    bool valid = false;
+    DBuf<float4> nodes;       // 4 float4 per internal node
+    DBuf<float4> tris;        // 3 float4 per leaf
+    DBuf<uint32_t> morton;    // leaf order
+    float build_ms = 0.f;
+};
+
+struct Scene {
+    uint32_t nv = 0, nt = 0, nb = 0, bin_mode = 0;
+    uint32_t ld = 0;  // row pitch of the dense operator: nb rounded up to 64
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    // host copies of the inputs
+    std::vector<float> xyz;
+    std::vector<uint32_t> idx;
+    std::vector<irtt_material> mat;
+    std::vector<uint8_t> flags;
+    std::vector<float> rho_d, rho_s, eps, tri_area;  // FP32 as the shader sees them
+
+    // per-bin physics (FP64), ThermalData::load / computeLumpedThermalMass /
+    // computeConductiveLaplacian
+    std::vector<double> bin_area, bin_eps_area, bin_mass, bin_source;
+    std::vector<double> bin_kelvin0;   // initial / fixed values
+    std::vector<uint8_t> bin_fixed;
+    std::vector<double> x;             // current state (K)
+    Csr lap;
+
+    // device geometry
+    DBuf<float> d_xyz;
+    DBuf<uint32_t> d_idx;
+    DBuf<float> d_rho_d, d_rho_s;
+    DBuf<uint8_t> d_flags;
+    DBuf<uint32_t> d_traceable;       // ids of traceable triangles
+    Bvh bvh;
+
+    // transport
+    bool has_transport = false;
+    irtt_transport_params tparams{};
+    double quantum = 0.0;
+    DBuf<uint64_t> d_q;               // per-triangle fixed-point weight
+    DBuf<unsigned long long> d_acc;   // nb*ld fixed-point accumulators
+    DBuf<float> d_T;                  // nb*ld FP32 operator, row-major
+    DBuf<double> d_diag;              // sigma*eps*A per bin
+    DBuf<unsigned long long> d_colacc;  // column sums of d_acc
+    std::vector<double> colsum;        // column sums of T (exact)
+    std::vector<double> Tdiag;         // diagonal of the FP32 operator as stored
+    DBuf<unsigned long long> d_stats;
+
+    // solver workspace
+    DBuf<double> d_vec;               // 12 vectors of nb doubles
+    DBuf<double> d_red;               // reduction scratch
+    DBuf<uint32_t> d_lap_rowptr, d_lap_col;
+    DBuf<double> d_lap_val;
+
+    uint64_t device_bytes() const;
+};
+
+// lbvh.hip
+void bvh_build(Scene &s);
+// trace.hip
+void trace_rays(Scene &s, const float *rays, uint32_t n, float tmin, float tmax, int32_t *prim,
+                float *t, float *b1, float *b2);
+void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_stats &st);
+// solver.hip
+void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st);
+void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r);
+
+}  // namespace irtt

@@ -1,0 +1,494 @@
+// solver.hip -- Newton solve of the radiative + conductive energy balance on
+// the device.
+//
+// Replaces ThermalSolver::residual / jacobian / solve
+// (src/implementations/thermal_renderer/thermal_solver.cpp:13-229), which run
+// on the host with Eigen.  Residual (thermal_solver.cpp:20-31, plus the
+// source and ambient terms of this library):
+//   steady:    r = T x^4 + L x - b
+//   transient: r = M (x - x_prev) + dt (T x^4 + L x - b)
+//   b = external absorbed power + T_amb^4 * colsum(T)   (ambient exchange)
+// Fixed bins hold their value and have r = 0.  Newton steps solve J dx = -r
+// with BiCGSTAB (right Jacobi preconditioning) using matrix-free products;
+// the Jacobian is the exact one or the reference's row-scaled form
+// (thermal_solver.cpp:39-96).  A damped line search halves the step while the
+// residual norm grows (thermal_solver.cpp:192-203).  The operator is FP32 in
+// HBM; products accumulate in FP64 and every vector is FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "irtt_internal.hpp"
+
+namespace irtt {
+namespace {
+
+constexpr int kRedBlocks = 256;
+constexpr int kRedThreads = 256;
+
+// y_i = sum_j T_ij u_j: one wave per row, float4 loads, FP64 accumulation.
+__global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, uint32_t nb,
+                                                   uint32_t ld, const double *__restrict__ u,
+                                                   double *__restrict__ y)
+{
+    uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    uint32_t lane = threadIdx.x & 63;
+    if (row >= nb) return;
+    const float4 *r = reinterpret_cast<const float4 *>(T + (size_t)row * ld);
+    double acc = 0.0;
+    const uint32_t n4 = ld >> 2;
+    for (uint32_t k = lane; k < n4; k += 64) {
+        float4 v = r[k];
+        uint32_t j = 4 * k;
+        // the padded tail of each row is zero and u is padded with zeros
+        acc += (double)v.x * u[j] + (double)v.y * u[j + 1] + (double)v.z * u[j + 2] +
+               (double)v.w * u[j + 3];
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) y[row] = acc;
+}
+
+struct Sys {
+    uint32_t nb;
+    const uint32_t *lrow, *lcol;
+    const double *lval;
+    const double *mass, *b, *kfix, *xprev;
+    const uint8_t *fixed;
+    double dt;
+    int steady, jac_ref;
+};
+
+__device__ __forceinline__ double lap_row(const Sys &s, uint32_t i, const double *v)
+{
+    if (!s.lrow) return 0.0;
+    double a = 0.0;
+    for (uint32_t k = s.lrow[i]; k < s.lrow[i + 1]; k++) a += s.lval[k] * v[s.lcol[k]];
+    return a;
+}
+
+// y = (fixed ? kfix : x)^4 ; xe = (fixed ? kfix : x)
+__global__ void k_pow4(Sys s, const double *__restrict__ x, double *__restrict__ xe,
+                       double *__restrict__ y)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    double v = s.fixed[i] ? s.kfix[i] : x[i];
+    xe[i] = v;
+    double v2 = v * v;
+    y[i] = v2 * v2;
+}
+
+// r from Ty (= T x^4) and xe; part[block] = sum r^2
+__global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *__restrict__ xe,
+                           double *__restrict__ r, double *part)
+{
+    __shared__ double sm[kRedThreads];
+    double acc = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.nb;
+         i += gridDim.x * blockDim.x) {
+        double v = 0.0;
+        if (!s.fixed[i]) {
+            double rad = Ty[i] + lap_row(s, i, xe) - s.b[i];
+            v = s.steady ? rad : s.mass[i] * (xe[i] - s.xprev[i]) + s.dt * rad;
+        }
+        r[i] = v;
+        acc += v * v;
+    }
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+// w = exact: fixed ? 0 : 4 x^3 v ; reference: fixed ? 0 : v
+__global__ void k_jv_pre(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
+                         double *__restrict__ w)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    double vi = s.fixed[i] ? 0.0 : v[i];
+    if (!s.jac_ref) {
+        double x = xe[i];
+        vi = 4.0 * x * x * x * vi;
+    }
+    w[i] = vi;
+}
+
+// out = J v from Tw (= T w)
+__global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
+                          const double *__restrict__ Tw, double *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    if (s.fixed[i]) {
+        out[i] = v[i];
+        return;
+    }
+    double lv = lap_row(s, i, v);
+    double rad = Tw[i];
+    if (s.jac_ref) {
+        double x = xe[i];
+        rad = 4.0 * x * x * x * rad;  // diag(4x^3) T v, thermal_solver.cpp:149
+    }
+    if (s.steady) {
+        double o = rad + lv;
+        if (s.jac_ref) o += v[i] / s.dt;  // I/dt regulariser, thermal_solver.cpp:150
+        out[i] = o;
+    } else {
+        out[i] = s.mass[i] * v[i] + s.dt * (rad + lv);
+    }
+}
+
+// deterministic multi-dot: part[k*kRedBlocks + block] = sum a_k * b_k
+__global__ void k_dots(uint32_t n, int m, const double *a0, const double *b0, const double *a1,
+                       const double *b1, const double *a2, const double *b2, double *part)
+{
+    __shared__ double sm[3][kRedThreads];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        acc[0] += a0[i] * b0[i];
+        if (m > 1) acc[1] += a1[i] * b1[i];
+        if (m > 2) acc[2] += a2[i] * b2[i];
+    }
+    for (int k = 0; k < 3; k++) sm[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < 3; k++) sm[k][threadIdx.x] += sm[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < m; k++) part[k * kRedBlocks + blockIdx.x] = sm[k][0];
+}
+
+__global__ void k_final_sum(const double *part, int m, int nparts, double *out)
+{
+    __shared__ double sm[kRedThreads];
+    for (int k = 0; k < m; k++) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < nparts; i += blockDim.x) a += part[k * kRedBlocks + i];
+        sm[threadIdx.x] = a;
+        __syncthreads();
+        for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = sm[0];
+        __syncthreads();
+    }
+}
+
+// generic fused vector updates
+// mode 0: p = r + beta (p - omega v)
+// mode 1: y = a * x (pointwise Jacobi: y = x * dinv)
+// mode 2: s = r - alpha v
+// mode 3: x += alpha ph + omega sh ; r = s - omega t
+__global__ void k_p_update(uint32_t n, double beta, double omega, const double *r, const double *v,
+                           double *p)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+}
+__global__ void k_scale(uint32_t n, const double *dinv, const double *x, double *y)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = dinv[i] * x[i];
+}
+__global__ void k_axpy2(uint32_t n, double alpha, const double *x, const double *y, double *out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = x[i] + alpha * y[i];
+}
+__global__ void k_step(uint32_t n, double alpha, double omega, const double *ph, const double *sh,
+                       const double *s, const double *t, double *x, double *r)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        x[i] += alpha * ph[i] + omega * sh[i];
+        r[i] = s[i] - omega * t[i];
+    }
+}
+// Jacobi diagonal: dinv_i = 1 / J_ii
+__global__ void k_jdiag(Sys s, const double *__restrict__ xe, const double *__restrict__ Tdiag,
+                        const double *__restrict__ Ldiag, double *__restrict__ dinv)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    if (s.fixed[i]) {
+        dinv[i] = 1.0;
+        return;
+    }
+    double x = xe[i];
+    double rad = 4.0 * x * x * x * Tdiag[i];
+    double d = s.steady ? rad + Ldiag[i] + (s.jac_ref ? 1.0 / s.dt : 0.0)
+                        : s.mass[i] + s.dt * (rad + Ldiag[i]);
+    dinv[i] = (d != 0.0 && isfinite(d)) ? 1.0 / d : 1.0;
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// vector slots in d_vec (each ld doubles, padded with zeros)
+enum {
+    V_X = 0, V_XE, V_Y, V_TY, V_R, V_DX, V_XT, V_P, V_V, V_S, V_T, V_PH, V_SH, V_RHAT, V_W, V_TW,
+    V_DINV, V_B, V_KFIX, V_XPREV, V_MASS, V_TDIAG, V_LDIAG, V_COUNT
+};
+
+struct Ctx {
+    Scene &s;
+    hipStream_t st;
+    uint32_t nb, ld;
+    Sys sys;
+    double *v[V_COUNT];
+    double *part, *out;
+    double hbuf[4];
+    explicit Ctx(Scene &sc) : s(sc), st(sc.stream), nb(sc.nb), ld(sc.ld) {}
+
+    void gemv(const double *u, double *y)
+    {
+        k_gemv_rows<<<cdiv(nb, 4), 256, 0, st>>>(s.d_T.p, nb, ld, u, y);
+    }
+    // returns sum of m dot products
+    void dots(int m, const double *a0, const double *b0, const double *a1, const double *b1,
+              const double *a2, const double *b2, double *res)
+    {
+        k_dots<<<kRedBlocks, kRedThreads, 0, st>>>(nb, m, a0, b0, a1, b1, a2, b2, part);
+        k_final_sum<<<1, kRedThreads, 0, st>>>(part, m, kRedBlocks, out);
+        IRTT_HIP(hipMemcpyAsync(hbuf, out, m * sizeof(double), hipMemcpyDeviceToHost, st));
+        IRTT_HIP(hipStreamSynchronize(st));
+        for (int k = 0; k < m; k++) res[k] = hbuf[k];
+    }
+    // r(x) into V_R, returns ||r||^2; V_XE, V_Y, V_TY updated
+    double residual(const double *x)
+    {
+        uint32_t g = cdiv(nb, 256);
+        k_pow4<<<g, 256, 0, st>>>(sys, x, v[V_XE], v[V_Y]);
+        gemv(v[V_Y], v[V_TY]);
+        k_residual<<<kRedBlocks, kRedThreads, 0, st>>>(sys, v[V_TY], v[V_XE], v[V_R], part);
+        k_final_sum<<<1, kRedThreads, 0, st>>>(part, 1, kRedBlocks, out);
+        IRTT_HIP(hipMemcpyAsync(hbuf, out, sizeof(double), hipMemcpyDeviceToHost, st));
+        IRTT_HIP(hipStreamSynchronize(st));
+        return hbuf[0];
+    }
+    void jv(const double *vin, double *outv)
+    {
+        uint32_t g = cdiv(nb, 256);
+        k_jv_pre<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_W]);
+        gemv(v[V_W], v[V_TW]);
+        k_jv_post<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_TW], outv);
+    }
+};
+
+void upload_static(Scene &s, Ctx &c, const irtt_solver_params &p)
+{
+    const uint32_t nb = s.nb, ld = s.ld;
+    s.d_vec.alloc((size_t)V_COUNT * ld);
+    s.d_red.alloc(4 * kRedBlocks + 8);
+    IRTT_HIP(hipMemsetAsync(s.d_vec.p, 0, s.d_vec.bytes(), s.stream));
+    for (int k = 0; k < V_COUNT; k++) c.v[k] = s.d_vec.p + (size_t)k * ld;
+    c.part = s.d_red.p;
+    c.out = s.d_red.p + 4 * kRedBlocks;
+    std::vector<double> b(nb), kfix(nb), ldiag(nb, 0.0);
+    const double Ta = p.ambient_kelvin;
+    for (uint32_t i = 0; i < nb; i++) {
+        double amb = Ta >= 0.0 ? Ta * Ta * Ta * Ta * s.colsum[i] : 0.0;
+        b[i] = s.bin_source[i] + amb;
+        kfix[i] = s.bin_kelvin0[i];
+    }
+    if (!s.lap.rowptr.empty())
+        for (uint32_t i = 0; i < nb; i++)
+            for (uint32_t k = s.lap.rowptr[i]; k < s.lap.rowptr[i + 1]; k++)
+                if (s.lap.col[k] == i) ldiag[i] += s.lap.val[k];
+    auto up = [&](int slot, const std::vector<double> &h) {
+        IRTT_HIP(hipMemcpyAsync(c.v[slot], h.data(), nb * sizeof(double), hipMemcpyHostToDevice,
+                                s.stream));
+    };
+    up(V_B, b);
+    up(V_KFIX, kfix);
+    up(V_MASS, s.bin_mass);
+    up(V_TDIAG, s.Tdiag);
+    up(V_LDIAG, ldiag);
+    up(V_XPREV, s.x);
+    if (!s.lap.rowptr.empty() && s.d_lap_rowptr.n == 0) {
+        s.d_lap_rowptr.alloc(s.lap.rowptr.size());
+        s.d_lap_col.alloc(std::max<size_t>(1, s.lap.col.size()));
+        s.d_lap_val.alloc(std::max<size_t>(1, s.lap.val.size()));
+        IRTT_HIP(hipMemcpyAsync(s.d_lap_rowptr.p, s.lap.rowptr.data(),
+                                s.lap.rowptr.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                s.stream));
+        if (!s.lap.col.empty()) {
+            IRTT_HIP(hipMemcpyAsync(s.d_lap_col.p, s.lap.col.data(), s.lap.col.size() * sizeof(uint32_t),
+                                    hipMemcpyHostToDevice, s.stream));
+            IRTT_HIP(hipMemcpyAsync(s.d_lap_val.p, s.lap.val.data(), s.lap.val.size() * sizeof(double),
+                                    hipMemcpyHostToDevice, s.stream));
+        }
+    }
+}
+
+}  // namespace
+
+// fixed flags live in a device byte array allocated per solve
+struct FixedBuf {
+    DBuf<uint8_t> b;
+};
+
+static void setup(Scene &s, Ctx &c, const irtt_solver_params &p, FixedBuf &fb)
+{
+    if (!s.has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+    upload_static(s, c, p);
+    fb.b.alloc(s.nb);
+    IRTT_HIP(hipMemcpyAsync(fb.b.p, s.bin_fixed.data(), s.nb, hipMemcpyHostToDevice, s.stream));
+    Sys &y = c.sys;
+    y.nb = s.nb;
+    bool lap = !s.lap.rowptr.empty() && s.lap.val.size() > 0;
+    y.lrow = lap ? s.d_lap_rowptr.p : nullptr;
+    y.lcol = lap ? s.d_lap_col.p : nullptr;
+    y.lval = lap ? s.d_lap_val.p : nullptr;
+    y.mass = c.v[V_MASS];
+    y.b = c.v[V_B];
+    y.kfix = c.v[V_KFIX];
+    y.xprev = c.v[V_XPREV];
+    y.fixed = fb.b.p;
+    y.dt = p.dt > 0.0 ? p.dt : 300.0;
+    y.steady = p.steady ? 1 : 0;
+    y.jac_ref = p.jacobian == IRTT_JAC_REFERENCE ? 1 : 0;
+}
+
+void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r)
+{
+    Ctx c(s);
+    FixedBuf fb;
+    setup(s, c, p, fb);
+    IRTT_HIP(hipMemcpyAsync(c.v[V_XT], x, s.nb * sizeof(double), hipMemcpyHostToDevice, s.stream));
+    c.residual(c.v[V_XT]);
+    IRTT_HIP(hipMemcpyAsync(r, c.v[V_R], s.nb * sizeof(double), hipMemcpyDeviceToHost, s.stream));
+    IRTT_HIP(hipStreamSynchronize(s.stream));
+}
+
+void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
+{
+    Ctx c(s);
+    FixedBuf fb;
+    setup(s, c, p, fb);
+    const uint32_t nb = s.nb;
+    const uint32_t g = cdiv(nb, 256);
+    hipStream_t stm = s.stream;
+    IRTT_HIP(hipEventRecord(s.ev0, stm));
+
+    // initial guess, thermal_renderer.cpp:544-551: fixed -> initial value,
+    // zeros -> 1 K in steady state
+    std::vector<double> x0(s.x);
+    for (uint32_t i = 0; i < nb; i++) {
+        if (s.bin_fixed[i]) x0[i] = s.bin_kelvin0[i];
+        else if (p.steady && x0[i] == 0.0) x0[i] = 1.0;
+    }
+    IRTT_HIP(hipMemcpyAsync(c.v[V_X], x0.data(), nb * sizeof(double), hipMemcpyHostToDevice, stm));
+
+    const uint32_t max_newton = p.max_newton ? p.max_newton : 100;
+    const uint32_t max_lin = p.max_linear ? p.max_linear : 200;
+    const double f_tol = p.f_tol > 0 ? p.f_tol : 1e-6;
+    const double x_tol = p.x_tol > 0 ? p.x_tol : 1e-4;
+    const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
+    st = irtt_solve_stats{};
+
+    double rn = c.residual(c.v[V_X]);
+    for (uint32_t it = 0; it < max_newton; it++) {
+        st.newton_iters = it + 1;
+        if (rn < f_tol) {
+            st.converged = 1;
+            break;
+        }
+        // Jacobi diagonal at the current x (V_XE holds x with fixed values)
+        k_jdiag<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_TDIAG], c.v[V_LDIAG], c.v[V_DINV]);
+        // BiCGSTAB on J dx = -r, dx0 = 0: r0 = -r
+        double *R = c.v[V_RHAT];
+        IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
+        IRTT_HIP(hipMemsetAsync(c.v[V_P], 0, nb * sizeof(double), stm));
+        IRTT_HIP(hipMemsetAsync(c.v[V_V], 0, nb * sizeof(double), stm));
+        // lin residual in V_S as working "r_lin"; keep Newton r in V_R
+        k_axpy2<<<g, 256, 0, stm>>>(nb, -2.0, c.v[V_R], c.v[V_R], c.v[V_S]);  // s = -r
+        IRTT_HIP(hipMemcpyAsync(R, c.v[V_S], nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
+        double *rl = c.v[V_XT];  // linear residual
+        IRTT_HIP(hipMemcpyAsync(rl, c.v[V_S], nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
+        double rho = 1.0, alpha = 1.0, omega = 1.0;
+        const double bnorm2 = rn;
+        double res[3];
+        uint32_t li = 0;
+        for (; li < max_lin; li++) {
+            c.dots(1, R, rl, nullptr, nullptr, nullptr, nullptr, res);
+            double rho_new = res[0];
+            if (rho_new == 0.0 || !std::isfinite(rho_new)) break;
+            double beta = (li == 0) ? 0.0 : (rho_new / rho) * (alpha / omega);
+            k_p_update<<<g, 256, 0, stm>>>(nb, beta, omega, rl, c.v[V_V], c.v[V_P]);
+            k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
+            c.jv(c.v[V_PH], c.v[V_V]);
+            c.dots(1, R, c.v[V_V], nullptr, nullptr, nullptr, nullptr, res);
+            if (res[0] == 0.0 || !std::isfinite(res[0])) break;
+            alpha = rho_new / res[0];
+            k_axpy2<<<g, 256, 0, stm>>>(nb, -alpha, rl, c.v[V_V], c.v[V_S]);
+            c.dots(1, c.v[V_S], c.v[V_S], nullptr, nullptr, nullptr, nullptr, res);
+            if (res[0] <= lin_rtol * lin_rtol * bnorm2) {
+                k_axpy2<<<g, 256, 0, stm>>>(nb, alpha, c.v[V_DX], c.v[V_PH], c.v[V_DX]);
+                li++;
+                break;
+            }
+            k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
+            c.jv(c.v[V_SH], c.v[V_T]);
+            c.dots(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], nullptr, nullptr, res);
+            if (res[1] == 0.0) break;
+            omega = res[0] / res[1];
+            k_step<<<g, 256, 0, stm>>>(nb, alpha, omega, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T],
+                                       c.v[V_DX], rl);
+            c.dots(1, rl, rl, nullptr, nullptr, nullptr, nullptr, res);
+            rho = rho_new;
+            if (res[0] <= lin_rtol * lin_rtol * bnorm2) {
+                li++;
+                break;
+            }
+            if (omega == 0.0) break;
+        }
+        st.linear_iters += li;
+        // damped line search, thermal_solver.cpp:192-203
+        double a = 1.0;
+        k_axpy2<<<g, 256, 0, stm>>>(nb, a, c.v[V_X], c.v[V_DX], c.v[V_XT]);
+        double rn_new = c.residual(c.v[V_XT]);
+        uint32_t ls = 0;
+        while ((rn_new > rn || !std::isfinite(rn_new)) && ls < 20) {
+            ls++;
+            a *= 0.5;
+            k_axpy2<<<g, 256, 0, stm>>>(nb, a, c.v[V_X], c.v[V_DX], c.v[V_XT]);
+            rn_new = c.residual(c.v[V_XT]);
+        }
+        st.line_search_halvings += ls;
+        IRTT_HIP(hipMemcpyAsync(c.v[V_X], c.v[V_XT], nb * sizeof(double), hipMemcpyDeviceToDevice,
+                                stm));
+        // max |a dx| over free bins
+        std::vector<double> dx(nb);
+        IRTT_HIP(hipMemcpyAsync(dx.data(), c.v[V_DX], nb * sizeof(double), hipMemcpyDeviceToHost, stm));
+        IRTT_HIP(hipStreamSynchronize(stm));
+        double mdx = 0.0;
+        for (uint32_t i = 0; i < nb; i++)
+            if (!s.bin_fixed[i]) mdx = std::max(mdx, std::fabs(a * dx[i]));
+        st.max_dx = mdx;
+        rn = rn_new;
+        if (!std::isfinite(rn)) break;
+        if (rn < f_tol || mdx < x_tol || a < 1e-6) {
+            st.converged = (rn < f_tol || mdx < x_tol) ? 1 : 0;
+            break;
+        }
+    }
+    st.residual_sq = rn;
+    std::vector<double> xr(nb);
+    // V_XE holds the final x with fixed values applied (set by the last residual call)
+    IRTT_HIP(hipMemcpyAsync(xr.data(), c.v[V_XE], nb * sizeof(double), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    IRTT_HIP(hipEventElapsedTime(&st.solve_ms, s.ev0, s.ev1));
+    s.x = xr;
+}
+
+}  // namespace irtt

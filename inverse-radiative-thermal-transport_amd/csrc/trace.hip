@@ -1,0 +1,498 @@
+// trace.hip -- closest-hit traversal of the LBVH and the forward photon
+// transport kernel.
+//
+// Replaces the reference's ray-tracing pipeline launch
+// (src/implementations/thermal_renderer/thermal_transport.cpp:599-738) and its
+// raygen program (assets/shader/raytracing_thermal/thermal_tracing_unified.glsl:
+// 247-601, TRACING_MODE 1).  CDNA4 has no ray-tracing units, so traversal is a
+// per-lane stack walk over 64-byte two-child nodes with the stack in LDS.
+// Photon hits are accumulated as exact fixed-point integers (one 64-bit
+// integer add per absorbed photon, weight q_t of its source triangle), so the
+// operator is bit-reproducible and independent of launch order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "device_math.hpp"
+#include "irtt_internal.hpp"
+
+namespace irtt {
+namespace {
+
+constexpr int kTraceBlock = 64;  // one wave per block: the LDS stack is per lane
+
+struct Hit {
+    int32_t prim;
+    float t, b1, b2;
+};
+
+// conservative slab test; the far distance is widened by 1 + 2^-21 (>= 1 + 2*gamma_3).
+// Pruning against the best hit keeps a 1e-4 relative margin: box entry and
+// triangle distances round differently, and exact ties must still be visited.
+__device__ __forceinline__ bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz,
+                                        V3 o, V3 inv, float tmin, float tbest, float &tnear)
+{
+    float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
+    float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
+    float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+    float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000048f;
+    tnear = n;
+    return (n <= f) && (n <= tbest * 1.0001f);
+}
+
+__device__ __forceinline__ void leaf_test(const float4 *__restrict__ tris, uint32_t leaf,
+                                          const RayPre &rp, V3 o, float tmin, Hit &h)
+{
+    float4 a = tris[3 * leaf], b = tris[3 * leaf + 1], c = tris[3 * leaf + 2];
+    int32_t id = (int32_t)__float_as_uint(a.w);
+    float t, b1, b2;
+    if (tri_hit(rp, o, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, tmin, h.t, t, b1,
+                b2)) {
+        if (t < h.t || h.prim < 0 || (t == h.t && id < h.prim)) {
+            h.prim = id;
+            h.t = t;
+            h.b1 = b1;
+            h.b2 = b2;
+        }
+    }
+}
+
+// Closest hit (smallest t, ties to the smallest triangle id), near child
+// first; a popped entry is skipped when its box starts beyond the best hit.
+// stk_n / stk_t: this lane's LDS stack, entry k at [k * kTraceBlock].
+template <int STACK>
+__device__ __forceinline__ Hit trace_closest(const float4 *__restrict__ nodes,
+                                             const float4 *__restrict__ tris, uint32_t n_leaves,
+                                             V3 o, V3 d, float tmin, float tmax, uint32_t *stk_n,
+                                             float *stk_t, uint32_t &nvis, uint32_t &ntri)
+{
+    Hit h{-1, tmax, 0.0f, 0.0f};
+    if (n_leaves == 0) return h;
+    RayPre rp = ray_pre(d);
+    if (n_leaves == 1) {
+        ntri++;
+        leaf_test(tris, 0, rp, o, tmin, h);
+        if (h.prim < 0) h.t = tmax;
+        return h;
+    }
+    V3 inv{1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
+           1.0f / (d.z != 0.0f ? d.z : 1e-30f)};
+    int sp = 0;
+    uint32_t cur = 0;
+    for (;;) {
+        if (!(cur & kLeafBit)) {
+            nvis++;
+            const float4 *nd = nodes + 4 * (size_t)cur;
+            float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+            float t0, t1;
+            bool h0 = box_hit(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, o, inv, tmin, h.t, t0);
+            bool h1 = box_hit(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, o, inv, tmin, h.t, t1);
+            uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
+            if (h0 && h1) {
+                uint32_t nr, fr;
+                float ft;
+                if (t1 < t0) {
+                    nr = c1; fr = c0; ft = t0;
+                } else {
+                    nr = c0; fr = c1; ft = t1;
+                }
+                stk_n[sp * kTraceBlock] = fr;
+                stk_t[sp * kTraceBlock] = ft;
+                sp++;
+                cur = nr;
+                continue;
+            } else if (h0) {
+                cur = c0;
+                continue;
+            } else if (h1) {
+                cur = c1;
+                continue;
+            }
+        } else {
+            ntri++;
+            leaf_test(tris, cur & ~kLeafBit, rp, o, tmin, h);
+        }
+        bool found = false;
+        while (sp > 0) {
+            sp--;
+            if (stk_t[sp * kTraceBlock] <= h.t * 1.0001f) {
+                cur = stk_n[sp * kTraceBlock];
+                found = true;
+                break;
+            }
+        }
+        if (!found) break;
+    }
+    if (h.prim < 0) h.t = tmax;
+    return h;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_rays(
+    const float4 *__restrict__ nodes, const float4 *__restrict__ tris, uint32_t n_leaves,
+    const float *__restrict__ rays, uint32_t n, float tmin, float tmax, int32_t *prim, float *tout,
+    float *b1o, float *b2o)
+{
+    __shared__ uint32_t stk_n[STACK * kTraceBlock];
+    __shared__ float stk_t[STACK * kTraceBlock];
+    uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= n) return;
+    const float *r = rays + 6 * (size_t)i;
+    uint32_t nv = 0, nt = 0;
+    Hit h = trace_closest<STACK>(nodes, tris, n_leaves, V3{r[0], r[1], r[2]},
+                                 V3{r[3], r[4], r[5]}, tmin, tmax, stk_n + threadIdx.x,
+                                 stk_t + threadIdx.x, nv, nt);
+    prim[i] = h.prim;
+    tout[i] = h.t;
+    b1o[i] = h.b1;
+    b2o[i] = h.b2;
+}
+
+// statistic slots
+enum { S_RAYS = 0, S_ABS, S_MISS, S_CLOSE, S_BACK, S_DEG, S_FORCED, S_COUNT };
+
+struct PhotonArgs {
+    const float *xyz;
+    const uint32_t *idx;
+    const float *rho_d, *rho_s;
+    const uint8_t *flags;
+    const unsigned long long *q;
+    const float4 *nodes, *tris;
+    uint32_t n_leaves;
+    uint32_t nt, nb, ld, bin_mode;
+    uint32_t threads_per_tri, rays_per_thread, per_tri, batch_seed, max_depth;
+    float tiny_dist;
+    uint64_t photon_begin, photon_end;
+    unsigned long long *acc;
+    unsigned long long *stats;
+};
+
+__device__ __forceinline__ V3 load_v(const float *xyz, uint32_t v)
+{
+    return V3{xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
+}
+
+// One photon per lane per iteration; lanes of a wave take consecutive photons
+// of the same source triangle, so first-bounce rays are spatially coherent.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
+{
+    __shared__ uint32_t stk_n[STACK * kTraceBlock];
+    __shared__ float stk_t[STACK * kTraceBlock];
+    uint32_t *sn = stk_n + threadIdx.x;
+    float *stt = stk_t + threadIdx.x;
+    uint32_t cnt[S_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    uint32_t nvis = 0, ntri = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    for (uint64_t g = a.photon_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+         g < a.photon_end; g += stride) {
+        uint32_t t = (uint32_t)(g / a.per_tri);
+        uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+        uint32_t x = k / a.rays_per_thread, nn = k - x * a.rays_per_thread;
+        // thermal_tracing_unified.glsl:291
+        uint32_t seed = tea16(a.nt * x + t, a.batch_seed + nn);
+        uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+        V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
+        // getEmissionRay, thermal_tracing_unified.glsl:125-161
+        float u0 = rnd(seed), u1 = rnd(seed);
+        (void)rnd(seed);
+        float su = sqrt_rn(u0);
+        float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
+        V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
+        float len;
+        V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
+        V3 o = offset_ray(pos, nrm);
+        V3 d = nrm;
+        if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
+            float r0 = rnd(seed), r1 = rnd(seed);
+            (void)rnd(seed);
+            d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
+        }
+        if (!(len > 0.0f)) {
+            cnt[S_DEG]++;
+            continue;
+        }
+        int sc = max_corner(b0, b1, b2);
+        uint32_t src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
+        const unsigned long long qt = a.q[t];
+        uint32_t depth = 0;
+        for (;;) {
+            ++depth;
+            if (depth > a.max_depth) break;
+            Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn, stt,
+                                         nvis, ntri);
+            cnt[S_RAYS]++;
+            if (h.prim < 0) {
+                cnt[S_MISS]++;
+                break;
+            }
+            if (h.t < a.tiny_dist) {
+                cnt[S_CLOSE]++;
+                break;
+            }
+            uint32_t hp = (uint32_t)h.prim;
+            uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+            V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
+            V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
+            if (dot3(hn, d) > -1e-3f) {
+                cnt[S_BACK]++;
+                break;
+            }
+            float hb0 = 1.0f - h.b1 - h.b2;
+            // getNextRay, thermal_tracing_unified.glsl:163-197
+            V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+            o = offset_ray(hpos, hn);
+            float u = rnd(seed);
+            float rd = a.rho_d[hp], rs = a.rho_s[hp];
+            bool absorbed = false;
+            if (u >= rd + rs) {
+                absorbed = true;
+            } else if (u >= rd) {
+                V3 in = normalize3(d);
+                float dn = 2.0f * dot3(hn, in);
+                d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
+            } else {
+                float r0 = rnd(seed), r1 = rnd(seed);
+                (void)rnd(seed);
+                d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
+            }
+            if (absorbed || depth == a.max_depth) {
+                // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
+                int hc = max_corner(hb0, h.b1, h.b2);
+                uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
+                atomicAdd(a.acc + (uint64_t)ab * a.ld + src, qt);
+                if (absorbed) cnt[S_ABS]++;
+                else cnt[S_FORCED]++;
+                break;
+            }
+        }
+    }
+    // wave-level reduction of the counters (one wave per block)
+    for (int s = 0; s < S_COUNT; s++) {
+        uint32_t v = cnt[s];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (threadIdx.x == 0 && v) atomicAdd(a.stats + s, (unsigned long long)v);
+    }
+    // traversal work: internal nodes fetched and triangles tested
+    unsigned long long nv64 = nvis, nt64 = ntri;
+    for (int off = 32; off > 0; off >>= 1) {
+        nv64 += __shfl_down(nv64, off, 64);
+        nt64 += __shfl_down(nt64, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(a.stats + S_COUNT, nv64);
+        atomicAdd(a.stats + S_COUNT + 1, nt64);
+    }
+}
+
+// T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
+// integer column sums of acc are accumulated per 256-row tile.
+constexpr int kFinRows = 256;
+__global__ void k_finalize(const unsigned long long *__restrict__ acc, const double *__restrict__ diag,
+                           uint32_t nb, uint32_t ld, double Q, float *__restrict__ T,
+                           unsigned long long *colacc)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    uint32_t r0 = blockIdx.y * kFinRows, r1 = min(r0 + kFinRows, nb);
+    unsigned long long s = 0;
+    for (uint32_t i = r0; i < r1; i++) {
+        size_t o = (size_t)i * ld + j;
+        unsigned long long v = acc[o];
+        s += v;
+        double val = (i == j ? diag[i] : 0.0) - (double)v * Q;
+        T[o] = j < nb ? (float)val : 0.0f;
+    }
+    if (s && j < nb) atomicAdd(colacc + j, s);
+}
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+int stack_size_for(uint32_t depth)
+{
+    if (depth + 1 <= 32) return 32;
+    if (depth + 1 <= 64) return 64;
+    if (depth + 1 <= 128) return 128;
+    throw Error(IRTT_ERR_LIMIT, "LBVH depth " + std::to_string(depth) +
+                                    " exceeds the traversal stack (128 entries)");
+}
+
+}  // namespace
+
+void trace_rays(Scene &s, const float *rays, uint32_t n, float tmin, float tmax, int32_t *prim,
+                float *t, float *b1, float *b2)
+{
+    if (!s.bvh.valid) bvh_build(s);
+    if (n == 0) return;
+    DBuf<float> d_rays, d_t, d_b1, d_b2;
+    DBuf<int32_t> d_prim;
+    d_rays.alloc(6 * (size_t)n);
+    d_t.alloc(n); d_b1.alloc(n); d_b2.alloc(n); d_prim.alloc(n);
+    hipStream_t st = s.stream;
+    IRTT_HIP(hipMemcpyAsync(d_rays.p, rays, d_rays.bytes(), hipMemcpyHostToDevice, st));
+    int S = stack_size_for(s.bvh.max_depth);
+    dim3 g(cdiv(n, kTraceBlock)), b(kTraceBlock);
+    const float4 *nd = s.bvh.nodes.p, *tr = s.bvh.tris.p;
+    if (S == 32)
+        k_trace_rays<32><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays.p, n, tmin, tmax, d_prim.p,
+                                          d_t.p, d_b1.p, d_b2.p);
+    else if (S == 64)
+        k_trace_rays<64><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays.p, n, tmin, tmax, d_prim.p,
+                                          d_t.p, d_b1.p, d_b2.p);
+    else
+        k_trace_rays<128><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays.p, n, tmin, tmax,
+                                           d_prim.p, d_t.p, d_b1.p, d_b2.p);
+    IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipMemcpyAsync(prim, d_prim.p, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(t, d_t.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(b1, d_b1.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(b2, d_b2.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipStreamSynchronize(st));
+}
+
+static uint32_t ceil_log2_u64(uint64_t x)
+{
+    uint32_t l = 0;
+    while (l < 64 && ((uint64_t)1 << l) < x) l++;
+    return l;
+}
+
+void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_stats &st)
+{
+    if (p.threads_per_tri == 0 || p.rays_per_thread == 0)
+        throw Error(IRTT_ERR_ARG, "threads_per_tri and rays_per_thread must be positive");
+    uint64_t per64 = (uint64_t)p.threads_per_tri * p.rays_per_thread;
+    if (per64 > 0xffffffffull)
+        throw Error(IRTT_ERR_LIMIT, "photons per triangle exceed 2^32");
+    const uint32_t per = (uint32_t)per64;
+    const uint32_t max_depth =
+        (p.max_depth == 0 || p.max_depth > kMaxDepthDefault) ? kMaxDepthDefault : p.max_depth;
+    if (!s.bvh.valid) bvh_build(s);
+    const uint32_t nb = s.nb;
+    const uint32_t ld = s.ld;
+    const size_t nb2 = (size_t)nb * ld;
+    if (nb == 0) throw Error(IRTT_ERR_STATE, "empty scene");
+
+    // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
+    // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s
+    std::vector<float> w(s.nt);
+    float wmax = 0.0f;
+    for (uint32_t t = 0; t < s.nt; t++) {
+        float wt = s.tri_area[t] / (float)per;
+        wt = wt * (kSigma * s.eps[t]);
+        w[t] = wt;
+        wmax = std::max(wmax, wt);
+    }
+    const uint64_t total = per64 * s.nt;
+    int sh = 62 - (int)ceil_log2_u64(total ? total : 1);
+    if (sh > 40) sh = 40;
+    const double Q = wmax > 0.0f ? std::ldexp((double)wmax, -sh) : 1.0;
+    std::vector<uint64_t> q(s.nt);
+    for (uint32_t t = 0; t < s.nt; t++) {
+        double r = wmax > 0.0f ? std::ldexp((double)w[t] / (double)wmax, sh) : 0.0;
+        q[t] = (uint64_t)std::rint(r > 0.0 ? r : 0.0);
+    }
+    s.quantum = Q;
+    s.tparams = p;
+
+    hipStream_t stm = s.stream;
+    s.d_q.alloc(s.nt);
+    IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice, stm));
+    s.d_acc.alloc(nb2);
+    s.d_T.alloc(nb2);
+    s.d_diag.alloc(nb);
+    s.d_colacc.alloc(nb);
+    s.d_stats.alloc(16);
+    std::vector<double> diag(nb);
+    for (uint32_t i = 0; i < nb; i++) diag[i] = kSigmaD * s.bin_eps_area[i];
+    IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice, stm));
+    IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+    IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
+    IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
+
+    PhotonArgs a{};
+    a.xyz = s.d_xyz.p;
+    a.idx = s.d_idx.p;
+    a.rho_d = s.d_rho_d.p;
+    a.rho_s = s.d_rho_s.p;
+    a.flags = s.d_flags.p;
+    a.q = reinterpret_cast<const unsigned long long *>(s.d_q.p);
+    a.nodes = s.bvh.nodes.p;
+    a.tris = s.bvh.tris.p;
+    a.n_leaves = s.bvh.n_leaves;
+    a.nt = s.nt;
+    a.nb = nb;
+    a.ld = ld;
+    a.bin_mode = s.bin_mode;
+    a.threads_per_tri = p.threads_per_tri;
+    a.rays_per_thread = p.rays_per_thread;
+    a.per_tri = per;
+    a.batch_seed = p.batch_seed;
+    a.max_depth = max_depth;
+    a.tiny_dist = p.tiny_dist;
+    a.photon_begin = 0;
+    a.photon_end = total;
+    a.acc = s.d_acc.p;
+    a.stats = s.d_stats.p;
+
+    int dev = 0, ncu = 256;
+    IRTT_HIP(hipGetDevice(&dev));
+    IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int S = stack_size_for(s.bvh.max_depth);
+    // enough waves to cover every CU several times; lanes loop over photons
+    uint64_t want = cdiv(total, kTraceBlock);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * 64);
+    if (grid == 0) grid = 1;
+    IRTT_HIP(hipEventRecord(s.ev0, stm));
+    if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 64) k_photons<64><<<grid, kTraceBlock, 0, stm>>>(a);
+    else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
+    IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
+
+    IRTT_HIP(hipEventRecord(s.ev0, stm));
+    dim3 fg(cdiv(ld, 256), cdiv(nb, kFinRows));
+    k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p);
+    IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    std::vector<unsigned long long> colacc(nb), hstats(16);
+    IRTT_HIP(hipMemcpyAsync(colacc.data(), s.d_colacc.p, nb * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(hstats.data(), s.d_stats.p, 16 * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    IRTT_HIP(hipEventElapsedTime(&st.finalize_ms, s.ev0, s.ev1));
+
+    s.colsum.assign(nb, 0.0);
+    for (uint32_t j = 0; j < nb; j++) s.colsum[j] = diag[j] - (double)colacc[j] * Q;
+    // diagonal of the stored operator, for the Jacobi preconditioner
+    s.Tdiag.assign(nb, 0.0);
+    {
+        std::vector<float> dcol(nb);
+        // strided copy of the diagonal: nb elements at stride nb+1
+        IRTT_HIP(hipMemcpy2DAsync(dcol.data(), sizeof(float), s.d_T.p, (ld + 1) * sizeof(float),
+                                  sizeof(float), nb, hipMemcpyDeviceToHost, stm));
+        IRTT_HIP(hipStreamSynchronize(stm));
+        for (uint32_t i = 0; i < nb; i++) s.Tdiag[i] = dcol[i];
+    }
+    st.photons = total;
+    st.rays = hstats[S_RAYS];
+    st.absorbed = hstats[S_ABS];
+    st.miss = hstats[S_MISS];
+    st.too_close = hstats[S_CLOSE];
+    st.backface = hstats[S_BACK];
+    st.degenerate = hstats[S_DEG];
+    st.forced_absorb = hstats[S_FORCED];
+    st.node_visits = hstats[S_COUNT];
+    st.tri_tests = hstats[S_COUNT + 1];
+    st.quantum = Q;
+    st.bvh_ms = s.bvh.build_ms;
+    s.has_transport = true;
+}
+
+}  // namespace irtt

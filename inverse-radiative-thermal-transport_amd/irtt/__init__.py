@@ -1,0 +1,331 @@
+"""ctypes binding of libirtt.so, the MI355X radiative-thermal core.
+
+The product is the C ABI declared in ``include/irtt.h`` (HIP kernels for
+gfx950 behind it).  This module only marshals numpy arrays across that ABI; it
+has no compute path of its own and raises when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libirtt.so")
+_lib = None
+
+BIN_VERTEX = 0
+BIN_TRIANGLE = 1
+MAT_DIFFUSE_EMISSION = 1
+MAT_TRACEABLE = 2
+JAC_EXACT = 0
+JAC_REFERENCE = 1
+
+
+class IrttError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"irtt error {code}: {msg}")
+        self.code = code
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("diffuse_reflectance", ctypes.c_float),
+                ("specular_reflectance", ctypes.c_float),
+                ("thickness", ctypes.c_float), ("density", ctypes.c_float),
+                ("heat_capacity", ctypes.c_float),
+                ("heat_conductivity", ctypes.c_float),
+                ("heat_flux", ctypes.c_float), ("flags", ctypes.c_uint32)]
+
+
+MATERIAL_DTYPE = np.dtype([("diffuse_reflectance", "<f4"), ("specular_reflectance", "<f4"),
+                           ("thickness", "<f4"), ("density", "<f4"),
+                           ("heat_capacity", "<f4"), ("heat_conductivity", "<f4"),
+                           ("heat_flux", "<f4"), ("flags", "<u4")])
+
+
+class SceneInfo(ctypes.Structure):
+    _fields_ = [("n_vertices", ctypes.c_uint32), ("n_triangles", ctypes.c_uint32),
+                ("n_bins", ctypes.c_uint32), ("n_traceable", ctypes.c_uint32),
+                ("bvh_max_depth", ctypes.c_uint32), ("bin_mode", ctypes.c_uint32),
+                ("device_bytes", ctypes.c_uint64)]
+
+
+class TransportParams(ctypes.Structure):
+    _fields_ = [("threads_per_tri", ctypes.c_uint32), ("rays_per_thread", ctypes.c_uint32),
+                ("batch_seed", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
+                ("tiny_dist", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class TransportStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("photons", "rays", "absorbed", "miss", "too_close", "backface",
+                 "degenerate", "forced_absorb")] + [
+        ("quantum", ctypes.c_double), ("trace_ms", ctypes.c_float),
+        ("finalize_ms", ctypes.c_float), ("bvh_ms", ctypes.c_float),
+        ("reserved0", ctypes.c_uint32), ("node_visits", ctypes.c_uint64),
+        ("tri_tests", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: (getattr(self, n)) for n, _ in self._fields_}
+
+
+class SolverParams(ctypes.Structure):
+    _fields_ = [("steady", ctypes.c_uint32), ("jacobian", ctypes.c_uint32),
+                ("max_newton", ctypes.c_uint32), ("max_linear", ctypes.c_uint32),
+                ("dt", ctypes.c_double), ("ambient_kelvin", ctypes.c_double),
+                ("x_tol", ctypes.c_double), ("f_tol", ctypes.c_double),
+                ("lin_rtol", ctypes.c_double)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("newton_iters", ctypes.c_uint32), ("linear_iters", ctypes.c_uint32),
+                ("line_search_halvings", ctypes.c_uint32), ("converged", ctypes.c_uint32),
+                ("residual_sq", ctypes.c_double), ("max_dx", ctypes.c_double),
+                ("solve_ms", ctypes.c_float)]
+
+    def as_dict(self):
+        return {n: (getattr(self, n)) for n, _ in self._fields_}
+
+
+_vp = ctypes.c_void_p
+
+
+def _declare(L):
+    L.irtt_version.restype = ctypes.c_int
+    L.irtt_last_error.restype = ctypes.c_char_p
+    L.irtt_device_info.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+    L.irtt_scene_create.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp,
+                                    ctypes.c_uint32, ctypes.POINTER(_vp)]
+    L.irtt_scene_destroy.argtypes = [_vp]
+    L.irtt_scene_info_get.argtypes = [_vp, ctypes.POINTER(SceneInfo)]
+    L.irtt_scene_set_vertex_state.argtypes = [_vp, _vp, _vp]
+    L.irtt_scene_set_bin_state.argtypes = [_vp, _vp, _vp]
+    L.irtt_scene_update_vertices.argtypes = [_vp, _vp]
+    L.irtt_scene_get_bin_data.argtypes = [_vp, _vp, _vp, _vp]
+    L.irtt_build_bvh.argtypes = [_vp]
+    L.irtt_get_bvh.argtypes = [_vp, _vp, _vp, _vp, ctypes.POINTER(ctypes.c_uint32),
+                               ctypes.POINTER(ctypes.c_uint32)]
+    L.irtt_trace_rays.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_float, ctypes.c_float,
+                                  _vp, _vp, _vp, _vp]
+    L.irtt_build_transport.argtypes = [_vp, ctypes.POINTER(TransportParams),
+                                       ctypes.POINTER(TransportStats)]
+    L.irtt_get_transport.argtypes = [_vp, _vp]
+    L.irtt_get_transport_fixed.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_double)]
+    L.irtt_get_column_sums.argtypes = [_vp, _vp]
+    L.irtt_solve.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.POINTER(SolveStats)]
+    L.irtt_get_temperatures.argtypes = [_vp, _vp]
+    L.irtt_set_temperatures.argtypes = [_vp, _vp]
+    L.irtt_residual.argtypes = [_vp, ctypes.POINTER(SolverParams), _vp, _vp]
+
+
+def lib():
+    """Loads libirtt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise IrttError(-3, f"{LIB_PATH} is missing: run __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().irtt_last_error()
+        raise IrttError(rc, msg.decode() if msg else "")
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    cu = ctypes.c_int()
+    _check(lib().irtt_device_info(buf, 256, ctypes.byref(cu)))
+    return buf.value.decode(), cu.value
+
+
+def make_materials(nt, rho_d=0.0, rho_s=0.0, thickness=1.0, density=1.0, heat_capacity=1.0,
+                   conductivity=0.0, heat_flux=0.0, flags=MAT_DIFFUSE_EMISSION | MAT_TRACEABLE):
+    m = np.zeros(nt, MATERIAL_DTYPE)
+    m["diffuse_reflectance"] = rho_d
+    m["specular_reflectance"] = rho_s
+    m["thickness"] = thickness
+    m["density"] = density
+    m["heat_capacity"] = heat_capacity
+    m["heat_conductivity"] = conductivity
+    m["heat_flux"] = heat_flux
+    m["flags"] = flags
+    return m
+
+
+def transport_params(threads_per_tri=32768, rays_per_thread=8, batch_seed=1234, max_depth=512,
+                     tiny_dist=1e-3):
+    p = TransportParams()
+    p.threads_per_tri = threads_per_tri
+    p.rays_per_thread = rays_per_thread
+    p.batch_seed = batch_seed
+    p.max_depth = max_depth
+    p.tiny_dist = tiny_dist
+    return p
+
+
+def solver_params(steady=True, jacobian=JAC_EXACT, max_newton=100, max_linear=300, dt=300.0,
+                  ambient_kelvin=-1.0, x_tol=1e-5, f_tol=1e-12, lin_rtol=1e-10):
+    p = SolverParams()
+    p.steady = 1 if steady else 0
+    p.jacobian = jacobian
+    p.max_newton = max_newton
+    p.max_linear = max_linear
+    p.dt = dt
+    p.ambient_kelvin = ambient_kelvin
+    p.x_tol = x_tol
+    p.f_tol = f_tol
+    p.lin_rtol = lin_rtol
+    return p
+
+
+class Scene:
+    """One scene/design resident in HBM (an ``irtt_scene`` handle)."""
+
+    def __init__(self, xyz, idx, materials, bin_mode=BIN_VERTEX):
+        self._h = None
+        self.xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        mats = np.ascontiguousarray(materials, MATERIAL_DTYPE)
+        if mats.shape[0] != self.idx.shape[0]:
+            raise ValueError("one material per triangle")
+        h = ctypes.c_void_p()
+        _check(lib().irtt_scene_create(_ptr(self.xyz), self.xyz.shape[0], _ptr(self.idx),
+                                       self.idx.shape[0], _ptr(mats), bin_mode,
+                                       ctypes.byref(h)))
+        self._h = h
+        self.info = self.get_info()
+
+    @classmethod
+    def from_data(cls, sd, bin_mode=BIN_VERTEX):
+        """From an ``irtt.scenes.SceneData``."""
+        s = cls(sd.xyz, sd.idx, sd.materials(), bin_mode)
+        s.set_vertex_state(sd.kelvin, sd.fixed)
+        return s
+
+    def close(self):
+        if self._h is not None and self._h.value:
+            lib().irtt_scene_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def get_info(self):
+        i = SceneInfo()
+        _check(lib().irtt_scene_info_get(self._h, ctypes.byref(i)))
+        return {n: int(getattr(i, n)) for n, _ in i._fields_}
+
+    @property
+    def n_bins(self):
+        return self.info["n_bins"]
+
+    def set_vertex_state(self, kelvin, fixed=None):
+        k = np.ascontiguousarray(kelvin, np.float32)
+        f = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        _check(lib().irtt_scene_set_vertex_state(self._h, _ptr(k), _ptr(f)))
+
+    def set_bin_state(self, kelvin, fixed=None):
+        k = np.ascontiguousarray(kelvin, np.float32)
+        f = None if fixed is None else np.ascontiguousarray(fixed, np.uint8)
+        _check(lib().irtt_scene_set_bin_state(self._h, _ptr(k), _ptr(f)))
+
+    def update_vertices(self, xyz):
+        x = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+        _check(lib().irtt_scene_update_vertices(self._h, _ptr(x)))
+        self.xyz = x
+
+    def bin_data(self):
+        n = self.n_bins
+        a, e, m = np.empty(n), np.empty(n), np.empty(n)
+        _check(lib().irtt_scene_get_bin_data(self._h, _ptr(a), _ptr(e), _ptr(m)))
+        return a, e, m
+
+    def build_bvh(self):
+        _check(lib().irtt_build_bvh(self._h))
+        self.info = self.get_info()
+
+    def get_bvh(self):
+        n = ctypes.c_uint32()
+        d = ctypes.c_uint32()
+        _check(lib().irtt_get_bvh(self._h, None, None, None, ctypes.byref(n), ctypes.byref(d)))
+        n = n.value
+        nodes = np.zeros((max(n - 1, 0), 16), np.uint32)
+        tris = np.zeros((n, 12), np.uint32)
+        morton = np.zeros(n, np.uint32)
+        _check(lib().irtt_get_bvh(self._h, _ptr(nodes) if n > 1 else None, _ptr(tris),
+                                  _ptr(morton), None, None))
+        return nodes, tris, morton, d.value
+
+    def trace_rays(self, rays, tmin=1e-12, tmax=1e12):
+        r = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        n = r.shape[0]
+        prim = np.empty(n, np.int32)
+        t = np.empty(n, np.float32)
+        b1 = np.empty(n, np.float32)
+        b2 = np.empty(n, np.float32)
+        _check(lib().irtt_trace_rays(self._h, _ptr(r), n, tmin, tmax, _ptr(prim), _ptr(t),
+                                     _ptr(b1), _ptr(b2)))
+        return prim, t, b1, b2
+
+    def build_transport(self, params=None, **kw):
+        p = params if params is not None else transport_params(**kw)
+        st = TransportStats()
+        _check(lib().irtt_build_transport(self._h, ctypes.byref(p), ctypes.byref(st)))
+        self.info = self.get_info()
+        return st.as_dict()
+
+    def transport(self):
+        n = self.n_bins
+        T = np.empty((n, n), np.float32)
+        _check(lib().irtt_get_transport(self._h, _ptr(T)))
+        return T
+
+    def transport_fixed(self):
+        n = self.n_bins
+        acc = np.empty((n, n), np.uint64)
+        q = ctypes.c_double()
+        _check(lib().irtt_get_transport_fixed(self._h, _ptr(acc), ctypes.byref(q)))
+        return acc, q.value
+
+    def column_sums(self):
+        c = np.empty(self.n_bins)
+        _check(lib().irtt_get_column_sums(self._h, _ptr(c)))
+        return c
+
+    def solve(self, params=None, **kw):
+        p = params if params is not None else solver_params(**kw)
+        st = SolveStats()
+        _check(lib().irtt_solve(self._h, ctypes.byref(p), ctypes.byref(st)))
+        return st.as_dict()
+
+    def temperatures(self):
+        x = np.empty(self.n_bins)
+        _check(lib().irtt_get_temperatures(self._h, _ptr(x)))
+        return x
+
+    def set_temperatures(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        _check(lib().irtt_set_temperatures(self._h, _ptr(x)))
+
+    def residual(self, x, params=None, **kw):
+        p = params if params is not None else solver_params(**kw)
+        x = np.ascontiguousarray(x, np.float64)
+        r = np.empty(self.n_bins)
+        _check(lib().irtt_residual(self._h, ctypes.byref(p), _ptr(x), _ptr(r)))
+        return r

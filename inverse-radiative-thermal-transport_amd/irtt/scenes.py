@@ -1,0 +1,229 @@
+"""Synthetic scene generators (numpy, seeded).
+
+* ``parallel_plates`` -- two coaxial 1 m x 1 m plates 1 m apart, each a shared
+  17 x 17 vertex grid of 16 x 16 quads (512 triangles), facing each other.
+* ``city_block`` -- 8 x 8 box buildings (footprints 10-20 m, heights 6-40 m,
+  >= 5 m streets) on a ground plane, every surface subdivided to roughly equal
+  triangles (about 50k in total by default); per-patch emissivity, solar
+  absorptivity and areal heat capacity; direct sun at a given elevation.
+* ``closed_box`` -- the inside of a closed unit cube (every photon is absorbed
+  somewhere), used for conservation checks.
+
+Each returns a ``SceneData`` with per-triangle materials and per-vertex
+initial / fixed temperatures, the per-object inputs of the reference's
+``ObjectProperties`` (src/implementations/thermal_renderer/thermal_api.cpp:25-45)
+flattened to triangles.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import MATERIAL_DTYPE, MAT_DIFFUSE_EMISSION, MAT_TRACEABLE
+
+
+@dataclass
+class SceneData:
+    name: str
+    xyz: np.ndarray          # (nv, 3) float32
+    idx: np.ndarray          # (nt, 3) uint32
+    rho_d: np.ndarray        # (nt,) float32
+    rho_s: np.ndarray        # (nt,) float32
+    flags: np.ndarray        # (nt,) uint8
+    thickness: np.ndarray    # (nt,)
+    density: np.ndarray
+    heat_capacity: np.ndarray
+    conductivity: np.ndarray
+    heat_flux: np.ndarray    # absorbed external flux, W/m^2
+    kelvin: np.ndarray       # (nv,) initial / fixed temperature
+    fixed: np.ndarray        # (nv,) uint8
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nv(self):
+        return self.xyz.shape[0]
+
+    @property
+    def nt(self):
+        return self.idx.shape[0]
+
+    def materials(self):
+        m = np.zeros(self.nt, MATERIAL_DTYPE)
+        m["diffuse_reflectance"] = self.rho_d
+        m["specular_reflectance"] = self.rho_s
+        m["thickness"] = self.thickness
+        m["density"] = self.density
+        m["heat_capacity"] = self.heat_capacity
+        m["heat_conductivity"] = self.conductivity
+        m["heat_flux"] = self.heat_flux
+        m["flags"] = self.flags
+        return m
+
+    def triangle_areas(self):
+        p = self.xyz.astype(np.float64)[self.idx]
+        c = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+        return 0.5 * np.linalg.norm(c, axis=1)
+
+    def normals(self):
+        p = self.xyz.astype(np.float64)[self.idx]
+        c = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+        return c / np.linalg.norm(c, axis=1)[:, None]
+
+
+def _uniform_scene(name, xyz, idx, rho_d=0.0, rho_s=0.0, flags=MAT_DIFFUSE_EMISSION | MAT_TRACEABLE,
+                   thickness=1.0, density=1.0, heat_capacity=1.0, conductivity=0.0,
+                   heat_flux=0.0, kelvin=0.0):
+    nt = idx.shape[0]
+    nv = xyz.shape[0]
+
+    def f(v, dt=np.float32):
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (nt,)))
+
+    return SceneData(name, np.ascontiguousarray(xyz, np.float32),
+                     np.ascontiguousarray(idx, np.uint32), f(rho_d), f(rho_s), f(flags, np.uint8),
+                     f(thickness), f(density), f(heat_capacity), f(conductivity), f(heat_flux),
+                     np.ascontiguousarray(np.broadcast_to(np.float32(kelvin), (nv,))),
+                     np.zeros(nv, np.uint8))
+
+
+def grid_patch(origin, u, v, nu, nv_):
+    """Vertices and triangles of a planar patch origin + a*u + b*v, a,b in
+    [0,1], subdivided nu x nv_; normal = u x v."""
+    origin, u, v = (np.asarray(a, np.float64) for a in (origin, u, v))
+    a = np.linspace(0.0, 1.0, nu + 1)
+    b = np.linspace(0.0, 1.0, nv_ + 1)
+    A, B = np.meshgrid(a, b, indexing="ij")
+    P = origin + A[..., None] * u + B[..., None] * v
+    xyz = P.reshape(-1, 3)
+    i = np.arange(nu)[:, None]
+    j = np.arange(nv_)[None, :]
+    v00 = (i * (nv_ + 1) + j).ravel()
+    v10 = ((i + 1) * (nv_ + 1) + j).ravel()
+    v11 = ((i + 1) * (nv_ + 1) + j + 1).ravel()
+    v01 = (i * (nv_ + 1) + j + 1).ravel()
+    tri = np.concatenate([np.stack([v00, v10, v11], 1), np.stack([v00, v11, v01], 1)])
+    # interleave so the two triangles of a quad are adjacent (quad q -> 2q, 2q+1)
+    nq = v00.size
+    order = np.empty(2 * nq, np.int64)
+    order[0::2] = np.arange(nq)
+    order[1::2] = np.arange(nq) + nq
+    return xyz, tri[order]
+
+
+def parallel_plates(n=16, gap=1.0, emissivity=0.9, heat_flux_a=500.0, kelvin=0.0):
+    """Plate A at z=0 facing +z, plate B at z=gap facing -z, both [0,1]^2."""
+    xa, ta = grid_patch([0, 0, 0], [1, 0, 0], [0, 1, 0], n, n)
+    xb, tb = grid_patch([0, 0, gap], [0, 1, 0], [1, 0, 0], n, n)  # swapped axes: normal -z
+    xyz = np.concatenate([xa, xb])
+    idx = np.concatenate([ta, tb + xa.shape[0]])
+    nt = idx.shape[0]
+    flux = np.zeros(nt, np.float32)
+    flux[: ta.shape[0]] = heat_flux_a
+    sd = _uniform_scene("parallel_plates", xyz, idx, rho_d=1.0 - emissivity, heat_flux=flux,
+                        kelvin=kelvin)
+    sd.meta.update(n=n, gap=gap, emissivity=emissivity, plate_a_tris=ta.shape[0],
+                   plate_a_verts=xa.shape[0])
+    return sd
+
+
+def closed_box(n=4, emissivity=1.0):
+    """Inside of the unit cube, every face subdivided n x n, normals inward."""
+    faces = [  # normal = u x v
+        ([0, 0, 0], [1, 0, 0], [0, 1, 0]),  # z=0, normal +z
+        ([0, 0, 1], [0, 1, 0], [1, 0, 0]),  # z=1, normal -z
+        ([0, 0, 0], [0, 0, 1], [1, 0, 0]),  # y=0, normal +y
+        ([0, 1, 0], [1, 0, 0], [0, 0, 1]),  # y=1, normal -y
+        ([0, 0, 0], [0, 1, 0], [0, 0, 1]),  # x=0, normal +x
+        ([1, 0, 0], [0, 0, 1], [0, 1, 0]),  # x=1, normal -x
+    ]
+    xs, ts, off = [], [], 0
+    for o, u, v in faces:
+        x, t = grid_patch(o, u, v, n, n)
+        xs.append(x)
+        ts.append(t + off)
+        off += x.shape[0]
+    return _uniform_scene("closed_box", np.concatenate(xs), np.concatenate(ts),
+                          rho_d=1.0 - emissivity)
+
+
+def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fmax=20.0,
+               hmin=6.0, hmax=40.0, ground_cell=2.5, sun_elevation_deg=45.0,
+               sun_azimuth_deg=135.0, sun_flux=800.0, eps_range=(0.85, 0.95),
+               alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5), kelvin=293.0):
+    """Synthetic city block; returns SceneData with meta on the layout.
+
+    Footprints are multiples of ``ground_cell`` so walls meet the ground grid
+    exactly.  Patch = one quad (two triangles).  The direct sun term is the
+    unshadowed absorbed flux alpha * flux * max(0, n.s) per patch."""
+    rng = np.random.default_rng(seed)
+    pitch = fmax + street
+    gcells_per_pitch = int(round(pitch / ground_cell))
+    ncx, ncy = nx * gcells_per_pitch, ny * gcells_per_pitch
+    occupied = np.zeros((ncx, ncy), bool)
+    bld = []
+    for bi in range(nx):
+        for bj in range(ny):
+            wc = int(round(rng.uniform(fmin, fmax) / ground_cell))
+            dc = int(round(rng.uniform(fmin, fmax) / ground_cell))
+            h = float(rng.uniform(hmin, hmax))
+            ox = bi * gcells_per_pitch + (gcells_per_pitch - wc) // 2
+            oy = bj * gcells_per_pitch + (gcells_per_pitch - dc) // 2
+            occupied[ox:ox + wc, oy:oy + dc] = True
+            bld.append((ox, oy, wc, dc, h))
+    # total surface to size the wall/roof subdivision
+    ground_area = (~occupied).sum() * ground_cell ** 2
+    wall_area = sum(2 * (wc + dc) * ground_cell * h for _, _, wc, dc, h in bld)
+    roof_area = sum(wc * dc * ground_cell ** 2 for _, _, wc, dc, _ in bld)
+    ground_tris = 2 * int((~occupied).sum())
+    rest = max(target_tris - ground_tris, 1000)
+    edge = np.sqrt(2.0 * (wall_area + roof_area) / rest)
+    xs, ts, off = [], [], 0
+
+    def add(o, u, v, nu, nv_):
+        nonlocal off
+        x, t = grid_patch(o, u, v, nu, nv_)
+        xs.append(x)
+        ts.append(t + off)
+        off += x.shape[0]
+
+    # ground: one patch per maximal run of free cells along x in each row,
+    # subdivided per cell so vertex spacing matches the walls' feet
+    g = ground_cell
+    for cy in range(ncy):
+        cx = 0
+        while cx < ncx:
+            if occupied[cx, cy]:
+                cx += 1
+                continue
+            c0 = cx
+            while cx < ncx and not occupied[cx, cy]:
+                cx += 1
+            add([c0 * g, cy * g, 0.0], [(cx - c0) * g, 0, 0], [0, g, 0], cx - c0, 1)
+    for ox, oy, wc, dc, h in bld:
+        x0, y0, x1, y1 = ox * g, oy * g, (ox + wc) * g, (oy + dc) * g
+        nh = max(1, int(round(h / edge)))
+        nwx = max(1, int(round((x1 - x0) / edge)))
+        nwy = max(1, int(round((y1 - y0) / edge)))
+        # walls, outward normals (u x v points out)
+        add([x0, y0, 0], [x1 - x0, 0, 0], [0, 0, h], nwx, nh)   # y=y0, normal -y
+        add([x1, y1, 0], [x0 - x1, 0, 0], [0, 0, h], nwx, nh)   # y=y1, normal +y
+        add([x1, y0, 0], [0, y1 - y0, 0], [0, 0, h], nwy, nh)   # x=x1, normal +x
+        add([x0, y1, 0], [0, y0 - y1, 0], [0, 0, h], nwy, nh)   # x=x0, normal -x
+        add([x0, y0, h], [x1 - x0, 0, 0], [0, y1 - y0, 0], nwx, nwy)  # roof, +z
+    xyz = np.concatenate(xs)
+    idx = np.concatenate(ts)
+    nt = idx.shape[0]
+    nq = nt // 2
+    eps = np.repeat(rng.uniform(*eps_range, nq), 2).astype(np.float32)
+    alpha = np.repeat(rng.uniform(*alpha_range, nq), 2)
+    cap = np.repeat(rng.uniform(*capacity_range, nq), 2).astype(np.float32)
+    el, az = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg)
+    sun = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    sd = _uniform_scene("city_block", xyz, idx, rho_d=1.0 - eps, heat_capacity=cap,
+                        kelvin=kelvin)
+    cosi = np.clip(sd.normals() @ sun, 0.0, None)
+    sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
+    areas = sd.triangle_areas()
+    sd.meta.update(seed=seed, buildings=len(bld), edge=float(edge), sun=sun.tolist(),
+                   mean_triangle_area=float(areas.mean()), total_area=float(areas.sum()),
+                   alpha=alpha.astype(np.float32))
+    return sd
