@@ -57,7 +57,7 @@ struct Sys {
     const double *mass, *b, *kfix, *xprev;
     const uint8_t *fixed;
     double dt;
-    int steady, jac_ref;
+    int steady, jmode;  // jmode: J_EXACT, J_REFERENCE or J_PREDICT
 };
 
 __device__ __forceinline__ double lap_row(const Sys &s, uint32_t i, const double *v)
@@ -105,21 +105,25 @@ __global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *_
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
-// w = exact: fixed ? 0 : 4 x^3 v ; reference: fixed ? 0 : v
+// Jacobian-vector product modes
+enum { J_EXACT = 0, J_REFERENCE = 1, J_PREDICT = 2 };
+
+// w = exact: fixed ? 0 : 4 x^3 v ; reference / predictor: fixed ? 0 : v
 __global__ void k_jv_pre(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
                          double *__restrict__ w)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nb) return;
     double vi = s.fixed[i] ? 0.0 : v[i];
-    if (!s.jac_ref) {
+    if (s.jmode == J_EXACT) {
         double x = xe[i];
         vi = 4.0 * x * x * x * vi;
     }
     w[i] = vi;
 }
 
-// out = J v from Tw (= T w)
+// out = J v from Tw (= T w).  J_PREDICT is the operator of the steady
+// radiative balance in y = x^4 alone (T), used for the initial guess.
 __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
                           const double *__restrict__ Tw, double *__restrict__ out)
 {
@@ -129,15 +133,19 @@ __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__
         out[i] = v[i];
         return;
     }
+    if (s.jmode == J_PREDICT) {
+        out[i] = Tw[i];
+        return;
+    }
     double lv = lap_row(s, i, v);
     double rad = Tw[i];
-    if (s.jac_ref) {
+    if (s.jmode == J_REFERENCE) {
         double x = xe[i];
         rad = 4.0 * x * x * x * rad;  // diag(4x^3) T v, thermal_solver.cpp:149
     }
     if (s.steady) {
         double o = rad + lv;
-        if (s.jac_ref) o += v[i] / s.dt;  // I/dt regulariser, thermal_solver.cpp:150
+        if (s.jmode == J_REFERENCE) o += v[i] / s.dt;  // I/dt regulariser, thermal_solver.cpp:150
         out[i] = o;
     } else {
         out[i] = s.mass[i] * v[i] + s.dt * (rad + lv);
@@ -225,9 +233,27 @@ __global__ void k_jdiag(Sys s, const double *__restrict__ xe, const double *__re
     }
     double x = xe[i];
     double rad = 4.0 * x * x * x * Tdiag[i];
-    double d = s.steady ? rad + Ldiag[i] + (s.jac_ref ? 1.0 / s.dt : 0.0)
-                        : s.mass[i] + s.dt * (rad + Ldiag[i]);
+    double d;
+    if (s.jmode == J_PREDICT) d = Tdiag[i];
+    else if (s.steady) d = rad + Ldiag[i] + (s.jmode == J_REFERENCE ? 1.0 / s.dt : 0.0);
+    else d = s.mass[i] + s.dt * (rad + Ldiag[i]);
     dinv[i] = (d != 0.0 && isfinite(d)) ? 1.0 / d : 1.0;
+}
+
+// after the predictor solve T dy = -r (at x_g): x = (x_g^4 + dy)^(1/4) on
+// free bins whose updated y stays positive
+__global__ void k_y_update(Sys s, const double *__restrict__ xe, const double *__restrict__ dy,
+                           double *__restrict__ x)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    double xg = xe[i];
+    if (s.fixed[i]) {
+        x[i] = xg;
+        return;
+    }
+    double y = xg * xg * xg * xg + dy[i];
+    x[i] = (y > 0.0 && isfinite(y)) ? sqrt(sqrt(y)) : xg;
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -355,7 +381,7 @@ static void setup(Scene &s, Ctx &c, const irtt_solver_params &p, FixedBuf &fb)
     y.fixed = fb.b.p;
     y.dt = p.dt > 0.0 ? p.dt : 300.0;
     y.steady = p.steady ? 1 : 0;
-    y.jac_ref = p.jacobian == IRTT_JAC_REFERENCE ? 1 : 0;
+    y.jmode = p.jacobian == IRTT_JAC_REFERENCE ? J_REFERENCE : J_EXACT;
 }
 
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r)
@@ -367,6 +393,63 @@ void residual_host(Scene &s, const irtt_solver_params &p, const double *x, doubl
     c.residual(c.v[V_XT]);
     IRTT_HIP(hipMemcpyAsync(r, c.v[V_R], s.nb * sizeof(double), hipMemcpyDeviceToHost, s.stream));
     IRTT_HIP(hipStreamSynchronize(s.stream));
+}
+
+// BiCGSTAB (van der Vorst 1992), right Jacobi preconditioning, on J dx = -r
+// with J the current Jacobian mode at V_XE and r in V_R; dx0 = 0, result in
+// V_DX.  Stops at ||r_lin||^2 <= lin_rtol^2 * rn (rn = ||r||^2).  Returns the
+// iteration count.
+static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
+{
+    const uint32_t nb = c.nb;
+    const uint32_t g = cdiv(nb, 256);
+    hipStream_t stm = c.st;
+    k_jdiag<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_TDIAG], c.v[V_LDIAG], c.v[V_DINV]);
+    double *R = c.v[V_RHAT];
+    IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
+    IRTT_HIP(hipMemsetAsync(c.v[V_P], 0, nb * sizeof(double), stm));
+    IRTT_HIP(hipMemsetAsync(c.v[V_V], 0, nb * sizeof(double), stm));
+    double *rl = c.v[V_XT];  // linear residual, starts at -r
+    k_axpy2<<<g, 256, 0, stm>>>(nb, -2.0, c.v[V_R], c.v[V_R], rl);
+    IRTT_HIP(hipMemcpyAsync(R, rl, nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
+    double rho = 1.0, alpha = 1.0, omega = 1.0;
+    const double stop = lin_rtol * lin_rtol * rn;
+    double res[3];
+    uint32_t li = 0;
+    for (; li < max_lin; li++) {
+        c.dots(1, R, rl, nullptr, nullptr, nullptr, nullptr, res);
+        double rho_new = res[0];
+        if (rho_new == 0.0 || !std::isfinite(rho_new)) break;
+        double beta = (li == 0) ? 0.0 : (rho_new / rho) * (alpha / omega);
+        k_p_update<<<g, 256, 0, stm>>>(nb, beta, omega, rl, c.v[V_V], c.v[V_P]);
+        k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
+        c.jv(c.v[V_PH], c.v[V_V]);
+        c.dots(1, R, c.v[V_V], nullptr, nullptr, nullptr, nullptr, res);
+        if (res[0] == 0.0 || !std::isfinite(res[0])) break;
+        alpha = rho_new / res[0];
+        k_axpy2<<<g, 256, 0, stm>>>(nb, -alpha, rl, c.v[V_V], c.v[V_S]);
+        c.dots(1, c.v[V_S], c.v[V_S], nullptr, nullptr, nullptr, nullptr, res);
+        if (res[0] <= stop) {
+            k_axpy2<<<g, 256, 0, stm>>>(nb, alpha, c.v[V_DX], c.v[V_PH], c.v[V_DX]);
+            li++;
+            break;
+        }
+        k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
+        c.jv(c.v[V_SH], c.v[V_T]);
+        c.dots(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], nullptr, nullptr, res);
+        if (res[1] == 0.0) break;
+        omega = res[0] / res[1];
+        k_step<<<g, 256, 0, stm>>>(nb, alpha, omega, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T],
+                                   c.v[V_DX], rl);
+        c.dots(1, rl, rl, nullptr, nullptr, nullptr, nullptr, res);
+        rho = rho_new;
+        if (res[0] <= stop) {
+            li++;
+            break;
+        }
+        if (omega == 0.0) break;
+    }
+    return li;
 }
 
 void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
@@ -396,62 +479,24 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
     st = irtt_solve_stats{};
 
     double rn = c.residual(c.v[V_X]);
+    if (p.steady && p.jacobian != IRTT_JAC_REFERENCE) {
+        // initial guess: the steady radiative balance is linear in y = x^4, so
+        // solve T dy = -r(x_g) and take x = (x_g^4 + dy)^(1/4); exact without
+        // conduction, a close start otherwise.  (The reference relies on its
+        // I/dt regulariser instead, thermal_solver.cpp:150.)
+        c.sys.jmode = J_PREDICT;
+        st.linear_iters += bicgstab(c, max_lin, lin_rtol, rn);
+        k_y_update<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_DX], c.v[V_X]);
+        c.sys.jmode = J_EXACT;
+        rn = c.residual(c.v[V_X]);
+    }
     for (uint32_t it = 0; it < max_newton; it++) {
         st.newton_iters = it + 1;
         if (rn < f_tol) {
             st.converged = 1;
             break;
         }
-        // Jacobi diagonal at the current x (V_XE holds x with fixed values)
-        k_jdiag<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_TDIAG], c.v[V_LDIAG], c.v[V_DINV]);
-        // BiCGSTAB on J dx = -r, dx0 = 0: r0 = -r
-        double *R = c.v[V_RHAT];
-        IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
-        IRTT_HIP(hipMemsetAsync(c.v[V_P], 0, nb * sizeof(double), stm));
-        IRTT_HIP(hipMemsetAsync(c.v[V_V], 0, nb * sizeof(double), stm));
-        // lin residual in V_S as working "r_lin"; keep Newton r in V_R
-        k_axpy2<<<g, 256, 0, stm>>>(nb, -2.0, c.v[V_R], c.v[V_R], c.v[V_S]);  // s = -r
-        IRTT_HIP(hipMemcpyAsync(R, c.v[V_S], nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
-        double *rl = c.v[V_XT];  // linear residual
-        IRTT_HIP(hipMemcpyAsync(rl, c.v[V_S], nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
-        double rho = 1.0, alpha = 1.0, omega = 1.0;
-        const double bnorm2 = rn;
-        double res[3];
-        uint32_t li = 0;
-        for (; li < max_lin; li++) {
-            c.dots(1, R, rl, nullptr, nullptr, nullptr, nullptr, res);
-            double rho_new = res[0];
-            if (rho_new == 0.0 || !std::isfinite(rho_new)) break;
-            double beta = (li == 0) ? 0.0 : (rho_new / rho) * (alpha / omega);
-            k_p_update<<<g, 256, 0, stm>>>(nb, beta, omega, rl, c.v[V_V], c.v[V_P]);
-            k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
-            c.jv(c.v[V_PH], c.v[V_V]);
-            c.dots(1, R, c.v[V_V], nullptr, nullptr, nullptr, nullptr, res);
-            if (res[0] == 0.0 || !std::isfinite(res[0])) break;
-            alpha = rho_new / res[0];
-            k_axpy2<<<g, 256, 0, stm>>>(nb, -alpha, rl, c.v[V_V], c.v[V_S]);
-            c.dots(1, c.v[V_S], c.v[V_S], nullptr, nullptr, nullptr, nullptr, res);
-            if (res[0] <= lin_rtol * lin_rtol * bnorm2) {
-                k_axpy2<<<g, 256, 0, stm>>>(nb, alpha, c.v[V_DX], c.v[V_PH], c.v[V_DX]);
-                li++;
-                break;
-            }
-            k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
-            c.jv(c.v[V_SH], c.v[V_T]);
-            c.dots(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], nullptr, nullptr, res);
-            if (res[1] == 0.0) break;
-            omega = res[0] / res[1];
-            k_step<<<g, 256, 0, stm>>>(nb, alpha, omega, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T],
-                                       c.v[V_DX], rl);
-            c.dots(1, rl, rl, nullptr, nullptr, nullptr, nullptr, res);
-            rho = rho_new;
-            if (res[0] <= lin_rtol * lin_rtol * bnorm2) {
-                li++;
-                break;
-            }
-            if (omega == 0.0) break;
-        }
-        st.linear_iters += li;
+        st.linear_iters += bicgstab(c, max_lin, lin_rtol, rn);
         // damped line search, thermal_solver.cpp:192-203
         double a = 1.0;
         k_axpy2<<<g, 256, 0, stm>>>(nb, a, c.v[V_X], c.v[V_DX], c.v[V_XT]);

@@ -1,0 +1,112 @@
+"""GPU solver parity: the device Newton/BiCGSTAB solve against the FP64
+oracle on the same operator (downloaded from the device).  Tolerance: max
+|dT| <= 0.05 K (the stated bar); the measured error is printed."""
+import numpy as np
+import pytest
+
+from oracle import physics, solver
+
+pytestmark = pytest.mark.gpu
+
+TOL_K = 0.05
+
+
+def _oracle_rhs(s, sd, bin_mode, Ta):
+    _, _, _, src = physics.bins(sd, bin_mode)
+    return solver.rhs(src, s.column_sums(), Ta)
+
+
+def test_bin_assembly_matches_restatement(irtt):
+    from irtt import scenes
+    sd = scenes.city_block(seed=1, nx=2, ny=2, target_tris=2000)
+    for mode in (0, 1):
+        with irtt.Scene.from_data(sd, mode) as s:
+            a, ea, m = s.bin_data()
+        oa, oea, om, _ = physics.bins(sd, mode)
+        assert np.allclose(a, oa, rtol=1e-12) and np.allclose(ea, oea, rtol=1e-12)
+        assert np.allclose(m, om, rtol=1e-12)
+
+
+def test_single_plate_closed_form(irtt):
+    """A lone plate sees nothing: x = (q / (sigma eps) + Ta^4)^(1/4) exactly."""
+    from irtt import scenes
+    sd = scenes.parallel_plates(n=4, emissivity=0.9)
+    na = sd.meta["plate_a_tris"]
+    nva = sd.meta["plate_a_verts"]
+    xyz, idx = sd.xyz[:nva], sd.idx[:na]
+    mats = sd.materials()[:na]
+    with irtt.Scene(xyz, idx, mats, irtt.BIN_TRIANGLE) as s:
+        s.build_transport(threads_per_tri=64, rays_per_thread=4)
+        st = s.solve(irtt.solver_params(ambient_kelvin=293.0))
+        x = s.temperatures()
+    want = (500.0 / (5.670374419e-8 * 0.9) + 293.0 ** 4) ** 0.25
+    print(f"closed form {want:.6f} K, max |dT| {np.abs(x - want).max():.2e} K, {st}")
+    assert st["converged"] and np.abs(x - want).max() < 1e-6
+
+
+@pytest.mark.parametrize("jac", [0, 1])
+def test_plates_steady_vs_fp64(irtt, jac):
+    """Config 0 physics (eps 0.9, 500 W/m^2 on A, ambient 293 K, triangle bins,
+    1024 x 1024), fewer photons: device solve vs FP64 solve of the same T."""
+    from irtt import scenes
+    sd = scenes.parallel_plates(emissivity=0.9)
+    with irtt.Scene.from_data(sd, irtt.BIN_TRIANGLE) as s:
+        s.build_transport(threads_per_tri=512, rays_per_thread=8, batch_seed=0)
+        T = s.transport().astype(np.float64)
+        b = _oracle_rhs(s, sd, 1, 293.0)
+        st = s.solve(irtt.solver_params(jacobian=jac, ambient_kelvin=293.0, x_tol=1e-6,
+                                        f_tol=1e-14))
+        x = s.temperatures()
+    fixed = np.zeros(T.shape[0], bool)
+    x64 = solver.steady_linear(T, b, fixed, np.zeros(T.shape[0]))
+    err = np.abs(x - x64).max()
+    print(f"jacobian={jac} max|dT| vs FP64 = {err:.3e} K; plate A mean {x[:512].mean():.3f} K, "
+          f"plate B mean {x[512:].mean():.3f} K; {st}")
+    assert st["converged"] and err < TOL_K
+
+
+def test_conduction_and_fixed_vs_fp64(irtt):
+    """Vertex bins with the cotangent Laplacian and fixed bins: device solve vs
+    FP64 Newton with the restated Laplacian."""
+    from irtt import scenes
+    sd = scenes.closed_box(4, emissivity=0.85)
+    sd.conductivity[:] = 1.5
+    sd.thickness[:] = 0.1
+    sd.heat_flux[:32] = 300.0
+    sd.kelvin[:] = 290.0
+    sd.fixed[:25] = 1          # first face's vertices pinned
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=256, rays_per_thread=4, batch_seed=5)
+        T = s.transport().astype(np.float64)
+        b = _oracle_rhs(s, sd, 0, -1.0)
+        st = s.solve(irtt.solver_params(ambient_kelvin=-1.0, x_tol=1e-7, f_tol=1e-16))
+        x = s.temperatures()
+    L = physics.cotan_laplacian(sd.xyz, sd.idx, sd.conductivity * sd.thickness, sd.nv)
+    fixed = sd.fixed.astype(bool)
+    x64, _ = solver.newton(T, b, fixed, sd.kelvin.astype(np.float64), sd.kelvin.astype(np.float64),
+                           lap=L, tol=1e-12)
+    err = np.abs(x - x64).max()
+    print(f"max|dT| vs FP64 = {err:.3e} K; {st}")
+    assert np.all(x[:25] == 290.0)
+    assert st["converged"] and err < TOL_K
+
+
+def test_transient_step_vs_fp64(irtt):
+    from irtt import scenes
+    sd = scenes.parallel_plates(n=8, emissivity=0.9, kelvin=293.0)
+    sd.heat_capacity[:] = 2e4  # areal capacity (density 1, thickness 1)
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=256, rays_per_thread=8, batch_seed=3)
+        T = s.transport().astype(np.float64)
+        b = _oracle_rhs(s, sd, 0, 293.0)
+        _, _, mass = s.bin_data()
+        x0 = s.temperatures()
+        st = s.solve(irtt.solver_params(steady=False, dt=300.0, ambient_kelvin=293.0,
+                                        x_tol=1e-8, f_tol=1e-16))
+        x1 = s.temperatures()
+    fixed = np.zeros(len(x0), bool)
+    x64, _ = solver.newton(T, b, fixed, x0, x0, mass=mass, xprev=x0, dt=300.0, steady=False,
+                           tol=1e-12)
+    err = np.abs(x1 - x64).max()
+    print(f"transient step: max|dT| vs FP64 = {err:.3e} K, mean rise {np.mean(x1 - x0):.4f} K; {st}")
+    assert st["converged"] and err < TOL_K
