@@ -213,6 +213,7 @@ uint64_t Scene::device_bytes() const
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
            d_traceable.bytes() + bvh.nodes.bytes() + bvh.tris.bytes() + bvh.morton.bytes() +
            d_q.bytes() + d_acc.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
+           d_tdiag.bytes() +
            d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
            d_lap_col.bytes() + d_lap_val.bytes();
 }

@@ -117,6 +117,7 @@ struct Scene {
     DBuf<float> d_T;                  // nb*ld FP32 operator, row-major
     DBuf<double> d_diag;              // sigma*eps*A per bin
     DBuf<unsigned long long> d_colacc;  // column sums of d_acc
+    DBuf<float> d_tdiag;              // diagonal of d_T
     std::vector<double> colsum;        // column sums of T (exact)
     std::vector<double> Tdiag;         // diagonal of the FP32 operator as stored
     DBuf<unsigned long long> d_stats;

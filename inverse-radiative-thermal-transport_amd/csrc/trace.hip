@@ -289,11 +289,12 @@ __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
 }
 
 // T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
-// integer column sums of acc are accumulated per 256-row tile.
+// integer column sums of acc are accumulated per 256-row tile, and the stored
+// diagonal is gathered into tdiag.
 constexpr int kFinRows = 256;
 __global__ void k_finalize(const unsigned long long *__restrict__ acc, const double *__restrict__ diag,
                            uint32_t nb, uint32_t ld, double Q, float *__restrict__ T,
-                           unsigned long long *colacc)
+                           unsigned long long *colacc, float *__restrict__ tdiag)
 {
     uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= ld) return;
@@ -304,7 +305,9 @@ __global__ void k_finalize(const unsigned long long *__restrict__ acc, const dou
         unsigned long long v = acc[o];
         s += v;
         double val = (i == j ? diag[i] : 0.0) - (double)v * Q;
-        T[o] = j < nb ? (float)val : 0.0f;
+        float f = j < nb ? (float)val : 0.0f;
+        T[o] = f;
+        if (i == j) tdiag[j] = f;
     }
     if (s && j < nb) atomicAdd(colacc + j, s);
 }
@@ -457,29 +460,26 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
 
     IRTT_HIP(hipEventRecord(s.ev0, stm));
     dim3 fg(cdiv(ld, 256), cdiv(nb, kFinRows));
-    k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p);
+    s.d_tdiag.alloc(nb);
+    k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p,
+                                    s.d_tdiag.p);
     IRTT_HIP(hipGetLastError());
     IRTT_HIP(hipEventRecord(s.ev1, stm));
     std::vector<unsigned long long> colacc(nb), hstats(16);
+    std::vector<float> dcol(nb);
     IRTT_HIP(hipMemcpyAsync(colacc.data(), s.d_colacc.p, nb * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(hstats.data(), s.d_stats.p, 16 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(dcol.data(), s.d_tdiag.p, nb * sizeof(float), hipMemcpyDeviceToHost,
+                            stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     IRTT_HIP(hipEventElapsedTime(&st.finalize_ms, s.ev0, s.ev1));
 
     s.colsum.assign(nb, 0.0);
     for (uint32_t j = 0; j < nb; j++) s.colsum[j] = diag[j] - (double)colacc[j] * Q;
     // diagonal of the stored operator, for the Jacobi preconditioner
-    s.Tdiag.assign(nb, 0.0);
-    {
-        std::vector<float> dcol(nb);
-        // strided copy of the diagonal: nb elements at stride nb+1
-        IRTT_HIP(hipMemcpy2DAsync(dcol.data(), sizeof(float), s.d_T.p, (ld + 1) * sizeof(float),
-                                  sizeof(float), nb, hipMemcpyDeviceToHost, stm));
-        IRTT_HIP(hipStreamSynchronize(stm));
-        for (uint32_t i = 0; i < nb; i++) s.Tdiag[i] = dcol[i];
-    }
+    s.Tdiag.assign(dcol.begin(), dcol.end());
     st.photons = total;
     st.rays = hstats[S_RAYS];
     st.absorbed = hstats[S_ABS];

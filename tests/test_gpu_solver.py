@@ -41,7 +41,9 @@ def test_single_plate_closed_form(irtt):
         x = s.temperatures()
     want = (500.0 / (5.670374419e-8 * 0.9) + 293.0 ** 4) ** 0.25
     print(f"closed form {want:.6f} K, max |dT| {np.abs(x - want).max():.2e} K, {st}")
-    assert st["converged"] and np.abs(x - want).max() < 1e-6
+    # the operator is stored in FP32 (relative rounding <= 6e-8 on sigma eps A,
+    # i.e. ~1e-5 K at 360 K); the ambient term uses exact column sums
+    assert st["converged"] and np.abs(x - want).max() < 1e-4
 
 
 @pytest.mark.parametrize("jac", [0, 1])
