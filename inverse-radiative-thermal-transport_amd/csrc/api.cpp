@@ -208,6 +208,8 @@ void upload_geometry(Scene &s)
 
 }  // namespace
 
+void irtt::set_last_error(const std::string &msg) { g_err = msg; }
+
 uint64_t Scene::device_bytes() const
 {
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +

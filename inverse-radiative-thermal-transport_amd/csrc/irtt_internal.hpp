@@ -131,6 +131,9 @@ struct Scene {
     uint64_t device_bytes() const;
 };
 
+// api.cpp: thread-local message returned by irtt_last_error()
+void set_last_error(const std::string &msg);
+
 // lbvh.hip
 void bvh_build(Scene &s);
 // trace.hip

@@ -1,0 +1,62 @@
+"""The C-ABI library loads and exports every function the headers declare;
+struct layouts match the ctypes mirrors; entry points that need no device
+behave as the reference's (thermal_api.cpp:347-358)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(?:int|const char \*|void)\s*\**\s*([a-z_][a-z0-9_]*)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header", ["irtt.h", "thermal_api_compat.h"])
+def test_every_declared_symbol_is_exported(irtt, header):
+    names = _declared(header)
+    assert len(names) >= 15
+    L = ctypes.CDLL(irtt.LIB_PATH)
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts(irtt):
+    assert ctypes.sizeof(irtt.Material) == 32
+    assert irtt.MATERIAL_DTYPE.itemsize == 32
+    assert ctypes.sizeof(irtt.TransportParams) == 32
+    assert ctypes.sizeof(irtt.SolverParams) == 56
+
+
+class ObjectProperties(ctypes.Structure):
+    """thermal_api.cpp:25-45"""
+    _fields_ = [("vertex_offset", ctypes.c_uint), ("vertex_count", ctypes.c_uint),
+                ("indices_offset", ctypes.c_uint), ("indices_count", ctypes.c_uint),
+                ("kelvin", ctypes.c_float), ("temperature_fixed", ctypes.c_bool),
+                ("thickness", ctypes.c_float), ("density", ctypes.c_float),
+                ("heat_capacity", ctypes.c_float), ("heat_conductivity", ctypes.c_float),
+                ("diffuse_reflectance", ctypes.c_float),
+                ("specular_reflectance", ctypes.c_float),
+                ("diffuse_emission", ctypes.c_bool), ("traceable", ctypes.c_bool)]
+
+
+def test_object_properties_layout():
+    assert ctypes.sizeof(ObjectProperties) == 52
+    assert ObjectProperties.temperature_fixed.offset == 20
+    assert ObjectProperties.thickness.offset == 24
+    assert ObjectProperties.diffuse_emission.offset == 48
+    assert ObjectProperties.traceable.offset == 49
+
+
+def test_reference_test_entry_points(irtt):
+    L = ctypes.CDLL(irtt.LIB_PATH)
+    assert L.test_lib(21) == 42
+    arr = np.arange(1, 6, dtype=np.float32)
+    assert L.test_fill_array(arr.ctypes.data_as(ctypes.c_void_p), 5) == 0
+    assert np.array_equal(arr, np.arange(1, 6, dtype=np.float32) * np.arange(5) * 3)
