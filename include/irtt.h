@@ -183,6 +183,55 @@ int irtt_set_temperatures(irtt_scene h, const double *kelvin); /* current state 
 int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x,
                   double *r);
 
+/* ---------------------------------------------------------------- adjoint */
+/* Replaces ThermalRenderer::adjointTimestep (thermal_renderer.cpp:932-956):
+ * solves J(x)^T lambda = rhs with the transposed Jacobian of irtt_solve's
+ * system (p->jacobian selects the exact or the reference form; p->steady and
+ * p->dt select the steady or implicit-Euler system).  Fixed bins: rhs and
+ * lambda are zero there.  x: state to linearise at (n_bins), NULL = current
+ * state.  lambda: n_bins out. */
+int irtt_adjoint_solve(irtt_scene h, const irtt_solver_params *p, const double *x,
+                       const double *rhs, double *lambda, irtt_solve_stats *st);
+
+#define IRTT_ADJ_AREA 1u               /* source-area terms */
+#define IRTT_ADJ_CONTINUOUS 2u         /* geometric-term derivatives */
+#define IRTT_ADJ_EDGE 4u               /* approximate edge sampling */
+#define IRTT_ADJ_REFERENCE_DROPPED 8u  /* dropped photons still contribute with absorber bin 0
+                                          (the reference shader's behaviour) */
+
+typedef struct irtt_adjoint_params {
+    float edge_ratio;   /* ThermalVars_s::gradEdgeRatio (0.05), thermal_common.hpp:124 */
+    float tiny_eps;     /* ThermalVars_s::tinyEps (1e-2), thermal_common.hpp:96 */
+    uint32_t flags;     /* IRTT_ADJ_*; 0 = area | continuous | edge */
+    uint32_t reserved;
+} irtt_adjoint_params;
+
+typedef struct irtt_adjoint_stats {
+    uint64_t photons, absorbed, segments, continuous, edge_hits, occluded, rays;
+    float trace_ms;
+    uint32_t reserved;
+} irtt_adjoint_stats;
+
+/* Replaces ThermalTransport::computeAdjoint (thermal_transport.cpp:740-832):
+ * replays the photons of the last irtt_build_transport (same seeds and
+ * counts) and accumulates into grad_x (3 * n_vertices doubles, overwritten)
+ *     grad_x = - sum_h lambda_h^T (dT/dv) x4_h,
+ * the vertex gradient of the objective through the operator: each photon
+ * term is weighted by sum_h lambda_h[absorbing bin] * x4_h[emitting bin]
+ * (lambda and x4 are n_hist x n_bins, history-major; x4 is the emission
+ * factor x^4 per bin, as the reference uploads it, or any other weights). */
+int irtt_adjoint_trace(irtt_scene h, const irtt_adjoint_params *ap, uint32_t n_hist,
+                       const double *lambda, const double *x4, double *grad_x,
+                       irtt_adjoint_stats *st);
+
+/* Replaces Parametrization::projectGradient (thermal_params.cpp:130-156):
+ * dp[param[k]] += dxdp[3k..3k+2] . grad_x[3 vertex[k] ..] for k < nnz, where
+ * dxdp holds d(vertex position)/d(parameter) (world space).  dp: n_params,
+ * accumulated.  Host-side; needs no device. */
+int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nnz,
+                          const uint32_t *vertex, const uint32_t *param, const float *dxdp,
+                          uint32_t n_params, double *dp);
+
 #ifdef __cplusplus
 }
 #endif

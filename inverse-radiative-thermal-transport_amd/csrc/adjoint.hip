@@ -1,0 +1,440 @@
+// adjoint.hip -- adjoint photon trace: d(objective)/d(vertex positions).
+//
+// Replaces ThermalTransport::computeAdjoint (thermal_transport.cpp:740-832)
+// and the raygen program in TRACING_MODE 2
+// (assets/shader/raytracing_thermal/thermal_tracing_unified.glsl:292-599).
+// Each photon is replayed twice from its seed: pass 0 finds the absorbing bin,
+// pass 1 accumulates, weighted by sum_h lambda_h[bin] * x_h^4[source bin]
+// (sumOverHistory, :227-244, indexed history-major so every history length
+// is handled, not only one step):
+//   (i)   -/+ dw/dv of the three source vertices at emission / absorption,
+//   (ii)  w * d ln|G| / dv for the six vertices of every path segment, with
+//         G = -(N_i.d)(N_j.d) / (|N_i||N_j|(r+eps)^4), d = p_j - p_i,
+//   (iii) the approximate edge term: a hit within edge_ratio of an edge casts
+//         one occlusion ray from just outside the edge; a front-facing,
+//         non-adjacent occluded surface receives +, the path's origin vertex +,
+//         the edge vertex - of the edge-irradiance derivative.
+// Contributions are FP32 (as the shader computes them), scaled and summed in
+// FP64 with hardware atomics on a 3 x n_vertices buffer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "device_math.hpp"
+#include "irtt_internal.hpp"
+#include "traverse.hpp"
+
+namespace irtt {
+namespace {
+
+__device__ __forceinline__ V3 add3(V3 a, V3 b) { return V3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ V3 scl3(V3 a, float s) { return V3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ bool finite3(V3 a)
+{
+    return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
+}
+__device__ __forceinline__ int min_corner(float b0, float b1, float b2)
+{
+    int idx = 0;
+    float v = b0;
+    if (b1 < v) {
+        idx = 1;
+        v = b1;
+    }
+    if (b2 < v) idx = 2;
+    return idx;
+}
+
+// d|N|/2 (area) w.r.t. the three vertices
+__device__ __forceinline__ void area_grad(V3 A, V3 B, V3 C, V3 g[3])
+{
+    V3 e1 = B - A, e2 = C - A;
+    V3 n = normalize3(cross3(e1, e2));
+    V3 db = scl3(cross3(e2, n), 0.5f);
+    V3 dc = scl3(cross3(n, e1), 0.5f);
+    g[0] = scl3(add3(db, dc), -1.0f);
+    g[1] = db;
+    g[2] = dc;
+}
+
+// G and d ln|G| / d{a_i,b_i,c_i,a_j,b_j,c_j}
+__device__ __forceinline__ float geom_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3 Cj, float bi1,
+                                           float bi2, float bj1, float bj2, float eps, V3 out[6])
+{
+    float wi0 = 1.0f - bi1 - bi2, wj0 = 1.0f - bj1 - bj2;
+    V3 pi = bary_point(Ai, Bi, Ci, wi0, bi1, bi2);
+    V3 pj = bary_point(Aj, Bj, Cj, wj0, bj1, bj2);
+    V3 d = pj - pi;
+    float r = sqrt_rn(dot3(d, d));
+    V3 dh = scl3(d, 1.0f / r);
+    V3 e1i = Bi - Ai, e2i = Ci - Ai, e1j = Bj - Aj, e2j = Cj - Aj;
+    V3 Ni = cross3(e1i, e2i), Nj = cross3(e1j, e2j);
+    float li = sqrt_rn(dot3(Ni, Ni)), lj = sqrt_rn(dot3(Nj, Nj));
+    V3 ni = scl3(Ni, 1.0f / li), nj = scl3(Nj, 1.0f / lj);
+    float ci = dot3(Ni, d), cj = dot3(Nj, d);
+    float re = r + eps;
+    float re2 = re * re;
+    float G = -(ci * cj) / (li * lj * (re2 * re2));
+    const float kd[6] = {-wi0, -bi1, -bi2, wj0, bj1, bj2};
+    V3 e2xd_i = cross3(e2i, d), dxe1_i = cross3(d, e1i);
+    V3 e2xd_j = cross3(e2j, d), dxe1_j = cross3(d, e1j);
+    V3 ab_i = cross3(e2i, ni), ac_i = cross3(ni, e1i);
+    V3 ab_j = cross3(e2j, nj), ac_j = cross3(nj, e1j);
+    V3 dNi[3] = {scl3(add3(e2xd_i, dxe1_i), -1.0f), e2xd_i, dxe1_i};
+    V3 dNj[3] = {scl3(add3(e2xd_j, dxe1_j), -1.0f), e2xd_j, dxe1_j};
+    V3 dLi[3] = {scl3(add3(ab_i, ac_i), -1.0f), ab_i, ac_i};
+    V3 dLj[3] = {scl3(add3(ab_j, ac_j), -1.0f), ab_j, ac_j};
+#pragma unroll
+    for (int v = 0; v < 6; v++) {
+        V3 g_ci = scl3(Ni, kd[v]);
+        V3 g_cj = scl3(Nj, kd[v]);
+        V3 g_li{0.0f, 0.0f, 0.0f}, g_lj{0.0f, 0.0f, 0.0f};
+        if (v < 3) {
+            g_ci = add3(g_ci, dNi[v]);
+            g_li = dLi[v];
+        } else {
+            g_cj = add3(g_cj, dNj[v - 3]);
+            g_lj = dLj[v - 3];
+        }
+        V3 g_r = scl3(dh, kd[v]);
+        out[v] = add3(add3(scl3(g_ci, 1.0f / ci), scl3(g_cj, 1.0f / cj)),
+                      scl3(add3(add3(scl3(g_li, 1.0f / li), scl3(g_lj, 1.0f / lj)),
+                                scl3(g_r, 4.0f / re)),
+                           -1.0f));
+    }
+    return G;
+}
+
+__device__ __forceinline__ void acc3(double *g, uint32_t v, V3 d, double f)
+{
+    unsafeAtomicAdd(g + 3 * (size_t)v, f * (double)d.x);
+    unsafeAtomicAdd(g + 3 * (size_t)v + 1, f * (double)d.y);
+    unsafeAtomicAdd(g + 3 * (size_t)v + 2, f * (double)d.z);
+}
+
+struct AdjArgs {
+    const float *xyz;
+    const uint32_t *idx;
+    const float *rho_d, *rho_s;
+    const uint8_t *flags;
+    const float *w, *area;
+    const float4 *nodes, *tris;
+    uint32_t n_leaves;
+    uint32_t nt, nb, bin_mode;
+    uint32_t per_tri, rays_per_thread, batch_seed, max_depth;
+    float tiny_dist, eps, ratio;
+    uint32_t aflags, n_hist;
+    const double *lambda;  // n_hist x nb
+    const double *x4;      // n_hist x nb
+    uint64_t photons;
+    double *grad;
+    unsigned long long *stats;
+};
+
+enum { A_ABS = 0, A_SEG, A_CONT, A_EDGE, A_OCCL, A_RAYS, A_COUNT };
+
+__device__ __forceinline__ double hist_factor(const AdjArgs &a, uint32_t bin_l, uint32_t bin_x)
+{
+    double f = 0.0;
+    for (uint32_t h = 0; h < a.n_hist; h++)
+        f += a.lambda[(size_t)h * a.nb + bin_l] * a.x4[(size_t)h * a.nb + bin_x];
+    return f;
+}
+
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
+{
+    __shared__ uint32_t stk_n[STACK * kTraceBlock];
+    __shared__ float stk_t[STACK * kTraceBlock];
+    uint32_t *sn = stk_n + threadIdx.x;
+    float *stt = stk_t + threadIdx.x;
+    uint32_t cnt[A_COUNT] = {0, 0, 0, 0, 0, 0};
+    uint32_t nvis = 0, ntri = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x; g < a.photons; g += stride) {
+        uint32_t t = (uint32_t)(g / a.per_tri);
+        uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+        uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
+        const uint32_t seed0 = tea16(a.nt * xx + t, a.batch_seed + nn);
+        const uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+        const V3 p0{a.xyz[3 * i0], a.xyz[3 * i0 + 1], a.xyz[3 * i0 + 2]};
+        const V3 p1{a.xyz[3 * i1], a.xyz[3 * i1 + 1], a.xyz[3 * i1 + 2]};
+        const V3 p2{a.xyz[3 * i2], a.xyz[3 * i2 + 1], a.xyz[3 * i2 + 2]};
+        uint32_t abs_bin = 0;
+        bool absorbed_any = false;
+        for (int pass = 0; pass <= 1; pass++) {
+            uint32_t seed = seed0;
+            float u0 = rnd(seed), u1 = rnd(seed);
+            (void)rnd(seed);
+            float su = sqrt_rn(u0);
+            float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
+            V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
+            float len;
+            V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
+            V3 o = offset_ray(pos, nrm);
+            V3 d = nrm;
+            if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
+                float r0 = rnd(seed), r1 = rnd(seed);
+                (void)rnd(seed);
+                d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
+            }
+            if (!(len > 0.0f)) break;
+            int sc = max_corner(b0, b1, b2);
+            const uint32_t src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
+            const float wt = a.w[t];
+            V3 dw[3];
+            if (pass == 1 && (a.aflags & IRTT_ADJ_AREA)) {
+                // emission-side area term, thermal_tracing_unified.glsl:327-351
+                V3 ga[3];
+                area_grad(p0, p1, p2, ga);
+                float s = wt / a.area[t];
+                const double fsrc = hist_factor(a, src, src);
+                const uint32_t tv[3] = {i0, i1, i2};
+                for (int c = 0; c < 3; c++) {
+                    dw[c] = scl3(ga[c], s);
+                    if (finite3(dw[c])) acc3(a.grad, tv[c], dw[c], -fsrc);
+                }
+            }
+            if (pass == 1 && !absorbed_any && !(a.aflags & IRTT_ADJ_REFERENCE_DROPPED)) break;
+            const double fpath = pass ? hist_factor(a, abs_bin, src) : 0.0;
+            uint32_t prev0 = i0, prev1 = i1, prev2 = i2;
+            float pb1 = b1, pb2 = b2;
+            int prev_max = sc;
+            uint32_t depth = 0;
+            for (;;) {
+                ++depth;
+                if (depth > a.max_depth) break;
+                Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn,
+                                             stt, nvis, ntri);
+                if (pass == 0) cnt[A_RAYS]++;
+                if (h.prim < 0 || h.t < a.tiny_dist) break;
+                const uint32_t hp = (uint32_t)h.prim;
+                const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+                const V3 q0{a.xyz[3 * j0], a.xyz[3 * j0 + 1], a.xyz[3 * j0 + 2]};
+                const V3 q1{a.xyz[3 * j1], a.xyz[3 * j1 + 1], a.xyz[3 * j1 + 2]};
+                const V3 q2{a.xyz[3 * j2], a.xyz[3 * j2 + 1], a.xyz[3 * j2 + 2]};
+                V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
+                if (dot3(hn, d) > -1e-3f) break;
+                const float hb0 = 1.0f - h.b1 - h.b2;
+                if (pass == 1) {
+                    cnt[A_SEG]++;
+                    if (a.aflags & IRTT_ADJ_CONTINUOUS) {
+                        // continuous geometric term, :391-448
+                        const V3 r0{a.xyz[3 * prev0], a.xyz[3 * prev0 + 1], a.xyz[3 * prev0 + 2]};
+                        const V3 r1{a.xyz[3 * prev1], a.xyz[3 * prev1 + 1], a.xyz[3 * prev1 + 2]};
+                        const V3 r2{a.xyz[3 * prev2], a.xyz[3 * prev2 + 1], a.xyz[3 * prev2 + 2]};
+                        V3 gg[6];
+                        float G = geom_grad(r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps, gg);
+                        if (fabsf(G) > a.eps * a.eps) {
+                            cnt[A_CONT]++;
+                            const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
+                            for (int v = 0; v < 6; v++) {
+                                V3 dv = scl3(gg[v], wt);
+                                if (finite3(dv)) acc3(a.grad, vid[v], dv, fpath);
+                            }
+                        }
+                    }
+                    const float hb[3] = {hb0, h.b1, h.b2};
+                    const int opp = min_corner(hb0, h.b1, h.b2);
+                    if ((a.aflags & IRTT_ADJ_EDGE) && hb[opp] <= a.ratio) {
+                        // approximate edge term, :454-567
+                        cnt[A_EDGE]++;
+                        const int nr = max_corner(hb0, h.b1, h.b2);
+                        int other = 0;
+                        if (opp == 0 || nr == 0) {
+                            other = 1;
+                            if (opp == 1 || nr == 1) other = 2;
+                        }
+                        const V3 qv[3] = {q0, q1, q2};
+                        const uint32_t jv[3] = {j0, j1, j2};
+                        const uint32_t edge_v = jv[nr];
+                        V3 edir = qv[nr] - qv[other];
+                        V3 oppe = qv[nr] - qv[opp];
+                        float elen = sqrt_rn(dot3(edir, edir));
+                        edir = scl3(edir, 1.0f / elen);
+                        V3 n_edg = normalize3(cross3(edir, hn));
+                        float th = dot3(oppe, n_edg);
+                        if (th < 0.0f) {
+                            th = -th;
+                            n_edg = scl3(n_edg, -1.0f);
+                        }
+                        float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
+                        V3 da = normalize3(cross3(edir, d));
+                        if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
+                        V3 dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * wt / larea);
+                        if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
+                        V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+                        to = add3(to, scl3(n_edg, a.eps + dot3(qv[nr] - to, n_edg)));
+                        to = to - scl3(d, a.eps);
+                        Hit oc = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, to, d, 1e-12f,
+                                                      1e12f, sn, stt, nvis, ntri);
+                        if (oc.prim >= 0 && oc.t >= a.tiny_dist) {
+                            const uint32_t op = (uint32_t)oc.prim;
+                            const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1],
+                                           c2 = a.idx[3 * op + 2];
+                            bool nbr = c0 == j0 || c0 == j1 || c0 == j2 || c1 == j0 || c1 == j1 ||
+                                       c1 == j2 || c2 == j0 || c2 == j1 || c2 == j2;
+                            const V3 s0{a.xyz[3 * c0], a.xyz[3 * c0 + 1], a.xyz[3 * c0 + 2]};
+                            const V3 s1{a.xyz[3 * c1], a.xyz[3 * c1 + 1], a.xyz[3 * c1 + 2]};
+                            const V3 s2{a.xyz[3 * c2], a.xyz[3 * c2 + 1], a.xyz[3 * c2 + 2]};
+                            V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
+                            if (!nbr && dot3(tn, d) < -1e-3f) {
+                                cnt[A_OCCL]++;
+                                const float ob0 = 1.0f - oc.b1 - oc.b2;
+                                const int sm = max_corner(ob0, oc.b1, oc.b2);
+                                const uint32_t shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
+                                const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
+                                const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
+                                const float d1 = h.t, d2 = oc.t, tot = d1 + d2;
+                                const double fs = hist_factor(a, shadow_bin, src);
+                                const uint32_t origin_v =
+                                    prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
+                                acc3(a.grad, shadow_v, scl3(dE, sa * (d1 / tot)), fs);
+                                acc3(a.grad, origin_v, scl3(dE, sa * (d2 / tot)), fs);
+                                acc3(a.grad, edge_v, scl3(dE, -sa), fs);
+                            }
+                        }
+                    }
+                    prev0 = j0;
+                    prev1 = j1;
+                    prev2 = j2;
+                    pb1 = h.b1;
+                    pb2 = h.b2;
+                    prev_max = max_corner(hb0, h.b1, h.b2);
+                }
+                // getNextRay, :163-197
+                V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+                o = offset_ray(hpos, hn);
+                float u = rnd(seed);
+                float rd = a.rho_d[hp], rs = a.rho_s[hp];
+                bool absorbed = false;
+                if (u >= rd + rs) {
+                    absorbed = true;
+                } else if (u >= rd) {
+                    V3 in = normalize3(d);
+                    float dn = 2.0f * dot3(hn, in);
+                    d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
+                } else {
+                    float r0 = rnd(seed), r1 = rnd(seed);
+                    (void)rnd(seed);
+                    d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
+                }
+                if (absorbed || depth == a.max_depth) {
+                    const int hc = max_corner(hb0, h.b1, h.b2);
+                    const uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
+                    if (pass == 0) {
+                        abs_bin = ab;
+                        absorbed_any = true;
+                        cnt[A_ABS]++;
+                    } else if (a.aflags & IRTT_ADJ_AREA) {
+                        // absorption-side area term, :588-592
+                        const uint32_t tv[3] = {i0, i1, i2};
+                        for (int c = 0; c < 3; c++)
+                            if (finite3(dw[c])) acc3(a.grad, tv[c], dw[c], fpath);
+                    }
+                    break;
+                }
+            }
+        }
+    }
+    for (int s = 0; s < A_COUNT; s++) {
+        uint32_t v = cnt[s];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (threadIdx.x == 0 && v) atomicAdd(a.stats + s, (unsigned long long)v);
+    }
+}
+
+}  // namespace
+
+void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
+                   const double *x, double *grad, irtt_adjoint_stats &st)
+{
+    if (!s.has_transport)
+        throw Error(IRTT_ERR_STATE, "build the transport first: the adjoint replays its photons");
+    if (n_hist == 0) throw Error(IRTT_ERR_ARG, "n_hist must be positive");
+    if (!s.bvh.valid) bvh_build(s);
+    const irtt_transport_params &tp = s.tparams;
+    const uint32_t per = tp.threads_per_tri * tp.rays_per_thread;
+    const uint32_t max_depth =
+        (tp.max_depth == 0 || tp.max_depth > kMaxDepthDefault) ? kMaxDepthDefault : tp.max_depth;
+    const uint32_t nb = s.nb;
+    std::vector<float> w(s.nt);
+    for (uint32_t t = 0; t < s.nt; t++) {
+        float wt = s.tri_area[t] / (float)per;
+        w[t] = wt * (kSigma * s.eps[t]);
+    }
+    const double *x4 = x;  // emission factors per bin, n_hist x nb
+    hipStream_t stm = s.stream;
+    DBuf<float> d_w, d_area;
+    DBuf<double> d_lam, d_x4, d_grad;
+    DBuf<unsigned long long> d_st;
+    d_w.alloc(s.nt);
+    d_area.alloc(s.nt);
+    d_lam.alloc((size_t)n_hist * nb);
+    d_x4.alloc((size_t)n_hist * nb);
+    d_grad.alloc(3 * (size_t)s.nv);
+    d_st.alloc(8);
+    IRTT_HIP(hipMemcpyAsync(d_w.p, w.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
+    IRTT_HIP(hipMemcpyAsync(d_area.p, s.tri_area.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
+    IRTT_HIP(hipMemcpyAsync(d_lam.p, lambda, d_lam.bytes(), hipMemcpyHostToDevice, stm));
+    IRTT_HIP(hipMemcpyAsync(d_x4.p, x4, d_x4.bytes(), hipMemcpyHostToDevice, stm));
+    IRTT_HIP(hipMemsetAsync(d_grad.p, 0, d_grad.bytes(), stm));
+    IRTT_HIP(hipMemsetAsync(d_st.p, 0, d_st.bytes(), stm));
+    AdjArgs a{};
+    a.xyz = s.d_xyz.p;
+    a.idx = s.d_idx.p;
+    a.rho_d = s.d_rho_d.p;
+    a.rho_s = s.d_rho_s.p;
+    a.flags = s.d_flags.p;
+    a.w = d_w.p;
+    a.area = d_area.p;
+    a.nodes = s.bvh.nodes.p;
+    a.tris = s.bvh.tris.p;
+    a.n_leaves = s.bvh.n_leaves;
+    a.nt = s.nt;
+    a.nb = nb;
+    a.bin_mode = s.bin_mode;
+    a.per_tri = per;
+    a.rays_per_thread = tp.rays_per_thread;
+    a.batch_seed = tp.batch_seed;
+    a.max_depth = max_depth;
+    a.tiny_dist = tp.tiny_dist;
+    a.eps = ap.tiny_eps > 0 ? ap.tiny_eps : 1e-2f;
+    a.ratio = ap.edge_ratio > 0 ? ap.edge_ratio : 0.05f;
+    a.aflags = ap.flags ? ap.flags : (IRTT_ADJ_AREA | IRTT_ADJ_CONTINUOUS | IRTT_ADJ_EDGE);
+    a.n_hist = n_hist;
+    a.lambda = d_lam.p;
+    a.x4 = d_x4.p;
+    a.photons = (uint64_t)per * s.nt;
+    a.grad = d_grad.p;
+    a.stats = d_st.p;
+    int dev = 0, ncu = 256;
+    IRTT_HIP(hipGetDevice(&dev));
+    IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int S = stack_size_for(s.bvh.max_depth);
+    uint64_t want = (a.photons + kTraceBlock - 1) / kTraceBlock;
+    uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ncu * 64));
+    IRTT_HIP(hipEventRecord(s.ev0, stm));
+    if (S == 32) k_adjoint<32><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 64) k_adjoint<64><<<grid, kTraceBlock, 0, stm>>>(a);
+    else k_adjoint<128><<<grid, kTraceBlock, 0, stm>>>(a);
+    IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    std::vector<unsigned long long> hs(8);
+    IRTT_HIP(hipMemcpyAsync(grad, d_grad.p, d_grad.bytes(), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(hs.data(), d_st.p, 8 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                            stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
+    st.photons = a.photons;
+    st.absorbed = hs[A_ABS];
+    st.segments = hs[A_SEG];
+    st.continuous = hs[A_CONT];
+    st.edge_hits = hs[A_EDGE];
+    st.occluded = hs[A_OCCL];
+    st.rays = hs[A_RAYS];
+}
+
+}  // namespace irtt

@@ -488,4 +488,47 @@ int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x, do
     });
 }
 
+int irtt_adjoint_solve(irtt_scene h, const irtt_solver_params *p, const double *x,
+                       const double *rhs, double *lambda, irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !rhs || !lambda) throw Error(IRTT_ERR_ARG, "null pointer");
+        irtt_solve_stats tmp{};
+        adjoint_solve(*s, *p, x, rhs, lambda, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_adjoint_trace(irtt_scene h, const irtt_adjoint_params *ap, uint32_t n_hist,
+                       const double *lambda, const double *x, double *grad_x,
+                       irtt_adjoint_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!lambda || !x || !grad_x) throw Error(IRTT_ERR_ARG, "null pointer");
+        irtt_adjoint_params def{};
+        irtt_adjoint_stats tmp{};
+        adjoint_trace(*s, ap ? *ap : def, n_hist, lambda, x, grad_x, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nnz,
+                          const uint32_t *vertex, const uint32_t *param, const float *dxdp,
+                          uint32_t n_params, double *dp)
+{
+    return guard([&] {
+        if (nnz && (!grad_x || !vertex || !param || !dxdp || !dp))
+            throw Error(IRTT_ERR_ARG, "null pointer");
+        for (uint32_t k = 0; k < nnz; k++) {
+            if (vertex[k] >= n_vertices || param[k] >= n_params)
+                throw Error(IRTT_ERR_ARG, "projection entry " + std::to_string(k) + " out of range");
+            const double *g = grad_x + 3 * (size_t)vertex[k];
+            dp[param[k]] += (double)dxdp[3 * k] * g[0] + (double)dxdp[3 * k + 1] * g[1] +
+                            (double)dxdp[3 * k + 2] * g[2];
+        }
+    });
+}
+
 }  // extern "C"

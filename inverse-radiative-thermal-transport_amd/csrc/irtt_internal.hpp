@@ -125,6 +125,7 @@ struct Scene {
     // solver workspace
     DBuf<double> d_vec;               // 12 vectors of nb doubles
     DBuf<double> d_red;               // reduction scratch
+    DBuf<double> d_tpart;             // row-chunk partial sums of T^T u
     DBuf<uint32_t> d_lap_rowptr, d_lap_col;
     DBuf<double> d_lap_val;
 
@@ -143,5 +144,10 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
 // solver.hip
 void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st);
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r);
+void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const double *rhs,
+                   double *lambda, irtt_solve_stats &st);
+// adjoint.hip
+void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
+                   const double *x, double *grad, irtt_adjoint_stats &st);
 
 }  // namespace irtt

@@ -50,6 +50,32 @@ __global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, 
     if (lane == 0) y[row] = acc;
 }
 
+// y = T^T u in two deterministic steps: partial column sums over row chunks
+// of kTRows rows (threads along columns read coalesced rows), then a fixed-
+// order sum over the chunks.
+constexpr uint32_t kTRows = 512;
+__global__ __launch_bounds__(256) void k_gemv_t_partial(const float *__restrict__ T, uint32_t nb,
+                                                        uint32_t ld, const double *__restrict__ u,
+                                                        double *__restrict__ part)
+{
+    uint32_t j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= ld) return;
+    uint32_t r0 = blockIdx.y * kTRows, r1 = min(r0 + kTRows, nb);
+    double acc = 0.0;
+    for (uint32_t i = r0; i < r1; i++) acc += (double)T[(size_t)i * ld + j] * u[i];
+    part[(size_t)blockIdx.y * ld + j] = acc;
+}
+
+__global__ void k_gemv_t_reduce(const double *__restrict__ part, uint32_t nchunk, uint32_t ld,
+                                double *__restrict__ y)
+{
+    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ld) return;
+    double a = 0.0;
+    for (uint32_t c = 0; c < nchunk; c++) a += part[(size_t)c * ld + j];
+    y[j] = a;
+}
+
 struct Sys {
     uint32_t nb;
     const uint32_t *lrow, *lcol;
@@ -58,6 +84,7 @@ struct Sys {
     const uint8_t *fixed;
     double dt;
     int steady, jmode;  // jmode: J_EXACT, J_REFERENCE or J_PREDICT
+    int transposed;     // products with J^T (adjoint solve)
 };
 
 __device__ __forceinline__ double lap_row(const Sys &s, uint32_t i, const double *v)
@@ -108,14 +135,22 @@ __global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *_
 // Jacobian-vector product modes
 enum { J_EXACT = 0, J_REFERENCE = 1, J_PREDICT = 2 };
 
-// w = exact: fixed ? 0 : 4 x^3 v ; reference / predictor: fixed ? 0 : v
+// The radiative block is T D (exact) or D T (reference), D = diag(4x^3); its
+// transpose swaps the two.  D is applied before the operator product when
+// (exact != transposed), after it otherwise.
+__device__ __forceinline__ bool d_before(const Sys &s)
+{
+    return (s.jmode == J_EXACT) != (s.transposed != 0);
+}
+
+// w = fixed ? 0 : (D before the product ? 4 x^3 v : v)
 __global__ void k_jv_pre(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
                          double *__restrict__ w)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nb) return;
     double vi = s.fixed[i] ? 0.0 : v[i];
-    if (s.jmode == J_EXACT) {
+    if (s.jmode != J_PREDICT && d_before(s)) {
         double x = xe[i];
         vi = 4.0 * x * x * x * vi;
     }
@@ -137,11 +172,11 @@ __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__
         out[i] = Tw[i];
         return;
     }
-    double lv = lap_row(s, i, v);
+    double lv = lap_row(s, i, v);  // the cotangent Laplacian is symmetric
     double rad = Tw[i];
-    if (s.jmode == J_REFERENCE) {
+    if (!d_before(s)) {
         double x = xe[i];
-        rad = 4.0 * x * x * x * rad;  // diag(4x^3) T v, thermal_solver.cpp:149
+        rad = 4.0 * x * x * x * rad;  // reference: diag(4x^3) T v, thermal_solver.cpp:149
     }
     if (s.steady) {
         double o = rad + lv;
@@ -278,6 +313,14 @@ struct Ctx {
     {
         k_gemv_rows<<<cdiv(nb, 4), 256, 0, st>>>(s.d_T.p, nb, ld, u, y);
     }
+    void gemv_t(const double *u, double *y)
+    {
+        const uint32_t nchunk = cdiv(nb, kTRows);
+        s.d_tpart.alloc((size_t)nchunk * ld);
+        k_gemv_t_partial<<<dim3(cdiv(ld, 256), nchunk), 256, 0, st>>>(s.d_T.p, nb, ld, u,
+                                                                       s.d_tpart.p);
+        k_gemv_t_reduce<<<cdiv(ld, 256), 256, 0, st>>>(s.d_tpart.p, nchunk, ld, y);
+    }
     // returns sum of m dot products
     void dots(int m, const double *a0, const double *b0, const double *a1, const double *b1,
               const double *a2, const double *b2, double *res)
@@ -304,7 +347,8 @@ struct Ctx {
     {
         uint32_t g = cdiv(nb, 256);
         k_jv_pre<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_W]);
-        gemv(v[V_W], v[V_TW]);
+        if (sys.transposed) gemv_t(v[V_W], v[V_TW]);
+        else gemv(v[V_W], v[V_TW]);
         k_jv_post<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_TW], outv);
     }
 };
@@ -382,6 +426,7 @@ static void setup(Scene &s, Ctx &c, const irtt_solver_params &p, FixedBuf &fb)
     y.dt = p.dt > 0.0 ? p.dt : 300.0;
     y.steady = p.steady ? 1 : 0;
     y.jmode = p.jacobian == IRTT_JAC_REFERENCE ? J_REFERENCE : J_EXACT;
+    y.transposed = 0;
 }
 
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r)
@@ -450,6 +495,52 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
         if (omega == 0.0) break;
     }
     return li;
+}
+
+namespace {
+// r = fixed ? 0 : -rhs
+__global__ void k_neg_masked(Sys s, const double *__restrict__ rhs, double *__restrict__ r)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.nb) r[i] = s.fixed[i] ? 0.0 : -rhs[i];
+}
+}  // namespace
+
+// Adjoint solve J(x)^T lambda = rhs (thermal_renderer.cpp:932-956): the same
+// Krylov solver on the transposed products; fixed bins get lambda = 0.
+void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const double *rhs,
+                   double *lambda, irtt_solve_stats &st)
+{
+    Ctx c(s);
+    FixedBuf fb;
+    setup(s, c, p, fb);
+    c.sys.transposed = 1;
+    const uint32_t nb = s.nb;
+    const uint32_t g = cdiv(nb, 256);
+    hipStream_t stm = s.stream;
+    IRTT_HIP(hipEventRecord(s.ev0, stm));
+    IRTT_HIP(hipMemcpyAsync(c.v[V_X], x ? x : s.x.data(), nb * sizeof(double), hipMemcpyHostToDevice,
+                            stm));
+    k_pow4<<<g, 256, 0, stm>>>(c.sys, c.v[V_X], c.v[V_XE], c.v[V_Y]);
+    IRTT_HIP(hipMemcpyAsync(c.v[V_W], rhs, nb * sizeof(double), hipMemcpyHostToDevice, stm));
+    k_neg_masked<<<g, 256, 0, stm>>>(c.sys, c.v[V_W], c.v[V_R]);
+    double rn;
+    c.dots(1, c.v[V_R], c.v[V_R], nullptr, nullptr, nullptr, nullptr, &rn);
+    st = irtt_solve_stats{};
+    const uint32_t max_lin = p.max_linear ? p.max_linear : 200;
+    const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
+    if (rn > 0.0) st.linear_iters = bicgstab(c, max_lin, lin_rtol, rn);
+    else IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
+    std::vector<double> lam(nb);
+    IRTT_HIP(hipMemcpyAsync(lam.data(), c.v[V_DX], nb * sizeof(double), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    IRTT_HIP(hipEventElapsedTime(&st.solve_ms, s.ev0, s.ev1));
+    for (uint32_t i = 0; i < nb; i++) lambda[i] = s.bin_fixed[i] ? 0.0 : lam[i];
+    // residual of the linear solve, for the caller
+    st.newton_iters = 0;
+    st.converged = st.linear_iters < max_lin ? 1 : 0;
+    st.residual_sq = rn;
 }
 
 void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)

@@ -115,6 +115,44 @@ def _declare(L):
     L.irtt_get_temperatures.argtypes = [_vp, _vp]
     L.irtt_set_temperatures.argtypes = [_vp, _vp]
     L.irtt_residual.argtypes = [_vp, ctypes.POINTER(SolverParams), _vp, _vp]
+    L.irtt_adjoint_solve.argtypes = [_vp, ctypes.POINTER(SolverParams), _vp, _vp, _vp,
+                                     ctypes.POINTER(SolveStats)]
+    L.irtt_adjoint_trace.argtypes = [_vp, ctypes.POINTER(AdjointParams), ctypes.c_uint32, _vp,
+                                     _vp, _vp, ctypes.POINTER(AdjointStats)]
+    L.irtt_project_gradient.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
+                                        ctypes.c_uint32, _vp]
+
+
+ADJ_AREA = 1
+ADJ_CONTINUOUS = 2
+ADJ_EDGE = 4
+ADJ_REFERENCE_DROPPED = 8
+
+
+class AdjointParams(ctypes.Structure):
+    _fields_ = [("edge_ratio", ctypes.c_float), ("tiny_eps", ctypes.c_float),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded",
+                 "rays")] + [("trace_ms", ctypes.c_float), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def project_gradient(grad_x, vertex, param, dxdp, n_params):
+    """dp[param[k]] += dxdp[k] . grad_x[vertex[k]] (irtt_project_gradient)."""
+    g = np.ascontiguousarray(grad_x, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(vertex, np.uint32)
+    p = np.ascontiguousarray(param, np.uint32)
+    d = np.ascontiguousarray(dxdp, np.float32).reshape(-1, 3)
+    dp = np.zeros(n_params)
+    _check(lib().irtt_project_gradient(_ptr(g), g.shape[0], v.size, _ptr(v), _ptr(p), _ptr(d),
+                                       n_params, _ptr(dp)))
+    return dp
 
 
 def lib():
@@ -329,3 +367,30 @@ class Scene:
         r = np.empty(self.n_bins)
         _check(lib().irtt_residual(self._h, ctypes.byref(p), _ptr(x), _ptr(r)))
         return r
+
+    def adjoint_solve(self, rhs, x=None, params=None, **kw):
+        """lambda with J(x)^T lambda = rhs; x = None linearises at the current state."""
+        p = params if params is not None else solver_params(**kw)
+        r = np.ascontiguousarray(rhs, np.float64)
+        xx = None if x is None else np.ascontiguousarray(x, np.float64)
+        lam = np.empty(self.n_bins)
+        st = SolveStats()
+        _check(lib().irtt_adjoint_solve(self._h, ctypes.byref(p), _ptr(xx), _ptr(r), _ptr(lam),
+                                        ctypes.byref(st)))
+        return lam, st.as_dict()
+
+    def adjoint_trace(self, lam, x4, edge_ratio=0.05, tiny_eps=1e-2,
+                      flags=ADJ_AREA | ADJ_CONTINUOUS | ADJ_EDGE):
+        """Vertex gradient (n_vertices, 3) = -sum_h lam_h^T (dT/dv) x4_h over
+        the history rows of lam / x4 (shape (n_hist, n_bins) or (n_bins,));
+        x4 is the emission factor (x^4 for temperatures x)."""
+        lam = np.ascontiguousarray(np.atleast_2d(lam), np.float64)
+        x = np.ascontiguousarray(np.atleast_2d(x4), np.float64)
+        if lam.shape != x.shape or lam.shape[1] != self.n_bins:
+            raise ValueError("lambda and x4 must be (n_hist, n_bins)")
+        g = np.empty(3 * self.info["n_vertices"])
+        ap = AdjointParams(edge_ratio, tiny_eps, flags, 0)
+        st = AdjointStats()
+        _check(lib().irtt_adjoint_trace(self._h, ctypes.byref(ap), lam.shape[0], _ptr(lam), _ptr(x),
+                                        _ptr(g), ctypes.byref(st)))
+        return g.reshape(-1, 3), st.as_dict()

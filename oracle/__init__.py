@@ -297,3 +297,72 @@ def transport(sc, params, tri_begin=0, tri_end=None, dense=True, records=0,
         raise RuntimeError("or_transport failed")
     recs = (ra[: cnt.value], rs[: cnt.value]) if records else None
     return acc, recs, st.as_dict()
+
+
+ADJ_AREA = 1
+ADJ_CONTINUOUS = 2
+ADJ_EDGE = 4
+ADJ_REFERENCE_DROPPED = 8
+
+
+class AdjointParams(ctypes.Structure):
+    _fields_ = [("edge_ratio", ctypes.c_float), ("tiny_eps", ctypes.c_float),
+                ("flags", ctypes.c_uint32)]
+
+
+class AdjointStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in
+                ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded",
+                 "rays")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+def _declare_adjoint(L):
+    _f64p = ctypes.POINTER(ctypes.c_double)
+    L.or_area_gradient.argtypes = [_f32p] * 6
+    L.or_geom_term_grad.restype = ctypes.c_float
+    L.or_geom_term_grad.argtypes = [_f32p] * 6 + [ctypes.c_float] * 5 + [_f32p]
+    L.or_adjoint.restype = ctypes.c_int
+    L.or_adjoint.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params),
+                             ctypes.POINTER(AdjointParams), _f64p, _f64p, _f64p, ctypes.c_int,
+                             ctypes.POINTER(AdjointStats)]
+
+
+def area_gradient(a, b, c):
+    arrs = [np.ascontiguousarray(v, np.float32) for v in (a, b, c)]
+    out = [np.empty(3, np.float32) for _ in range(3)]
+    L = lib()
+    _declare_adjoint(L)
+    L.or_area_gradient(*[_p(v, _f32p) for v in arrs + out])
+    return out
+
+
+def geom_term_grad(verts6, bi, bj, eps=1e-2):
+    vs = [np.ascontiguousarray(v, np.float32) for v in verts6]
+    out = np.empty(18, np.float32)
+    L = lib()
+    _declare_adjoint(L)
+    G = L.or_geom_term_grad(*[_p(v, _f32p) for v in vs], float(bi[0]), float(bi[1]),
+                            float(bj[0]), float(bj[1]), float(eps), _p(out, _f32p))
+    return G, out.reshape(6, 3)
+
+
+def adjoint(sc, params, lam, x4, edge_ratio=0.05, tiny_eps=1e-2,
+            flags=ADJ_AREA | ADJ_CONTINUOUS | ADJ_EDGE, n_threads=0):
+    """Adjoint photon trace; returns (grad (nv, 3) float64, stats)."""
+    L = lib()
+    _declare_adjoint(L)
+    lam = np.ascontiguousarray(lam, np.float64)
+    x4 = np.ascontiguousarray(x4, np.float64)
+    grad = np.zeros(3 * sc.nv)
+    ap = AdjointParams(edge_ratio, tiny_eps, flags)
+    st = AdjointStats()
+    _f64p = ctypes.POINTER(ctypes.c_double)
+    rc = L.or_adjoint(ctypes.byref(sc.c), ctypes.byref(params), ctypes.byref(ap),
+                      lam.ctypes.data_as(_f64p), x4.ctypes.data_as(_f64p),
+                      grad.ctypes.data_as(_f64p), n_threads, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("or_adjoint failed")
+    return grad.reshape(-1, 3), st.as_dict()

@@ -125,6 +125,36 @@ int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
                  uint32_t *rec_src, uint64_t rec_cap, uint64_t *rec_count,
                  int n_threads, or_stats *st);
 
+/* ---- adjoint photon trace (thermal_tracing_unified.glsl:292-599), irtt_oracle_adjoint.c ---- */
+#define OR_ADJ_AREA 1u
+#define OR_ADJ_CONTINUOUS 2u
+#define OR_ADJ_EDGE 4u
+#define OR_ADJ_REFERENCE_DROPPED 8u /* dropped photons keep absorber bin 0, as the shader does */
+
+typedef struct {
+    float edge_ratio;   /* ubo.gradEdgeRatio, 0.05 */
+    float tiny_eps;     /* ubo.tinyEps, 1e-2 */
+    uint32_t flags;     /* OR_ADJ_* */
+} or_adjoint_params;
+
+typedef struct {
+    uint64_t photons, absorbed, segments, continuous, edge_hits, occluded, rays;
+} or_adjoint_stats;
+
+/* dA/da, dA/db, dA/dc of A = |(b-a) x (c-a)| / 2 */
+void or_area_gradient(const float *a, const float *b, const float *c, float *ga, float *gb,
+                      float *gc);
+/* G = -(N_i.d)(N_j.d) / (|N_i||N_j|(r+eps)^4), d = p_j - p_i, and
+ * d ln|G| / d{a_i,b_i,c_i,a_j,b_j,c_j} (18 floats).  Returns G. */
+float or_geom_term_grad(const float *ai, const float *bi, const float *ci, const float *aj,
+                        const float *bj, const float *cj, float bi1, float bi2, float bj1,
+                        float bj2, float eps, float *out);
+/* grad (3*nv doubles, accumulated) += sum over photons of the adjoint terms,
+ * weighted by lambda[bin] * x4[bin] (one history step). */
+int or_adjoint(const or_scene *sc, const or_params *p, const or_adjoint_params *ap,
+               const double *lambda, const double *x4, double *grad, int n_threads,
+               or_adjoint_stats *st);
+
 #ifdef __cplusplus
 }
 #endif
