@@ -28,6 +28,10 @@ NODE_BYTES = 64      # two child boxes + two child refs
 TRI_BYTES = 40       # 3 x float3 vertices + id
 RAY_BYTES = 48       # 32 B ray in + 16 B hit out
 BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
+# DRAM bytes per k_photons launch on this workload, from rocprofv3 FETCH_SIZE
+# (x 2.0) + WRITE_SIZE (x 1.0) scaled by a 1 GiB copy calibration; the
+# commands and raw values are in profiles/k_photons_city_block_2p26.md
+TRAFFIC_BYTES_CITY = (1496483 * 2.0 + 1162026 * 1.0) * 1024
 
 
 def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):
@@ -121,7 +125,11 @@ def main():
                    "bins": s.n_bins, "triangles": sd.nt, "photons": st["photons"],
                    "ray_segments": rays},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (round(TRAFFIC_BYTES_CITY / 1e9, 3)
+                                 if (args.seed == 0 and args.threads_per_tri == 162
+                                     and args.rays_per_thread == 8) else None),
+                     "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
                      "kernel": "k_photons", "kernel_ms": round(trace_ms, 3),
                      "bytes_per_ray": round(b_launch / rays, 1),
                      "node_visits_per_ray": round(n_node, 2),

@@ -67,8 +67,11 @@ __device__ __forceinline__ V3 load_v(const float *xyz, uint32_t v)
     return V3{xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
 }
 
-// One photon per lane per iteration; lanes of a wave take consecutive photons
-// of the same source triangle, so first-bounce rays are spatially coherent.
+// Each loop iteration traces one ray segment per lane.  A lane whose photon
+// ends (absorbed, missed, dropped) starts its next photon in the following
+// iteration, so lanes do not wait for the longest path of the wave.  A lane
+// takes photons g, g + stride, ...; lanes of a wave start on consecutive
+// photons of the same source triangle (coherent first-bounce rays).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
 {
@@ -79,87 +82,102 @@ __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
     uint32_t cnt[S_COUNT] = {0, 0, 0, 0, 0, 0, 0};
     uint32_t nvis = 0, ntri = 0;
     const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
-    for (uint64_t g = a.photon_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
-         g < a.photon_end; g += stride) {
-        uint32_t t = (uint32_t)(g / a.per_tri);
-        uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
-        uint32_t x = k / a.rays_per_thread, nn = k - x * a.rays_per_thread;
-        // thermal_tracing_unified.glsl:291
-        uint32_t seed = tea16(a.nt * x + t, a.batch_seed + nn);
-        uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
-        V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
-        // getEmissionRay, thermal_tracing_unified.glsl:125-161
-        float u0 = rnd(seed), u1 = rnd(seed);
-        (void)rnd(seed);
-        float su = sqrt_rn(u0);
-        float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
-        V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
-        float len;
-        V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
-        V3 o = offset_ray(pos, nrm);
-        V3 d = nrm;
-        if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
-            float r0 = rnd(seed), r1 = rnd(seed);
-            (void)rnd(seed);
-            d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
-        }
-        if (!(len > 0.0f)) {
-            cnt[S_DEG]++;
-            continue;
-        }
-        int sc = max_corner(b0, b1, b2);
-        uint32_t src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
-        const unsigned long long qt = a.q[t];
-        uint32_t depth = 0;
-        for (;;) {
-            ++depth;
-            if (depth > a.max_depth) break;
-            Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn, stt,
-                                         nvis, ntri);
-            cnt[S_RAYS]++;
-            if (h.prim < 0) {
-                cnt[S_MISS]++;
+    uint64_t g = a.photon_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    bool live = false;
+    uint32_t seed = 0, src = 0, depth = 0;
+    unsigned long long qt = 0;
+    V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+    for (;;) {
+        if (!live) {
+            // start this lane's next photon; degenerate emitters are skipped
+            while (g < a.photon_end) {
+                uint32_t t = (uint32_t)(g / a.per_tri);
+                uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+                uint32_t x = k / a.rays_per_thread, nn = k - x * a.rays_per_thread;
+                // thermal_tracing_unified.glsl:291
+                seed = tea16(a.nt * x + t, a.batch_seed + nn);
+                uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+                V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
+                // getEmissionRay, thermal_tracing_unified.glsl:125-161
+                float u0 = rnd(seed), u1 = rnd(seed);
+                (void)rnd(seed);
+                float su = sqrt_rn(u0);
+                float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
+                V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
+                float len;
+                V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
+                o = offset_ray(pos, nrm);
+                d = nrm;
+                if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
+                    float r0 = rnd(seed), r1 = rnd(seed);
+                    (void)rnd(seed);
+                    d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
+                }
+                if (!(len > 0.0f)) {
+                    cnt[S_DEG]++;
+                    g += stride;
+                    continue;
+                }
+                int sc = max_corner(b0, b1, b2);
+                src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
+                qt = a.q[t];
+                depth = 0;
+                live = true;
                 break;
             }
-            if (h.t < a.tiny_dist) {
-                cnt[S_CLOSE]++;
-                break;
-            }
+            if (!live) break;
+        }
+        // one ray segment of the photon in flight
+        ++depth;
+        bool done = true;
+        Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn, stt, nvis,
+                                     ntri);
+        cnt[S_RAYS]++;
+        if (h.prim < 0) {
+            cnt[S_MISS]++;
+        } else if (h.t < a.tiny_dist) {
+            cnt[S_CLOSE]++;
+        } else {
             uint32_t hp = (uint32_t)h.prim;
             uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
             V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
             V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
             if (dot3(hn, d) > -1e-3f) {
                 cnt[S_BACK]++;
-                break;
-            }
-            float hb0 = 1.0f - h.b1 - h.b2;
-            // getNextRay, thermal_tracing_unified.glsl:163-197
-            V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
-            o = offset_ray(hpos, hn);
-            float u = rnd(seed);
-            float rd = a.rho_d[hp], rs = a.rho_s[hp];
-            bool absorbed = false;
-            if (u >= rd + rs) {
-                absorbed = true;
-            } else if (u >= rd) {
-                V3 in = normalize3(d);
-                float dn = 2.0f * dot3(hn, in);
-                d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
             } else {
-                float r0 = rnd(seed), r1 = rnd(seed);
-                (void)rnd(seed);
-                d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
+                float hb0 = 1.0f - h.b1 - h.b2;
+                // getNextRay, thermal_tracing_unified.glsl:163-197
+                V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+                o = offset_ray(hpos, hn);
+                float u = rnd(seed);
+                float rd = a.rho_d[hp], rs = a.rho_s[hp];
+                bool absorbed = false;
+                if (u >= rd + rs) {
+                    absorbed = true;
+                } else if (u >= rd) {
+                    V3 in = normalize3(d);
+                    float dn = 2.0f * dot3(hn, in);
+                    d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
+                } else {
+                    float r0 = rnd(seed), r1 = rnd(seed);
+                    (void)rnd(seed);
+                    d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
+                }
+                if (absorbed || depth == a.max_depth) {
+                    // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
+                    int hc = max_corner(hb0, h.b1, h.b2);
+                    uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
+                    atomicAdd(a.acc + (uint64_t)ab * a.ld + src, qt);
+                    if (absorbed) cnt[S_ABS]++;
+                    else cnt[S_FORCED]++;
+                } else {
+                    done = false;
+                }
             }
-            if (absorbed || depth == a.max_depth) {
-                // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
-                int hc = max_corner(hb0, h.b1, h.b2);
-                uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
-                atomicAdd(a.acc + (uint64_t)ab * a.ld + src, qt);
-                if (absorbed) cnt[S_ABS]++;
-                else cnt[S_FORCED]++;
-                break;
-            }
+        }
+        if (done) {
+            live = false;
+            g += stride;
         }
     }
     // wave-level reduction of the counters (one wave per block)

@@ -69,7 +69,9 @@ def test_nop_parameter_exact(irtt):
         xp = solver.steady_linear(T + p * 1e-8 * np.eye(len(x)), b, fixed, np.zeros(len(x)))
         return 0.5 * np.sum(w * (xp - target) ** 2)
 
-    h = 1e-3
+    # x moves ~2e3 K per unit p here, so the step must keep the second-order
+    # term (relative to the 1 K target offset) small
+    h = 1e-6
     fd = (L(h) - L(-h)) / (2 * h)
     print(f"NOP test: adjoint {adj:.8e} FD {fd:.8e}")
     assert abs(adj - fd) < 1e-4 * abs(fd)
