@@ -147,7 +147,7 @@ template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
-    __shared__ float stk_t[STACK * kTraceBlock];
+    __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
     uint32_t cnt[A_COUNT] = {0, 0, 0, 0, 0, 0};

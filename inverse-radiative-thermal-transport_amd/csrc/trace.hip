@@ -29,7 +29,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_rays(
     float *b1o, float *b2o)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
-    __shared__ float stk_t[STACK * kTraceBlock];
+    __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= n) return;
     const float *r = rays + 6 * (size_t)i;
@@ -76,7 +76,7 @@ template <int STACK>
 __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
-    __shared__ float stk_t[STACK * kTraceBlock];
+    __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
     uint32_t cnt[S_COUNT] = {0, 0, 0, 0, 0, 0, 0};

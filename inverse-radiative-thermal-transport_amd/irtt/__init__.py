@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libirtt.so")
+# IRTT_LIB selects another build of the same library (e.g. a variant built
+# with different compile-time switches for an A/B measurement)
+LIB_PATH = os.environ.get("IRTT_LIB") or os.path.join(_HERE, "libirtt.so")
 _lib = None
 
 BIN_VERTEX = 0

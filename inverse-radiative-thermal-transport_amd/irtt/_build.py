@@ -27,17 +27,19 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, jobs=None):
-    """Compile each source to an object in parallel, then link libirtt.so."""
+def build(verbose=False, out=None, extra_flags=(), objdir=None):
+    """Compile each source to an object in parallel, then link libirtt.so
+    (or `out`, with `extra_flags` added to every compile)."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    objdir = os.path.join(PKG, "_build")
+    lib = out or LIB
+    objdir = objdir or os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     procs = []
     objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".", "_") + ".o")
         objs.append(obj)
-        cmd = [hipcc] + [f for f in FLAGS if f != "-shared"] + [
+        cmd = [hipcc] + [f for f in FLAGS if f != "-shared"] + list(extra_flags) + [
             "-I", os.path.join(PKG, "..", "include"), "-c",
             "-x", "hip", os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -55,13 +57,13 @@ def build(verbose=False, jobs=None):
     if failed:
         msg = "\n".join(f"--- {s} ---\n{t}" for s, t in failed)
         raise RuntimeError("hipcc failed:\n" + msg)
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace"))
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -472,10 +472,10 @@ static inline void leaf_test(const or_bvh *bvh, uint32_t leaf, const float o[3],
     }
 }
 
-/* Closest-hit traversal, near child first.  Node entries carry their box
- * entry distance; a popped entry is skipped when it starts beyond the current
- * best hit (entries at exactly the best distance are kept so ties resolve to
- * the smallest id). */
+/* Closest-hit traversal, near child first (farther child pushed).  A popped
+ * node's children are box-tested against the current best hit, with a 1e-4
+ * relative margin so entries at exactly the best distance are kept and ties
+ * resolve to the smallest id. */
 static void trace_one(const or_bvh *bvh, const float o[3], const float d[3],
                       float tmin, float tmax, hit_rec *h, or_trav_stats *st)
 {
@@ -519,7 +519,10 @@ static void trace_one(const or_bvh *bvh, const float o[3], const float d[3],
         int found = 0;
         while (sp > 0) {
             sp--;
-            if (stt[sp] <= h->t * 1.0001f) { cur = stk[sp]; found = 1; break; }
+            /* entries are popped without a distance check, as the device
+             * traversal does (its stack keeps node refs only) */
+            (void)stt;
+            cur = stk[sp]; found = 1; break;
         }
         if (!found) break;
     }
