@@ -149,8 +149,57 @@ int irtt_get_transport_fixed(irtt_scene h, uint64_t *acc, double *quantum);
 /* Column sums of the operator, exact from the fixed-point accumulators (FP64). */
 int irtt_get_column_sums(irtt_scene h, double *colsum);
 
+/* --------------------------------------------------------- operator forms */
+#define IRTT_OP_DENSE 0u      /* FP32 dense operator of irtt_build_transport */
+#define IRTT_OP_ACA 1u        /* U V + diag(C) of irtt_compress_transport */
+#define IRTT_OP_DENSE_F16 2u  /* off-diagonal FP16 with an FP32 scale per row, FP64
+                                 diagonal; Newton residuals keep the FP32 operator */
+
+/* Operator used by irtt_solve / irtt_residual / irtt_adjoint_solve (the
+ * reference's transportCompressed switch, thermal_solver.cpp:126-131).
+ * IRTT_OP_DENSE_F16 quantises the dense operator on first use;
+ * IRTT_OP_ACA needs irtt_compress_transport.  A new irtt_build_transport
+ * returns to IRTT_OP_DENSE and drops the other forms. */
+int irtt_select_operator(irtt_scene h, uint32_t op);
+
+#define IRTT_ACA_FULL 0u      /* simpleAdaptiveCrossApproximation, thermal_transport.cpp:360-386 */
+#define IRTT_ACA_PARTIAL 1u   /* completeAdaptiveCrossApproximation (reference default), :388-461 */
+#define IRTT_ACA_PLUS 2u      /* adaptiveCrossApproximationPlus, :522-597 */
+#define IRTT_STORE_F32 0u
+#define IRTT_STORE_F16 1u     /* FP16 with one FP32 scale per stored row */
+
+typedef struct irtt_aca_params {
+    uint32_t algorithm;             /* IRTT_ACA_* (ThermalVars_s::compressionAlgorithm = 1) */
+    uint32_t rank;                  /* ThermalVars_s::compressionMaxIters (50); 0 -> 50 */
+    uint32_t column_sum_correction; /* ThermalVars_s::applyCollumnSumCorrection (0) */
+    uint32_t seed;                  /* ACA+ reference-row draws: C runtime rand() state, 0 -> 1 */
+    uint32_t precision;             /* IRTT_STORE_*: factors read by the solver */
+    uint32_t reserved[3];
+} irtt_aca_params;
+
+typedef struct irtt_aca_stats {
+    double err_inf, err_frob;         /* ||T_off - U V||, max-abs and Frobenius */
+    double err_inf_rel, err_frob_rel; /* relative to ||T_off|| */
+    float compress_ms;
+    uint32_t rank;
+    uint32_t n_corrected;             /* columns raised by the column-sum correction */
+    uint32_t reserved;
+} irtt_aca_stats;
+
+/* Replaces ThermalTransport::compressTransportMatrix + collumnSumCorrection
+ * (thermal_transport.cpp:281-358, :634-656): rank-k cross approximation of
+ * the photon part T_off of the operator (the dense operator without its
+ * emission diagonal), T ~ U V + diag(C), C = sigma eps A (+ correction).
+ * Factors are built in FP64 on the device; selects IRTT_OP_ACA. */
+int irtt_compress_transport(irtt_scene h, const irtt_aca_params *p, irtt_aca_stats *st);
+/* U: n_bins x rank row-major, V: rank x n_bins row-major (FP64 as built),
+ * cdiag: n_bins, piv_rows / piv_cols: rank pivots each.  Any pointer may be
+ * NULL. */
+int irtt_get_aca(irtt_scene h, double *U, double *V, double *cdiag, int32_t *piv_rows,
+                 int32_t *piv_cols);
+
 /* ------------------------------------------------------------------ solve */
-#define IRTT_JAC_EXACT 0u       /* J = T diag(4x^3) + L (+ M/dt) */
+#define IRTT_JAC_EXACT 0u      /* J = T diag(4x^3) + L (+ M/dt) */
 #define IRTT_JAC_REFERENCE 1u   /* thermal_solver.cpp:109-166: diag(4x^3) T + L + I/dt */
 
 typedef struct irtt_solver_params {

@@ -204,6 +204,7 @@ void upload_geometry(Scene &s)
     IRTT_HIP(hipStreamSynchronize(st));
     s.bvh.valid = false;
     s.has_transport = false;
+    op_reset(s);
 }
 
 }  // namespace
@@ -215,8 +216,9 @@ uint64_t Scene::device_bytes() const
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
            d_traceable.bytes() + bvh.nodes.bytes() + bvh.tris.bytes() + bvh.morton.bytes() +
            d_q.bytes() + d_acc.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
-           d_tdiag.bytes() +
-           d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
+           d_tdiag.bytes() + d_h16.bytes() + d_h16s.bytes() + aca.d_ut.bytes() +
+           aca.d_v.bytes() + aca.d_uf.bytes() + aca.d_vf.bytes() + aca.d_uh.bytes() +
+           aca.d_vh.bytes() + d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
            d_lap_col.bytes() + d_lap_val.bytes();
 }
 
@@ -415,6 +417,28 @@ int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_tran
         transport_build(*s, *p, tmp);
         if (st) *st = tmp;
     });
+}
+
+int irtt_select_operator(irtt_scene h, uint32_t op)
+{
+    return guard([&] { op_select(*S(h), op); });
+}
+
+int irtt_compress_transport(irtt_scene h, const irtt_aca_params *p, irtt_aca_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p) throw Error(IRTT_ERR_ARG, "params is null");
+        irtt_aca_stats tmp{};
+        aca_compress(*s, *p, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_get_aca(irtt_scene h, double *U, double *V, double *cdiag, int32_t *piv_rows,
+                 int32_t *piv_cols)
+{
+    return guard([&] { aca_get(*S(h), U, V, cdiag, piv_rows, piv_cols); });
 }
 
 int irtt_get_transport(irtt_scene h, float *dense)

@@ -79,6 +79,41 @@ struct Bvh {
     float build_ms = 0.f;
 };
 
+constexpr uint32_t kAcaMaxRank = 256;
+
+// ACA-compressed operator U V + diag(C) (aca.hip)
+struct Aca {
+    bool valid = false;
+    uint32_t k = 0, precision = 0;
+    DBuf<double> d_ut, d_v;           // k x ld: columns of U, rows of V (FP64)
+    DBuf<float> d_uf, d_vf;           // FP32 copies read by the solver
+    DBuf<uint16_t> d_uh, d_vh;        // FP16 copies (bit patterns) ...
+    DBuf<float> d_us, d_vs;           // ... and their per-row scales
+    DBuf<float> d_od;                 // off-diagonal operator's diagonal, FP32
+    DBuf<double> d_u, d_w, d_part, d_mag, d_t;
+    DBuf<int32_t> d_piv;
+    DBuf<uint8_t> d_excl;
+    std::vector<int32_t> prow, pcol;
+    std::vector<double> cdiag;        // C: sigma eps A (+ column-sum correction)
+    std::vector<double> opdiag;       // diag(U V) + C
+    double max_uv = 0.0;
+    void reset()
+    {
+        valid = false;
+        k = 0;
+        for (DBuf<double> *b : {&d_ut, &d_v, &d_u, &d_w, &d_part, &d_mag, &d_t}) b->release();
+        for (DBuf<float> *b : {&d_uf, &d_vf, &d_us, &d_vs, &d_od}) b->release();
+        d_uh.release();
+        d_vh.release();
+        d_piv.release();
+        d_excl.release();
+        prow.clear();
+        pcol.clear();
+        cdiag.clear();
+        opdiag.clear();
+    }
+};
+
 struct Scene {
     uint32_t nv = 0, nt = 0, nb = 0, bin_mode = 0;
     uint32_t ld = 0;  // row pitch of the dense operator: nb rounded up to 64
@@ -122,6 +157,14 @@ struct Scene {
     std::vector<double> Tdiag;         // diagonal of the FP32 operator as stored
     DBuf<unsigned long long> d_stats;
 
+    // operator forms (operator.hip, aca.hip)
+    uint32_t op_kind = IRTT_OP_DENSE;
+    bool f16_valid = false;
+    DBuf<uint16_t> d_h16;             // nb*ld FP16 off-diagonal operator (bit patterns)
+    DBuf<float> d_h16s;               // its per-row scales
+    DBuf<float> d_u32;                // FP32 copy of a product's input vector
+    Aca aca;
+
     // solver workspace
     DBuf<double> d_vec;               // 12 vectors of nb doubles
     DBuf<double> d_red;               // reduction scratch
@@ -146,6 +189,16 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st);
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r);
 void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const double *rhs,
                    double *lambda, irtt_solve_stats &st);
+// operator.hip
+void op_reset(Scene &s);
+void op_select(Scene &s, uint32_t kind);
+void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine);
+const std::vector<double> &op_diag_outside(const Scene &s, bool refine);
+const std::vector<double> &op_diag_total(const Scene &s);
+// aca.hip
+void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st);
+void aca_apply(Scene &s, const double *w, double *y, bool transposed);
+void aca_get(Scene &s, double *U, double *V, double *cdiag, int32_t *prow, int32_t *pcol);
 // adjoint.hip
 void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
                    const double *x, double *grad, irtt_adjoint_stats &st);

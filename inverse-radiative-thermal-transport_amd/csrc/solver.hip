@@ -12,8 +12,11 @@
 // with BiCGSTAB (right Jacobi preconditioning) using matrix-free products;
 // the Jacobian is the exact one or the reference's row-scaled form
 // (thermal_solver.cpp:39-96).  A damped line search halves the step while the
-// residual norm grows (thermal_solver.cpp:192-203).  The operator is FP32 in
-// HBM; products accumulate in FP64 and every vector is FP64.
+// residual norm grows (thermal_solver.cpp:192-203).  Operator products go
+// through op_apply (dense FP32, dense FP16 + row scales, or ACA); with the
+// FP16 form the Krylov products are FP16 while every Newton residual uses
+// the FP32 operator with FP64 accumulation, so Newton is the refinement loop.
+// Every vector is FP64.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,54 +31,6 @@ namespace {
 constexpr int kRedBlocks = 256;
 constexpr int kRedThreads = 256;
 
-// y_i = sum_j T_ij u_j: one wave per row, float4 loads, FP64 accumulation.
-__global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, uint32_t nb,
-                                                   uint32_t ld, const double *__restrict__ u,
-                                                   double *__restrict__ y)
-{
-    uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    uint32_t lane = threadIdx.x & 63;
-    if (row >= nb) return;
-    const float4 *r = reinterpret_cast<const float4 *>(T + (size_t)row * ld);
-    double acc = 0.0;
-    const uint32_t n4 = ld >> 2;
-    for (uint32_t k = lane; k < n4; k += 64) {
-        float4 v = r[k];
-        uint32_t j = 4 * k;
-        // the padded tail of each row is zero and u is padded with zeros
-        acc += (double)v.x * u[j] + (double)v.y * u[j + 1] + (double)v.z * u[j + 2] +
-               (double)v.w * u[j + 3];
-    }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) y[row] = acc;
-}
-
-// y = T^T u in two deterministic steps: partial column sums over row chunks
-// of kTRows rows (threads along columns read coalesced rows), then a fixed-
-// order sum over the chunks.
-constexpr uint32_t kTRows = 512;
-__global__ __launch_bounds__(256) void k_gemv_t_partial(const float *__restrict__ T, uint32_t nb,
-                                                        uint32_t ld, const double *__restrict__ u,
-                                                        double *__restrict__ part)
-{
-    uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= ld) return;
-    uint32_t r0 = blockIdx.y * kTRows, r1 = min(r0 + kTRows, nb);
-    double acc = 0.0;
-    for (uint32_t i = r0; i < r1; i++) acc += (double)T[(size_t)i * ld + j] * u[i];
-    part[(size_t)blockIdx.y * ld + j] = acc;
-}
-
-__global__ void k_gemv_t_reduce(const double *__restrict__ part, uint32_t nchunk, uint32_t ld,
-                                double *__restrict__ y)
-{
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ld) return;
-    double a = 0.0;
-    for (uint32_t c = 0; c < nchunk; c++) a += part[(size_t)c * ld + j];
-    y[j] = a;
-}
-
 struct Sys {
     uint32_t nb;
     const uint32_t *lrow, *lcol;
@@ -85,6 +40,11 @@ struct Sys {
     double dt;
     int steady, jmode;  // jmode: J_EXACT, J_REFERENCE or J_PREDICT
     int transposed;     // products with J^T (adjoint solve)
+    // operator diagonal kept outside the stored product (NULL = none): in the
+    // residual (cdr) and in the Krylov products (cdj)
+    const double *cdr, *cdj;
+    int c_unscaled;     // reference ACA Jacobian: D U V + C, C not scaled by D
+                        // (thermal_solver.cpp:44-75, thermal_aca_matrix.hpp:128)
 };
 
 __device__ __forceinline__ double lap_row(const Sys &s, uint32_t i, const double *v)
@@ -107,9 +67,9 @@ __global__ void k_pow4(Sys s, const double *__restrict__ x, double *__restrict__
     y[i] = v2 * v2;
 }
 
-// r from Ty (= T x^4) and xe; part[block] = sum r^2
-__global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *__restrict__ xe,
-                           double *__restrict__ r, double *part)
+// r from Ty (= P x^4), y (= x^4) and xe; part[block] = sum r^2
+__global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *__restrict__ y,
+                           const double *__restrict__ xe, double *__restrict__ r, double *part)
 {
     __shared__ double sm[kRedThreads];
     double acc = 0.0;
@@ -117,7 +77,8 @@ __global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *_
          i += gridDim.x * blockDim.x) {
         double v = 0.0;
         if (!s.fixed[i]) {
-            double rad = Ty[i] + lap_row(s, i, xe) - s.b[i];
+            double ty = Ty[i] + (s.cdr ? s.cdr[i] * y[i] : 0.0);
+            double rad = ty + lap_row(s, i, xe) - s.b[i];
             v = s.steady ? rad : s.mass[i] * (xe[i] - s.xprev[i]) + s.dt * rad;
         }
         r[i] = v;
@@ -160,7 +121,8 @@ __global__ void k_jv_pre(Sys s, const double *__restrict__ xe, const double *__r
 // out = J v from Tw (= T w).  J_PREDICT is the operator of the steady
 // radiative balance in y = x^4 alone (T), used for the initial guess.
 __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
-                          const double *__restrict__ Tw, double *__restrict__ out)
+                          const double *__restrict__ w, const double *__restrict__ Tw,
+                          double *__restrict__ out)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nb) return;
@@ -168,16 +130,19 @@ __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__
         out[i] = v[i];
         return;
     }
+    const double cd = s.cdj ? s.cdj[i] : 0.0;
     if (s.jmode == J_PREDICT) {
-        out[i] = Tw[i];
+        out[i] = Tw[i] + cd * w[i];
         return;
     }
     double lv = lap_row(s, i, v);  // the cotangent Laplacian is symmetric
     double rad = Tw[i];
+    if (!s.c_unscaled) rad += cd * w[i];
     if (!d_before(s)) {
         double x = xe[i];
         rad = 4.0 * x * x * x * rad;  // reference: diag(4x^3) T v, thermal_solver.cpp:149
     }
+    if (s.c_unscaled) rad += cd * v[i];
     if (s.steady) {
         double o = rad + lv;
         if (s.jmode == J_REFERENCE) o += v[i] / s.dt;  // I/dt regulariser, thermal_solver.cpp:150
@@ -268,6 +233,7 @@ __global__ void k_jdiag(Sys s, const double *__restrict__ xe, const double *__re
     }
     double x = xe[i];
     double rad = 4.0 * x * x * x * Tdiag[i];
+    if (s.c_unscaled) rad = 4.0 * x * x * x * (Tdiag[i] - s.cdj[i]) + s.cdj[i];
     double d;
     if (s.jmode == J_PREDICT) d = Tdiag[i];
     else if (s.steady) d = rad + Ldiag[i] + (s.jmode == J_REFERENCE ? 1.0 / s.dt : 0.0);
@@ -296,7 +262,8 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 // vector slots in d_vec (each ld doubles, padded with zeros)
 enum {
     V_X = 0, V_XE, V_Y, V_TY, V_R, V_DX, V_XT, V_P, V_V, V_S, V_T, V_PH, V_SH, V_RHAT, V_W, V_TW,
-    V_DINV, V_B, V_KFIX, V_XPREV, V_MASS, V_TDIAG, V_LDIAG, V_COUNT
+    V_DINV, V_B, V_KFIX, V_XPREV, V_MASS, V_TDIAG, V_LDIAG, V_CDR, V_CDJ, V_LAM, V_RHS, V_AUX,
+    V_COUNT
 };
 
 struct Ctx {
@@ -309,18 +276,8 @@ struct Ctx {
     double hbuf[4];
     explicit Ctx(Scene &sc) : s(sc), st(sc.stream), nb(sc.nb), ld(sc.ld) {}
 
-    void gemv(const double *u, double *y)
-    {
-        k_gemv_rows<<<cdiv(nb, 4), 256, 0, st>>>(s.d_T.p, nb, ld, u, y);
-    }
-    void gemv_t(const double *u, double *y)
-    {
-        const uint32_t nchunk = cdiv(nb, kTRows);
-        s.d_tpart.alloc((size_t)nchunk * ld);
-        k_gemv_t_partial<<<dim3(cdiv(ld, 256), nchunk), 256, 0, st>>>(s.d_T.p, nb, ld, u,
-                                                                       s.d_tpart.p);
-        k_gemv_t_reduce<<<cdiv(ld, 256), 256, 0, st>>>(s.d_tpart.p, nchunk, ld, y);
-    }
+    // operator product of the residual side
+    void gemv_res(const double *u, double *y) { op_apply(s, u, y, false, true); }
     // returns sum of m dot products
     void dots(int m, const double *a0, const double *b0, const double *a1, const double *b1,
               const double *a2, const double *b2, double *res)
@@ -336,20 +293,24 @@ struct Ctx {
     {
         uint32_t g = cdiv(nb, 256);
         k_pow4<<<g, 256, 0, st>>>(sys, x, v[V_XE], v[V_Y]);
-        gemv(v[V_Y], v[V_TY]);
-        k_residual<<<kRedBlocks, kRedThreads, 0, st>>>(sys, v[V_TY], v[V_XE], v[V_R], part);
+        gemv_res(v[V_Y], v[V_TY]);
+        k_residual<<<kRedBlocks, kRedThreads, 0, st>>>(sys, v[V_TY], v[V_Y], v[V_XE], v[V_R],
+                                                       part);
         k_final_sum<<<1, kRedThreads, 0, st>>>(part, 1, kRedBlocks, out);
         IRTT_HIP(hipMemcpyAsync(hbuf, out, sizeof(double), hipMemcpyDeviceToHost, st));
         IRTT_HIP(hipStreamSynchronize(st));
         return hbuf[0];
     }
-    void jv(const double *vin, double *outv)
+    // out = J v (J^T v when transposed); refine: with the residual side's
+    // operator (FP32 when the Krylov side is FP16)
+    void jv(const double *vin, double *outv, bool refine = false)
     {
         uint32_t g = cdiv(nb, 256);
-        k_jv_pre<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_W]);
-        if (sys.transposed) gemv_t(v[V_W], v[V_TW]);
-        else gemv(v[V_W], v[V_TW]);
-        k_jv_post<<<g, 256, 0, st>>>(sys, v[V_XE], vin, v[V_TW], outv);
+        Sys y = sys;
+        if (refine) y.cdj = y.cdr;
+        k_jv_pre<<<g, 256, 0, st>>>(y, v[V_XE], vin, v[V_W]);
+        op_apply(s, v[V_W], v[V_TW], sys.transposed != 0, refine);
+        k_jv_post<<<g, 256, 0, st>>>(y, v[V_XE], vin, v[V_W], v[V_TW], outv);
     }
 };
 
@@ -380,8 +341,11 @@ void upload_static(Scene &s, Ctx &c, const irtt_solver_params &p)
     up(V_B, b);
     up(V_KFIX, kfix);
     up(V_MASS, s.bin_mass);
-    up(V_TDIAG, s.Tdiag);
+    up(V_TDIAG, op_diag_total(s));
     up(V_LDIAG, ldiag);
+    const std::vector<double> &cdr = op_diag_outside(s, true), &cdj = op_diag_outside(s, false);
+    if (!cdr.empty()) up(V_CDR, cdr);
+    if (!cdj.empty()) up(V_CDJ, cdj);
     up(V_XPREV, s.x);
     if (!s.lap.rowptr.empty() && s.d_lap_rowptr.n == 0) {
         s.d_lap_rowptr.alloc(s.lap.rowptr.size());
@@ -427,6 +391,9 @@ static void setup(Scene &s, Ctx &c, const irtt_solver_params &p, FixedBuf &fb)
     y.steady = p.steady ? 1 : 0;
     y.jmode = p.jacobian == IRTT_JAC_REFERENCE ? J_REFERENCE : J_EXACT;
     y.transposed = 0;
+    y.cdr = op_diag_outside(s, true).empty() ? nullptr : c.v[V_CDR];
+    y.cdj = op_diag_outside(s, false).empty() ? nullptr : c.v[V_CDJ];
+    y.c_unscaled = (s.op_kind == IRTT_OP_ACA && y.jmode == J_REFERENCE) ? 1 : 0;
 }
 
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r)
@@ -504,7 +471,21 @@ __global__ void k_neg_masked(Sys s, const double *__restrict__ rhs, double *__re
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < s.nb) r[i] = s.fixed[i] ? 0.0 : -rhs[i];
 }
+// r = fixed ? 0 : a - b
+__global__ void k_sub_masked(Sys s, const double *__restrict__ a, const double *__restrict__ b,
+                             double *__restrict__ r)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < s.nb) r[i] = s.fixed[i] ? 0.0 : a[i] - b[i];
+}
 }  // namespace
+
+// Krylov tolerance: with FP16 products a linear solve cannot go much below the
+// quantisation level, the outer (FP32-residual) iteration refines instead
+static double krylov_rtol(const Scene &s, double lin_rtol)
+{
+    return s.op_kind == IRTT_OP_DENSE_F16 ? std::max(lin_rtol, 1e-6) : lin_rtol;
+}
 
 // Adjoint solve J(x)^T lambda = rhs (thermal_renderer.cpp:932-956): the same
 // Krylov solver on the transposed products; fixed bins get lambda = 0.
@@ -529,17 +510,36 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
     st = irtt_solve_stats{};
     const uint32_t max_lin = p.max_linear ? p.max_linear : 200;
     const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
-    if (rn > 0.0) st.linear_iters = bicgstab(c, max_lin, lin_rtol, rn);
+    double *res_lam = c.v[V_DX];
+    if (rn > 0.0) st.linear_iters = bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
     else IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
+    if (rn > 0.0 && s.op_kind == IRTT_OP_DENSE_F16) {
+        // iterative refinement: r = J^T lambda - rhs with the FP32 operator,
+        // correction from the FP16 Krylov solve
+        IRTT_HIP(hipMemcpyAsync(c.v[V_RHS], rhs, nb * sizeof(double), hipMemcpyHostToDevice, stm));
+        IRTT_HIP(hipMemcpyAsync(c.v[V_LAM], c.v[V_DX], nb * sizeof(double),
+                                hipMemcpyDeviceToDevice, stm));
+        const double goal = lin_rtol * lin_rtol * rn;
+        for (int it = 0; it < 20; it++) {
+            c.jv(c.v[V_LAM], c.v[V_AUX], true);
+            k_sub_masked<<<g, 256, 0, stm>>>(c.sys, c.v[V_AUX], c.v[V_RHS], c.v[V_R]);
+            double r2;
+            c.dots(1, c.v[V_R], c.v[V_R], nullptr, nullptr, nullptr, nullptr, &r2);
+            st.newton_iters = it + 1;
+            if (r2 <= goal || !std::isfinite(r2)) break;
+            st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), r2);
+            k_axpy2<<<g, 256, 0, stm>>>(nb, 1.0, c.v[V_LAM], c.v[V_DX], c.v[V_LAM]);
+        }
+        res_lam = c.v[V_LAM];
+    }
     std::vector<double> lam(nb);
-    IRTT_HIP(hipMemcpyAsync(lam.data(), c.v[V_DX], nb * sizeof(double), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(lam.data(), res_lam, nb * sizeof(double), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipEventRecord(s.ev1, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     IRTT_HIP(hipEventElapsedTime(&st.solve_ms, s.ev0, s.ev1));
     for (uint32_t i = 0; i < nb; i++) lambda[i] = s.bin_fixed[i] ? 0.0 : lam[i];
-    // residual of the linear solve, for the caller
-    st.newton_iters = 0;
-    st.converged = st.linear_iters < max_lin ? 1 : 0;
+    // newton_iters counts refinement passes (FP16 operator form), 0 otherwise
+    st.converged = st.linear_iters < max_lin * (st.newton_iters + 1) ? 1 : 0;
     st.residual_sq = rn;
 }
 
@@ -576,7 +576,7 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
         // conduction, a close start otherwise.  (The reference relies on its
         // I/dt regulariser instead, thermal_solver.cpp:150.)
         c.sys.jmode = J_PREDICT;
-        st.linear_iters += bicgstab(c, max_lin, lin_rtol, rn);
+        st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
         k_y_update<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_DX], c.v[V_X]);
         c.sys.jmode = J_EXACT;
         rn = c.residual(c.v[V_X]);
@@ -587,7 +587,7 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
             st.converged = 1;
             break;
         }
-        st.linear_iters += bicgstab(c, max_lin, lin_rtol, rn);
+        st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
         // damped line search, thermal_solver.cpp:192-203
         double a = 1.0;
         k_axpy2<<<g, 256, 0, stm>>>(nb, a, c.v[V_X], c.v[V_DX], c.v[V_XT]);

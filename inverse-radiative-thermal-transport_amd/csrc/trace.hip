@@ -288,6 +288,8 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     const uint32_t ld = s.ld;
     const size_t nb2 = (size_t)nb * ld;
     if (nb == 0) throw Error(IRTT_ERR_STATE, "empty scene");
+    op_reset(s);
+    s.has_transport = false;
 
     // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
     // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s

@@ -123,6 +123,35 @@ def _declare(L):
                                      _vp, _vp, ctypes.POINTER(AdjointStats)]
     L.irtt_project_gradient.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
                                         ctypes.c_uint32, _vp]
+    L.irtt_select_operator.argtypes = [_vp, ctypes.c_uint32]
+    L.irtt_compress_transport.argtypes = [_vp, ctypes.POINTER(AcaParams), ctypes.POINTER(AcaStats)]
+    L.irtt_get_aca.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+
+
+OP_DENSE = 0
+OP_ACA = 1
+OP_DENSE_F16 = 2
+ACA_FULL = 0
+ACA_PARTIAL = 1
+ACA_PLUS = 2
+STORE_F32 = 0
+STORE_F16 = 1
+
+
+class AcaParams(ctypes.Structure):
+    _fields_ = [("algorithm", ctypes.c_uint32), ("rank", ctypes.c_uint32),
+                ("column_sum_correction", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+                ("precision", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
+
+
+class AcaStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("err_inf", "err_frob", "err_inf_rel", "err_frob_rel")] + [
+        ("compress_ms", ctypes.c_float), ("rank", ctypes.c_uint32),
+        ("n_corrected", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
 
 
 ADJ_AREA = 1
@@ -347,6 +376,31 @@ class Scene:
         c = np.empty(self.n_bins)
         _check(lib().irtt_get_column_sums(self._h, _ptr(c)))
         return c
+
+    def select_operator(self, op):
+        """OP_DENSE, OP_DENSE_F16 or OP_ACA for the solves that follow."""
+        _check(lib().irtt_select_operator(self._h, op))
+
+    def compress(self, algorithm=ACA_PARTIAL, rank=50, column_sum_correction=False, seed=1,
+                 precision=STORE_F32):
+        """ACA of the operator's photon part; selects OP_ACA (irtt_compress_transport)."""
+        p = AcaParams(algorithm, rank, int(bool(column_sum_correction)), seed, precision)
+        st = AcaStats()
+        _check(lib().irtt_compress_transport(self._h, ctypes.byref(p), ctypes.byref(st)))
+        self._aca_rank = st.rank
+        return st.as_dict()
+
+    def aca_factors(self):
+        """(Ut (k x n), V (k x n), C (n), pivot rows (k), pivot columns (k));
+        Ut holds the columns of U."""
+        n, k = self.n_bins, self._aca_rank
+        U = np.empty((n, k))
+        V = np.empty((k, n))
+        C = np.empty(n)
+        pr = np.empty(k, np.int32)
+        pc = np.empty(k, np.int32)
+        _check(lib().irtt_get_aca(self._h, _ptr(U), _ptr(V), _ptr(C), _ptr(pr), _ptr(pc)))
+        return np.ascontiguousarray(U.T), V, C, pr, pc
 
     def solve(self, params=None, **kw):
         p = params if params is not None else solver_params(**kw)

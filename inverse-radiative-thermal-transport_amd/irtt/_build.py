@@ -17,7 +17,8 @@ PKG = os.path.dirname(HERE)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libirtt.so")
 
-SOURCES = ["api.cpp", "compat_api.cpp", "lbvh.hip", "trace.hip", "solver.hip", "adjoint.hip"]
+SOURCES = ["api.cpp", "compat_api.cpp", "lbvh.hip", "trace.hip", "operator.hip", "aca.hip",
+           "solver.hip", "adjoint.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

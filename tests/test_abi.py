@@ -32,6 +32,39 @@ def test_struct_layouts(irtt):
     assert irtt.MATERIAL_DTYPE.itemsize == 32
     assert ctypes.sizeof(irtt.TransportParams) == 32
     assert ctypes.sizeof(irtt.SolverParams) == 56
+    assert ctypes.sizeof(irtt.AcaParams) == 32
+    assert ctypes.sizeof(irtt.AcaStats) == 48
+
+
+def test_struct_layouts_match_the_c_header(irtt, tmp_path):
+    """sizeof and every field offset as a C compiler lays out include/irtt.h
+    equal the ctypes mirrors."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("no C compiler")
+    pairs = {"irtt_material": irtt.Material, "irtt_transport_params": irtt.TransportParams,
+             "irtt_transport_stats": irtt.TransportStats, "irtt_solver_params": irtt.SolverParams,
+             "irtt_solve_stats": irtt.SolveStats, "irtt_adjoint_params": irtt.AdjointParams,
+             "irtt_adjoint_stats": irtt.AdjointStats, "irtt_aca_params": irtt.AcaParams,
+             "irtt_aca_stats": irtt.AcaStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "irtt.h"', "int main(void){"]
+    for cname, cls in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, value = line.split()
+        cls = pairs[cname]
+        got = ctypes.sizeof(cls) if field == "size" else getattr(cls, field).offset
+        assert got == int(value), (cname, field, got, value)
 
 
 class ObjectProperties(ctypes.Structure):
