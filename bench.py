@@ -61,6 +61,46 @@ def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):
                       f"included), {dt:.1f} s"}
 
 
+def solve_rates(irtt, s, sd, reps=3):
+    """Forward steady solve + adjoint solve on the built operator, FP32 form
+    and FP16 form (FP16 Krylov products, FP32-operator Newton residuals).
+    Returns solves/s of each and the largest temperature difference; the
+    FP32-form solve converges the FP64-accumulated residual of the FP32
+    operator, i.e. it is the FP64 solution of that operator to the Newton
+    tolerance (|r|^2 < f_tol)."""
+    import numpy as np
+    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14,
+                            lin_rtol=1e-10)
+    _, ea, _ = s.bin_data()
+    out = {}
+    xs = {}
+    for name, op in (("fp32", irtt.OP_DENSE), ("fp16", irtt.OP_DENSE_F16)):
+        s.select_operator(op)
+        times = []
+        for _ in range(reps):
+            s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+            t0 = time.perf_counter()
+            fst = s.solve(sp)
+            x = s.temperatures()
+            # objective gradient of sum_i A_i (x_i - mean x)^2 (thermal_objective.cpp:47-57)
+            rhs = ea * (x - x.mean())
+            lam, ast = s.adjoint_solve(rhs, params=sp)
+            times.append(time.perf_counter() - t0)
+        xs[name] = x
+        t = min(times)
+        out[name] = {"solves_per_s": round(1.0 / t, 2), "ms": round(t * 1e3, 2),
+                     "forward_ms": round(fst["solve_ms"], 2),
+                     "adjoint_ms": round(ast["solve_ms"], 2),
+                     "newton_iters": fst["newton_iters"],
+                     "linear_iters": fst["linear_iters"] + ast["linear_iters"],
+                     "residual_sq": fst["residual_sq"]}
+    s.select_operator(irtt.OP_DENSE)
+    out["max_dT_fp16_vs_fp32_K"] = float(np.abs(xs["fp16"] - xs["fp32"]).max())
+    out["speedup_fp16"] = round(out["fp16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
+    out["unit"] = "forward steady + adjoint solves per second (city block operator resident)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,6 +145,7 @@ def main():
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-10)
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
+    solves = solve_rates(irtt, s, sd)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -139,6 +180,7 @@ def main():
                          "bvh_build": round(st["bvh_ms"], 3)},
         "steady_solve": {"ms": round(sol["solve_ms"], 1), "newton_iters": sol["newton_iters"],
                          "linear_iters": sol["linear_iters"], "converged": sol["converged"]},
+        "solves": solves,
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread)

@@ -281,6 +281,72 @@ int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nn
                           const uint32_t *vertex, const uint32_t *param, const float *dxdp,
                           uint32_t n_params, double *dp);
 
+/* ---------------------------------------------------------------- serving */
+/* Request loop over design variants (no counterpart inside the reference,
+ * whose ThermalRenderer runs one scene at a time, thermal_renderer.cpp:958-1133).
+ * Each request is one design: scene -> LBVH -> transport -> steady (or one
+ * transient) solve -> adjoint solve -> adjoint trace -> dL/dp.  Requests are
+ * admitted in arrival order as a slot and their HBM footprint fit (first fit
+ * over the queue: a small request may pass a large one waiting for memory),
+ * and run concurrently, one HIP stream per slot, so work of different
+ * requests overlaps on the device.  Every kernel is deterministic, so a
+ * request's results do not depend on what runs beside it. */
+typedef struct irtt_server_s *irtt_server;
+
+typedef struct irtt_serve_config {
+    uint32_t slots;             /* requests on the device at once (0 -> 4) */
+    uint32_t reserved0;
+    uint64_t memory_budget;     /* HBM bytes for requests (0 -> 90% of free memory) */
+} irtt_serve_config;
+
+/* Inputs are copied at submit; outputs are written when the request ends. */
+typedef struct irtt_request {
+    const float *xyz;           /* 3 * nv */
+    uint32_t nv, nt;
+    const uint32_t *tri;        /* 3 * nt */
+    const irtt_material *mat;   /* nt */
+    uint32_t bin_mode;
+    uint32_t operator_kind;     /* IRTT_OP_DENSE or IRTT_OP_DENSE_F16 */
+    const float *kelvin0;       /* per vertex (vertex bins) or per triangle; NULL = 0 K */
+    const uint8_t *fixed;       /* same indexing; NULL = none fixed */
+    irtt_transport_params transport;
+    irtt_solver_params solver;
+    /* objective L = w * sum_i A_i (x_i - target_i)^2 (thermal_objective.cpp:34,47-57);
+       target NULL: forward solve only */
+    const double *target;       /* n_bins */
+    double objective_weight;
+    /* design parameters: dp[param[k]] += dxdp[3k..3k+2] . dL/dvertex[vertex[k]] */
+    uint32_t nnz, n_params;
+    const uint32_t *vertex, *param;
+    const float *dxdp;
+    /* outputs (caller-owned) */
+    double *kelvin;             /* n_bins, or NULL */
+    double *dp;                 /* n_params, or NULL */
+    double *objective;          /* 1, or NULL */
+} irtt_request;
+
+#define IRTT_REQ_QUEUED 0
+#define IRTT_REQ_RUNNING 1
+#define IRTT_REQ_DONE 2        /* negative: failed with that status code */
+
+typedef struct irtt_request_status {
+    int32_t state;
+    uint32_t n_bins;
+    double queued_ms;           /* submit -> admission */
+    double run_ms;              /* admission -> done */
+    double latency_ms;          /* submit -> done */
+    uint64_t footprint_bytes;   /* HBM reserved for the request */
+} irtt_request_status;
+
+int irtt_serve_create(const irtt_serve_config *cfg, irtt_server *out);
+int irtt_serve_submit(irtt_server sv, const irtt_request *req, uint64_t *ticket);
+/* non-blocking status; irtt_serve_wait blocks until the request has ended.
+ * The status of an ended request stays readable until irtt_serve_destroy. */
+int irtt_serve_poll(irtt_server sv, uint64_t ticket, irtt_request_status *st);
+int irtt_serve_wait(irtt_server sv, uint64_t ticket, irtt_request_status *st);
+/* waits for every submitted request, then releases the server */
+int irtt_serve_destroy(irtt_server sv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,7 +46,9 @@ int guard(F &&f)
 Scene *S(irtt_scene h)
 {
     if (!h) throw Error(IRTT_ERR_ARG, "null scene handle");
-    return reinterpret_cast<Scene *>(h);
+    Scene *s = reinterpret_cast<Scene *>(h);
+    t_alloc_stream = s->stream;
+    return s;
 }
 
 // cotangent of the angle between a and b, thermal_data.cpp:253-258; a
@@ -211,6 +213,8 @@ void upload_geometry(Scene &s)
 
 void irtt::set_last_error(const std::string &msg) { g_err = msg; }
 
+thread_local hipStream_t irtt::t_alloc_stream = nullptr;
+
 uint64_t Scene::device_bytes() const
 {
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
@@ -274,7 +278,17 @@ int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32
         IRTT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         IRTT_HIP(hipEventCreate(&s->ev0));
         IRTT_HIP(hipEventCreate(&s->ev1));
-        upload_geometry(*s);
+        t_alloc_stream = s->stream;
+        try {
+            upload_geometry(*s);
+        } catch (...) {
+            hipStream_t st = s->stream;
+            s.reset();
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+            t_alloc_stream = nullptr;
+            throw;
+        }
         *out = reinterpret_cast<irtt_scene>(s.release());
     });
 }
@@ -284,10 +298,12 @@ int irtt_scene_destroy(irtt_scene h)
     return guard([&] {
         if (!h) return;
         Scene *s = S(h);
+        t_alloc_stream = nullptr;
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         hipStream_t st = s->stream;
         hipEvent_t e0 = s->ev0, e1 = s->ev1;
-        delete s;
+        delete s;  // buffers go back to the pool in the scene's stream order
+        if (st) (void)hipStreamSynchronize(st);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (st) (void)hipStreamDestroy(st);

@@ -32,30 +32,53 @@ constexpr double kSigmaD = 5.670374419e-8;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kMaxDepthDefault = 512;     // MAX_MAX_DEPTH, thermal_tracing_unified.glsl:7
 
+// Stream on which the calling thread's device buffers are allocated and freed
+// (stream-ordered hipMallocAsync / hipFreeAsync), set by every entry point
+// that takes a scene to that scene's stream: scenes served concurrently from
+// different threads allocate and free without synchronising the device.
+// nullptr: plain hipMalloc / hipFree.
+extern thread_local hipStream_t t_alloc_stream;
+
 // Device buffer with RAII release.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    hipStream_t st = nullptr;  // stream the buffer was allocated on (nullptr: hipMalloc)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (st) (void)hipFreeAsync(p, st);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        st = nullptr;
     }
     void alloc(size_t count)
     {
         if (count == n && p) return;
         release();
         if (count == 0) return;
-        hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+        const size_t b = count * sizeof(T);
+        hipError_t e = hipErrorNotSupported;
+        if (t_alloc_stream) {
+            e = hipMallocAsync(reinterpret_cast<void **>(&p), b, t_alloc_stream);
+            if (e == hipSuccess) st = t_alloc_stream;
+        }
+        if (e == hipErrorNotSupported) {
+            (void)hipGetLastError();
+            e = hipMalloc(reinterpret_cast<void **>(&p), b);
+        }
         if (e != hipSuccess) {
             p = nullptr;
-            throw Error(IRTT_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+            st = nullptr;
+            (void)hipGetLastError();
+            throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(b) +
                                             " bytes failed: " + hipGetErrorString(e));
         }
         n = count;

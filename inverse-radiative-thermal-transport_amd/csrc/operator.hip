@@ -24,42 +24,66 @@ namespace {
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// y_i = sum_j T_ij u_j: one wave per row, float4 loads, FP64 accumulation.
+// Rows per wave in the row-major products: the wave reads each input-vector
+// chunk once for kGemvRows rows, so the vector's cache traffic is a quarter
+// of the matrix's.
+constexpr uint32_t kGemvRows = 4;
+
+// y_i = sum_j T_ij u_j: four rows per wave, float4 loads, FP64 accumulation.
 __global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, uint32_t nb,
                                                    uint32_t ld, const double *__restrict__ u,
                                                    double *__restrict__ y)
 {
-    uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    uint32_t lane = threadIdx.x & 63;
-    if (row >= nb) return;
-    const float4 *r = reinterpret_cast<const float4 *>(T + (size_t)row * ld);
-    double acc = 0.0;
+    const uint32_t row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kGemvRows;
+    const uint32_t lane = threadIdx.x & 63;
+    if (row0 >= nb) return;
+    const float4 *r[kGemvRows];
+    for (uint32_t q = 0; q < kGemvRows; q++)
+        r[q] = reinterpret_cast<const float4 *>(T + (size_t)min(row0 + q, nb - 1) * ld);
+    double acc[kGemvRows] = {};
     const uint32_t n4 = ld >> 2;
     for (uint32_t k = lane; k < n4; k += 64) {
-        float4 v = r[k];
-        uint32_t j = 4 * k;
-        // the padded tail of each row is zero and u is padded with zeros
-        acc += (double)v.x * u[j] + (double)v.y * u[j + 1] + (double)v.z * u[j + 2] +
-               (double)v.w * u[j + 3];
+        const double4 uu = reinterpret_cast<const double4 *>(u)[k];
+        for (uint32_t q = 0; q < kGemvRows; q++) {
+            float4 v = r[q][k];
+            // the padded tail of each row is zero and u is padded with zeros
+            acc[q] += (double)v.x * uu.x + (double)v.y * uu.y + (double)v.z * uu.z +
+                      (double)v.w * uu.w;
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) y[row] = acc;
+    for (uint32_t q = 0; q < kGemvRows; q++) {
+        double a = acc[q];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0 && row0 + q < nb) y[row0 + q] = a;
+    }
 }
 
 // y = T^T u in two deterministic steps: partial column sums over row chunks
 // of kTRows rows (threads along columns read coalesced rows), then a fixed-
 // order sum over the chunks.
 constexpr uint32_t kTRows = 512;
+// four columns per thread (float4 loads)
 __global__ __launch_bounds__(256) void k_gemv_t_partial(const float *__restrict__ T, uint32_t nb,
                                                         uint32_t ld, const double *__restrict__ u,
                                                         double *__restrict__ part)
 {
-    uint32_t j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= ld) return;
+    uint32_t j4 = blockIdx.x * 256 + threadIdx.x;
+    if (4 * j4 >= ld) return;
     uint32_t r0 = blockIdx.y * kTRows, r1 = min(r0 + kTRows, nb);
-    double acc = 0.0;
-    for (uint32_t i = r0; i < r1; i++) acc += (double)T[(size_t)i * ld + j] * u[i];
-    part[(size_t)blockIdx.y * ld + j] = acc;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    for (uint32_t i = r0; i < r1; i++) {
+        const float4 v = reinterpret_cast<const float4 *>(T + (size_t)i * ld)[j4];
+        const double w = u[i];
+        a0 += (double)v.x * w;
+        a1 += (double)v.y * w;
+        a2 += (double)v.z * w;
+        a3 += (double)v.w * w;
+    }
+    double *o = part + (size_t)blockIdx.y * ld + 4 * j4;
+    o[0] = a0;
+    o[1] = a1;
+    o[2] = a2;
+    o[3] = a3;
 }
 
 __global__ void k_chunk_reduce(const double *__restrict__ part, uint32_t nchunk, uint32_t ld,
@@ -108,33 +132,38 @@ __global__ void k_vec_f32(const double *__restrict__ u, uint32_t n, float *__res
     if (i < n) o[i] = (float)u[i];
 }
 
-// y_r = scale_r * sum_c H_rc u_c: one wave per row, 8 halves (16 B) per lane
-// per step, FP32 products and partial sums, FP64 across lanes
+// y_r = scale_r * sum_c H_rc u_c: four rows per wave, 8 halves (16 B) per
+// lane and row per step, FP32 products and partial sums, FP64 across lanes
 __global__ __launch_bounds__(256) void k_gemv16_rows(const __half *__restrict__ H,
                                                      const float *__restrict__ scale, uint32_t nb,
                                                      uint32_t ld, const float *__restrict__ u,
                                                      double *__restrict__ y)
 {
-    uint32_t row = blockIdx.x * 4 + (threadIdx.x >> 6);
-    uint32_t lane = threadIdx.x & 63;
-    if (row >= nb) return;
-    const uint4 *r = reinterpret_cast<const uint4 *>(H + (size_t)row * ld);
+    const uint32_t row0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kGemvRows;
+    const uint32_t lane = threadIdx.x & 63;
+    if (row0 >= nb) return;
+    const uint4 *r[kGemvRows];
+    for (uint32_t q = 0; q < kGemvRows; q++)
+        r[q] = reinterpret_cast<const uint4 *>(H + (size_t)min(row0 + q, nb - 1) * ld);
     const float4 *u4 = reinterpret_cast<const float4 *>(u);
-    float acc = 0.0f;
+    float acc[kGemvRows] = {};
     const uint32_t n8 = ld >> 3;
     for (uint32_t k = lane; k < n8; k += 64) {
-        uint4 h = r[k];
-        float4 ua = u4[2 * k], ub = u4[2 * k + 1];
-        __half2 h0 = *reinterpret_cast<__half2 *>(&h.x), h1 = *reinterpret_cast<__half2 *>(&h.y);
-        __half2 h2 = *reinterpret_cast<__half2 *>(&h.z), h3 = *reinterpret_cast<__half2 *>(&h.w);
-        float2 f0 = __half22float2(h0), f1 = __half22float2(h1);
-        float2 f2 = __half22float2(h2), f3 = __half22float2(h3);
-        acc += f0.x * ua.x + f0.y * ua.y + f1.x * ua.z + f1.y * ua.w + f2.x * ub.x + f2.y * ub.y +
-               f3.x * ub.z + f3.y * ub.w;
+        const float4 ua = u4[2 * k], ub = u4[2 * k + 1];
+        for (uint32_t q = 0; q < kGemvRows; q++) {
+            uint4 h = r[q][k];
+            const __half2 *hp = reinterpret_cast<const __half2 *>(&h);
+            float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
+            float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
+            acc[q] += f0.x * ua.x + f0.y * ua.y + f1.x * ua.z + f1.y * ua.w + f2.x * ub.x +
+                      f2.y * ub.y + f3.x * ub.z + f3.y * ub.w;
+        }
     }
-    double a = acc;
-    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
-    if (lane == 0) y[row] = a * (double)scale[row];
+    for (uint32_t q = 0; q < kGemvRows; q++) {
+        double a = acc[q];
+        for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+        if (lane == 0 && row0 + q < nb) y[row0 + q] = a * (double)scale[row0 + q];
+    }
 }
 
 // part[chunk][c] = sum_{r in chunk} scale_r u_r H_rc, 8 columns per thread
@@ -229,10 +258,10 @@ void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine
     const uint32_t nchunk = cdiv(nb, kTRows);
     if (kind == IRTT_OP_DENSE) {
         if (!transposed) {
-            k_gemv_rows<<<cdiv(nb, 4), 256, 0, st>>>(s.d_T.p, nb, ld, w, y);
+            k_gemv_rows<<<cdiv(nb, 4 * kGemvRows), 256, 0, st>>>(s.d_T.p, nb, ld, w, y);
         } else {
             s.d_tpart.alloc((size_t)nchunk * ld);
-            k_gemv_t_partial<<<dim3(cdiv(ld, 256), nchunk), 256, 0, st>>>(s.d_T.p, nb, ld, w,
+            k_gemv_t_partial<<<dim3(cdiv(ld / 4, 256), nchunk), 256, 0, st>>>(s.d_T.p, nb, ld, w,
                                                                            s.d_tpart.p);
             k_chunk_reduce<<<cdiv(ld, 256), 256, 0, st>>>(s.d_tpart.p, nchunk, ld, y);
         }
@@ -242,7 +271,7 @@ void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine
     if (!transposed) {
         s.d_u32.alloc(ld);
         k_vec_f32<<<cdiv(ld, 256), 256, 0, st>>>(w, ld, s.d_u32.p);
-        k_gemv16_rows<<<cdiv(nb, 4), 256, 0, st>>>(H, s.d_h16s.p, nb, ld, s.d_u32.p, y);
+        k_gemv16_rows<<<cdiv(nb, 4 * kGemvRows), 256, 0, st>>>(H, s.d_h16s.p, nb, ld, s.d_u32.p, y);
     } else {
         s.d_tpart.alloc((size_t)nchunk * ld);
         k_gemv16_t_partial<<<dim3(cdiv(ld / 8, 256), nchunk), 256, 0, st>>>(H, s.d_h16s.p, nb, ld,

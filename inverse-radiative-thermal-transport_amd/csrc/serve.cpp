@@ -1,0 +1,356 @@
+// serve.cpp -- request loop over design variants with continuous admission.
+//
+// The reference runs one scene at a time inside ThermalRenderer
+// (thermal_renderer.cpp:958-1133: forward simulation, adjoint, gradient); this
+// server runs many designs concurrently on one GPU.  Worker threads (slots)
+// take requests in arrival order, first fit against the HBM budget, so a
+// request that arrives while others run is admitted as soon as a slot and
+// its footprint are free.  Each request owns a scene, and with it a HIP
+// stream and stream-ordered device buffers, so the kernels of different
+// requests overlap on the device and allocation does not synchronise it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <functional>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "irtt_internal.hpp"
+
+namespace irtt {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Req {
+    uint64_t ticket = 0;
+    // owned copies of the inputs
+    std::vector<float> xyz, kelvin0, dxdp;
+    std::vector<uint32_t> tri, vertex, param;
+    std::vector<irtt_material> mat;
+    std::vector<uint8_t> fixed;
+    std::vector<double> target;
+    irtt_request r{};  // scalars and output pointers
+    uint32_t nb = 0;
+    uint64_t footprint = 0;
+    int state = IRTT_REQ_QUEUED;
+    Clock::time_point t_submit, t_start, t_end;
+};
+
+// HBM needed by one request: fixed-point + FP32 operator (+ FP16 copy),
+// solver vectors and transposed-product partials, geometry and LBVH
+uint64_t footprint(const irtt_request &q, uint32_t nb)
+{
+    const uint64_t ld = (nb + 63) / 64 * 64;
+    uint64_t b = (uint64_t)nb * ld * 12;
+    if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
+    b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;
+    b += (uint64_t)q.nt * 256 + (uint64_t)q.nv * 64 + (64ull << 20);
+    return b;
+}
+
+}  // namespace
+
+struct Server {
+    irtt_serve_config cfg{};
+    uint64_t budget = 0, in_use = 0;
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Req *> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::vector<std::thread> workers;
+
+    // first queued request whose footprint fits; nullptr if none
+    Req *admit_locked()
+    {
+        for (auto it = queue.begin(); it != queue.end(); ++it) {
+            if ((*it)->footprint <= budget - in_use) {
+                Req *q = *it;
+                queue.erase(it);
+                in_use += q->footprint;
+                q->state = IRTT_REQ_RUNNING;
+                q->t_start = Clock::now();
+                return q;
+            }
+        }
+        return nullptr;
+    }
+
+    void worker()
+    {
+        for (;;) {
+            Req *q = nullptr;
+            {
+                std::unique_lock<std::mutex> lk(m);
+                cv_work.wait(lk, [&] { return stop || (q = admit_locked()) != nullptr; });
+                if (!q) return;  // stop with nothing admissible left
+            }
+            int rc = run(*q);
+            {
+                std::lock_guard<std::mutex> lk(m);
+                in_use -= q->footprint;
+                q->t_end = Clock::now();
+                q->state = rc == IRTT_OK ? IRTT_REQ_DONE : rc;
+            }
+            cv_done.notify_all();
+            cv_work.notify_all();
+        }
+    }
+
+    static int run(Req &q);
+};
+
+// One design: scene -> transport -> solve -> adjoint solve -> adjoint trace
+// -> projection (the steps of ThermalRenderer::runForwardSim / runAdjointSim
+// for one state, thermal_renderer.cpp:958-1133).
+int Server::run(Req &q)
+{
+    const irtt_request &r = q.r;
+    irtt_scene h = nullptr;
+    int rc = irtt_scene_create(q.xyz.data(), r.nv, q.tri.data(), r.nt, q.mat.data(), r.bin_mode, &h);
+    if (rc) return rc;
+    auto done = [&](int code) {
+        irtt_scene_destroy(h);
+        return code;
+    };
+    const uint32_t nb = q.nb;
+    if (!q.kelvin0.empty() || !q.fixed.empty()) {
+        const uint32_t n = r.bin_mode == IRTT_BIN_VERTEX ? r.nv : r.nt;
+        std::vector<float> k0 = q.kelvin0.empty() ? std::vector<float>(n, 0.0f) : q.kelvin0;
+        std::vector<uint8_t> fx = q.fixed.empty() ? std::vector<uint8_t>(n, 0) : q.fixed;
+        rc = r.bin_mode == IRTT_BIN_VERTEX ? irtt_scene_set_vertex_state(h, k0.data(), fx.data())
+                                           : irtt_scene_set_bin_state(h, k0.data(), fx.data());
+        if (rc) return done(rc);
+    }
+    irtt_transport_stats ts{};
+    if ((rc = irtt_build_transport(h, &r.transport, &ts))) return done(rc);
+    if (r.operator_kind != IRTT_OP_DENSE && (rc = irtt_select_operator(h, r.operator_kind)))
+        return done(rc);
+    irtt_solve_stats ss{};
+    if ((rc = irtt_solve(h, &r.solver, &ss))) return done(rc);
+    std::vector<double> x(nb);
+    if ((rc = irtt_get_temperatures(h, x.data()))) return done(rc);
+    if (r.kelvin) std::memcpy(r.kelvin, x.data(), nb * sizeof(double));
+    if (q.target.empty()) return done(IRTT_OK);
+
+    // objective and its gradient in x (thermal_objective.cpp:34,47-57)
+    std::vector<double> area(nb), dLdx(nb);
+    if ((rc = irtt_scene_get_bin_data(h, area.data(), nullptr, nullptr))) return done(rc);
+    double L = 0.0;
+    for (uint32_t i = 0; i < nb; i++) {
+        double d = x[i] - q.target[i];
+        L += r.objective_weight * area[i] * d * d;
+        dLdx[i] = 2.0 * r.objective_weight * area[i] * d;
+    }
+    if (r.objective) *r.objective = L;
+    std::vector<double> lam(nb);
+    if ((rc = irtt_adjoint_solve(h, &r.solver, nullptr, dLdx.data(), lam.data(), &ss)))
+        return done(rc);
+    // history rows: (lambda, x^4) for the operator term, (-Ta^4, lambda) for
+    // the ambient term Ta^4 colsum(T); a transient step carries dt
+    const double scale = r.solver.steady ? 1.0 : (r.solver.dt > 0 ? r.solver.dt : 300.0);
+    const bool amb = r.solver.ambient_kelvin >= 0.0;
+    const uint32_t nh = amb ? 2 : 1;
+    std::vector<double> lrows(nh * (size_t)nb), xrows(nh * (size_t)nb);
+    const double Ta = r.solver.ambient_kelvin, Ta4 = Ta * Ta * Ta * Ta;
+    for (uint32_t i = 0; i < nb; i++) {
+        double x2 = x[i] * x[i];
+        lrows[i] = scale * lam[i];
+        xrows[i] = x2 * x2;
+        if (amb) {
+            lrows[nb + i] = -scale * Ta4;
+            xrows[nb + i] = lam[i];
+        }
+    }
+    std::vector<double> gx(3 * (size_t)r.nv);
+    irtt_adjoint_params ap{0.05f, 1e-2f, 0u, 0u};
+    irtt_adjoint_stats as{};
+    if ((rc = irtt_adjoint_trace(h, &ap, nh, lrows.data(), xrows.data(), gx.data(), &as)))
+        return done(rc);
+    if (r.dp && r.n_params) {
+        std::fill(r.dp, r.dp + r.n_params, 0.0);
+        if ((rc = irtt_project_gradient(gx.data(), r.nv, r.nnz, q.vertex.data(), q.param.data(),
+                                        q.dxdp.data(), r.n_params, r.dp)))
+            return done(rc);
+    }
+    return done(IRTT_OK);
+}
+
+}  // namespace irtt
+
+using irtt::Error;
+using irtt::Req;
+using irtt::Server;
+
+namespace {
+int sguard(const std::function<void()> &f)
+{
+    try {
+        f();
+        return IRTT_OK;
+    } catch (const Error &e) {
+        irtt::set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        irtt::set_last_error("host allocation failed");
+        return IRTT_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        irtt::set_last_error(e.what());
+        return IRTT_ERR_STATE;
+    }
+}
+
+Server *SV(irtt_server h)
+{
+    if (!h) throw Error(IRTT_ERR_ARG, "null server handle");
+    return reinterpret_cast<Server *>(h);
+}
+
+void fill_status(const Req &q, irtt_request_status *st)
+{
+    if (!st) return;
+    *st = irtt_request_status{};
+    st->state = q.state;
+    st->n_bins = q.nb;
+    st->footprint_bytes = q.footprint;
+    auto now = irtt::Clock::now();
+    if (q.state == IRTT_REQ_QUEUED) {
+        st->queued_ms = irtt::ms_between(q.t_submit, now);
+        return;
+    }
+    st->queued_ms = irtt::ms_between(q.t_submit, q.t_start);
+    auto end = q.state == IRTT_REQ_RUNNING ? now : q.t_end;
+    st->run_ms = irtt::ms_between(q.t_start, end);
+    st->latency_ms = irtt::ms_between(q.t_submit, end);
+}
+}  // namespace
+
+extern "C" {
+
+int irtt_serve_create(const irtt_serve_config *cfg, irtt_server *out)
+{
+    return sguard([&] {
+        if (!out) throw Error(IRTT_ERR_ARG, "out is null");
+        auto sv = std::make_unique<Server>();
+        if (cfg) sv->cfg = *cfg;
+        if (!sv->cfg.slots) sv->cfg.slots = 4;
+        if (sv->cfg.slots > 64) throw Error(IRTT_ERR_ARG, "at most 64 slots");
+        sv->budget = sv->cfg.memory_budget;
+        if (!sv->budget) {
+            size_t fr = 0, tot = 0;
+            IRTT_HIP(hipMemGetInfo(&fr, &tot));
+            sv->budget = (uint64_t)(0.9 * (double)fr);
+        }
+        Server *p = sv.get();
+        for (uint32_t i = 0; i < sv->cfg.slots; i++) sv->workers.emplace_back([p] { p->worker(); });
+        *out = reinterpret_cast<irtt_server>(sv.release());
+    });
+}
+
+int irtt_serve_submit(irtt_server h, const irtt_request *r, uint64_t *ticket)
+{
+    return sguard([&] {
+        Server *sv = SV(h);
+        if (!r || !r->xyz || !r->tri || !r->mat || !r->nv || !r->nt)
+            throw Error(IRTT_ERR_ARG, "request without geometry");
+        if (r->bin_mode > IRTT_BIN_TRIANGLE) throw Error(IRTT_ERR_ARG, "unknown bin mode");
+        if (r->operator_kind != IRTT_OP_DENSE && r->operator_kind != IRTT_OP_DENSE_F16)
+            throw Error(IRTT_ERR_ARG, "requests use the dense FP32 or FP16 operator");
+        if (r->nnz && (!r->vertex || !r->param || !r->dxdp))
+            throw Error(IRTT_ERR_ARG, "parameter map without arrays");
+        auto q = std::make_unique<Req>();
+        q->r = *r;
+        q->nb = r->bin_mode == IRTT_BIN_VERTEX ? r->nv : r->nt;
+        q->xyz.assign(r->xyz, r->xyz + 3ull * r->nv);
+        q->tri.assign(r->tri, r->tri + 3ull * r->nt);
+        q->mat.assign(r->mat, r->mat + r->nt);
+        const uint32_t ns = r->bin_mode == IRTT_BIN_VERTEX ? r->nv : r->nt;
+        if (r->kelvin0) q->kelvin0.assign(r->kelvin0, r->kelvin0 + ns);
+        if (r->fixed) q->fixed.assign(r->fixed, r->fixed + ns);
+        if (r->target) q->target.assign(r->target, r->target + q->nb);
+        if (r->nnz) {
+            q->vertex.assign(r->vertex, r->vertex + r->nnz);
+            q->param.assign(r->param, r->param + r->nnz);
+            q->dxdp.assign(r->dxdp, r->dxdp + 3ull * r->nnz);
+        }
+        q->footprint = irtt::footprint(*r, q->nb);
+        if (q->footprint > sv->budget)
+            throw Error(IRTT_ERR_LIMIT, "request needs " + std::to_string(q->footprint) +
+                                            " bytes of HBM, the budget is " +
+                                            std::to_string(sv->budget));
+        q->t_submit = irtt::Clock::now();
+        {
+            std::lock_guard<std::mutex> lk(sv->m);
+            if (sv->stop) throw Error(IRTT_ERR_STATE, "server is shutting down");
+            q->ticket = sv->next_ticket++;
+            if (ticket) *ticket = q->ticket;
+            sv->queue.push_back(q.get());
+            sv->reqs.emplace(q->ticket, std::move(q));
+        }
+        sv->cv_work.notify_one();
+    });
+}
+
+int irtt_serve_poll(irtt_server h, uint64_t ticket, irtt_request_status *st)
+{
+    return sguard([&] {
+        Server *sv = SV(h);
+        std::lock_guard<std::mutex> lk(sv->m);
+        auto it = sv->reqs.find(ticket);
+        if (it == sv->reqs.end()) throw Error(IRTT_ERR_ARG, "unknown ticket");
+        fill_status(*it->second, st);
+    });
+}
+
+int irtt_serve_wait(irtt_server h, uint64_t ticket, irtt_request_status *st)
+{
+    return sguard([&] {
+        Server *sv = SV(h);
+        std::unique_lock<std::mutex> lk(sv->m);
+        auto it = sv->reqs.find(ticket);
+        if (it == sv->reqs.end()) throw Error(IRTT_ERR_ARG, "unknown ticket");
+        Req *q = it->second.get();
+        sv->cv_done.wait(lk, [&] { return q->state != IRTT_REQ_QUEUED &&
+                                          q->state != IRTT_REQ_RUNNING; });
+        fill_status(*q, st);
+    });
+}
+
+int irtt_serve_destroy(irtt_server h)
+{
+    return sguard([&] {
+        if (!h) return;
+        Server *sv = SV(h);
+        {
+            // let the workers drain the queue, then stop
+            std::unique_lock<std::mutex> lk(sv->m);
+            sv->cv_done.wait(lk, [&] {
+                for (auto &kv : sv->reqs)
+                    if (kv.second->state == IRTT_REQ_QUEUED || kv.second->state == IRTT_REQ_RUNNING)
+                        return false;
+                return true;
+            });
+            sv->stop = true;
+        }
+        sv->cv_work.notify_all();
+        for (auto &t : sv->workers) t.join();
+        delete sv;
+    });
+}
+
+}  // extern "C"

@@ -91,6 +91,30 @@ class SolveStats(ctypes.Structure):
 _vp = ctypes.c_void_p
 
 
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("reserved0", ctypes.c_uint32),
+                ("memory_budget", ctypes.c_uint64)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("xyz", _vp), ("nv", ctypes.c_uint32), ("nt", ctypes.c_uint32), ("tri", _vp),
+                ("mat", _vp), ("bin_mode", ctypes.c_uint32), ("operator_kind", ctypes.c_uint32),
+                ("kelvin0", _vp), ("fixed", _vp), ("transport", TransportParams),
+                ("solver", SolverParams), ("target", _vp), ("objective_weight", ctypes.c_double),
+                ("nnz", ctypes.c_uint32), ("n_params", ctypes.c_uint32), ("vertex", _vp),
+                ("param", _vp), ("dxdp", _vp), ("kelvin", _vp), ("dp", _vp),
+                ("objective", _vp)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("n_bins", ctypes.c_uint32),
+                ("queued_ms", ctypes.c_double), ("run_ms", ctypes.c_double),
+                ("latency_ms", ctypes.c_double), ("footprint_bytes", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def _declare(L):
     L.irtt_version.restype = ctypes.c_int
     L.irtt_last_error.restype = ctypes.c_char_p
@@ -123,6 +147,11 @@ def _declare(L):
                                      _vp, _vp, ctypes.POINTER(AdjointStats)]
     L.irtt_project_gradient.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
                                         ctypes.c_uint32, _vp]
+    L.irtt_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(_vp)]
+    L.irtt_serve_submit.argtypes = [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]
+    L.irtt_serve_poll.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]
+    L.irtt_serve_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]
+    L.irtt_serve_destroy.argtypes = [_vp]
     L.irtt_select_operator.argtypes = [_vp, ctypes.c_uint32]
     L.irtt_compress_transport.argtypes = [_vp, ctypes.POINTER(AcaParams), ctypes.POINTER(AcaStats)]
     L.irtt_get_aca.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
@@ -450,3 +479,97 @@ class Scene:
         _check(lib().irtt_adjoint_trace(self._h, ctypes.byref(ap), lam.shape[0], _ptr(lam), _ptr(x),
                                         _ptr(g), ctypes.byref(st)))
         return g.reshape(-1, 3), st.as_dict()
+
+
+REQ_QUEUED = 0
+REQ_RUNNING = 1
+REQ_DONE = 2
+
+
+class Server:
+    """Request loop over design variants (irtt_serve_*): each request is a
+    scene + transport + solve (+ adjoint and dL/dp when a target is given);
+    requests run concurrently on the device as slots and HBM allow."""
+
+    def __init__(self, slots=4, memory_budget=0):
+        h = ctypes.c_void_p()
+        cfg = ServeConfig(slots, 0, memory_budget)
+        _check(lib().irtt_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}
+
+    def submit(self, sd, bin_mode=BIN_VERTEX, operator=OP_DENSE, transport=None, solver=None,
+               kelvin0=None, fixed=None, target=None, objective_weight=1.0, params=None):
+        """sd: SceneData.  params: (vertex, param, dxdp, n_params) for dL/dp.
+        Returns a ticket; outputs come back from wait()."""
+        nb = sd.nv if bin_mode == BIN_VERTEX else sd.nt
+        xyz = np.ascontiguousarray(sd.xyz, np.float32)
+        tri = np.ascontiguousarray(sd.idx, np.uint32)
+        mat = sd.materials()
+        arrays = [xyz, tri, mat]
+        r = Request()
+        r.xyz, r.nv, r.nt, r.tri, r.mat = _ptr(xyz), sd.nv, sd.nt, _ptr(tri), _ptr(mat)
+        r.bin_mode, r.operator_kind = bin_mode, operator
+        if kelvin0 is not None:
+            k0 = np.ascontiguousarray(kelvin0, np.float32)
+            arrays.append(k0)
+            r.kelvin0 = _ptr(k0)
+        if fixed is not None:
+            fx = np.ascontiguousarray(fixed, np.uint8)
+            arrays.append(fx)
+            r.fixed = _ptr(fx)
+        r.transport = transport if transport is not None else transport_params()
+        r.solver = solver if solver is not None else solver_params()
+        kelvin = np.zeros(nb)
+        objective = np.zeros(1)
+        dp = None
+        r.kelvin, r.objective = _ptr(kelvin), _ptr(objective)
+        if target is not None:
+            t = np.ascontiguousarray(np.broadcast_to(target, (nb,)), np.float64)
+            arrays.append(t)
+            r.target, r.objective_weight = _ptr(t), objective_weight
+            if params is not None:
+                v, p, d, n_params = params
+                v = np.ascontiguousarray(v, np.uint32)
+                p = np.ascontiguousarray(p, np.uint32)
+                d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+                arrays += [v, p, d]
+                dp = np.zeros(n_params)
+                r.nnz, r.n_params = v.size, n_params
+                r.vertex, r.param, r.dxdp, r.dp = _ptr(v), _ptr(p), _ptr(d), _ptr(dp)
+        tk = ctypes.c_uint64()
+        _check(lib().irtt_serve_submit(self._h, ctypes.byref(r), ctypes.byref(tk)))
+        self._keep[tk.value] = {"kelvin": kelvin, "dp": dp, "objective": objective}
+        return tk.value
+
+    def poll(self, ticket):
+        st = RequestStatus()
+        _check(lib().irtt_serve_poll(self._h, ticket, ctypes.byref(st)))
+        return st.as_dict()
+
+    def wait(self, ticket):
+        """(status dict, {"kelvin", "dp", "objective"}); raises if the request failed."""
+        st = RequestStatus()
+        _check(lib().irtt_serve_wait(self._h, ticket, ctypes.byref(st)))
+        if st.state < 0:
+            raise IrttError(st.state, f"request {ticket} failed with status {st.state}")
+        out = self._keep.pop(ticket)
+        out = {"kelvin": out["kelvin"], "dp": out["dp"], "objective": float(out["objective"][0])}
+        return st.as_dict(), out
+
+    def close(self):
+        if self._h:
+            _check(lib().irtt_serve_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -177,10 +177,13 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
     rest = max(target_tris - ground_tris, 1000)
     edge = np.sqrt(2.0 * (wall_area + roof_area) / rest)
     xs, ts, off = [], [], 0
+    owner = []   # building index per vertex, -1 on the ground
+    current = [-1]
 
     def add(o, u, v, nu, nv_):
         nonlocal off
         x, t = grid_patch(o, u, v, nu, nv_)
+        owner.append(np.full(x.shape[0], current[0], np.int32))
         xs.append(x)
         ts.append(t + off)
         off += x.shape[0]
@@ -198,7 +201,8 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
             while cx < ncx and not occupied[cx, cy]:
                 cx += 1
             add([c0 * g, cy * g, 0.0], [(cx - c0) * g, 0, 0], [0, g, 0], cx - c0, 1)
-    for ox, oy, wc, dc, h in bld:
+    for b, (ox, oy, wc, dc, h) in enumerate(bld):
+        current[0] = b
         x0, y0, x1, y1 = ox * g, oy * g, (ox + wc) * g, (oy + dc) * g
         nh = max(1, int(round(h / edge)))
         nwx = max(1, int(round((x1 - x0) / edge)))
@@ -225,5 +229,22 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
     areas = sd.triangle_areas()
     sd.meta.update(seed=seed, buildings=len(bld), edge=float(edge), sun=sun.tolist(),
                    mean_triangle_area=float(areas.mean()), total_area=float(areas.sum()),
-                   alpha=alpha.astype(np.float32))
+                   alpha=alpha.astype(np.float32), vertex_building=np.concatenate(owner),
+                   heights=np.array([b[4] for b in bld]))
     return sd
+
+
+def height_parametrization(sd, n_params=None):
+    """Building heights as design parameters of a city block: raising
+    building b by dh moves each of its vertices by z / h_b * dh along z (walls
+    stretch, the roof lifts).  Returns (vertex, param, dxdp, n_params) for
+    irtt_project_gradient; building b maps to parameter b % n_params.
+    (The reference's vertex-subset parametrizations, thermal_params.hpp:41-130.)"""
+    owner = sd.meta["vertex_building"]
+    h = sd.meta["heights"]
+    n = len(h) if n_params is None else n_params
+    v = np.nonzero(owner >= 0)[0].astype(np.uint32)
+    b = owner[v]
+    d = np.zeros((v.size, 3), np.float32)
+    d[:, 2] = sd.xyz[v, 2] / h[b]
+    return v, (b % n).astype(np.uint32), d, n
