@@ -162,7 +162,16 @@ int irtt_get_column_sums(irtt_scene h, double *colsum);
  * returns to IRTT_OP_DENSE and drops the other forms. */
 int irtt_select_operator(irtt_scene h, uint32_t op);
 
-#define IRTT_ACA_FULL 0u      /* simpleAdaptiveCrossApproximation, thermal_transport.cpp:360-386 */
+/* Y = T X (transposed: T^T X) for n_rhs vectors with the selected operator
+ * form (X, Y: n_rhs x n_bins, row-major, FP64).  IRTT_OP_DENSE_F16 runs one
+ * FP16 MFMA GEMM per 64 vectors (each vector quantised to FP16 with one FP32
+ * scale, FP32 accumulation); the other forms one product per vector.  The
+ * reference's single product x = T x (solver mode 1, thermal_solver.cpp:141-150)
+ * for many vectors at once. */
+int irtt_apply_operator(irtt_scene h, uint32_t n_rhs, const double *X, double *Y,
+                        uint32_t transposed);
+
+#define IRTT_ACA_FULL 0u    /* simpleAdaptiveCrossApproximation, thermal_transport.cpp:360-386 */
 #define IRTT_ACA_PARTIAL 1u   /* completeAdaptiveCrossApproximation (reference default), :388-461 */
 #define IRTT_ACA_PLUS 2u      /* adaptiveCrossApproximationPlus, :522-597 */
 #define IRTT_STORE_F32 0u
@@ -231,6 +240,21 @@ int irtt_set_temperatures(irtt_scene h, const double *kelvin); /* current state 
 /* residual r(x) of the assembled system at x (n_bins), FP64 */
 int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x,
                   double *r);
+
+/* n_rhs steady solves that share the operator, conduction, fixed bins and
+ * ambient, each with its own external absorbed power (source: n_rhs x n_bins,
+ * W per bin; NULL = the scene's), in lockstep: every Krylov product is one
+ * multi-vector product (an FP16 MFMA GEMM with IRTT_OP_DENSE_F16, whose Newton
+ * iteration then finishes on residuals of the FP32 operator).  Each column is
+ * irtt_solve's steady solve (exact Jacobian), started from the current state.
+ * kelvin: n_rhs x n_bins out.  The scene state is not changed. */
+int irtt_solve_batch(irtt_scene h, const irtt_solver_params *p, uint32_t n_rhs,
+                     const double *source, double *kelvin, irtt_solve_stats *st);
+/* n_rhs adjoint solves J(x_c)^T lambda_c = rhs_c (steady, exact Jacobian) in
+ * lockstep; x, rhs, lambda: n_rhs x n_bins. */
+int irtt_adjoint_solve_batch(irtt_scene h, const irtt_solver_params *p, uint32_t n_rhs,
+                             const double *x, const double *rhs, double *lambda,
+                             irtt_solve_stats *st);
 
 /* ---------------------------------------------------------------- adjoint */
 /* Replaces ThermalRenderer::adjointTimestep (thermal_renderer.cpp:932-956):

@@ -440,6 +440,55 @@ int irtt_select_operator(irtt_scene h, uint32_t op)
     return guard([&] { op_select(*S(h), op); });
 }
 
+int irtt_apply_operator(irtt_scene h, uint32_t n_rhs, const double *X, double *Y,
+                        uint32_t transposed)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (n_rhs == 0) return;
+        if (!X || !Y) throw Error(IRTT_ERR_ARG, "null pointer");
+        const size_t nb = s->nb, ld = s->ld;
+        DBuf<double> dx, dy;
+        dx.alloc((size_t)n_rhs * ld);
+        dy.alloc((size_t)n_rhs * ld);
+        hipStream_t st = s->stream;
+        IRTT_HIP(hipMemsetAsync(dx.p, 0, dx.bytes(), st));
+        IRTT_HIP(hipMemcpy2DAsync(dx.p, ld * 8, X, nb * 8, nb * 8, n_rhs, hipMemcpyHostToDevice, st));
+        op_apply_batch(*s, n_rhs, dx.p, dy.p, transposed != 0);
+        IRTT_HIP(hipGetLastError());
+        IRTT_HIP(hipMemcpy2DAsync(Y, nb * 8, dy.p, ld * 8, nb * 8, n_rhs, hipMemcpyDeviceToHost, st));
+        IRTT_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int irtt_solve_batch(irtt_scene h, const irtt_solver_params *p, uint32_t n_rhs,
+                     const double *source, double *kelvin, irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !kelvin) throw Error(IRTT_ERR_ARG, "null pointer");
+        if (n_rhs == 0) return;
+        irtt_solve_stats tmp{};
+        solve_batch(*s, *p, n_rhs, source, kelvin, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_adjoint_solve_batch(irtt_scene h, const irtt_solver_params *p, uint32_t n_rhs,
+                             const double *x, const double *rhs, double *lambda,
+                             irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !x || !rhs || !lambda) throw Error(IRTT_ERR_ARG, "null pointer");
+        if (n_rhs == 0) return;
+        irtt_solve_stats tmp{};
+        adjoint_solve_batch(*s, *p, n_rhs, x, rhs, lambda, tmp);
+        if (st) *st = tmp;
+    });
+}
+
 int irtt_compress_transport(irtt_scene h, const irtt_aca_params *p, irtt_aca_stats *st)
 {
     return guard([&] {

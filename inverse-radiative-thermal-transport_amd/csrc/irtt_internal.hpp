@@ -186,6 +186,12 @@ struct Scene {
     DBuf<uint16_t> d_h16;             // nb*ld FP16 off-diagonal operator (bit patterns)
     DBuf<float> d_h16s;               // its per-row scales
     DBuf<float> d_u32;                // FP32 copy of a product's input vector
+    DBuf<double> d_cd;                // diagonal outside the stored product
+    DBuf<float> d_xs;                 // per-vector FP16 scales of a batched product
+    DBuf<uint16_t> d_xh;              // FP16 vectors of a batched product (bit patterns)
+    DBuf<float> d_gpart;              // split-K partial sums of the MFMA GEMM
+    DBuf<double> d_x64;               // padded vectors of the FP64 MFMA GEMM
+    DBuf<double> d_gpart64;           // its split-K partial sums
     Aca aca;
 
     // solver workspace
@@ -216,8 +222,15 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
 void op_reset(Scene &s);
 void op_select(Scene &s, uint32_t kind);
 void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine);
+void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool transposed,
+                    bool refine = false);
 const std::vector<double> &op_diag_outside(const Scene &s, bool refine);
 const std::vector<double> &op_diag_total(const Scene &s);
+// batch.hip
+void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *source,
+                 double *kelvin, irtt_solve_stats &st);
+void adjoint_solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *x,
+                         const double *rhs, double *lambda, irtt_solve_stats &st);
 // aca.hip
 void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st);
 void aca_apply(Scene &s, const double *w, double *y, bool transposed);

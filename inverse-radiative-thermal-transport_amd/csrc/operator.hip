@@ -191,6 +191,289 @@ __global__ __launch_bounds__(256) void k_gemv16_t_partial(const __half *__restri
     for (int q = 0; q < 8; q++) o[q] = acc[q];
 }
 
+// ---- many right-hand sides: FP16 MFMA GEMM --------------------------------
+// Y = P X for R vectors at once with v_mfma_f32_32x32x16_f16 (FP16 inputs,
+// FP32 accumulation).  Each wave owns a 32-row (or, transposed, 32-column)
+// tile of the operator and one or two 32-vector tiles of X; every lane
+// supplies 32 consecutive k of its row for four MFMAs (the k order inside a
+// dot product is free, and A and B use the same order), so the operator is
+// read once, 64 B per lane per step.  K is split over grid.y for occupancy;
+// the partial sums are reduced in a fixed order.
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+constexpr uint32_t kMfmaStep = 64;  // k per wave iteration
+
+__device__ __forceinline__ half8_t as_half8(uint4 v)
+{
+    return __builtin_bit_cast(half8_t, v);
+}
+
+// xs[r] = max_i |rs_i x_r[i]| / 2^14 (rs = nullptr: 1)
+__global__ __launch_bounds__(256) void k_rhs_scale(const double *__restrict__ X, uint32_t n,
+                                                   uint32_t ld, const float *__restrict__ rs,
+                                                   float *__restrict__ xs)
+{
+    __shared__ double sm[256];
+    const double *x = X + (size_t)blockIdx.x * ld;
+    double m = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256)
+        m = fmax(m, fabs(x[i] * (rs ? (double)rs[i] : 1.0)));
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) xs[blockIdx.x] = (float)(sm[0] * 0x1p-14);
+}
+
+// Xh[r][i] = FP16(rs_i x_r[i] / xs[r]), zero for r >= R and i >= n
+__global__ void k_rhs_quant(const double *__restrict__ X, uint32_t n, uint32_t ld, uint32_t R,
+                            uint32_t rpad, const float *__restrict__ rs,
+                            const float *__restrict__ xs, _Float16 *__restrict__ Xh)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)rpad * ld) return;
+    uint32_t r = (uint32_t)(t / ld), i = (uint32_t)(t % ld);
+    float v = 0.0f;
+    if (r < R && i < n && xs[r] > 0.0f)
+        v = (float)(X[(size_t)r * ld + i] * (rs ? (double)rs[i] : 1.0) / (double)xs[r]);
+    Xh[t] = (_Float16)v;
+}
+
+// part[split][r][i] = sum_{k in split} H[i][k] Xh[r][k]
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm16(const __half *__restrict__ H, uint32_t nb,
+                                                uint32_t ld, const _Float16 *__restrict__ Xh,
+                                                uint32_t kchunk, float *__restrict__ part)
+{
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t m = lane & 31, hh = lane >> 5;
+    const uint32_t row0 = (blockIdx.x * 4 + wave) * 32;
+    if (row0 >= nb) return;
+    const uint32_t row = min(row0 + m, nb - 1);
+    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
+    const uint4 *arow = reinterpret_cast<const uint4 *>(H + (size_t)row * ld);
+    const uint4 *brow[NT];
+    for (int nt = 0; nt < NT; nt++)
+        brow[nt] = reinterpret_cast<const uint4 *>(Xh + (size_t)(nt * 32 + m) * ld);
+    f32x16_t acc[NT];
+    for (int nt = 0; nt < NT; nt++)
+        for (int v = 0; v < 16; v++) acc[nt][v] = 0.0f;
+    for (uint32_t k = k0; k < k1; k += kMfmaStep) {
+        const uint32_t q = (k + 32 * hh) >> 3;
+        uint4 a[4];
+        for (int t = 0; t < 4; t++) a[t] = arow[q + t];
+        for (int nt = 0; nt < NT; nt++) {
+            uint4 b[4];
+            for (int t = 0; t < 4; t++) b[t] = brow[nt][q + t];
+            for (int t = 0; t < 4; t++)
+                acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(a[t]), as_half8(b[t]),
+                                                                 acc[nt], 0, 0, 0);
+        }
+    }
+    // output tile: lane holds column m of each vector tile, rows
+    // 8 (v / 4) + 4 hh + v % 4
+    float *o = part + (size_t)blockIdx.y * NT * 32 * ld;
+    for (int nt = 0; nt < NT; nt++)
+        for (int v = 0; v < 16; v++) {
+            uint32_t i = row0 + 8 * (v >> 2) + 4 * hh + (v & 3);
+            if (i < nb) o[(size_t)(nt * 32 + m) * ld + i] = acc[nt][v];
+        }
+}
+
+// transposed: part[split][r][c] = sum_{i in split} H[i][c] Xh[r][i]; the
+// operator tile is read down its columns (32 columns x 2 B per row and
+// instruction: 64 B contiguous per half-wave)
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm16_t(const __half *__restrict__ H, uint32_t nb,
+                                                  uint32_t ld, const _Float16 *__restrict__ Xh,
+                                                  uint32_t kchunk, float *__restrict__ part)
+{
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t m = lane & 31, hh = lane >> 5;
+    const uint32_t col0 = (blockIdx.x * 4 + wave) * 32;
+    if (col0 >= ld) return;
+    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
+    const _Float16 *Hh = reinterpret_cast<const _Float16 *>(H);
+    const uint4 *brow[NT];
+    for (int nt = 0; nt < NT; nt++)
+        brow[nt] = reinterpret_cast<const uint4 *>(Xh + (size_t)(nt * 32 + m) * ld);
+    f32x16_t acc[NT];
+    for (int nt = 0; nt < NT; nt++)
+        for (int v = 0; v < 16; v++) acc[nt][v] = 0.0f;
+    for (uint32_t k = k0; k < k1; k += kMfmaStep) {
+        const uint32_t kb = k + 32 * hh;
+        half8_t a[4];
+        for (int t = 0; t < 4; t++)
+            for (int j = 0; j < 8; j++) {
+                // rows beyond nb are zero padding of Xh; clamp the read
+                uint32_t i = min(kb + 8 * t + j, nb - 1);
+                a[t][j] = Hh[(size_t)i * ld + col0 + m];
+            }
+        for (int nt = 0; nt < NT; nt++) {
+            uint4 b[4];
+            for (int t = 0; t < 4; t++) b[t] = brow[nt][(kb >> 3) + t];
+            for (int t = 0; t < 4; t++)
+                acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[t], as_half8(b[t]), acc[nt], 0,
+                                                                 0, 0);
+        }
+    }
+    float *o = part + (size_t)blockIdx.y * NT * 32 * ld;
+    for (int nt = 0; nt < NT; nt++)
+        for (int v = 0; v < 16; v++) {
+            uint32_t c = col0 + 8 * (v >> 2) + 4 * hh + (v & 3);
+            o[(size_t)(nt * 32 + m) * ld + c] = acc[nt][v];
+        }
+}
+
+// Y[r][i] = ys_i xs[r] sum_split part + cd_i X[r][i] for r < R
+__global__ void k_gemm_reduce(const float *__restrict__ part, uint32_t nsplit, uint32_t rpad,
+                              uint32_t R, uint32_t n, uint32_t ld, const float *__restrict__ ys,
+                              const float *__restrict__ xs, const double *__restrict__ cd,
+                              const double *__restrict__ X, double *__restrict__ Y)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)R * ld) return;
+    uint32_t r = (uint32_t)(t / ld), i = (uint32_t)(t % ld);
+    if (i >= n) {
+        Y[t] = 0.0;
+        return;
+    }
+    double a = 0.0;
+    for (uint32_t sp = 0; sp < nsplit; sp++) a += (double)part[((size_t)sp * rpad + r) * ld + i];
+    a *= (double)xs[r] * (ys ? (double)ys[i] : 1.0);
+    if (cd) a += cd[i] * X[t];
+    Y[t] = a;
+}
+
+// ---- many right-hand sides on the FP32 operator: FP64 MFMA GEMM -----------
+// Y = T X with v_mfma_f64_16x16x4_f64: the FP32 entries are widened exactly,
+// products and sums are FP64, so the result has the accuracy of the FP64-
+// accumulated single-vector product (the refinement residual of the FP16
+// form runs here).  A wave owns 64 rows (four 16-row tiles) and up to 32
+// vectors (two 16-vector tiles); per 16 k every lane loads one float4 of each
+// of its rows and one double4 of each vector and feeds them to four MFMAs
+// (k slot g of MFMA t holds k = 4 g + t in both operands).
+typedef double f64x4_t __attribute__((ext_vector_type(4)));
+constexpr uint32_t kM64 = 4;  // 16-row tiles per wave
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm64(const float *__restrict__ T, uint32_t nb,
+                                                uint32_t ld, const double *__restrict__ X,
+                                                uint32_t kchunk, double *__restrict__ part)
+{
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t m = lane & 15, g = lane >> 4;
+    const uint32_t row0 = (blockIdx.x * 4 + wave) * 16 * kM64;
+    if (row0 >= nb) return;
+    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
+    const float *arow[kM64];
+    for (uint32_t a = 0; a < kM64; a++)
+        arow[a] = T + (size_t)min(row0 + 16 * a + m, nb - 1) * ld + 4 * g;
+    const double *brow[NT];
+    for (int b = 0; b < NT; b++) brow[b] = X + (size_t)(16 * b + m) * ld + 4 * g;
+    f64x4_t acc[kM64][NT];
+    for (uint32_t a = 0; a < kM64; a++)
+        for (int b = 0; b < NT; b++) acc[a][b] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = k0; k < k1; k += 16) {
+        float4 av[kM64];
+        for (uint32_t a = 0; a < kM64; a++) av[a] = *reinterpret_cast<const float4 *>(arow[a] + k);
+        double4 bv[NT];
+        for (int b = 0; b < NT; b++) bv[b] = *reinterpret_cast<const double4 *>(brow[b] + k);
+        for (uint32_t a = 0; a < kM64; a++)
+            for (int b = 0; b < NT; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[a].x, bv[b].x,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[a].y, bv[b].y,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[a].z, bv[b].z,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[a].w, bv[b].w,
+                                                                 acc[a][b], 0, 0, 0);
+            }
+    }
+    // output tile: lane holds vector m of each tile, rows 4 v + g
+    double *o = part + (size_t)blockIdx.y * NT * 16 * ld;
+    for (uint32_t a = 0; a < kM64; a++)
+        for (int b = 0; b < NT; b++)
+            for (int v = 0; v < 4; v++) {
+                uint32_t i = row0 + 16 * a + 4 * v + g;
+                if (i < nb) o[(size_t)(16 * b + m) * ld + i] = acc[a][b][v];
+            }
+}
+
+// transposed: part[split][r][c] = sum_i T[i][c] X[r][i]; the wave owns 64
+// columns, each lane reads its column down four rows per 16 k
+template <int NT>
+__global__ __launch_bounds__(256) void k_gemm64_t(const float *__restrict__ T, uint32_t nb,
+                                                  uint32_t ld, const double *__restrict__ X,
+                                                  uint32_t kchunk, double *__restrict__ part)
+{
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t m = lane & 15, g = lane >> 4;
+    const uint32_t col0 = (blockIdx.x * 4 + wave) * 16 * kM64;
+    if (col0 >= ld) return;
+    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
+    const double *brow[NT];
+    for (int b = 0; b < NT; b++) brow[b] = X + (size_t)(16 * b + m) * ld + 4 * g;
+    f64x4_t acc[kM64][NT];
+    for (uint32_t a = 0; a < kM64; a++)
+        for (int b = 0; b < NT; b++) acc[a][b] = f64x4_t{0.0, 0.0, 0.0, 0.0};
+    for (uint32_t k = k0; k < k1; k += 16) {
+        double4 bv[NT];
+        for (int b = 0; b < NT; b++) bv[b] = *reinterpret_cast<const double4 *>(brow[b] + k);
+        for (uint32_t a = 0; a < kM64; a++) {
+            const uint32_t c = col0 + 16 * a + m;
+            float av[4];
+            for (int t = 0; t < 4; t++) {
+                // rows beyond nb meet zero padding in X; clamp the read
+                const uint32_t i = min(k + 4 * g + t, nb - 1);
+                av[t] = T[(size_t)i * ld + c];
+            }
+            for (int b = 0; b < NT; b++) {
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[0], bv[b].x,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[1], bv[b].y,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[2], bv[b].z,
+                                                                 acc[a][b], 0, 0, 0);
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[3], bv[b].w,
+                                                                 acc[a][b], 0, 0, 0);
+            }
+        }
+    }
+    double *o = part + (size_t)blockIdx.y * NT * 16 * ld;
+    for (uint32_t a = 0; a < kM64; a++)
+        for (int b = 0; b < NT; b++)
+            for (int v = 0; v < 4; v++) {
+                uint32_t c = col0 + 16 * a + 4 * v + g;
+                o[(size_t)(16 * b + m) * ld + c] = acc[a][b][v];
+            }
+}
+
+// Y[r][i] = sum_split part (fixed order), zero past n
+__global__ void k_gemm64_reduce(const double *__restrict__ part, uint32_t nsplit, uint32_t rpad,
+                                uint32_t R, uint32_t n, uint32_t ld, double *__restrict__ Y)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)R * ld) return;
+    uint32_t r = (uint32_t)(t / ld), i = (uint32_t)(t % ld);
+    double a = 0.0;
+    if (i < n)
+        for (uint32_t sp = 0; sp < nsplit; sp++) a += part[((size_t)sp * rpad + r) * ld + i];
+    Y[t] = a;
+}
+
+// y += cd * x
+__global__ void k_add_diag(uint32_t n, const double *__restrict__ cd, const double *__restrict__ x,
+                           double *__restrict__ y)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += cd[i] * x[i];
+}
+
 }  // namespace
 
 // back to the dense FP32 form; the other forms are released (a new
@@ -277,6 +560,96 @@ void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine
         k_gemv16_t_partial<<<dim3(cdiv(ld / 8, 256), nchunk), 256, 0, st>>>(H, s.d_h16s.p, nb, ld,
                                                                             w, s.d_tpart.p);
         k_chunk_reduce<<<cdiv(ld, 256), 256, 0, st>>>(s.d_tpart.p, nchunk, ld, y);
+    }
+}
+
+// Y = Op X (or Op^T X) for R vectors: X, Y are R x ld device arrays (FP64,
+// padded with zeros), Op the full operator of the Krylov side (P plus the
+// diagonal kept outside it).  The FP16 form runs one MFMA GEMM per 64
+// vectors; the other forms one product per vector.
+void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool transposed,
+                    bool refine)
+{
+    const uint32_t nb = s.nb, ld = s.ld;
+    hipStream_t st = s.stream;
+    if (s.op_kind == IRTT_OP_DENSE || (refine && s.op_kind == IRTT_OP_DENSE_F16)) {
+        // the FP32 operator (diagonal inside) with FP64 MFMA, 32 vectors per pass
+        const uint32_t tiles = transposed ? ld / (16 * kM64) : cdiv(nb, 16 * kM64);
+        const uint32_t ksteps = ld / 16;
+        uint32_t ksplit = std::min<uint32_t>(std::max<uint32_t>(1, cdiv(4096, tiles)), ksteps);
+        const uint32_t kchunk = cdiv(ksteps, ksplit) * 16;
+        ksplit = cdiv(ld, kchunk);
+        s.d_x64.alloc((size_t)32 * ld);
+        s.d_gpart64.alloc((size_t)ksplit * 32 * ld);
+        for (uint32_t r0 = 0; r0 < R; r0 += 32) {
+            const uint32_t rc = std::min<uint32_t>(32, R - r0), nt = cdiv(rc, 16), rpad = 16 * nt;
+            IRTT_HIP(hipMemcpyAsync(s.d_x64.p, X + (size_t)r0 * ld, (size_t)rc * ld * 8,
+                                    hipMemcpyDeviceToDevice, st));
+            if (rpad > rc)
+                IRTT_HIP(hipMemsetAsync(s.d_x64.p + (size_t)rc * ld, 0, (size_t)(rpad - rc) * ld * 8,
+                                        st));
+            dim3 grid(cdiv(tiles, 4), ksplit);
+            if (!transposed) {
+                if (nt == 1) k_gemm64<1><<<grid, 256, 0, st>>>(s.d_T.p, nb, ld, s.d_x64.p, kchunk,
+                                                               s.d_gpart64.p);
+                else k_gemm64<2><<<grid, 256, 0, st>>>(s.d_T.p, nb, ld, s.d_x64.p, kchunk,
+                                                        s.d_gpart64.p);
+            } else {
+                if (nt == 1) k_gemm64_t<1><<<grid, 256, 0, st>>>(s.d_T.p, nb, ld, s.d_x64.p, kchunk,
+                                                                 s.d_gpart64.p);
+                else k_gemm64_t<2><<<grid, 256, 0, st>>>(s.d_T.p, nb, ld, s.d_x64.p, kchunk,
+                                                          s.d_gpart64.p);
+            }
+            k_gemm64_reduce<<<cdiv((size_t)rc * ld, 256), 256, 0, st>>>(
+                s.d_gpart64.p, ksplit, rpad, rc, nb, ld, Y + (size_t)r0 * ld);
+        }
+        return;
+    }
+    const std::vector<double> &cdv = op_diag_outside(s, false);
+    const double *cd = nullptr;
+    if (!cdv.empty()) {
+        s.d_cd.alloc(ld);
+        IRTT_HIP(hipMemsetAsync(s.d_cd.p, 0, s.d_cd.bytes(), st));
+        IRTT_HIP(hipMemcpyAsync(s.d_cd.p, cdv.data(), nb * sizeof(double), hipMemcpyHostToDevice,
+                                st));
+        cd = s.d_cd.p;
+    }
+    if (s.op_kind != IRTT_OP_DENSE_F16) {
+        for (uint32_t r = 0; r < R; r++) {
+            op_apply(s, X + (size_t)r * ld, Y + (size_t)r * ld, transposed, false);
+            if (cd) k_add_diag<<<cdiv(nb, 256), 256, 0, st>>>(nb, cd, X + (size_t)r * ld,
+                                                              Y + (size_t)r * ld);
+        }
+        return;
+    }
+    const __half *H = reinterpret_cast<const __half *>(s.d_h16.p);
+    const uint32_t tiles = transposed ? ld / 32 : cdiv(nb, 32);  // 32-wide wave tiles
+    const uint32_t ksteps = ld / kMfmaStep;
+    uint32_t ksplit = std::min<uint32_t>(std::max<uint32_t>(1, cdiv(4096, tiles)), ksteps);
+    const uint32_t kchunk = cdiv(ksteps, ksplit) * kMfmaStep;
+    ksplit = cdiv(ld, kchunk);
+    s.d_xs.alloc(64);
+    s.d_xh.alloc((size_t)64 * ld);
+    s.d_gpart.alloc((size_t)ksplit * 64 * ld);
+    _Float16 *Xh = reinterpret_cast<_Float16 *>(s.d_xh.p);
+    const float *rs = s.d_h16s.p;
+    for (uint32_t r0 = 0; r0 < R; r0 += 64) {
+        const uint32_t rc = std::min<uint32_t>(64, R - r0), nt = cdiv(rc, 32), rpad = 32 * nt;
+        const double *Xc = X + (size_t)r0 * ld;
+        k_rhs_scale<<<rc, 256, 0, st>>>(Xc, nb, ld, transposed ? rs : nullptr, s.d_xs.p);
+        k_rhs_quant<<<cdiv((size_t)rpad * ld, 256), 256, 0, st>>>(
+            Xc, nb, ld, rc, rpad, transposed ? rs : nullptr, s.d_xs.p, Xh);
+        dim3 grid(cdiv(tiles, 4), ksplit);
+        if (!transposed) {
+            if (nt == 1) k_gemm16<1><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
+            else k_gemm16<2><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
+        } else {
+            if (nt == 1) k_gemm16_t<1><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
+            else k_gemm16_t<2><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
+        }
+        k_gemm_reduce<<<cdiv((size_t)rc * ld, 256), 256, 0, st>>>(
+            s.d_gpart.p, ksplit, rpad, rc, nb, ld, transposed ? nullptr : rs, s.d_xs.p, cd, Xc,
+            Y + (size_t)r0 * ld);
     }
 }
 

@@ -153,6 +153,11 @@ def _declare(L):
     L.irtt_serve_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]
     L.irtt_serve_destroy.argtypes = [_vp]
     L.irtt_select_operator.argtypes = [_vp, ctypes.c_uint32]
+    L.irtt_apply_operator.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32]
+    L.irtt_solve_batch.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32, _vp, _vp,
+                                   ctypes.POINTER(SolveStats)]
+    L.irtt_adjoint_solve_batch.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32,
+                                           _vp, _vp, _vp, ctypes.POINTER(SolveStats)]
     L.irtt_compress_transport.argtypes = [_vp, ctypes.POINTER(AcaParams), ctypes.POINTER(AcaStats)]
     L.irtt_get_aca.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
 
@@ -409,6 +414,42 @@ class Scene:
     def select_operator(self, op):
         """OP_DENSE, OP_DENSE_F16 or OP_ACA for the solves that follow."""
         _check(lib().irtt_select_operator(self._h, op))
+
+    def solve_batch(self, sources=None, n_rhs=None, params=None, **kw):
+        """Steady solves for the rows of sources (n_rhs, n_bins) in lockstep
+        (irtt_solve_batch); returns (kelvin (n_rhs, n_bins), stats)."""
+        p = params if params is not None else solver_params(**kw)
+        if sources is not None:
+            src = np.ascontiguousarray(np.atleast_2d(sources), np.float64)
+            R = src.shape[0]
+        else:
+            src, R = None, n_rhs
+        X = np.empty((R, self.n_bins))
+        st = SolveStats()
+        _check(lib().irtt_solve_batch(self._h, ctypes.byref(p), R, _ptr(src), _ptr(X),
+                                      ctypes.byref(st)))
+        return X, st.as_dict()
+
+    def adjoint_solve_batch(self, x, rhs, params=None, **kw):
+        """lambda_c with J(x_c)^T lambda_c = rhs_c for the rows of x and rhs."""
+        p = params if params is not None else solver_params(**kw)
+        x = np.ascontiguousarray(np.atleast_2d(x), np.float64)
+        r = np.ascontiguousarray(np.atleast_2d(rhs), np.float64)
+        lam = np.empty_like(r)
+        st = SolveStats()
+        _check(lib().irtt_adjoint_solve_batch(self._h, ctypes.byref(p), r.shape[0], _ptr(x),
+                                              _ptr(r), _ptr(lam), ctypes.byref(st)))
+        return lam, st.as_dict()
+
+    def apply_operator(self, X, transposed=False):
+        """T X (or T^T X) for the rows of X (n_rhs, n_bins) with the selected
+        operator form (irtt_apply_operator)."""
+        X = np.ascontiguousarray(np.atleast_2d(X), np.float64)
+        if X.shape[1] != self.n_bins:
+            raise ValueError("X must be (n_rhs, n_bins)")
+        Y = np.empty_like(X)
+        _check(lib().irtt_apply_operator(self._h, X.shape[0], _ptr(X), _ptr(Y), int(transposed)))
+        return Y
 
     def compress(self, algorithm=ACA_PARTIAL, rank=50, column_sum_correction=False, seed=1,
                  precision=STORE_F32):

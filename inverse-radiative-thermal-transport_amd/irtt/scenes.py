@@ -234,6 +234,28 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
     return sd
 
 
+def sun_flux(sd, elevation_deg, azimuth_deg, flux=800.0):
+    """Absorbed direct-sun flux per triangle (W/m^2) for a city block:
+    alpha * flux * max(0, n.s), unshadowed."""
+    el, az = np.radians(elevation_deg), np.radians(azimuth_deg)
+    sun = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    cosi = np.clip(sd.normals() @ sun, 0.0, None)
+    return (sd.meta["alpha"] * flux * cosi).astype(np.float32)
+
+
+def bin_sources(sd, heat_flux, bin_mode=0):
+    """External absorbed power per bin (W) for a per-triangle flux, assigned
+    as the library assembles it: flux * area to a triangle bin, a third of it
+    to each corner's vertex bin."""
+    w = np.asarray(heat_flux, np.float32).astype(np.float64) * sd.triangle_areas()
+    if bin_mode == 1:
+        return w
+    out = np.zeros(sd.nv)
+    for c in range(3):
+        np.add.at(out, sd.idx[:, c], w / 3.0)
+    return out
+
+
 def height_parametrization(sd, n_params=None):
     """Building heights as design parameters of a city block: raising
     building b by dh moves each of its vertices by z / h_b * dh along z (walls
