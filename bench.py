@@ -129,6 +129,51 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
     return out
 
 
+def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=50000, seed=7):
+    """Poisson stream of design variants through irtt_serve_*: city blocks of
+    log-uniform size (tmin..tmax triangles) with random heights and seeds,
+    each a forward steady solve plus adjoint over up to 64 building-height
+    parameters.  Returns latency percentiles and throughput."""
+    import numpy as np
+    from irtt import scenes
+    rng = np.random.default_rng(seed)
+    designs = []
+    for k in range(n_req):
+        t = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
+        g = int(np.clip(round(np.sqrt(t / 800.0)), 2, 8))
+        sd = scenes.city_block(seed=1000 + k, nx=g, ny=g, target_tris=int(t))
+        designs.append((sd, scenes.height_parametrization(sd, 64)))
+    tp = irtt.transport_params(threads_per_tri=32, rays_per_thread=8, batch_seed=1234)
+    sp = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-8)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    with irtt.Server(slots=slots) as sv:
+        # one warm request so the first arrival does not pay for code loading
+        sd, par = designs[0]
+        sv.wait(sv.submit(sd, transport=tp, solver=sp, kelvin0=sd.kelvin, fixed=sd.fixed,
+                          target=300.0, objective_weight=1e-3, params=par))
+        tickets = []
+        t0 = time.perf_counter()
+        for (sd, par), at in zip(designs, arrivals):
+            dt = at - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            tickets.append(sv.submit(sd, transport=tp, solver=sp, kelvin0=sd.kelvin,
+                                     fixed=sd.fixed, target=300.0, objective_weight=1e-3,
+                                     params=par))
+        stats = [sv.wait(tk)[0] for tk in tickets]
+        wall = time.perf_counter() - t0
+    lat = np.array([st["latency_ms"] for st in stats])
+    return {"requests": n_req, "arrival_rate_per_s": rate, "slots": slots,
+            "triangles": [int(d[0].nt) for d in designs],
+            "p50_ms": round(float(np.percentile(lat, 50)), 1),
+            "p99_ms": round(float(np.percentile(lat, 99)), 1),
+            "mean_ms": round(float(lat.mean()), 1),
+            "mean_queue_ms": round(float(np.mean([st["queued_ms"] for st in stats])), 1),
+            "designs_per_s": round(n_req / wall, 2),
+            "work": "per request: LBVH, 256 photons per triangle, steady solve, adjoint solve, "
+                    "adjoint trace, dL/dp over up to 64 building heights"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +183,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--threads-per-tri", type=int, default=162)
     ap.add_argument("--rays-per-thread", type=int, default=8)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -210,6 +258,8 @@ def main():
                          "linear_iters": sol["linear_iters"], "converged": sol["converged"]},
         "solves": solves,
     }
+    if not args.no_serving:
+        line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread)
     s.close()

@@ -241,6 +241,13 @@ int irtt_set_temperatures(irtt_scene h, const double *kelvin); /* current state 
 int irtt_residual(irtt_scene h, const irtt_solver_params *p, const double *x,
                   double *r);
 
+/* Forward driver (ThermalRenderer::runForwardSim, thermal_renderer.cpp:958-1026):
+ * `steps` implicit-Euler steps of p->dt from the current state (one solve when
+ * p->steady), each step's temperatures stored in history (steps x n_bins).
+ * The last step becomes the current state. */
+int irtt_simulate(irtt_scene h, const irtt_solver_params *p, uint32_t steps, double *history,
+                  irtt_solve_stats *st);
+
 /* n_rhs steady solves that share the operator, conduction, fixed bins and
  * ambient, each with its own external absorbed power (source: n_rhs x n_bins,
  * W per bin; NULL = the scene's), in lockstep: every Krylov product is one
@@ -305,6 +312,21 @@ int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nn
                           const uint32_t *vertex, const uint32_t *param, const float *dxdp,
                           uint32_t n_params, double *dp);
 
+/* Adjoint over a forward history (ThermalRenderer::runAdjointSim,
+ * thermal_renderer.cpp:1028-1133, objective of thermal_objective.cpp:47-57):
+ *   L = s * sum_t 0.5 sum_i w_i (x_t,i - target_t,i)^2,  s = 1/steps (transient) or 1,
+ * backwards in time  lambda_t = J(x_t)^-T (dt g_t + M lambda_{t+1})  (steady: J^-T g),
+ * g_t = w (x_t - target_t), fixed bins zero; lambda_history[t] = s lambda_t.
+ * weights: n_bins (objective weight times bin area, as the reference forms it);
+ * targets: steps x n_bins, or n_bins shared by every step (target_stride 0).
+ * If grad_x is not NULL the adjoint trace runs too (ap NULL: reference
+ * defaults) and grad_x (3 n_vertices) receives dL/dvertex (history rows
+ * (lambda_t, x_t^4) and, with an ambient, (-Ta^4, sum_t lambda_t)). */
+int irtt_adjoint_history(irtt_scene h, const irtt_solver_params *p, uint32_t steps,
+                         const double *history, const double *weights, const double *targets,
+                         uint32_t target_stride, double *lambda_history, double *objective,
+                         const irtt_adjoint_params *ap, double *grad_x, irtt_solve_stats *st);
+
 /* ---------------------------------------------------------------- serving */
 /* Request loop over design variants (no counterpart inside the reference,
  * whose ThermalRenderer runs one scene at a time, thermal_renderer.cpp:958-1133).
@@ -335,7 +357,7 @@ typedef struct irtt_request {
     const uint8_t *fixed;       /* same indexing; NULL = none fixed */
     irtt_transport_params transport;
     irtt_solver_params solver;
-    /* objective L = w * sum_i A_i (x_i - target_i)^2 (thermal_objective.cpp:34,47-57);
+    /* objective L = 0.5 w sum_i A_i (x_i - target_i)^2 (thermal_objective.cpp:34,47-57);
        target NULL: forward solve only */
     const double *target;       /* n_bins */
     double objective_weight;

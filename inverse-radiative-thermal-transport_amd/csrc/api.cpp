@@ -462,6 +462,36 @@ int irtt_apply_operator(irtt_scene h, uint32_t n_rhs, const double *X, double *Y
     });
 }
 
+int irtt_simulate(irtt_scene h, const irtt_solver_params *p, uint32_t steps, double *history,
+                  irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !history) throw Error(IRTT_ERR_ARG, "null pointer");
+        irtt_solve_stats tmp{};
+        simulate_history(*s, *p, steps, history, tmp);
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_adjoint_history(irtt_scene h, const irtt_solver_params *p, uint32_t steps,
+                         const double *history, const double *weights, const double *targets,
+                         uint32_t target_stride, double *lambda_history, double *objective,
+                         const irtt_adjoint_params *ap, double *grad_x, irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !history || !weights || !targets || !lambda_history)
+            throw Error(IRTT_ERR_ARG, "null pointer");
+        if (target_stride != 0 && target_stride != s->nb)
+            throw Error(IRTT_ERR_ARG, "target_stride must be 0 or n_bins");
+        irtt_solve_stats tmp{};
+        adjoint_history(*s, *p, steps, history, weights, targets, target_stride, lambda_history,
+                        objective, ap, grad_x, tmp);
+        if (st) *st = tmp;
+    });
+}
+
 int irtt_solve_batch(irtt_scene h, const irtt_solver_params *p, uint32_t n_rhs,
                      const double *source, double *kelvin, irtt_solve_stats *st)
 {

@@ -226,6 +226,13 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
                     bool refine = false);
 const std::vector<double> &op_diag_outside(const Scene &s, bool refine);
 const std::vector<double> &op_diag_total(const Scene &s);
+// history.cpp
+void simulate_history(Scene &s, const irtt_solver_params &p, uint32_t steps, double *history,
+                      irtt_solve_stats &st);
+void adjoint_history(Scene &s, const irtt_solver_params &p, uint32_t steps, const double *hist,
+                     const double *w, const double *targets, uint32_t target_stride,
+                     double *lam_hist, double *objective, const irtt_adjoint_params *ap,
+                     double *grad_x, irtt_solve_stats &st);
 // batch.hip
 void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *source,
                  double *kelvin, irtt_solve_stats &st);

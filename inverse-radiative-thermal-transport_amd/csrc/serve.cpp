@@ -154,8 +154,8 @@ int Server::run(Req &q)
     double L = 0.0;
     for (uint32_t i = 0; i < nb; i++) {
         double d = x[i] - q.target[i];
-        L += r.objective_weight * area[i] * d * d;
-        dLdx[i] = 2.0 * r.objective_weight * area[i] * d;
+        L += 0.5 * r.objective_weight * area[i] * d * d;
+        dLdx[i] = r.objective_weight * area[i] * d;
     }
     if (r.objective) *r.objective = L;
     std::vector<double> lam(nb);

@@ -154,6 +154,13 @@ def _declare(L):
     L.irtt_serve_destroy.argtypes = [_vp]
     L.irtt_select_operator.argtypes = [_vp, ctypes.c_uint32]
     L.irtt_apply_operator.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32]
+    L.irtt_simulate.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32, _vp,
+                                ctypes.POINTER(SolveStats)]
+    L.irtt_adjoint_history.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32, _vp,
+                                       _vp, _vp, ctypes.c_uint32, _vp,
+                                       ctypes.POINTER(ctypes.c_double),
+                                       ctypes.POINTER(AdjointParams), _vp,
+                                       ctypes.POINTER(SolveStats)]
     L.irtt_solve_batch.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32, _vp, _vp,
                                    ctypes.POINTER(SolveStats)]
     L.irtt_adjoint_solve_batch.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32,
@@ -414,6 +421,34 @@ class Scene:
     def select_operator(self, op):
         """OP_DENSE, OP_DENSE_F16 or OP_ACA for the solves that follow."""
         _check(lib().irtt_select_operator(self._h, op))
+
+    def simulate(self, steps, params=None, **kw):
+        """steps implicit-Euler steps (one solve when steady); returns the
+        history (steps, n_bins) and stats (irtt_simulate)."""
+        p = params if params is not None else solver_params(**kw)
+        H = np.empty((steps, self.n_bins))
+        st = SolveStats()
+        _check(lib().irtt_simulate(self._h, ctypes.byref(p), steps, _ptr(H), ctypes.byref(st)))
+        return H, st.as_dict()
+
+    def adjoint_history(self, history, weights, targets, params=None, gradient=False,
+                        edge_ratio=0.05, tiny_eps=1e-2, flags=0, **kw):
+        """Backward adjoint over a history (irtt_adjoint_history).  Returns
+        (lambda_history, objective, grad_x or None, stats)."""
+        p = params if params is not None else solver_params(**kw)
+        Hh = np.ascontiguousarray(np.atleast_2d(history), np.float64)
+        w = np.ascontiguousarray(weights, np.float64)
+        tg = np.ascontiguousarray(targets, np.float64)
+        stride = self.n_bins if tg.ndim == 2 else 0
+        lam = np.empty_like(Hh)
+        obj = ctypes.c_double()
+        g = np.empty(3 * self.info["n_vertices"]) if gradient else None
+        ap = AdjointParams(edge_ratio, tiny_eps, flags, 0)
+        st = SolveStats()
+        _check(lib().irtt_adjoint_history(self._h, ctypes.byref(p), Hh.shape[0], _ptr(Hh), _ptr(w),
+                                          _ptr(tg), stride, _ptr(lam), ctypes.byref(obj),
+                                          ctypes.byref(ap), _ptr(g), ctypes.byref(st)))
+        return lam, obj.value, (g.reshape(-1, 3) if gradient else None), st.as_dict()
 
     def solve_batch(self, sources=None, n_rhs=None, params=None, **kw):
         """Steady solves for the rows of sources (n_rhs, n_bins) in lockstep

@@ -75,14 +75,14 @@ def _uniform_scene(name, xyz, idx, rho_d=0.0, rho_s=0.0, flags=MAT_DIFFUSE_EMISS
     nt = idx.shape[0]
     nv = xyz.shape[0]
 
-    def f(v, dt=np.float32):
-        return np.ascontiguousarray(np.broadcast_to(np.asarray(v, dt), (nt,)))
+    def f(v, dt=np.float32, n=nt):
+        # always a writable copy (a broadcast view of a full-size array is read-only)
+        return np.array(np.broadcast_to(np.asarray(v, dt), (n,)), dtype=dt)
 
     return SceneData(name, np.ascontiguousarray(xyz, np.float32),
                      np.ascontiguousarray(idx, np.uint32), f(rho_d), f(rho_s), f(flags, np.uint8),
                      f(thickness), f(density), f(heat_capacity), f(conductivity), f(heat_flux),
-                     np.ascontiguousarray(np.broadcast_to(np.float32(kelvin), (nv,))),
-                     np.zeros(nv, np.uint8))
+                     f(kelvin, np.float32, nv), np.zeros(nv, np.uint8))
 
 
 def grid_patch(origin, u, v, nu, nv_):
