@@ -356,13 +356,20 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     int dev = 0, ncu = 256;
     IRTT_HIP(hipGetDevice(&dev));
     IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int S = stack_size_for(s.bvh.max_depth);
+    // The photon kernel's occupancy is bound by its LDS stack: 24 entries
+    // (6 KiB per wave) admit 24 waves per CU against 20 for 32 entries, for
+    // trees up to 23 levels (the 51.8k-triangle city block has 21).  Measured
+    // on that block: 17.2 against 18.9 ms.  (A 16-entry LDS stack spilling
+    // deeper entries to global memory, with 64 VGPRs for 32 waves, measured
+    // 17.4 ms: the register spills cost what the occupancy gains.)
+    const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
     // enough waves to cover every CU several times; lanes loop over photons
     uint64_t want = cdiv(total, kTraceBlock);
     uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * 64);
     if (grid == 0) grid = 1;
     IRTT_HIP(hipEventRecord(s.ev0, stm));
-    if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
+    if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
     else if (S == 64) k_photons<64><<<grid, kTraceBlock, 0, stm>>>(a);
     else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
     IRTT_HIP(hipGetLastError());

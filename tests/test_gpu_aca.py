@@ -165,3 +165,45 @@ def test_fp16_operator_refines_to_the_fp32_solution(irtt):
           f"FP16 {st16}; adjoint refinement {ast}")
     assert st16["converged"] and err < 1e-6 and lerr < 1e-8
     np.testing.assert_array_equal(x32, x32b)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_aca_operator_products(irtt, precision):
+    """Products with the stored factors (FP32, or FP16 with a row scale):
+    within 2^-22 (FP32) / 2^-9 (FP16) of sum_l |U_il||V_lj||x_j| + |C_i x_i|
+    from the FP64 factors, forward and transposed."""
+    sd, s = _scene(irtt, "city")
+    rng = np.random.default_rng(precision)
+    with s:
+        s.compress(rank=30, precision=precision)
+        Ut, V, C, _, _ = s.aca_factors()
+        n = s.n_bins
+        X = rng.normal(size=(3, n))
+        Y = s.apply_operator(X)
+        Yt = s.apply_operator(X, transposed=True)
+    P = aca.lowrank(Ut, V) + np.diag(C)
+    A = aca.lowrank(np.abs(Ut), np.abs(V)) + np.diag(np.abs(C))
+    tol = 2.0 ** -22 if precision == 0 else 2.0 ** -9
+    for got, want, bound in ((Y, X @ P.T, np.abs(X) @ A.T), (Yt, X @ P, np.abs(X) @ A)):
+        ratio = (np.abs(got - want) / bound).max()
+        print(f"precision={precision}: max |err| / bound = {ratio:.2e}")
+        assert ratio < tol
+
+
+def test_aca_fp16_factors_solve(irtt):
+    """FP16 factors change the temperatures of the ACA solve by far less than
+    the 0.05 K bar."""
+    sd, s = _scene(irtt, "plates", mode=1, threads=256)
+    p = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-16)
+    with s:
+        s.compress(rank=40, precision=0)
+        s.set_temperatures(np.zeros(s.n_bins))
+        s.solve(p)
+        x32 = s.temperatures()
+        s.compress(rank=40, precision=1)
+        s.set_temperatures(np.zeros(s.n_bins))
+        st = s.solve(p)
+        x16 = s.temperatures()
+    err = np.abs(x16 - x32).max()
+    print(f"ACA FP16 factors vs FP32 factors: max |dT| {err:.2e} K; {st}")
+    assert st["converged"] and err < 0.05

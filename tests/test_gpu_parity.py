@@ -19,6 +19,7 @@ def _scenes():
         "box": lambda: scenes.closed_box(3, emissivity=0.8),
         "city_small": lambda: scenes.city_block(seed=2, nx=2, ny=2, target_tris=3000),
         "city_full": lambda: scenes.city_block(seed=0),
+        "city_deep": lambda: scenes.city_block(seed=2, nx=4, ny=4, target_tris=8000),
     }
 
 
@@ -68,6 +69,7 @@ CASES = [
     ("plates", 1, 64, 8),
     ("box", 1, 128, 4),
     ("city_small", 0, 16, 4),
+    ("city_deep", 0, 8, 4),   # 18-level tree, deeper than the LDS part of the photon stack
 ]
 
 
