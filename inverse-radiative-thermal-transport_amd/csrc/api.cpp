@@ -220,7 +220,8 @@ uint64_t Scene::device_bytes() const
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
            d_traceable.bytes() + bvh.nodes.bytes() + bvh.tris.bytes() + bvh.morton.bytes() +
            d_q.bytes() + d_acc.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
-           d_tdiag.bytes() + d_h16.bytes() + d_h16s.bytes() + aca.d_ut.bytes() +
+           d_tdiag.bytes() + d_h16.bytes() + d_h16s.bytes() + d_h16f.bytes() + d_h16ft.bytes() +
+           d_gpart.bytes() + d_gpart64.bytes() + aca.d_ut.bytes() +
            aca.d_v.bytes() + aca.d_uf.bytes() + aca.d_vf.bytes() + aca.d_uh.bytes() +
            aca.d_vh.bytes() + d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
            d_lap_col.bytes() + d_lap_val.bytes();

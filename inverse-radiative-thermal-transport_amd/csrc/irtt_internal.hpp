@@ -185,6 +185,7 @@ struct Scene {
     bool f16_valid = false;
     DBuf<uint16_t> d_h16;             // nb*ld FP16 off-diagonal operator (bit patterns)
     DBuf<float> d_h16s;               // its per-row scales
+    DBuf<uint16_t> d_h16f, d_h16ft;   // the same (and its transpose) in MFMA fragment order
     DBuf<float> d_u32;                // FP32 copy of a product's input vector
     DBuf<double> d_cd;                // diagonal outside the stored product
     DBuf<float> d_xs;                 // per-vector FP16 scales of a batched product

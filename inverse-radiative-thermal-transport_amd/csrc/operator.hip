@@ -228,46 +228,82 @@ __global__ __launch_bounds__(256) void k_rhs_scale(const double *__restrict__ X,
     if (threadIdx.x == 0) xs[blockIdx.x] = (float)(sm[0] * 0x1p-14);
 }
 
-// Xh[r][i] = FP16(rs_i x_r[i] / xs[r]), zero for r >= R and i >= n
+// Xh = FP16(rs_i x_r[i] / xs[r]), zero for r >= R and i >= n, stored in MFMA
+// fragment order like the operator: vector tile nt (32 vectors) x k-block kb
+// = [t][lane (hh, m)][8], lane holding vector 32 nt + m, k = 64 kb + 32 hh + 8 t
 __global__ void k_rhs_quant(const double *__restrict__ X, uint32_t n, uint32_t ld, uint32_t R,
                             uint32_t rpad, const float *__restrict__ rs,
                             const float *__restrict__ xs, _Float16 *__restrict__ Xh)
 {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)rpad * ld) return;
-    uint32_t r = (uint32_t)(t / ld), i = (uint32_t)(t % ld);
+    size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)rpad * ld) return;
+    uint32_t r = (uint32_t)(e / ld), i = (uint32_t)(e % ld);
     float v = 0.0f;
     if (r < R && i < n && xs[r] > 0.0f)
         v = (float)(X[(size_t)r * ld + i] * (rs ? (double)rs[i] : 1.0) / (double)xs[r]);
-    Xh[t] = (_Float16)v;
+    const uint32_t KB = ld / kMfmaStep, nt = r >> 5, m = r & 31, kb = i / kMfmaStep;
+    const uint32_t w = i % kMfmaStep, hh = w >> 5, t = (w & 31) >> 3, j = w & 7;
+    Xh[(((size_t)nt * KB + kb) * 256 + 64 * t + 32 * hh + m) * 8 + j] = (_Float16)v;
 }
 
-// part[split][r][i] = sum_{k in split} H[i][k] Xh[r][k]
+// The FP16 operator in MFMA fragment order: tile (32 rows) x k-block (64 k)
+// = 2048 halves stored as [t 0..3][lane 0..63][8], lane (hh, m) = 32 hh + m
+// holding row 32 tile + m, k = 64 kb + 32 hh + 8 t + 0..7.  One load
+// instruction of a wave then reads 1 KiB contiguously.  The transposed copy
+// holds the same for H^T (rows = operator columns, k = operator rows).
+__global__ void k_frag_pack(const __half *__restrict__ H, uint32_t nb, uint32_t ld,
+                            uint32_t tiles, int transposed, uint4 *__restrict__ F)
+{
+    const uint32_t KB = ld / kMfmaStep;
+    const size_t u = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= (size_t)tiles * KB * 256) return;
+    const uint32_t lane = (uint32_t)(u & 63), t = (uint32_t)((u >> 6) & 3);
+    const size_t blk = u >> 8;
+    const uint32_t kb = (uint32_t)(blk % KB), tile = (uint32_t)(blk / KB);
+    const uint32_t m = lane & 31, hh = lane >> 5;
+    const uint32_t k0 = kMfmaStep * kb + 32 * hh + 8 * t;
+    const uint16_t *Hb = reinterpret_cast<const uint16_t *>(H);
+    uint16_t v[8];
+    if (!transposed) {
+        const uint32_t row = 32 * tile + m;
+        for (int j = 0; j < 8; j++) v[j] = row < nb ? Hb[(size_t)row * ld + k0 + j] : 0;
+    } else {
+        const uint32_t col = 32 * tile + m;
+        for (int j = 0; j < 8; j++) v[j] = k0 + j < nb ? Hb[(size_t)(k0 + j) * ld + col] : 0;
+    }
+    uint4 o;
+    o.x = v[0] | ((uint32_t)v[1] << 16);
+    o.y = v[2] | ((uint32_t)v[3] << 16);
+    o.z = v[4] | ((uint32_t)v[5] << 16);
+    o.w = v[6] | ((uint32_t)v[7] << 16);
+    F[u] = o;
+}
+
+// part[split][r][i] = sum_{k in split} A[i][k] Xh[r][k] with A in fragment
+// order (the operator, or its transpose); rows i < nrows are written
 template <int NT>
-__global__ __launch_bounds__(256) void k_gemm16(const __half *__restrict__ H, uint32_t nb,
-                                                uint32_t ld, const _Float16 *__restrict__ Xh,
-                                                uint32_t kchunk, float *__restrict__ part)
+__global__ __launch_bounds__(256) void k_gemm16(const uint4 *__restrict__ F, uint32_t tiles,
+                                                uint32_t nrows, uint32_t ld,
+                                                const _Float16 *__restrict__ Xh, uint32_t kbchunk,
+                                                float *__restrict__ part)
 {
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const uint32_t m = lane & 31, hh = lane >> 5;
-    const uint32_t row0 = (blockIdx.x * 4 + wave) * 32;
-    if (row0 >= nb) return;
-    const uint32_t row = min(row0 + m, nb - 1);
-    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
-    const uint4 *arow = reinterpret_cast<const uint4 *>(H + (size_t)row * ld);
-    const uint4 *brow[NT];
-    for (int nt = 0; nt < NT; nt++)
-        brow[nt] = reinterpret_cast<const uint4 *>(Xh + (size_t)(nt * 32 + m) * ld);
+    const uint32_t tile = blockIdx.x * 4 + wave;
+    if (tile >= tiles) return;
+    const uint32_t KB = ld / kMfmaStep;
+    const uint32_t kb0 = blockIdx.y * kbchunk, kb1 = min(kb0 + kbchunk, KB);
+    const uint4 *fa = F + ((size_t)tile * KB) * 256 + lane;
+    const uint4 *fb = reinterpret_cast<const uint4 *>(Xh) + lane;
     f32x16_t acc[NT];
     for (int nt = 0; nt < NT; nt++)
         for (int v = 0; v < 16; v++) acc[nt][v] = 0.0f;
-    for (uint32_t k = k0; k < k1; k += kMfmaStep) {
-        const uint32_t q = (k + 32 * hh) >> 3;
+    for (uint32_t kb = kb0; kb < kb1; kb++) {
         uint4 a[4];
-        for (int t = 0; t < 4; t++) a[t] = arow[q + t];
+        for (int t = 0; t < 4; t++) a[t] = fa[(size_t)kb * 256 + 64 * t];
         for (int nt = 0; nt < NT; nt++) {
             uint4 b[4];
-            for (int t = 0; t < 4; t++) b[t] = brow[nt][q + t];
+            for (int t = 0; t < 4; t++) b[t] = fb[((size_t)nt * KB + kb) * 256 + 64 * t];
             for (int t = 0; t < 4; t++)
                 acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(as_half8(a[t]), as_half8(b[t]),
                                                                  acc[nt], 0, 0, 0);
@@ -276,55 +312,11 @@ __global__ __launch_bounds__(256) void k_gemm16(const __half *__restrict__ H, ui
     // output tile: lane holds column m of each vector tile, rows
     // 8 (v / 4) + 4 hh + v % 4
     float *o = part + (size_t)blockIdx.y * NT * 32 * ld;
+    const uint32_t row0 = 32 * tile;
     for (int nt = 0; nt < NT; nt++)
         for (int v = 0; v < 16; v++) {
             uint32_t i = row0 + 8 * (v >> 2) + 4 * hh + (v & 3);
-            if (i < nb) o[(size_t)(nt * 32 + m) * ld + i] = acc[nt][v];
-        }
-}
-
-// transposed: part[split][r][c] = sum_{i in split} H[i][c] Xh[r][i]; the
-// operator tile is read down its columns (32 columns x 2 B per row and
-// instruction: 64 B contiguous per half-wave)
-template <int NT>
-__global__ __launch_bounds__(256) void k_gemm16_t(const __half *__restrict__ H, uint32_t nb,
-                                                  uint32_t ld, const _Float16 *__restrict__ Xh,
-                                                  uint32_t kchunk, float *__restrict__ part)
-{
-    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const uint32_t m = lane & 31, hh = lane >> 5;
-    const uint32_t col0 = (blockIdx.x * 4 + wave) * 32;
-    if (col0 >= ld) return;
-    const uint32_t k0 = blockIdx.y * kchunk, k1 = min(k0 + kchunk, ld);
-    const _Float16 *Hh = reinterpret_cast<const _Float16 *>(H);
-    const uint4 *brow[NT];
-    for (int nt = 0; nt < NT; nt++)
-        brow[nt] = reinterpret_cast<const uint4 *>(Xh + (size_t)(nt * 32 + m) * ld);
-    f32x16_t acc[NT];
-    for (int nt = 0; nt < NT; nt++)
-        for (int v = 0; v < 16; v++) acc[nt][v] = 0.0f;
-    for (uint32_t k = k0; k < k1; k += kMfmaStep) {
-        const uint32_t kb = k + 32 * hh;
-        half8_t a[4];
-        for (int t = 0; t < 4; t++)
-            for (int j = 0; j < 8; j++) {
-                // rows beyond nb are zero padding of Xh; clamp the read
-                uint32_t i = min(kb + 8 * t + j, nb - 1);
-                a[t][j] = Hh[(size_t)i * ld + col0 + m];
-            }
-        for (int nt = 0; nt < NT; nt++) {
-            uint4 b[4];
-            for (int t = 0; t < 4; t++) b[t] = brow[nt][(kb >> 3) + t];
-            for (int t = 0; t < 4; t++)
-                acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[t], as_half8(b[t]), acc[nt], 0,
-                                                                 0, 0);
-        }
-    }
-    float *o = part + (size_t)blockIdx.y * NT * 32 * ld;
-    for (int nt = 0; nt < NT; nt++)
-        for (int v = 0; v < 16; v++) {
-            uint32_t c = col0 + 8 * (v >> 2) + 4 * hh + (v & 3);
-            o[(size_t)(nt * 32 + m) * ld + c] = acc[nt][v];
+            if (i < nrows) o[(size_t)(nt * 32 + m) * ld + i] = acc[nt][v];
         }
 }
 
@@ -484,6 +476,8 @@ void op_reset(Scene &s)
     s.f16_valid = false;
     s.d_h16.release();
     s.d_h16s.release();
+    s.d_h16f.release();
+    s.d_h16ft.release();
     s.aca.reset();
 }
 
@@ -622,12 +616,21 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
         }
         return;
     }
-    const __half *H = reinterpret_cast<const __half *>(s.d_h16.p);
     const uint32_t tiles = transposed ? ld / 32 : cdiv(nb, 32);  // 32-wide wave tiles
-    const uint32_t ksteps = ld / kMfmaStep;
-    uint32_t ksplit = std::min<uint32_t>(std::max<uint32_t>(1, cdiv(4096, tiles)), ksteps);
-    const uint32_t kchunk = cdiv(ksteps, ksplit) * kMfmaStep;
-    ksplit = cdiv(ld, kchunk);
+    const uint32_t nrows = transposed ? ld : nb;
+    const uint32_t KB = ld / kMfmaStep;
+    DBuf<uint16_t> &fr = transposed ? s.d_h16ft : s.d_h16f;
+    if (!fr.p) {
+        // fragment-order copy of the FP16 operator, built on first use
+        const __half *H = reinterpret_cast<const __half *>(s.d_h16.p);
+        fr.alloc((size_t)tiles * KB * 2048);
+        k_frag_pack<<<cdiv((size_t)tiles * KB * 256, 256), 256, 0, st>>>(
+            H, nb, ld, tiles, transposed ? 1 : 0, reinterpret_cast<uint4 *>(fr.p));
+    }
+    const uint4 *F = reinterpret_cast<const uint4 *>(fr.p);
+    uint32_t ksplit = std::min<uint32_t>(std::max<uint32_t>(1, cdiv(4096, tiles)), KB);
+    const uint32_t kbchunk = cdiv(KB, ksplit);
+    ksplit = cdiv(KB, kbchunk);
     s.d_xs.alloc(64);
     s.d_xh.alloc((size_t)64 * ld);
     s.d_gpart.alloc((size_t)ksplit * 64 * ld);
@@ -640,13 +643,9 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
         k_rhs_quant<<<cdiv((size_t)rpad * ld, 256), 256, 0, st>>>(
             Xc, nb, ld, rc, rpad, transposed ? rs : nullptr, s.d_xs.p, Xh);
         dim3 grid(cdiv(tiles, 4), ksplit);
-        if (!transposed) {
-            if (nt == 1) k_gemm16<1><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
-            else k_gemm16<2><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
-        } else {
-            if (nt == 1) k_gemm16_t<1><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
-            else k_gemm16_t<2><<<grid, 256, 0, st>>>(H, nb, ld, Xh, kchunk, s.d_gpart.p);
-        }
+        if (nt == 1)
+            k_gemm16<1><<<grid, 256, 0, st>>>(F, tiles, nrows, ld, Xh, kbchunk, s.d_gpart.p);
+        else k_gemm16<2><<<grid, 256, 0, st>>>(F, tiles, nrows, ld, Xh, kbchunk, s.d_gpart.p);
         k_gemm_reduce<<<cdiv((size_t)rc * ld, 256), 256, 0, st>>>(
             s.d_gpart.p, ksplit, rpad, rc, nb, ld, transposed ? nullptr : rs, s.d_xs.p, cd, Xc,
             Y + (size_t)r0 * ld);
