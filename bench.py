@@ -129,6 +129,50 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
     return out
 
 
+def transient(irtt, s, sd, steps=288, dt=300.0):
+    """The city block over 24 h: `steps` implicit-Euler steps of dt with the
+    sun sweeping east to west (elevation 60 sin(pi (t - 6 h) / 12 h), no sun
+    at night), per-patch areal heat capacity U[1e4, 1e5] J/(m^2 K).  FP16 form
+    (FP16 Krylov products, FP32-operator residuals with FP64 accumulation)
+    against the FP32 form; max |dT| between them per step."""
+    import numpy as np
+    from irtt import scenes
+    srcs = []
+    for k in range(steps):
+        th = (k + 1) * dt / 3600.0
+        el = 60.0 * np.sin(np.pi * (th - 6.0) / 12.0)
+        az = 90.0 + 15.0 * (th - 6.0)
+        flux = scenes.sun_flux(sd, el, az) if el > 0 else np.zeros(sd.nt, np.float32)
+        srcs.append(scenes.bin_sources(sd, flux))
+    p = irtt.solver_params(steady=False, dt=dt, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-14,
+                           lin_rtol=1e-8)
+    out, hist = {}, {}
+    for name, op in (("fp32", irtt.OP_DENSE), ("fp16", irtt.OP_DENSE_F16)):
+        s.select_operator(op)
+        s.set_vertex_state(np.full(sd.nv, 293.0, np.float32), sd.fixed)
+        H = np.empty((steps, s.n_bins))
+        t0 = time.perf_counter()
+        iters = 0
+        for k in range(steps):
+            s.set_source(srcs[k])
+            st = s.solve(p)
+            iters += st["linear_iters"]
+            H[k] = s.temperatures()
+        t = time.perf_counter() - t0
+        hist[name] = H
+        out[name] = {"ms_per_step": round(t * 1e3 / steps, 2), "linear_iters": iters}
+    s.select_operator(irtt.OP_DENSE)
+    d = np.abs(hist["fp16"] - hist["fp32"]).max(axis=1)
+    out.update({"steps": steps, "dt_s": dt,
+                "max_dT_fp16_vs_fp32_K": float(d.max()),
+                "max_dT_per_step_K": [float(f"{v:.3e}") for v in d[::24]],
+                "T_range_K": [round(float(hist["fp32"].min()), 2),
+                              round(float(hist["fp32"].max()), 2)],
+                "unit": "implicit-Euler steps of the city block, FP32-form solution = FP64-"
+                        "accumulated residual of the FP32 operator (max_dT_per_step every 2 h)"})
+    return out
+
+
 def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=50000, seed=7):
     """Poisson stream of design variants through irtt_serve_*: city blocks of
     log-uniform size (tmin..tmax triangles) with random heights and seeds,
@@ -184,6 +228,7 @@ def main():
     ap.add_argument("--threads-per-tri", type=int, default=162)
     ap.add_argument("--rays-per-thread", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
@@ -222,6 +267,7 @@ def main():
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
     solves = solve_rates(irtt, s, sd)
+    trans = None if args.no_transient else transient(irtt, s, sd)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -258,6 +304,8 @@ def main():
                          "linear_iters": sol["linear_iters"], "converged": sol["converged"]},
         "solves": solves,
     }
+    if trans is not None:
+        line["transient"] = trans
     if not args.no_serving:
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:

@@ -89,6 +89,11 @@ int irtt_scene_set_vertex_state(irtt_scene h, const float *kelvin,
 /* Per-bin state directly (n_bins entries). */
 int irtt_scene_set_bin_state(irtt_scene h, const float *kelvin,
                              const uint8_t *fixed);
+/* Replace the external absorbed power per bin (n_bins, W), e.g. a sun that
+ * moves between time steps (the reference updates its sky emitters instead,
+ * ThermalRenderer::updateSkyValues, thermal_renderer.cpp:499-512).  The
+ * transport operator is unaffected. */
+int irtt_scene_set_source(irtt_scene h, const double *source);
 /* Replace vertex positions (same topology) and rebuild areas, mass, Laplacian
  * and the BVH (ThermalRenderer::updateGeometryWithParameters,
  * thermal_renderer.cpp:1327-1337). */

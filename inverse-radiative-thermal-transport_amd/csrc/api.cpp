@@ -361,6 +361,18 @@ int irtt_scene_set_bin_state(irtt_scene h, const float *kelvin, const uint8_t *f
     });
 }
 
+int irtt_scene_set_source(irtt_scene h, const double *source)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!source) throw Error(IRTT_ERR_ARG, "source is null");
+        for (uint32_t i = 0; i < s->nb; i++) {
+            if (!std::isfinite(source[i])) throw Error(IRTT_ERR_ARG, "non-finite source");
+            s->bin_source[i] = source[i];
+        }
+    });
+}
+
 int irtt_scene_update_vertices(irtt_scene h, const float *xyz)
 {
     return guard([&] {

@@ -153,6 +153,7 @@ def _declare(L):
     L.irtt_serve_wait.argtypes = [_vp, ctypes.c_uint64, ctypes.POINTER(RequestStatus)]
     L.irtt_serve_destroy.argtypes = [_vp]
     L.irtt_select_operator.argtypes = [_vp, ctypes.c_uint32]
+    L.irtt_scene_set_source.argtypes = [_vp, _vp]
     L.irtt_apply_operator.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32]
     L.irtt_simulate.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.c_uint32, _vp,
                                 ctypes.POINTER(SolveStats)]
@@ -417,6 +418,13 @@ class Scene:
         c = np.empty(self.n_bins)
         _check(lib().irtt_get_column_sums(self._h, _ptr(c)))
         return c
+
+    def set_source(self, source):
+        """External absorbed power per bin (W), irtt_scene_set_source."""
+        src = np.ascontiguousarray(source, np.float64)
+        if src.shape != (self.n_bins,):
+            raise ValueError("source must have n_bins entries")
+        _check(lib().irtt_scene_set_source(self._h, _ptr(src)))
 
     def select_operator(self, op):
         """OP_DENSE, OP_DENSE_F16 or OP_ACA for the solves that follow."""
