@@ -223,6 +223,7 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
 void op_reset(Scene &s);
 void op_select(Scene &s, uint32_t kind);
 void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine);
+void op_prepare(Scene &s, bool transposed);
 void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool transposed,
                     bool refine = false);
 const std::vector<double> &op_diag_outside(const Scene &s, bool refine);

@@ -520,6 +520,14 @@ const std::vector<double> &op_diag_total(const Scene &s)
     return s.op_kind == IRTT_OP_ACA ? s.aca.opdiag : s.Tdiag;
 }
 
+// the allocations of op_apply, made ahead of a graph capture of products
+void op_prepare(Scene &s, bool transposed)
+{
+    const uint32_t nchunk = cdiv(s.nb, kTRows);
+    if (transposed) s.d_tpart.alloc((size_t)nchunk * s.ld);
+    if (s.op_kind == IRTT_OP_DENSE_F16) s.d_u32.alloc(s.ld);
+}
+
 // y = P w (or P^T w).  refine: the product of the residual ("refinement")
 // side, which uses the FP32 operator whenever the Krylov side is FP16.
 void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine)

@@ -273,6 +273,7 @@ struct Ctx {
     Sys sys;
     double *v[V_COUNT];
     double *part, *out;
+    double *sc;  // device scalars of the Krylov iteration
     double hbuf[4];
     explicit Ctx(Scene &sc) : s(sc), st(sc.stream), nb(sc.nb), ld(sc.ld) {}
 
@@ -318,11 +319,12 @@ void upload_static(Scene &s, Ctx &c, const irtt_solver_params &p)
 {
     const uint32_t nb = s.nb, ld = s.ld;
     s.d_vec.alloc((size_t)V_COUNT * ld);
-    s.d_red.alloc(4 * kRedBlocks + 8);
+    s.d_red.alloc(4 * kRedBlocks + 8 + 32);
     IRTT_HIP(hipMemsetAsync(s.d_vec.p, 0, s.d_vec.bytes(), s.stream));
     for (int k = 0; k < V_COUNT; k++) c.v[k] = s.d_vec.p + (size_t)k * ld;
     c.part = s.d_red.p;
     c.out = s.d_red.p + 4 * kRedBlocks;
+    c.sc = s.d_red.p + 4 * kRedBlocks + 8;
     std::vector<double> b(nb), kfix(nb), ldiag(nb, 0.0);
     const double Ta = p.ambient_kelvin;
     for (uint32_t i = 0; i < nb; i++) {
@@ -407,15 +409,133 @@ void residual_host(Scene &s, const irtt_solver_params &p, const double *x, doubl
     IRTT_HIP(hipStreamSynchronize(s.stream));
 }
 
+// ---- BiCGSTAB with its scalars in device memory --------------------------
+// The iteration's scalars (rho, alpha, omega, ...) live in a small device
+// array: reductions write into it, one-thread kernels derive the next
+// scalars, vector kernels read them, and a converged or broken-down
+// iteration sets `done`, after which every update is a no-op.  No host round
+// trip is needed inside an iteration, so K iterations are enqueued back to
+// back and the host reads `done` once per K.  (Capturing the K iterations
+// into a hipGraph per solve measured slower under concurrent serving: the
+// instantiation costs more than the launches it saves.)
+enum { B_RHO = 0, B_RHO_NEW, B_ALPHA, B_OMEGA, B_BETA, B_DONE, B_STOP, B_TMP, B_SS, B_TS, B_TT,
+       B_RR, B_ITERS, B_FIRST, B_HALF, B_NSC };
+
+__global__ void kd_init(double *sc, double stop)
+{
+    for (int k = 0; k < B_NSC; k++) sc[k] = 0.0;
+    sc[B_RHO] = sc[B_ALPHA] = sc[B_OMEGA] = 1.0;
+    sc[B_FIRST] = 1.0;
+    sc[B_STOP] = stop;
+}
+
+// rho_new from the last reduction; breakdown ends the solve
+__global__ void kd_rho(double *sc)
+{
+    if (sc[B_DONE] != 0.0) return;
+    const double rn = sc[B_RHO_NEW];
+    if (rn == 0.0 || !isfinite(rn)) {
+        sc[B_DONE] = 1.0;
+        return;
+    }
+    sc[B_BETA] = sc[B_FIRST] != 0.0 ? 0.0 : (rn / sc[B_RHO]) * (sc[B_ALPHA] / sc[B_OMEGA]);
+    sc[B_FIRST] = 0.0;
+}
+
+__global__ void kd_alpha(double *sc)
+{
+    if (sc[B_DONE] != 0.0) return;
+    const double d = sc[B_TMP];
+    if (d == 0.0 || !isfinite(d)) sc[B_DONE] = 1.0;
+    else sc[B_ALPHA] = sc[B_RHO_NEW] / d;
+}
+
+// ||s||^2 small enough: finish with dx += alpha ph (applied by kd_half_apply)
+__global__ void kd_half(double *sc)
+{
+    sc[B_HALF] = (sc[B_DONE] == 0.0 && sc[B_SS] <= sc[B_STOP]) ? 1.0 : 0.0;
+}
+
+__global__ void kd_half_done(double *sc)
+{
+    if (sc[B_HALF] != 0.0) {
+        sc[B_DONE] = 1.0;
+        sc[B_ITERS] += 1.0;
+    }
+}
+
+__global__ void kd_omega(double *sc)
+{
+    if (sc[B_DONE] != 0.0) return;
+    if (sc[B_TT] == 0.0) sc[B_DONE] = 1.0;
+    else sc[B_OMEGA] = sc[B_TS] / sc[B_TT];
+}
+
+__global__ void kd_end(double *sc)
+{
+    if (sc[B_DONE] != 0.0) return;
+    sc[B_RHO] = sc[B_RHO_NEW];
+    if (sc[B_RR] <= sc[B_STOP]) {
+        sc[B_ITERS] += 1.0;
+        sc[B_DONE] = 1.0;
+    } else if (sc[B_OMEGA] == 0.0) {
+        sc[B_DONE] = 1.0;
+    } else {
+        sc[B_ITERS] += 1.0;
+    }
+}
+
+__global__ void kd_p_update(uint32_t n, const double *__restrict__ sc, const double *r,
+                            const double *v, double *p)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || sc[B_DONE] != 0.0) return;
+    p[i] = r[i] + sc[B_BETA] * (p[i] - sc[B_OMEGA] * v[i]);
+}
+__global__ void kd_scale(uint32_t n, const double *__restrict__ sc, const double *dinv,
+                         const double *x, double *y)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || sc[B_DONE] != 0.0) return;
+    y[i] = dinv[i] * x[i];
+}
+// s = r - alpha v
+__global__ void kd_s(uint32_t n, const double *__restrict__ sc, const double *r, const double *v,
+                     double *s)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || sc[B_DONE] != 0.0) return;
+    s[i] = r[i] + (-sc[B_ALPHA]) * v[i];
+}
+__global__ void kd_half_apply(uint32_t n, const double *__restrict__ sc, const double *ph,
+                              double *dx)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || sc[B_HALF] == 0.0) return;
+    dx[i] = dx[i] + sc[B_ALPHA] * ph[i];
+}
+__global__ void kd_step(uint32_t n, const double *__restrict__ sc, const double *ph,
+                        const double *sh, const double *s, const double *t, double *x, double *r)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || sc[B_DONE] != 0.0) return;
+    x[i] += sc[B_ALPHA] * ph[i] + sc[B_OMEGA] * sh[i];
+    r[i] = s[i] - sc[B_OMEGA] * t[i];
+}
+
 // BiCGSTAB (van der Vorst 1992), right Jacobi preconditioning, on J dx = -r
 // with J the current Jacobian mode at V_XE and r in V_R; dx0 = 0, result in
 // V_DX.  Stops at ||r_lin||^2 <= lin_rtol^2 * rn (rn = ||r||^2).  Returns the
-// iteration count.
+// iteration count.  Operators above 64 MiB run one iteration per host check
+// (a product costs far more than the check); smaller ones enqueue eight
+// iterations per check.
 static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
 {
+    Scene &s = c.s;
     const uint32_t nb = c.nb;
     const uint32_t g = cdiv(nb, 256);
     hipStream_t stm = c.st;
+    double *sc = c.sc;
     k_jdiag<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_TDIAG], c.v[V_LDIAG], c.v[V_DINV]);
     double *R = c.v[V_RHAT];
     IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
@@ -424,44 +544,50 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
     double *rl = c.v[V_XT];  // linear residual, starts at -r
     k_axpy2<<<g, 256, 0, stm>>>(nb, -2.0, c.v[V_R], c.v[V_R], rl);
     IRTT_HIP(hipMemcpyAsync(R, rl, nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
-    double rho = 1.0, alpha = 1.0, omega = 1.0;
-    const double stop = lin_rtol * lin_rtol * rn;
-    double res[3];
-    uint32_t li = 0;
-    for (; li < max_lin; li++) {
-        c.dots(1, R, rl, nullptr, nullptr, nullptr, nullptr, res);
-        double rho_new = res[0];
-        if (rho_new == 0.0 || !std::isfinite(rho_new)) break;
-        double beta = (li == 0) ? 0.0 : (rho_new / rho) * (alpha / omega);
-        k_p_update<<<g, 256, 0, stm>>>(nb, beta, omega, rl, c.v[V_V], c.v[V_P]);
-        k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
+    kd_init<<<1, 1, 0, stm>>>(sc, lin_rtol * lin_rtol * rn);
+    op_prepare(s, c.sys.transposed != 0);
+    auto dot_into = [&](int m, const double *a0, const double *b0, const double *a1,
+                        const double *b1, double *dst) {
+        k_dots<<<kRedBlocks, kRedThreads, 0, stm>>>(nb, m, a0, b0, a1, b1, nullptr, nullptr,
+                                                    c.part);
+        k_final_sum<<<1, kRedThreads, 0, stm>>>(c.part, m, kRedBlocks, dst);
+    };
+    auto iteration = [&]() {
+        dot_into(1, R, rl, nullptr, nullptr, sc + B_RHO_NEW);
+        kd_rho<<<1, 1, 0, stm>>>(sc);
+        kd_p_update<<<g, 256, 0, stm>>>(nb, sc, rl, c.v[V_V], c.v[V_P]);
+        kd_scale<<<g, 256, 0, stm>>>(nb, sc, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
         c.jv(c.v[V_PH], c.v[V_V]);
-        c.dots(1, R, c.v[V_V], nullptr, nullptr, nullptr, nullptr, res);
-        if (res[0] == 0.0 || !std::isfinite(res[0])) break;
-        alpha = rho_new / res[0];
-        k_axpy2<<<g, 256, 0, stm>>>(nb, -alpha, rl, c.v[V_V], c.v[V_S]);
-        c.dots(1, c.v[V_S], c.v[V_S], nullptr, nullptr, nullptr, nullptr, res);
-        if (res[0] <= stop) {
-            k_axpy2<<<g, 256, 0, stm>>>(nb, alpha, c.v[V_DX], c.v[V_PH], c.v[V_DX]);
-            li++;
-            break;
-        }
-        k_scale<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
+        dot_into(1, R, c.v[V_V], nullptr, nullptr, sc + B_TMP);
+        kd_alpha<<<1, 1, 0, stm>>>(sc);
+        kd_s<<<g, 256, 0, stm>>>(nb, sc, rl, c.v[V_V], c.v[V_S]);
+        dot_into(1, c.v[V_S], c.v[V_S], nullptr, nullptr, sc + B_SS);
+        kd_half<<<1, 1, 0, stm>>>(sc);
+        kd_half_apply<<<g, 256, 0, stm>>>(nb, sc, c.v[V_PH], c.v[V_DX]);
+        kd_half_done<<<1, 1, 0, stm>>>(sc);
+        kd_scale<<<g, 256, 0, stm>>>(nb, sc, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
         c.jv(c.v[V_SH], c.v[V_T]);
-        c.dots(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], nullptr, nullptr, res);
-        if (res[1] == 0.0) break;
-        omega = res[0] / res[1];
-        k_step<<<g, 256, 0, stm>>>(nb, alpha, omega, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T],
-                                   c.v[V_DX], rl);
-        c.dots(1, rl, rl, nullptr, nullptr, nullptr, nullptr, res);
-        rho = rho_new;
-        if (res[0] <= stop) {
-            li++;
-            break;
-        }
-        if (omega == 0.0) break;
+        dot_into(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], sc + B_TS);
+        kd_omega<<<1, 1, 0, stm>>>(sc);
+        kd_step<<<g, 256, 0, stm>>>(nb, sc, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T], c.v[V_DX],
+                                    rl);
+        dot_into(1, rl, rl, nullptr, nullptr, sc + B_RR);
+        kd_end<<<1, 1, 0, stm>>>(sc);
+    };
+    const size_t op_bytes = (size_t)s.nb * s.ld * (s.op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
+    const uint32_t K = op_bytes > (64u << 20) ? 1u : 8u;
+    double hs[2] = {0.0, 0.0};
+    uint32_t launched = 0;
+    while (launched < max_lin) {
+        for (uint32_t k = 0; k < K; k++) iteration();
+        launched += K;
+        IRTT_HIP(hipMemcpyAsync(hs, sc + B_DONE, sizeof(double), hipMemcpyDeviceToHost, stm));
+        IRTT_HIP(hipMemcpyAsync(hs + 1, sc + B_ITERS, sizeof(double), hipMemcpyDeviceToHost,
+                                stm));
+        IRTT_HIP(hipStreamSynchronize(stm));
+        if (hs[0] != 0.0) break;
     }
-    return li;
+    return std::min<uint32_t>((uint32_t)hs[1], max_lin);
 }
 
 namespace {
