@@ -126,8 +126,22 @@ typedef struct irtt_transport_params {
     uint32_t batch_seed;        /* ThermalVars_s::randSeed (1234) */
     uint32_t max_depth;         /* 0 -> 512 (MAX_MAX_DEPTH) */
     float tiny_dist;            /* ThermalVars_s::tinyDist (1e-3) */
-    uint32_t reserved[3];
+    uint32_t layout;            /* IRTT_LAYOUT_*: how the operator is stored */
+    uint32_t reserved[2];
 } irtt_transport_params;
+
+/* Storage of the operator built by irtt_build_transport.  DENSE is the
+ * reference's n_bins x n_bins FP32 matrix (thermal_transport.cpp:612-660).
+ * SPARSE keeps only the entries some photon reached (plus the diagonal) in
+ * compressed rows and compressed columns: the entries are bit-identical to
+ * the dense ones, the memory is 8 B per stored entry instead of 4 n_bins^2
+ * (a Monte-Carlo operator is sparse whenever photons per bin are fewer than
+ * bins), and it has no n_bins limit (the reference caps dense indexing at
+ * 46 340 bins, thermal_transport.cpp:143).  Photon hits are combined in a
+ * device hash table with exact integer sums, then sorted by (row, column), so
+ * the stored operator is bit-reproducible. */
+#define IRTT_LAYOUT_DENSE 0u
+#define IRTT_LAYOUT_SPARSE 1u
 
 typedef struct irtt_transport_stats {
     uint64_t photons, rays, absorbed, miss, too_close, backface, degenerate,
@@ -136,9 +150,10 @@ typedef struct irtt_transport_stats {
     float trace_ms;             /* photon kernel, HIP events */
     float finalize_ms;          /* fixed point -> FP32 dense + diagonal */
     float bvh_ms;               /* last LBVH build */
-    uint32_t reserved0;
+    uint32_t layout;            /* IRTT_LAYOUT_* of the built operator */
     uint64_t node_visits;       /* internal BVH nodes fetched over all rays */
     uint64_t tri_tests;         /* leaf triangles tested over all rays */
+    uint64_t nnz;               /* stored entries (diagonal included) */
 } irtt_transport_stats;
 
 /* Replaces ThermalTransport::compute (thermal_transport.cpp:599-738):
@@ -149,8 +164,18 @@ typedef struct irtt_transport_stats {
 int irtt_build_transport(irtt_scene h, const irtt_transport_params *p,
                          irtt_transport_stats *st);
 int irtt_get_transport(irtt_scene h, float *dense);
-/* Raw fixed-point accumulators (n_bins^2, row-major) and the quantum. */
+/* Raw fixed-point accumulators (n_bins^2, row-major) and the quantum (dense
+ * layout only). */
 int irtt_get_transport_fixed(irtt_scene h, uint64_t *acc, double *quantum);
+/* The stored operator in compressed rows: *nnz receives the entry count;
+ * when rowptr, col and val are all non-NULL they receive n_bins + 1 row
+ * offsets, then the column and FP32 value of every entry, rows in order and
+ * columns ascending within a row.  Either layout (the dense one is
+ * compressed on the fly, zeros dropped, the diagonal kept).  No reference
+ * counterpart: the reference keeps only the dense host matrix
+ * (thermal_transport.cpp:612-660). */
+int irtt_get_transport_sparse(irtt_scene h, uint64_t *nnz, uint32_t *rowptr,
+                              uint32_t *col, float *val);
 /* Column sums of the operator, exact from the fixed-point accumulators (FP64). */
 int irtt_get_column_sums(irtt_scene h, double *colsum);
 
@@ -159,12 +184,14 @@ int irtt_get_column_sums(irtt_scene h, double *colsum);
 #define IRTT_OP_ACA 1u        /* U V + diag(C) of irtt_compress_transport */
 #define IRTT_OP_DENSE_F16 2u  /* off-diagonal FP16 with an FP32 scale per row, FP64
                                  diagonal; Newton residuals keep the FP32 operator */
+#define IRTT_OP_SPARSE 3u     /* FP32 compressed rows / columns (IRTT_LAYOUT_SPARSE) */
 
 /* Operator used by irtt_solve / irtt_residual / irtt_adjoint_solve (the
  * reference's transportCompressed switch, thermal_solver.cpp:126-131).
  * IRTT_OP_DENSE_F16 quantises the dense operator on first use;
  * IRTT_OP_ACA needs irtt_compress_transport.  A new irtt_build_transport
- * returns to IRTT_OP_DENSE and drops the other forms. */
+ * selects IRTT_OP_DENSE (dense layout) or IRTT_OP_SPARSE (sparse layout) and
+ * drops the other forms; the sparse layout offers no other form. */
 int irtt_select_operator(irtt_scene h, uint32_t op);
 
 /* Y = T X (transposed: T^T X) for n_rhs vectors with the selected operator

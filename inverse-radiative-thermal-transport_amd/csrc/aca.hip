@@ -411,6 +411,8 @@ static uint32_t reset_reference(CrtRand &rng, uint32_t n, const std::vector<uint
 void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st)
 {
     if (!s.has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+    if (s.layout != IRTT_LAYOUT_DENSE)
+        throw Error(IRTT_ERR_STATE, "global ACA compresses the dense transport layout");
     const uint32_t n = s.nb, ld = s.ld;
     const uint32_t k = p.rank ? p.rank : 50;
     if (k > kAcaMaxRank || k > n)

@@ -224,7 +224,9 @@ uint64_t Scene::device_bytes() const
            d_gpart.bytes() + d_gpart64.bytes() + aca.d_ut.bytes() +
            aca.d_v.bytes() + aca.d_uf.bytes() + aca.d_vf.bytes() + aca.d_uh.bytes() +
            aca.d_vh.bytes() + d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
-           d_lap_col.bytes() + d_lap_val.bytes();
+           d_lap_col.bytes() + d_lap_val.bytes() + sp.rowptr.bytes() + sp.col.bytes() +
+           sp.val.bytes() + sp.colptr.bytes() + sp.row.bytes() + sp.cval.bytes() +
+           sp.d_xt.bytes();
 }
 
 extern "C" {
@@ -434,14 +436,17 @@ int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_tran
     return guard([&] {
         Scene *s = S(h);
         if (!p) throw Error(IRTT_ERR_ARG, "params is null");
-        // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B)
-        size_t need = (size_t)s->nb * s->ld * 12;
-        size_t fr = 0, tot = 0;
-        IRTT_HIP(hipMemGetInfo(&fr, &tot));
-        size_t held = s->d_acc.bytes() + s->d_T.bytes();
-        if (need > fr + held)
-            throw Error(IRTT_ERR_LIMIT, "dense operator needs " + std::to_string(need) +
-                                            " bytes, " + std::to_string(fr + held) + " available");
+        if (p->layout == IRTT_LAYOUT_DENSE) {
+            // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B)
+            size_t need = (size_t)s->nb * s->ld * 12;
+            size_t fr = 0, tot = 0;
+            IRTT_HIP(hipMemGetInfo(&fr, &tot));
+            size_t held = s->d_acc.bytes() + s->d_T.bytes();
+            if (need > fr + held)
+                throw Error(IRTT_ERR_LIMIT, "dense operator needs " + std::to_string(need) +
+                                                " bytes, " + std::to_string(fr + held) +
+                                                " available");
+        }
         irtt_transport_stats tmp{};
         transport_build(*s, *p, tmp);
         if (st) *st = tmp;
@@ -555,8 +560,53 @@ int irtt_get_transport(irtt_scene h, float *dense)
         Scene *s = S(h);
         if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
         if (!dense) throw Error(IRTT_ERR_ARG, "dense is null");
-        IRTT_HIP(hipMemcpy2D(dense, (size_t)s->nb * 4, s->d_T.p, (size_t)s->ld * 4,
-                             (size_t)s->nb * 4, s->nb, hipMemcpyDeviceToHost));
+        const size_t nb = s->nb;
+        if (s->layout == IRTT_LAYOUT_SPARSE) {
+            std::vector<uint32_t> rp(nb + 1), col(s->sp.nnz);
+            std::vector<float> val(s->sp.nnz);
+            sp_download(*s, rp.data(), col.data(), val.data());
+            std::fill(dense, dense + nb * nb, 0.0f);
+            for (size_t i = 0; i < nb; i++)
+                for (uint32_t k = rp[i]; k < rp[i + 1]; k++) dense[i * nb + col[k]] = val[k];
+            return;
+        }
+        IRTT_HIP(hipMemcpy2D(dense, nb * 4, s->d_T.p, (size_t)s->ld * 4, nb * 4, nb,
+                             hipMemcpyDeviceToHost));
+    });
+}
+
+int irtt_get_transport_sparse(irtt_scene h, uint64_t *nnz, uint32_t *rowptr, uint32_t *col,
+                              float *val)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (!nnz) throw Error(IRTT_ERR_ARG, "nnz is null");
+        const bool fill = rowptr && col && val;
+        const size_t nb = s->nb;
+        if (s->layout == IRTT_LAYOUT_SPARSE) {
+            *nnz = s->sp.nnz;
+            if (fill) sp_download(*s, rowptr, col, val);
+            return;
+        }
+        std::vector<float> d(nb * nb);
+        IRTT_HIP(hipMemcpy2D(d.data(), nb * 4, s->d_T.p, (size_t)s->ld * 4, nb * 4, nb,
+                             hipMemcpyDeviceToHost));
+        uint64_t n = 0;
+        for (size_t i = 0; i < nb; i++) {
+            if (fill) rowptr[i] = (uint32_t)n;
+            for (size_t j = 0; j < nb; j++) {
+                const float v = d[i * nb + j];
+                if (v == 0.0f && i != j) continue;
+                if (fill) {
+                    col[n] = (uint32_t)j;
+                    val[n] = v;
+                }
+                n++;
+            }
+        }
+        if (fill) rowptr[nb] = (uint32_t)n;
+        *nnz = n;
     });
 }
 
@@ -565,6 +615,8 @@ int irtt_get_transport_fixed(irtt_scene h, uint64_t *acc, double *quantum)
     return guard([&] {
         Scene *s = S(h);
         if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (acc && s->layout != IRTT_LAYOUT_DENSE)
+            throw Error(IRTT_ERR_STATE, "fixed-point accumulators are kept by the dense layout only");
         if (quantum) *quantum = s->quantum;
         if (acc)
             IRTT_HIP(hipMemcpy2D(acc, (size_t)s->nb * 8, s->d_acc.p, (size_t)s->ld * 8,

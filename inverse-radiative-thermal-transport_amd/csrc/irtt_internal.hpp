@@ -102,6 +102,46 @@ struct Bvh {
     float build_ms = 0.f;
 };
 
+// Hash-table slot of the sparse build: key (absorbing bin << key_bits |
+// emitting bin), exact u64 sum of the photons' fixed-point weights.
+struct alignas(16) TabSlot {
+    unsigned long long key, val;
+};
+
+// Where the photon kernel bins an absorbed photon: the dense accumulators,
+// or (tab set) the 2^lg-slot hash table of the sparse layout.
+struct PhotonSink {
+    unsigned long long *acc = nullptr;
+    TabSlot *tab = nullptr;
+    uint32_t lg = 0, key_bits = 0;
+    uint32_t *overflow = nullptr;
+};
+
+// Operator in compressed rows and compressed columns (sparse.hip).  Every row
+// and column holds its diagonal entry.
+struct Sparse {
+    bool valid = false;
+    uint64_t nnz = 0;
+    DBuf<uint32_t> rowptr, col;       // n_bins + 1 offsets, column of each entry
+    DBuf<float> val;                  // FP32 entries, rows in order, columns ascending
+    DBuf<uint32_t> colptr, row;       // the same entries by column, rows ascending
+    DBuf<float> cval;
+    DBuf<double> d_xt, d_yt;          // vector-interleaved operands of a batched product
+    DBuf<TabSlot> tab;                // hash table, alive during a build only
+    DBuf<uint32_t> flag;
+    void reset()
+    {
+        valid = false;
+        nnz = 0;
+        for (DBuf<uint32_t> *b : {&rowptr, &col, &colptr, &row, &flag}) b->release();
+        val.release();
+        cval.release();
+        d_xt.release();
+        d_yt.release();
+        tab.release();
+    }
+};
+
 constexpr uint32_t kAcaMaxRank = 256;
 
 // ACA-compressed operator U V + diag(C) (aca.hip)
@@ -168,7 +208,9 @@ struct Scene {
 
     // transport
     bool has_transport = false;
+    uint32_t layout = IRTT_LAYOUT_DENSE;  // of the built operator
     irtt_transport_params tparams{};
+    Sparse sp;                        // IRTT_LAYOUT_SPARSE
     double quantum = 0.0;
     DBuf<uint64_t> d_q;               // per-triangle fixed-point weight
     DBuf<unsigned long long> d_acc;   // nb*ld fixed-point accumulators
@@ -240,6 +282,12 @@ void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double
                  double *kelvin, irtt_solve_stats &st);
 void adjoint_solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *x,
                          const double *rhs, double *lambda, irtt_solve_stats &st);
+// sparse.hip
+void sparse_table_begin(Scene &s, uint64_t photons, PhotonSink &sink);
+void sparse_table_finish(Scene &s, const PhotonSink &sink);
+void sp_apply(Scene &s, const double *w, double *y, bool transposed);
+void sp_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool transposed);
+void sp_download(Scene &s, uint32_t *rowptr, uint32_t *col, float *val);
 // aca.hip
 void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st);
 void aca_apply(Scene &s, const double *w, double *y, bool transposed);

@@ -468,11 +468,12 @@ __global__ void k_add_diag(uint32_t n, const double *__restrict__ cd, const doub
 
 }  // namespace
 
-// back to the dense FP32 form; the other forms are released (a new
-// operator, or new geometry)
+// back to the dense FP32 form; the other forms and the sparse layout are
+// released (a new operator, or new geometry)
 void op_reset(Scene &s)
 {
     s.op_kind = IRTT_OP_DENSE;
+    s.sp.reset();
     s.f16_valid = false;
     s.d_h16.release();
     s.d_h16s.release();
@@ -484,7 +485,13 @@ void op_reset(Scene &s)
 void op_select(Scene &s, uint32_t kind)
 {
     if (!s.has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
-    if (kind == IRTT_OP_ACA) {
+    if ((kind == IRTT_OP_SPARSE) != (s.layout == IRTT_LAYOUT_SPARSE))
+        throw Error(IRTT_ERR_STATE, s.layout == IRTT_LAYOUT_SPARSE
+                                        ? "the sparse transport layout offers IRTT_OP_SPARSE only"
+                                        : "IRTT_OP_SPARSE needs the sparse transport layout");
+    if (kind == IRTT_OP_SPARSE) {
+        // the compressed rows and columns of the build
+    } else if (kind == IRTT_OP_ACA) {
         if (!s.aca.valid) throw Error(IRTT_ERR_STATE, "no ACA operator (irtt_compress_transport)");
     } else if (kind == IRTT_OP_DENSE_F16) {
         if (!s.f16_valid) {
@@ -540,6 +547,10 @@ void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine
         aca_apply(s, w, y, transposed);
         return;
     }
+    if (kind == IRTT_OP_SPARSE) {
+        sp_apply(s, w, y, transposed);
+        return;
+    }
     const uint32_t nchunk = cdiv(nb, kTRows);
     if (kind == IRTT_OP_DENSE) {
         if (!transposed) {
@@ -574,6 +585,10 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
 {
     const uint32_t nb = s.nb, ld = s.ld;
     hipStream_t st = s.stream;
+    if (s.op_kind == IRTT_OP_SPARSE) {
+        sp_apply_batch(s, R, X, Y, transposed);
+        return;
+    }
     if (s.op_kind == IRTT_OP_DENSE || (refine && s.op_kind == IRTT_OP_DENSE_F16)) {
         // the FP32 operator (diagonal inside) with FP64 MFMA, 32 vectors per pass
         const uint32_t tiles = transposed ? ld / (16 * kM64) : cdiv(nb, 16 * kM64);

@@ -50,12 +50,23 @@ struct Req {
     Clock::time_point t_submit, t_start, t_end;
 };
 
-// HBM needed by one request: fixed-point + FP32 operator (+ FP16 copy),
-// solver vectors and transposed-product partials, geometry and LBVH
+// HBM needed by one request: fixed-point + FP32 operator (+ FP16 copy), or
+// for the sparse layout the peak of its build (hash table + the two key /
+// value arrays being sorted, at most one entry per photon or bin), solver
+// vectors and transposed-product partials, geometry and LBVH
 uint64_t footprint(const irtt_request &q, uint32_t nb)
 {
     const uint64_t ld = (nb + 63) / 64 * 64;
-    uint64_t b = (uint64_t)nb * ld * 12;
+    uint64_t b = 0;
+    if (q.transport.layout == IRTT_LAYOUT_SPARSE) {
+        const uint64_t keys = (uint64_t)q.transport.threads_per_tri * q.transport.rays_per_thread *
+                                  q.nt + nb;
+        uint64_t cap = 1024;
+        while (cap < keys + keys / 4) cap <<= 1;
+        b = 16 * cap + 32 * keys;
+    } else {
+        b = (uint64_t)nb * ld * 12;
+    }
     if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
     b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;
     b += (uint64_t)q.nt * 256 + (uint64_t)q.nv * 64 + (64ull << 20);

@@ -574,7 +574,9 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
         dot_into(1, rl, rl, nullptr, nullptr, sc + B_RR);
         kd_end<<<1, 1, 0, stm>>>(sc);
     };
-    const size_t op_bytes = (size_t)s.nb * s.ld * (s.op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
+    const size_t op_bytes = s.op_kind == IRTT_OP_SPARSE
+                                ? (size_t)s.sp.nnz * 8
+                                : (size_t)s.nb * s.ld * (s.op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
     const uint32_t K = op_bytes > (64u << 20) ? 1u : 8u;
     double hs[2] = {0.0, 0.0};
     uint32_t launched = 0;

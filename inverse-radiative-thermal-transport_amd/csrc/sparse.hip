@@ -1,0 +1,363 @@
+// sparse.hip -- the transport operator in compressed rows and columns.
+//
+// The reference stores T densely (thermal_transport.cpp:612-660) and caps it
+// at 46 340 bins (:143).  A Monte-Carlo operator reaches only the (absorber,
+// emitter) pairs some photon connected: with ~1300 photons per triangle over
+// 36k bins the city block's operator is a few percent full, and the 400k-
+// triangle scenes cannot be held densely at all.  Here the photon kernel adds
+// each absorbed photon's fixed-point weight into a device hash table keyed by
+// (absorbing bin, emitting bin) (sparse_table.hpp); the occupied slots are
+// then compacted, radix-sorted by key and converted to FP32 exactly as the
+// dense finalize converts (T_ij = delta_ij sigma eps_i A_i - acc_ij Q), so
+// every stored entry equals the dense layout's bit for bit.  A second sort by
+// (column, row) gives the compressed columns used by T^T products.  Every
+// step is integer arithmetic or a per-entry conversion, so the stored
+// operator does not depend on the order in which photons landed.
+#include <hip/hip_runtime.h>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "irtt_internal.hpp"
+#include "sparse_table.hpp"
+
+namespace irtt {
+namespace {
+
+inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+__global__ void k_tab_clear(TabSlot *tab, uint64_t n)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) {
+        tab[i].key = kTabEmpty;
+        tab[i].val = 0ull;
+    }
+}
+
+// every bin's diagonal entry exists (sigma eps A even when no photon returned)
+__global__ void k_tab_diag(TabSlot *tab, uint32_t lg, uint32_t bits, uint32_t nb,
+                           uint32_t *overflow)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) tab_add(tab, lg, tab_key(i, i, bits), 0ull, overflow);
+}
+
+// occupied slots, one atomic per wave
+__global__ void k_tab_count(const TabSlot *__restrict__ tab, uint64_t n,
+                            unsigned long long *count)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    uint32_t c = 0;
+    for (; i < n; i += stride) c += tab[i].key != kTabEmpty;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+}
+
+// occupied slots -> (key, value) arrays in arbitrary order (sorted next);
+// column sums of the accumulators on the way
+__global__ void k_tab_compact(const TabSlot *__restrict__ tab, uint64_t n, uint32_t bits,
+                              unsigned long long *__restrict__ keys,
+                              unsigned long long *__restrict__ vals,
+                              unsigned long long *cursor, unsigned long long *colacc)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint32_t lane = threadIdx.x & 63;
+    const unsigned long long cmask = (1ull << bits) - 1ull;
+    // every lane of a wave runs the same number of rounds (ballots need all)
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
+         base += stride) {
+        const uint64_t i = base + lane;
+        TabSlot t{kTabEmpty, 0ull};
+        if (i < n) t = tab[i];
+        const bool occ = t.key != kTabEmpty;
+        const unsigned long long m = __ballot(occ);
+        unsigned long long pos = 0;
+        if (lane == 0 && m) pos = atomicAdd(cursor, (unsigned long long)__popcll(m));
+        pos = __shfl(pos, 0, 64);
+        if (occ) {
+            const unsigned long long o = pos + __popcll(m & ((1ull << lane) - 1ull));
+            keys[o] = t.key;
+            vals[o] = t.val;
+            if (t.val) atomicAdd(colacc + (t.key & cmask), t.val);
+        }
+    }
+}
+
+// sorted (row, column) keys -> compressed rows; FP32 entry as the dense
+// finalize computes it
+__global__ void k_csr_fill(const unsigned long long *__restrict__ keys,
+                           const unsigned long long *__restrict__ vals, uint64_t nnz,
+                           uint32_t bits, uint32_t nb, const double *__restrict__ diag, double Q,
+                           uint32_t *__restrict__ col, float *__restrict__ val,
+                           uint32_t *__restrict__ rowptr, float *__restrict__ tdiag)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const unsigned long long key = keys[i];
+    const uint32_t r = (uint32_t)(key >> bits), c = (uint32_t)(key & ((1ull << bits) - 1ull));
+    const double v = (r == c ? diag[r] : 0.0) - (double)vals[i] * Q;
+    const float f = (float)v;
+    col[i] = c;
+    val[i] = f;
+    if (r == c) tdiag[r] = f;
+    if (i == 0 || (uint32_t)(keys[i - 1] >> bits) != r) rowptr[r] = (uint32_t)i;
+    if (i == nnz - 1) rowptr[nb] = (uint32_t)nnz;
+}
+
+__global__ void k_csc_keys(const unsigned long long *__restrict__ keys,
+                           const float *__restrict__ val, uint64_t nnz, uint32_t bits,
+                           unsigned long long *__restrict__ k2, uint32_t *__restrict__ v2)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const unsigned long long key = keys[i];
+    const unsigned long long cmask = (1ull << bits) - 1ull;
+    k2[i] = ((key & cmask) << bits) | (key >> bits);
+    v2[i] = __float_as_uint(val[i]);
+}
+
+__global__ void k_csc_fill(const unsigned long long *__restrict__ k2,
+                           const uint32_t *__restrict__ v2, uint64_t nnz, uint32_t bits,
+                           uint32_t nb, uint32_t *__restrict__ row, float *__restrict__ cval,
+                           uint32_t *__restrict__ colptr)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nnz) return;
+    const unsigned long long key = k2[i];
+    const uint32_t c = (uint32_t)(key >> bits);
+    row[i] = (uint32_t)(key & ((1ull << bits) - 1ull));
+    cval[i] = __uint_as_float(v2[i]);
+    if (i == 0 || (uint32_t)(k2[i - 1] >> bits) != c) colptr[c] = (uint32_t)i;
+    if (i == nnz - 1) colptr[nb] = (uint32_t)nnz;
+}
+
+// y_i = sum_k val_k u_{idx_k} over the entries of row i (or column i of the
+// compressed columns): one wave per row, FP64 accumulation, fixed-order
+// wave reduction, so the result is reproducible.
+__global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ ptr,
+                                              const uint32_t *__restrict__ idx,
+                                              const float *__restrict__ val, uint32_t n,
+                                              const double *__restrict__ u, double *__restrict__ y)
+{
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (r >= n) return;
+    const uint32_t b = ptr[r], e = ptr[r + 1];
+    double a = 0.0;
+    for (uint32_t k = b + lane; k < e; k += 64) a += (double)val[k] * u[idx[k]];
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    if (lane == 0) y[r] = a;
+}
+
+// X (rc vectors of pitch ld) -> Xt (n x RP, vector index fastest; zeros past rc)
+__global__ void k_interleave(const double *__restrict__ X, uint32_t ld, uint32_t n, uint32_t rc,
+                             uint32_t RP, double *__restrict__ Xt)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n * RP) return;
+    const uint32_t c = (uint32_t)(i / RP), r = (uint32_t)(i - (uint64_t)c * RP);
+    Xt[i] = r < rc ? X[(size_t)r * ld + c] : 0.0;
+}
+
+// Y[r][i] = sum_k val_k Xt[idx_k][r] for RP vectors: one wave per row, lane
+// (s, r) = (lane / RP, lane % RP) takes entries s, s + 64/RP, ... so one
+// gather of Xt reads RP consecutive doubles.
+template <int RP>
+__global__ __launch_bounds__(256) void k_spmm(const uint32_t *__restrict__ ptr,
+                                              const uint32_t *__restrict__ idx,
+                                              const float *__restrict__ val, uint32_t n,
+                                              const double *__restrict__ Xt, uint32_t rc,
+                                              uint32_t ld, double *__restrict__ Y)
+{
+    constexpr uint32_t S = 64 / RP;
+    const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (r >= n) return;
+    const uint32_t v = lane % RP, slot = lane / RP;
+    const uint32_t b = ptr[r], e = ptr[r + 1];
+    double a = 0.0;
+    for (uint32_t k = b + slot; k < e; k += S) a += (double)val[k] * Xt[(size_t)idx[k] * RP + v];
+    for (int off = 32; off >= RP; off >>= 1) a += __shfl_down(a, off, 64);
+    if (slot == 0 && v < rc) Y[(size_t)v * ld + r] = a;
+}
+
+template <typename K, typename V>
+size_t sort_temp_bytes(uint64_t n, uint32_t end_bit)
+{
+    size_t b = 0;
+    rocprim::double_buffer<K> kb(nullptr, nullptr);
+    rocprim::double_buffer<V> vb(nullptr, nullptr);
+    IRTT_HIP(rocprim::radix_sort_pairs(nullptr, b, kb, vb, (size_t)n, 0, end_bit));
+    return b;
+}
+
+}  // namespace
+
+// Hash table for the photons of one build: sized to 5/4 of (photons + bins)
+// rounded up to a power of two, an upper bound of the distinct keys, so the
+// load factor stays below 0.8 even if every photon reached a new pair.
+void sparse_table_begin(Scene &s, uint64_t photons, PhotonSink &sink)
+{
+    const uint32_t nb = s.nb;
+    uint32_t bits = 0;
+    while ((1ull << bits) < nb) bits++;
+    const uint64_t keys = photons + nb;
+    const uint64_t want = keys + keys / 4;
+    uint32_t lg = 10;
+    while (lg < 63 && (1ull << lg) < want) lg++;
+    if (lg > 36)
+        throw Error(IRTT_ERR_LIMIT, "sparse transport: " + std::to_string(photons) +
+                                        " photons need a hash table above 2^36 slots");
+    hipStream_t stm = s.stream;
+    s.sp.reset();
+    const uint64_t cap = 1ull << lg;
+    s.sp.tab.alloc(cap);
+    s.sp.flag.alloc(1);
+    IRTT_HIP(hipMemsetAsync(s.sp.flag.p, 0, sizeof(uint32_t), stm));
+    k_tab_clear<<<(uint32_t)std::min<uint64_t>(cdiv(cap, 256), 65536), 256, 0, stm>>>(s.sp.tab.p,
+                                                                                     cap);
+    k_tab_diag<<<cdiv(nb, 256), 256, 0, stm>>>(s.sp.tab.p, lg, bits, nb, s.sp.flag.p);
+    IRTT_HIP(hipGetLastError());
+    sink.acc = nullptr;
+    sink.tab = s.sp.tab.p;
+    sink.lg = lg;
+    sink.key_bits = bits;
+    sink.overflow = s.sp.flag.p;
+}
+
+// Table -> compressed rows and columns.  Needs s.d_diag, s.quantum and the
+// zeroed s.d_colacc; fills s.d_tdiag.
+void sparse_table_finish(Scene &s, const PhotonSink &sink)
+{
+    const uint32_t nb = s.nb, bits = sink.key_bits;
+    const uint64_t cap = 1ull << sink.lg;
+    hipStream_t stm = s.stream;
+    DBuf<unsigned long long> cnt;
+    cnt.alloc(2);
+    IRTT_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(unsigned long long), stm));
+    const uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(cap, 256), 65536);
+    k_tab_count<<<g, 256, 0, stm>>>(s.sp.tab.p, cap, cnt.p);
+    IRTT_HIP(hipGetLastError());
+    unsigned long long nnz = 0;
+    uint32_t overflow = 0;
+    IRTT_HIP(hipMemcpyAsync(&nnz, cnt.p, sizeof(nnz), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(&overflow, s.sp.flag.p, sizeof(overflow), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    if (overflow) throw Error(IRTT_ERR_LIMIT, "sparse transport: hash table overflow");
+    if (nnz >= 0xffffffffull)
+        throw Error(IRTT_ERR_LIMIT, "sparse transport: " + std::to_string(nnz) +
+                                        " entries exceed 32-bit offsets");
+    const uint32_t end_bit = std::max<uint32_t>(1, 2 * bits);
+
+    DBuf<unsigned long long> ka, kb, va, vb;
+    ka.alloc(nnz);
+    kb.alloc(nnz);
+    va.alloc(nnz);
+    vb.alloc(nnz);
+    k_tab_compact<<<g, 256, 0, stm>>>(s.sp.tab.p, cap, bits, ka.p, va.p, cnt.p + 1, s.d_colacc.p);
+    IRTT_HIP(hipGetLastError());
+    s.sp.tab.release();
+
+    const size_t t1 = sort_temp_bytes<unsigned long long, unsigned long long>(nnz, end_bit);
+    const size_t t2 = sort_temp_bytes<unsigned long long, uint32_t>(nnz, end_bit);
+    DBuf<uint8_t> tmp;
+    tmp.alloc(std::max<size_t>(std::max(t1, t2), 1));
+    size_t tb = t1;
+    rocprim::double_buffer<unsigned long long> kd(ka.p, kb.p);
+    rocprim::double_buffer<unsigned long long> vd(va.p, vb.p);
+    IRTT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kd, vd, (size_t)nnz, 0, end_bit, stm));
+
+    Sparse &p = s.sp;
+    p.rowptr.alloc((size_t)nb + 1);
+    p.col.alloc(nnz);
+    p.val.alloc(nnz);
+    s.d_tdiag.alloc(nb);
+    const uint32_t ge = cdiv(nnz, 256);
+    k_csr_fill<<<ge, 256, 0, stm>>>(kd.current(), vd.current(), nnz, bits, nb, s.d_diag.p,
+                                    s.quantum, p.col.p, p.val.p, p.rowptr.p, s.d_tdiag.p);
+    IRTT_HIP(hipGetLastError());
+
+    // compressed columns: the same entries keyed (column, row)
+    unsigned long long *k2a = kd.current() == ka.p ? kb.p : ka.p;
+    unsigned long long *k2b = kd.current();
+    va.release();
+    vb.release();
+    DBuf<uint32_t> wa, wb;
+    wa.alloc(nnz);
+    wb.alloc(nnz);
+    k_csc_keys<<<ge, 256, 0, stm>>>(kd.current(), p.val.p, nnz, bits, k2a, wa.p);
+    IRTT_HIP(hipGetLastError());
+    // k2b (the row-sorted keys) is no longer needed once k_csc_keys has read it
+    rocprim::double_buffer<unsigned long long> kd2(k2a, k2b);
+    rocprim::double_buffer<uint32_t> wd(wa.p, wb.p);
+    tb = t2;
+    IRTT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kd2, wd, (size_t)nnz, 0, end_bit, stm));
+    p.colptr.alloc((size_t)nb + 1);
+    p.row.alloc(nnz);
+    p.cval.alloc(nnz);
+    k_csc_fill<<<ge, 256, 0, stm>>>(kd2.current(), wd.current(), nnz, bits, nb, p.row.p,
+                                    p.cval.p, p.colptr.p);
+    IRTT_HIP(hipGetLastError());
+    p.flag.release();
+    p.nnz = nnz;
+    p.valid = true;
+}
+
+void sp_apply(Scene &s, const double *w, double *y, bool transposed)
+{
+    const Sparse &p = s.sp;
+    if (!p.valid) throw Error(IRTT_ERR_STATE, "no sparse operator");
+    k_spmv<<<cdiv(s.nb, 4), 256, 0, s.stream>>>(transposed ? p.colptr.p : p.rowptr.p,
+                                                 transposed ? p.row.p : p.col.p,
+                                                 transposed ? p.cval.p : p.val.p, s.nb, w, y);
+    IRTT_HIP(hipGetLastError());
+}
+
+// Y = T X (or T^T X) for R vectors of pitch ld, up to 64 per pass.
+void sp_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool transposed)
+{
+    Sparse &p = s.sp;
+    if (!p.valid) throw Error(IRTT_ERR_STATE, "no sparse operator");
+    const uint32_t nb = s.nb, ld = s.ld;
+    hipStream_t st = s.stream;
+    const uint32_t *ptr = transposed ? p.colptr.p : p.rowptr.p;
+    const uint32_t *idx = transposed ? p.row.p : p.col.p;
+    const float *val = transposed ? p.cval.p : p.val.p;
+    p.d_xt.alloc((size_t)64 * nb);
+    for (uint32_t r0 = 0; r0 < R; r0 += 64) {
+        const uint32_t rc = std::min<uint32_t>(64, R - r0);
+        uint32_t RP = 1;
+        while (RP < rc) RP <<= 1;
+        const double *Xc = X + (size_t)r0 * ld;
+        double *Yc = Y + (size_t)r0 * ld;
+        k_interleave<<<cdiv((uint64_t)nb * RP, 256), 256, 0, st>>>(Xc, ld, nb, rc, RP, p.d_xt.p);
+        const uint32_t g = cdiv(nb, 4);
+        switch (RP) {
+        case 1: k_spmm<1><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        case 2: k_spmm<2><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        case 4: k_spmm<4><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        case 8: k_spmm<8><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        case 16: k_spmm<16><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        case 32: k_spmm<32><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        default: k_spmm<64><<<g, 256, 0, st>>>(ptr, idx, val, nb, p.d_xt.p, rc, ld, Yc); break;
+        }
+        IRTT_HIP(hipGetLastError());
+    }
+}
+
+void sp_download(Scene &s, uint32_t *rowptr, uint32_t *col, float *val)
+{
+    const Sparse &p = s.sp;
+    hipStream_t st = s.stream;
+    IRTT_HIP(hipMemcpyAsync(rowptr, p.rowptr.p, ((size_t)s.nb + 1) * 4, hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(col, p.col.p, p.nnz * 4, hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(val, p.val.p, p.nnz * 4, hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipStreamSynchronize(st));
+}
+
+}  // namespace irtt

@@ -8,7 +8,9 @@
 // per-lane stack walk over 64-byte two-child nodes with the stack in LDS.
 // Photon hits are accumulated as exact fixed-point integers (one 64-bit
 // integer add per absorbed photon, weight q_t of its source triangle), so the
-// operator is bit-reproducible and independent of launch order.
+// operator is bit-reproducible and independent of launch order.  The dense
+// layout adds into an n_bins x ld array; the sparse layout into a hash table
+// keyed by (absorbing bin, emitting bin) (sparse_table.hpp).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -17,6 +19,7 @@
 
 #include "device_math.hpp"
 #include "irtt_internal.hpp"
+#include "sparse_table.hpp"
 #include "traverse.hpp"
 
 namespace irtt {
@@ -58,7 +61,7 @@ struct PhotonArgs {
     uint32_t threads_per_tri, rays_per_thread, per_tri, batch_seed, max_depth;
     float tiny_dist;
     uint64_t photon_begin, photon_end;
-    unsigned long long *acc;
+    PhotonSink sink;
     unsigned long long *stats;
 };
 
@@ -167,7 +170,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
                     // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
                     int hc = max_corner(hb0, h.b1, h.b2);
                     uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
-                    atomicAdd(a.acc + (uint64_t)ab * a.ld + src, qt);
+                    if (a.sink.tab)
+                        tab_add(a.sink.tab, a.sink.lg, tab_key(ab, src, a.sink.key_bits), qt,
+                                a.sink.overflow);
+                    else
+                        atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
                     if (absorbed) cnt[S_ABS]++;
                     else cnt[S_FORCED]++;
                 } else {
@@ -283,6 +290,9 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     const uint32_t per = (uint32_t)per64;
     const uint32_t max_depth =
         (p.max_depth == 0 || p.max_depth > kMaxDepthDefault) ? kMaxDepthDefault : p.max_depth;
+    if (p.layout != IRTT_LAYOUT_DENSE && p.layout != IRTT_LAYOUT_SPARSE)
+        throw Error(IRTT_ERR_ARG, "unknown transport layout");
+    const bool sparse = p.layout == IRTT_LAYOUT_SPARSE;
     if (!s.bvh.valid) bvh_build(s);
     const uint32_t nb = s.nb;
     const uint32_t ld = s.ld;
@@ -290,6 +300,10 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     if (nb == 0) throw Error(IRTT_ERR_STATE, "empty scene");
     op_reset(s);
     s.has_transport = false;
+    if (sparse) {
+        s.d_acc.release();
+        s.d_T.release();
+    }
 
     // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
     // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s
@@ -316,15 +330,23 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     hipStream_t stm = s.stream;
     s.d_q.alloc(s.nt);
     IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice, stm));
-    s.d_acc.alloc(nb2);
-    s.d_T.alloc(nb2);
+    if (!sparse) {
+        s.d_acc.alloc(nb2);
+        s.d_T.alloc(nb2);
+    }
     s.d_diag.alloc(nb);
     s.d_colacc.alloc(nb);
     s.d_stats.alloc(16);
     std::vector<double> diag(nb);
     for (uint32_t i = 0; i < nb; i++) diag[i] = kSigmaD * s.bin_eps_area[i];
     IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice, stm));
-    IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+    PhotonSink sink;
+    if (sparse) {
+        sparse_table_begin(s, total, sink);
+    } else {
+        IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+        sink.acc = s.d_acc.p;
+    }
     IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
     IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
 
@@ -350,7 +372,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     a.tiny_dist = p.tiny_dist;
     a.photon_begin = 0;
     a.photon_end = total;
-    a.acc = s.d_acc.p;
+    a.sink = sink;
     a.stats = s.d_stats.p;
 
     int dev = 0, ncu = 256;
@@ -378,11 +400,15 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
 
     IRTT_HIP(hipEventRecord(s.ev0, stm));
-    dim3 fg(cdiv(ld, 256), cdiv(nb, kFinRows));
-    s.d_tdiag.alloc(nb);
-    k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p,
-                                    s.d_tdiag.p);
-    IRTT_HIP(hipGetLastError());
+    if (sparse) {
+        sparse_table_finish(s, sink);
+    } else {
+        dim3 fg(cdiv(ld, 256), cdiv(nb, kFinRows));
+        s.d_tdiag.alloc(nb);
+        k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p,
+                                        s.d_tdiag.p);
+        IRTT_HIP(hipGetLastError());
+    }
     IRTT_HIP(hipEventRecord(s.ev1, stm));
     std::vector<unsigned long long> colacc(nb), hstats(16);
     std::vector<float> dcol(nb);
@@ -411,6 +437,10 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     st.tri_tests = hstats[S_COUNT + 1];
     st.quantum = Q;
     st.bvh_ms = s.bvh.build_ms;
+    st.layout = p.layout;
+    st.nnz = sparse ? s.sp.nnz : (uint64_t)nb * nb;
+    s.layout = p.layout;
+    s.op_kind = sparse ? IRTT_OP_SPARSE : IRTT_OP_DENSE;
     s.has_transport = true;
 }
 

@@ -54,7 +54,8 @@ class SceneInfo(ctypes.Structure):
 class TransportParams(ctypes.Structure):
     _fields_ = [("threads_per_tri", ctypes.c_uint32), ("rays_per_thread", ctypes.c_uint32),
                 ("batch_seed", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
-                ("tiny_dist", ctypes.c_float), ("reserved", ctypes.c_uint32 * 3)]
+                ("tiny_dist", ctypes.c_float), ("layout", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 2)]
 
 
 class TransportStats(ctypes.Structure):
@@ -63,8 +64,8 @@ class TransportStats(ctypes.Structure):
                  "degenerate", "forced_absorb")] + [
         ("quantum", ctypes.c_double), ("trace_ms", ctypes.c_float),
         ("finalize_ms", ctypes.c_float), ("bvh_ms", ctypes.c_float),
-        ("reserved0", ctypes.c_uint32), ("node_visits", ctypes.c_uint64),
-        ("tri_tests", ctypes.c_uint64)]
+        ("layout", ctypes.c_uint32), ("node_visits", ctypes.c_uint64),
+        ("tri_tests", ctypes.c_uint64), ("nnz", ctypes.c_uint64)]
 
     def as_dict(self):
         return {n: (getattr(self, n)) for n, _ in self._fields_}
@@ -136,6 +137,7 @@ def _declare(L):
                                        ctypes.POINTER(TransportStats)]
     L.irtt_get_transport.argtypes = [_vp, _vp]
     L.irtt_get_transport_fixed.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_double)]
+    L.irtt_get_transport_sparse.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), _vp, _vp, _vp]
     L.irtt_get_column_sums.argtypes = [_vp, _vp]
     L.irtt_solve.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.POINTER(SolveStats)]
     L.irtt_get_temperatures.argtypes = [_vp, _vp]
@@ -173,6 +175,9 @@ def _declare(L):
 OP_DENSE = 0
 OP_ACA = 1
 OP_DENSE_F16 = 2
+OP_SPARSE = 3
+LAYOUT_DENSE = 0
+LAYOUT_SPARSE = 1
 ACA_FULL = 0
 ACA_PARTIAL = 1
 ACA_PLUS = 2
@@ -271,13 +276,14 @@ def make_materials(nt, rho_d=0.0, rho_s=0.0, thickness=1.0, density=1.0, heat_ca
 
 
 def transport_params(threads_per_tri=32768, rays_per_thread=8, batch_seed=1234, max_depth=512,
-                     tiny_dist=1e-3):
+                     tiny_dist=1e-3, layout=0):
     p = TransportParams()
     p.threads_per_tri = threads_per_tri
     p.rays_per_thread = rays_per_thread
     p.batch_seed = batch_seed
     p.max_depth = max_depth
     p.tiny_dist = tiny_dist
+    p.layout = layout
     return p
 
 
@@ -414,6 +420,17 @@ class Scene:
         _check(lib().irtt_get_transport_fixed(self._h, _ptr(acc), ctypes.byref(q)))
         return acc, q.value
 
+    def transport_sparse(self):
+        """The operator in compressed rows: (rowptr, col, val) (irtt_get_transport_sparse)."""
+        nnz = ctypes.c_uint64()
+        _check(lib().irtt_get_transport_sparse(self._h, ctypes.byref(nnz), None, None, None))
+        rp = np.empty(self.n_bins + 1, np.uint32)
+        col = np.empty(nnz.value, np.uint32)
+        val = np.empty(nnz.value, np.float32)
+        _check(lib().irtt_get_transport_sparse(self._h, ctypes.byref(nnz), _ptr(rp), _ptr(col),
+                                               _ptr(val)))
+        return rp, col, val
+
     def column_sums(self):
         c = np.empty(self.n_bins)
         _check(lib().irtt_get_column_sums(self._h, _ptr(c)))
@@ -427,7 +444,8 @@ class Scene:
         _check(lib().irtt_scene_set_source(self._h, _ptr(src)))
 
     def select_operator(self, op):
-        """OP_DENSE, OP_DENSE_F16 or OP_ACA for the solves that follow."""
+        """OP_DENSE, OP_DENSE_F16 or OP_ACA (dense layout), OP_SPARSE (sparse
+        layout) for the solves that follow."""
         _check(lib().irtt_select_operator(self._h, op))
 
     def simulate(self, steps, params=None, **kw):
