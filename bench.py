@@ -120,12 +120,80 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
                          "newton_iters": fst["newton_iters"],
                          "linear_iters": fst["linear_iters"] + ast["linear_iters"],
                          "residual_sq_max": fst["residual_sq"]}
+    out["x_fp32"] = xs["fp32"]
     out["max_dT_fp16_vs_fp32_K"] = float(np.abs(xs["fp16"] - xs["fp32"]).max())
     out["max_dT_fp16_batch_vs_fp32_K"] = float(np.abs(X[0] - xs["fp32"]).max())
     out["speedup_fp16"] = round(out["fp16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
     out["speedup_fp16_batch"] = round(out["fp16_batch"]["solves_per_s"] /
                                       out["fp32"]["solves_per_s"], 3)
     out["unit"] = "forward steady + adjoint solves per second (city block operator resident)"
+    return out
+
+
+def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
+    """The same photon transport stored in the sparse layout (device hash
+    table -> compressed rows and columns, entries bit-identical to the dense
+    ones): build time, stored entries, and forward steady + adjoint solves/s
+    on it, single and R designs in lockstep, with max |dT| against the
+    dense FP32-form solution x_ref."""
+    import ctypes
+    import numpy as np
+    from irtt import scenes
+    q = irtt.TransportParams()
+    ctypes.pointer(q)[0] = p
+    q.layout = irtt.LAYOUT_SPARSE
+    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14,
+                            lin_rtol=1e-10)
+    out = {}
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_bvh()
+        s.build_transport(q)
+        walls, stats = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            stats.append(s.build_transport(q))
+            walls.append(time.perf_counter() - t0)
+        st = stats[-1]
+        n = s.n_bins
+        out["build"] = {"ms": round(min(walls) * 1e3, 2),
+                        "photon_kernel_ms": round(min(x["trace_ms"] for x in stats), 3),
+                        "rows_columns_ms": round(min(x["finalize_ms"] for x in stats), 3),
+                        "mrays_per_s": round(st["rays"] / (min(walls) * 1e6), 1),
+                        "nnz": int(st["nnz"]), "fill": round(st["nnz"] / float(n) ** 2, 5),
+                        "operator_MB": round(st["nnz"] * 16 / 1e6, 1),
+                        "dense_operator_MB": round(n * ((n + 63) // 64 * 64) * 4 / 1e6, 1)}
+        _, ea, _ = s.bin_data()
+        times = []
+        for _ in range(reps):
+            s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+            t0 = time.perf_counter()
+            fst = s.solve(sp)
+            x = s.temperatures()
+            lam, ast = s.adjoint_solve(ea * (x - x.mean()), params=sp)
+            times.append(time.perf_counter() - t0)
+        t = min(times)
+        out["single"] = {"solves_per_s": round(1.0 / t, 2), "ms": round(t * 1e3, 2),
+                         "linear_iters": fst["linear_iters"] + ast["linear_iters"],
+                         "max_dT_vs_dense_fp32_K": float(np.abs(x - x_ref).max())}
+        srcs = [scenes.bin_sources(sd, sd.heat_flux)]
+        for k in range(1, batch):
+            srcs.append(scenes.bin_sources(sd, scenes.sun_flux(sd, 45.0, 360.0 * k / batch)))
+        srcs = np.stack(srcs)
+        s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+        times = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            X, fst = s.solve_batch(srcs, params=sp)
+            rhs = ea[None, :] * (X - X.mean(axis=1, keepdims=True))
+            lam, ast = s.adjoint_solve_batch(X, rhs, params=sp)
+            times.append(time.perf_counter() - t0)
+        t = min(times)
+        out["batch"] = {"solves_per_s": round(batch / t, 2), "ms": round(t * 1e3, 2),
+                        "designs": batch,
+                        "linear_iters": fst["linear_iters"] + ast["linear_iters"],
+                        "max_dT_vs_dense_fp32_K": float(np.abs(X[0] - x_ref).max())}
+    out["unit"] = ("sparse layout of the city block operator: build, and forward steady + "
+                   "adjoint solves per second (FP32 entries, FP64 sums)")
     return out
 
 
@@ -173,11 +241,13 @@ def transient(irtt, s, sd, steps=288, dt=300.0):
     return out
 
 
-def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=50000, seed=7):
+def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=200000, seed=7):
     """Poisson stream of design variants through irtt_serve_*: city blocks of
     log-uniform size (tmin..tmax triangles) with random heights and seeds,
     each a forward steady solve plus adjoint over up to 64 building-height
-    parameters.  Returns latency percentiles and throughput."""
+    parameters, operators in the sparse layout (a dense operator of the
+    largest designs would take 40-120 GB).  Returns latency percentiles and
+    throughput."""
     import numpy as np
     from irtt import scenes
     rng = np.random.default_rng(seed)
@@ -187,7 +257,8 @@ def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=50000, seed=7):
         g = int(np.clip(round(np.sqrt(t / 800.0)), 2, 8))
         sd = scenes.city_block(seed=1000 + k, nx=g, ny=g, target_tris=int(t))
         designs.append((sd, scenes.height_parametrization(sd, 64)))
-    tp = irtt.transport_params(threads_per_tri=32, rays_per_thread=8, batch_seed=1234)
+    tp = irtt.transport_params(threads_per_tri=32, rays_per_thread=8, batch_seed=1234,
+                               layout=irtt.LAYOUT_SPARSE)
     sp = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-8)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
     with irtt.Server(slots=slots) as sv:
@@ -214,8 +285,9 @@ def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=50000, seed=7):
             "mean_ms": round(float(lat.mean()), 1),
             "mean_queue_ms": round(float(np.mean([st["queued_ms"] for st in stats])), 1),
             "designs_per_s": round(n_req / wall, 2),
-            "work": "per request: LBVH, 256 photons per triangle, steady solve, adjoint solve, "
-                    "adjoint trace, dL/dp over up to 64 building heights"}
+            "work": "per request: LBVH, 256 photons per triangle into the sparse layout, "
+                    "steady solve, adjoint solve, adjoint trace, dL/dp over up to 64 "
+                    "building heights"}
 
 
 def main():
@@ -229,6 +301,7 @@ def main():
     ap.add_argument("--rays-per-thread", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
+    ap.add_argument("--no-sparse", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
@@ -267,7 +340,12 @@ def main():
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
     solves = solve_rates(irtt, s, sd)
+    x_fp32 = solves.pop("x_fp32")
     trans = None if args.no_transient else transient(irtt, s, sd)
+    sparse = None
+    if not args.no_sparse:
+        s.close()  # the dense operator (16 GB with its accumulators) is no longer needed
+        sparse = sparse_layout(irtt, sd, p, x_fp32)
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -306,6 +384,8 @@ def main():
     }
     if trans is not None:
         line["transient"] = trans
+    if sparse is not None:
+        line["sparse"] = sparse
     if not args.no_serving:
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:

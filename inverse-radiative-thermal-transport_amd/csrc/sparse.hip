@@ -46,45 +46,89 @@ __global__ void k_tab_diag(TabSlot *tab, uint32_t lg, uint32_t bits, uint32_t nb
     if (i < nb) tab_add(tab, lg, tab_key(i, i, bits), 0ull, overflow);
 }
 
-// occupied slots, one atomic per wave
-__global__ void k_tab_count(const TabSlot *__restrict__ tab, uint64_t n,
-                            unsigned long long *count)
+// The table passes give each 256-thread block one chunk of kChunk slots
+// (thread t reads slots t, t + 256, ...): the occupied slots of a chunk are
+// counted, the chunk counts scanned, and each chunk written at its offset,
+// so no global atomics are needed.
+constexpr uint32_t kChunkIters = 16;
+constexpr uint32_t kChunk = 256 * kChunkIters;
+
+__global__ __launch_bounds__(256) void k_tab_count(const TabSlot *__restrict__ tab, uint64_t n,
+                                                   uint32_t *__restrict__ ccount)
 {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    __shared__ uint32_t ws[4];
+    const uint64_t base = (uint64_t)blockIdx.x * kChunk + threadIdx.x;
     uint32_t c = 0;
-    for (; i < n; i += stride) c += tab[i].key != kTabEmpty;
+    for (uint32_t j = 0; j < kChunkIters; j++) {
+        const uint64_t i = base + (uint64_t)j * 256;
+        if (i < n && tab[i].key != kTabEmpty) c++;
+    }
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(count, (unsigned long long)c);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) ccount[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-// occupied slots -> (key, value) arrays in arbitrary order (sorted next);
-// column sums of the accumulators on the way
-__global__ void k_tab_compact(const TabSlot *__restrict__ tab, uint64_t n, uint32_t bits,
-                              unsigned long long *__restrict__ keys,
-                              unsigned long long *__restrict__ vals,
-                              unsigned long long *cursor, unsigned long long *colacc)
+// exclusive scan of m chunk counts into 64-bit offsets, total in off[m]; one
+// block of 1024 threads
+__global__ __launch_bounds__(1024) void k_scan_chunks(const uint32_t *__restrict__ c, uint32_t m,
+                                                      unsigned long long *__restrict__ off)
 {
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint32_t lane = threadIdx.x & 63;
-    const unsigned long long cmask = (1ull << bits) - 1ull;
-    // every lane of a wave runs the same number of rounds (ballots need all)
-    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n;
-         base += stride) {
-        const uint64_t i = base + lane;
+    __shared__ unsigned long long sums[1024];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t per = (m + 1023u) / 1024u;
+    const uint32_t b = min(tid * per, m), e = min(b + per, m);
+    unsigned long long s = 0;
+    for (uint32_t i = b; i < e; i++) s += c[i];
+    sums[tid] = s;
+    __syncthreads();
+    for (uint32_t o = 1; o < 1024; o <<= 1) {
+        const unsigned long long v = tid >= o ? sums[tid - o] : 0ull;
+        __syncthreads();
+        sums[tid] += v;
+        __syncthreads();
+    }
+    unsigned long long run = sums[tid] - s;
+    for (uint32_t i = b; i < e; i++) {
+        off[i] = run;
+        run += c[i];
+    }
+    if (tid == 1023) off[m] = sums[1023];
+}
+
+// occupied slots of each chunk -> (key, value) at the chunk's offset, in
+// slot order (the slot of a key depends on the insertion race, so the
+// entries are sorted next)
+__global__ __launch_bounds__(256) void k_tab_compact(const TabSlot *__restrict__ tab, uint64_t n,
+                                                     const unsigned long long *__restrict__ coff,
+                                                     unsigned long long *__restrict__ keys,
+                                                     unsigned long long *__restrict__ vals)
+{
+    __shared__ uint32_t wc[4];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint64_t base = (uint64_t)blockIdx.x * kChunk + threadIdx.x;
+    unsigned long long run = coff[blockIdx.x];
+    for (uint32_t j = 0; j < kChunkIters; j++) {
+        const uint64_t i = base + (uint64_t)j * 256;
         TabSlot t{kTabEmpty, 0ull};
         if (i < n) t = tab[i];
         const bool occ = t.key != kTabEmpty;
         const unsigned long long m = __ballot(occ);
-        unsigned long long pos = 0;
-        if (lane == 0 && m) pos = atomicAdd(cursor, (unsigned long long)__popcll(m));
-        pos = __shfl(pos, 0, 64);
+        if (lane == 0) wc[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (uint32_t w = 0; w < 4; w++) {
+            const uint32_t x = wc[w];
+            before += w < wave ? x : 0u;
+            total += x;
+        }
         if (occ) {
-            const unsigned long long o = pos + __popcll(m & ((1ull << lane) - 1ull));
+            const unsigned long long o = run + before + __popcll(m & ((1ull << lane) - 1ull));
             keys[o] = t.key;
             vals[o] = t.val;
-            if (t.val) atomicAdd(colacc + (t.key & cmask), t.val);
         }
+        run += total;
+        __syncthreads();
     }
 }
 
@@ -110,30 +154,48 @@ __global__ void k_csr_fill(const unsigned long long *__restrict__ keys,
 }
 
 __global__ void k_csc_keys(const unsigned long long *__restrict__ keys,
-                           const float *__restrict__ val, uint64_t nnz, uint32_t bits,
-                           unsigned long long *__restrict__ k2, uint32_t *__restrict__ v2)
+                           const unsigned long long *__restrict__ acc, uint64_t nnz,
+                           uint32_t bits, unsigned long long *__restrict__ k2,
+                           unsigned long long *__restrict__ a2)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nnz) return;
     const unsigned long long key = keys[i];
     const unsigned long long cmask = (1ull << bits) - 1ull;
     k2[i] = ((key & cmask) << bits) | (key >> bits);
-    v2[i] = __float_as_uint(val[i]);
+    a2[i] = acc[i];
 }
 
+// sorted (column, row) keys -> compressed columns, entries converted as in
+// k_csr_fill (hence equal to the row form's)
 __global__ void k_csc_fill(const unsigned long long *__restrict__ k2,
-                           const uint32_t *__restrict__ v2, uint64_t nnz, uint32_t bits,
-                           uint32_t nb, uint32_t *__restrict__ row, float *__restrict__ cval,
+                           const unsigned long long *__restrict__ a2, uint64_t nnz, uint32_t bits,
+                           uint32_t nb, const double *__restrict__ diag, double Q,
+                           uint32_t *__restrict__ row, float *__restrict__ cval,
                            uint32_t *__restrict__ colptr)
 {
     uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nnz) return;
     const unsigned long long key = k2[i];
-    const uint32_t c = (uint32_t)(key >> bits);
-    row[i] = (uint32_t)(key & ((1ull << bits) - 1ull));
-    cval[i] = __uint_as_float(v2[i]);
+    const uint32_t c = (uint32_t)(key >> bits), r = (uint32_t)(key & ((1ull << bits) - 1ull));
+    row[i] = r;
+    cval[i] = (float)((r == c ? diag[c] : 0.0) - (double)a2[i] * Q);
     if (i == 0 || (uint32_t)(k2[i - 1] >> bits) != c) colptr[c] = (uint32_t)i;
     if (i == nnz - 1) colptr[nb] = (uint32_t)nnz;
+}
+
+// integer column sums of the accumulators, one wave per column
+__global__ __launch_bounds__(256) void k_col_sums(const uint32_t *__restrict__ colptr,
+                                                  const unsigned long long *__restrict__ a2,
+                                                  uint32_t nb, unsigned long long *__restrict__ colacc)
+{
+    const uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (c >= nb) return;
+    unsigned long long s = 0;
+    for (uint32_t k = colptr[c] + lane; k < colptr[c + 1]; k += 64) s += a2[k];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) colacc[c] = s;
 }
 
 // y_i = sum_k val_k u_{idx_k} over the entries of row i (or column i of the
@@ -230,22 +292,26 @@ void sparse_table_begin(Scene &s, uint64_t photons, PhotonSink &sink)
     sink.overflow = s.sp.flag.p;
 }
 
-// Table -> compressed rows and columns.  Needs s.d_diag, s.quantum and the
-// zeroed s.d_colacc; fills s.d_tdiag.
+// Table -> compressed rows and columns.  Needs s.d_diag and s.quantum;
+// fills s.d_tdiag and s.d_colacc (exact integer column sums).
 void sparse_table_finish(Scene &s, const PhotonSink &sink)
 {
     const uint32_t nb = s.nb, bits = sink.key_bits;
     const uint64_t cap = 1ull << sink.lg;
     hipStream_t stm = s.stream;
-    DBuf<unsigned long long> cnt;
-    cnt.alloc(2);
-    IRTT_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(unsigned long long), stm));
-    const uint32_t g = (uint32_t)std::min<uint64_t>(cdiv(cap, 256), 65536);
-    k_tab_count<<<g, 256, 0, stm>>>(s.sp.tab.p, cap, cnt.p);
+    const uint64_t nchunk64 = (cap + kChunk - 1) / kChunk;
+    if (nchunk64 > 0x7fffffffull) throw Error(IRTT_ERR_LIMIT, "sparse transport: table too large");
+    const uint32_t nchunk = (uint32_t)nchunk64;
+    DBuf<uint32_t> ccount;
+    DBuf<unsigned long long> coff;
+    ccount.alloc(nchunk);
+    coff.alloc((size_t)nchunk + 1);
+    k_tab_count<<<nchunk, 256, 0, stm>>>(s.sp.tab.p, cap, ccount.p);
+    k_scan_chunks<<<1, 1024, 0, stm>>>(ccount.p, nchunk, coff.p);
     IRTT_HIP(hipGetLastError());
     unsigned long long nnz = 0;
     uint32_t overflow = 0;
-    IRTT_HIP(hipMemcpyAsync(&nnz, cnt.p, sizeof(nnz), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipMemcpyAsync(&nnz, coff.p + nchunk, sizeof(nnz), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(&overflow, s.sp.flag.p, sizeof(overflow), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     if (overflow) throw Error(IRTT_ERR_LIMIT, "sparse transport: hash table overflow");
@@ -254,19 +320,20 @@ void sparse_table_finish(Scene &s, const PhotonSink &sink)
                                         " entries exceed 32-bit offsets");
     const uint32_t end_bit = std::max<uint32_t>(1, 2 * bits);
 
+    // ka/va: compacted, kb/vb: the sort's other halves; the column sort
+    // reuses all four
     DBuf<unsigned long long> ka, kb, va, vb;
     ka.alloc(nnz);
     kb.alloc(nnz);
     va.alloc(nnz);
     vb.alloc(nnz);
-    k_tab_compact<<<g, 256, 0, stm>>>(s.sp.tab.p, cap, bits, ka.p, va.p, cnt.p + 1, s.d_colacc.p);
+    k_tab_compact<<<nchunk, 256, 0, stm>>>(s.sp.tab.p, cap, coff.p, ka.p, va.p);
     IRTT_HIP(hipGetLastError());
     s.sp.tab.release();
 
     const size_t t1 = sort_temp_bytes<unsigned long long, unsigned long long>(nnz, end_bit);
-    const size_t t2 = sort_temp_bytes<unsigned long long, uint32_t>(nnz, end_bit);
     DBuf<uint8_t> tmp;
-    tmp.alloc(std::max<size_t>(std::max(t1, t2), 1));
+    tmp.alloc(std::max<size_t>(t1, 1));
     size_t tb = t1;
     rocprim::double_buffer<unsigned long long> kd(ka.p, kb.p);
     rocprim::double_buffer<unsigned long long> vd(va.p, vb.p);
@@ -282,26 +349,23 @@ void sparse_table_finish(Scene &s, const PhotonSink &sink)
                                     s.quantum, p.col.p, p.val.p, p.rowptr.p, s.d_tdiag.p);
     IRTT_HIP(hipGetLastError());
 
-    // compressed columns: the same entries keyed (column, row)
+    // compressed columns: the same entries keyed (column, row), written into
+    // the halves the row sort left free; the row-sorted halves are free once
+    // k_csc_keys has read them and serve as the second sort's other halves
     unsigned long long *k2a = kd.current() == ka.p ? kb.p : ka.p;
-    unsigned long long *k2b = kd.current();
-    va.release();
-    vb.release();
-    DBuf<uint32_t> wa, wb;
-    wa.alloc(nnz);
-    wb.alloc(nnz);
-    k_csc_keys<<<ge, 256, 0, stm>>>(kd.current(), p.val.p, nnz, bits, k2a, wa.p);
+    unsigned long long *a2a = vd.current() == va.p ? vb.p : va.p;
+    k_csc_keys<<<ge, 256, 0, stm>>>(kd.current(), vd.current(), nnz, bits, k2a, a2a);
     IRTT_HIP(hipGetLastError());
-    // k2b (the row-sorted keys) is no longer needed once k_csc_keys has read it
-    rocprim::double_buffer<unsigned long long> kd2(k2a, k2b);
-    rocprim::double_buffer<uint32_t> wd(wa.p, wb.p);
-    tb = t2;
-    IRTT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kd2, wd, (size_t)nnz, 0, end_bit, stm));
+    rocprim::double_buffer<unsigned long long> kd2(k2a, kd.current());
+    rocprim::double_buffer<unsigned long long> ad2(a2a, vd.current());
+    tb = t1;
+    IRTT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kd2, ad2, (size_t)nnz, 0, end_bit, stm));
     p.colptr.alloc((size_t)nb + 1);
     p.row.alloc(nnz);
     p.cval.alloc(nnz);
-    k_csc_fill<<<ge, 256, 0, stm>>>(kd2.current(), wd.current(), nnz, bits, nb, p.row.p,
-                                    p.cval.p, p.colptr.p);
+    k_csc_fill<<<ge, 256, 0, stm>>>(kd2.current(), ad2.current(), nnz, bits, nb, s.d_diag.p,
+                                    s.quantum, p.row.p, p.cval.p, p.colptr.p);
+    k_col_sums<<<cdiv(nb, 4), 256, 0, stm>>>(p.colptr.p, ad2.current(), nb, s.d_colacc.p);
     IRTT_HIP(hipGetLastError());
     p.flag.release();
     p.nnz = nnz;
