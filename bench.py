@@ -197,6 +197,49 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
     return out
 
 
+def facade(irtt, fd_params=8, h=1e-3, gd_steps=10, threads_per_tri=336, rays_per_thread=8):
+    """Procedural facade: 20 buildings, 401k triangles, 363k vertex bins,
+    256 balcony / fin extrusion depths; 336 x 8 = 2688 photons per triangle
+    (1.08e9 ~ 2^30) into the sparse layout; steady solve; design gradient of
+    the mean ground temperature (adjoint solve, adjoint trace, source term,
+    projection); central differences (h, common random numbers) on
+    `fd_params` depths; `gd_steps` steps of normalised gradient descent."""
+    import numpy as np
+    from irtt import scenes
+    from irtt.design import FacadeDesign
+    t0 = time.perf_counter()
+    sd = scenes.facade()
+    gen_s = time.perf_counter() - t0
+    tp = irtt.transport_params(threads_per_tri, rays_per_thread, batch_seed=1234,
+                               layout=irtt.LAYOUT_SPARSE)
+    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-12,
+                            lin_rtol=1e-10)
+    out = {"triangles": sd.nt, "bins": sd.nv, "params": int(sd.meta["n_params"]),
+           "photons": threads_per_tri * rays_per_thread * sd.nt,
+           "generate_s": round(gen_s, 2)}
+    with FacadeDesign(irtt, sd, tp, sp) as fz:
+        d0 = fz.depths.copy()
+        fz.forward(d0)
+        L0, x = fz.forward(d0)
+        fwd = dict(fz.timing)
+        dp = fz.gradient()
+        out.update({"objective_K": round(L0, 6), "forward": {k: (round(v, 2) if isinstance(
+            v, float) else v) for k, v in fwd.items()},
+            "gradient_ms": round(fz.timing["gradient_ms"], 1),
+            "dLdp_range": [float(dp.min()), float(dp.max())]})
+        ks = np.unique(np.linspace(0, sd.meta["n_params"] - 1, fd_params).astype(int))
+        rows = []
+        for k in ks:
+            fd = fz.central_difference(d0, int(k), h)
+            rows.append({"param": int(k), "adjoint": float(dp[k]), "fd": float(fd)})
+        out["fd_check"] = {"h_m": h, "rows": rows}
+        t1 = time.perf_counter()
+        hist, best = fz.descend(gd_steps)
+        out["descent"] = {"steps": gd_steps, "objective_K": [round(float(v), 5) for v in hist],
+                          "s_per_step": round((time.perf_counter() - t1) / max(gd_steps, 1), 2)}
+    return out
+
+
 def transient(irtt, s, sd, steps=288, dt=300.0):
     """The city block over 24 h: `steps` implicit-Euler steps of dt with the
     sun sweeping east to west (elevation 60 sin(pi (t - 6 h) / 12 h), no sun
@@ -302,6 +345,12 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
+    ap.add_argument("--facade", action="store_true",
+                    help="add the 400k-triangle facade design section (about a minute)")
+    ap.add_argument("--facade-fd", type=int, default=8)
+    ap.add_argument("--facade-gd", type=int, default=10)
+    ap.add_argument("--facade-only", action="store_true",
+                    help="run only the facade section and print it")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
     args = ap.parse_args()
@@ -314,6 +363,10 @@ def main():
     import irtt
     from irtt import scenes
 
+    if args.facade_only:
+        print(json.dumps({"facade": facade(irtt, fd_params=args.facade_fd,
+                                           gd_steps=args.facade_gd)}), flush=True)
+        return
     sd = scenes.city_block(seed=args.seed)
     s = irtt.Scene.from_data(sd, irtt.BIN_VERTEX)
     s.build_bvh()
@@ -386,6 +439,8 @@ def main():
         line["transient"] = trans
     if sparse is not None:
         line["sparse"] = sparse
+    if args.facade:
+        line["facade"] = facade(irtt, fd_params=args.facade_fd, gd_steps=args.facade_gd)
     if not args.no_serving:
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:

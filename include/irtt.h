@@ -344,6 +344,28 @@ int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nn
                           const uint32_t *vertex, const uint32_t *param, const float *dxdp,
                           uint32_t n_params, double *dp);
 
+/* grad_x (3 n_vertices, accumulated) += lambda^T d b / d vertex for the
+ * external power b assembled from the materials' heat_flux (flux times
+ * triangle area, a third to each corner bin in vertex mode): the geometry
+ * derivative of the source term, which the reference's fixed-temperature
+ * emitters carry inside T instead.  A source set per bin with
+ * irtt_scene_set_source is a fixed power and adds nothing.  Host-side. */
+int irtt_source_gradient(irtt_scene h, const double *lambda, double *grad_x);
+
+/* One design gradient at the current (solved) state: lambda from
+ * J^T lambda = dL/dx (irtt_adjoint_solve with p), the adjoint trace with the
+ * rows (s lambda, x^4) and, with an ambient temperature, (-s Ta^4, lambda)
+ * (s = 1 steady, dt for one implicit-Euler step), the source term
+ * (irtt_source_gradient with s lambda), then dp = projection through dxdp
+ * (irtt_project_gradient; dp overwritten).  grad_x (3 n_vertices) may be
+ * NULL; lambda (n_bins) may be NULL.  The reference's gradient step
+ * (ThermalRenderer::runAdjointSim + Parametrization::projectGradient,
+ * thermal_renderer.cpp:1028-1133, thermal_params.cpp:130-156) for one state. */
+int irtt_design_gradient(irtt_scene h, const irtt_solver_params *p, const double *dLdx,
+                         const irtt_adjoint_params *ap, uint32_t nnz, const uint32_t *vertex,
+                         const uint32_t *param, const float *dxdp, uint32_t n_params,
+                         double *dp, double *grad_x, double *lambda, irtt_solve_stats *st);
+
 /* Adjoint over a forward history (ThermalRenderer::runAdjointSim,
  * thermal_renderer.cpp:1028-1133, objective of thermal_objective.cpp:47-57):
  *   L = s * sum_t 0.5 sum_i w_i (x_t,i - target_t,i)^2,  s = 1/steps (transient) or 1,
@@ -353,7 +375,8 @@ int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nn
  * targets: steps x n_bins, or n_bins shared by every step (target_stride 0).
  * If grad_x is not NULL the adjoint trace runs too (ap NULL: reference
  * defaults) and grad_x (3 n_vertices) receives dL/dvertex (history rows
- * (lambda_t, x_t^4) and, with an ambient, (-Ta^4, sum_t lambda_t)). */
+ * (lambda_t, x_t^4) and, with an ambient, (-Ta^4, sum_t lambda_t); plus the
+ * source term of irtt_source_gradient with sum_t lambda_t). */
 int irtt_adjoint_history(irtt_scene h, const irtt_solver_params *p, uint32_t steps,
                          const double *history, const double *weights, const double *targets,
                          uint32_t target_stride, double *lambda_history, double *objective,

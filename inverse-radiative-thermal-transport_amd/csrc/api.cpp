@@ -110,6 +110,7 @@ void assemble(Scene &s)
     s.bin_eps_area.assign(nb, 0.0);
     s.bin_mass.assign(nb, 0.0);
     s.bin_source.assign(nb, 0.0);
+    s.source_explicit = false;
     bool any_k = false;
     for (uint32_t t = 0; t < nt; t++) {
         const irtt_material &m = s.mat[t];
@@ -372,6 +373,7 @@ int irtt_scene_set_source(irtt_scene h, const double *source)
             if (!std::isfinite(source[i])) throw Error(IRTT_ERR_ARG, "non-finite source");
             s->bin_source[i] = source[i];
         }
+        s->source_explicit = true;
     });
 }
 
@@ -712,6 +714,33 @@ int irtt_project_gradient(const double *grad_x, uint32_t n_vertices, uint32_t nn
             dp[param[k]] += (double)dxdp[3 * k] * g[0] + (double)dxdp[3 * k + 1] * g[1] +
                             (double)dxdp[3 * k + 2] * g[2];
         }
+    });
+}
+
+int irtt_source_gradient(irtt_scene h, const double *lambda, double *grad_x)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!lambda || !grad_x) throw Error(IRTT_ERR_ARG, "null pointer");
+        source_gradient(*s, lambda, grad_x);
+    });
+}
+
+int irtt_design_gradient(irtt_scene h, const irtt_solver_params *p, const double *dLdx,
+                         const irtt_adjoint_params *ap, uint32_t nnz, const uint32_t *vertex,
+                         const uint32_t *param, const float *dxdp, uint32_t n_params,
+                         double *dp, double *grad_x, double *lambda, irtt_solve_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p || !dLdx) throw Error(IRTT_ERR_ARG, "null pointer");
+        if (dp && nnz && (!vertex || !param || !dxdp))
+            throw Error(IRTT_ERR_ARG, "null projection arrays");
+        irtt_adjoint_params a = ap ? *ap : irtt_adjoint_params{0.05f, 1e-2f, 0u, 0u};
+        irtt_solve_stats tmp{};
+        design_gradient(*s, *p, dLdx, a, nnz, vertex, param, dxdp, n_params, dp, grad_x, lambda,
+                        tmp);
+        if (st) *st = tmp;
     });
 }
 

@@ -97,6 +97,11 @@ void adjoint_history(Scene &s, const irtt_solver_params &p, uint32_t steps, cons
     irtt_adjoint_params a = ap ? *ap : irtt_adjoint_params{0.05f, 1e-2f, 0u, 0u};
     irtt_adjoint_stats ast{};
     adjoint_trace(s, a, rows, lr.data(), xr.data(), grad_x, ast);
+    // the source term b (the same every step): sum_t lambda_t^T db/dv
+    std::vector<double> lsum(nb, 0.0);
+    for (uint32_t t = 0; t < steps; t++)
+        for (uint32_t i = 0; i < nb; i++) lsum[i] += lam_hist[(size_t)t * nb + i];
+    source_gradient(s, lsum.data(), grad_x);
 }
 
 }  // namespace irtt

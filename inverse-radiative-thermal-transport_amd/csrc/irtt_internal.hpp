@@ -193,6 +193,7 @@ struct Scene {
     // per-bin physics (FP64), ThermalData::load / computeLumpedThermalMass /
     // computeConductiveLaplacian
     std::vector<double> bin_area, bin_eps_area, bin_mass, bin_source;
+    bool source_explicit = false;     // bin_source set by irtt_scene_set_source
     std::vector<double> bin_kelvin0;   // initial / fixed values
     std::vector<uint8_t> bin_fixed;
     std::vector<double> x;             // current state (K)
@@ -277,6 +278,12 @@ void adjoint_history(Scene &s, const irtt_solver_params &p, uint32_t steps, cons
                      const double *w, const double *targets, uint32_t target_stride,
                      double *lam_hist, double *objective, const irtt_adjoint_params *ap,
                      double *grad_x, irtt_solve_stats &st);
+// design.cpp
+void source_gradient(const Scene &s, const double *lam, double *grad);
+void design_gradient(Scene &s, const irtt_solver_params &p, const double *dLdx,
+                     const irtt_adjoint_params &ap, uint32_t nnz, const uint32_t *vertex,
+                     const uint32_t *param, const float *dxdp, uint32_t n_params, double *dp,
+                     double *grad_out, double *lam_out, irtt_solve_stats &st);
 // batch.hip
 void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *source,
                  double *kelvin, irtt_solve_stats &st);

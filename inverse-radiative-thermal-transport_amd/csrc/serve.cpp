@@ -169,36 +169,14 @@ int Server::run(Req &q)
         dLdx[i] = r.objective_weight * area[i] * d;
     }
     if (r.objective) *r.objective = L;
-    std::vector<double> lam(nb);
-    if ((rc = irtt_adjoint_solve(h, &r.solver, nullptr, dLdx.data(), lam.data(), &ss)))
-        return done(rc);
-    // history rows: (lambda, x^4) for the operator term, (-Ta^4, lambda) for
-    // the ambient term Ta^4 colsum(T); a transient step carries dt
-    const double scale = r.solver.steady ? 1.0 : (r.solver.dt > 0 ? r.solver.dt : 300.0);
-    const bool amb = r.solver.ambient_kelvin >= 0.0;
-    const uint32_t nh = amb ? 2 : 1;
-    std::vector<double> lrows(nh * (size_t)nb), xrows(nh * (size_t)nb);
-    const double Ta = r.solver.ambient_kelvin, Ta4 = Ta * Ta * Ta * Ta;
-    for (uint32_t i = 0; i < nb; i++) {
-        double x2 = x[i] * x[i];
-        lrows[i] = scale * lam[i];
-        xrows[i] = x2 * x2;
-        if (amb) {
-            lrows[nb + i] = -scale * Ta4;
-            xrows[nb + i] = lam[i];
-        }
-    }
-    std::vector<double> gx(3 * (size_t)r.nv);
+    // adjoint solve, adjoint trace, source term, projection onto the parameters
     irtt_adjoint_params ap{0.05f, 1e-2f, 0u, 0u};
-    irtt_adjoint_stats as{};
-    if ((rc = irtt_adjoint_trace(h, &ap, nh, lrows.data(), xrows.data(), gx.data(), &as)))
+    const bool want_dp = r.dp && r.n_params;
+    if ((rc = irtt_design_gradient(h, &r.solver, dLdx.data(), &ap, want_dp ? r.nnz : 0,
+                                   q.vertex.data(), q.param.data(), q.dxdp.data(),
+                                   want_dp ? r.n_params : 0, want_dp ? r.dp : nullptr, nullptr,
+                                   nullptr, &ss)))
         return done(rc);
-    if (r.dp && r.n_params) {
-        std::fill(r.dp, r.dp + r.n_params, 0.0);
-        if ((rc = irtt_project_gradient(gx.data(), r.nv, r.nnz, q.vertex.data(), q.param.data(),
-                                        q.dxdp.data(), r.n_params, r.dp)))
-            return done(rc);
-    }
     return done(IRTT_OK);
 }
 

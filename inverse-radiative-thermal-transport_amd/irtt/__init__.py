@@ -147,6 +147,11 @@ def _declare(L):
                                      ctypes.POINTER(SolveStats)]
     L.irtt_adjoint_trace.argtypes = [_vp, ctypes.POINTER(AdjointParams), ctypes.c_uint32, _vp,
                                      _vp, _vp, ctypes.POINTER(AdjointStats)]
+    L.irtt_source_gradient.argtypes = [_vp, _vp, _vp]
+    L.irtt_design_gradient.argtypes = [_vp, ctypes.POINTER(SolverParams), _vp,
+                                       ctypes.POINTER(AdjointParams), ctypes.c_uint32, _vp, _vp,
+                                       _vp, ctypes.c_uint32, _vp, _vp, _vp,
+                                       ctypes.POINTER(SolveStats)]
     L.irtt_project_gradient.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _vp,
                                         ctypes.c_uint32, _vp]
     L.irtt_serve_create.argtypes = [ctypes.POINTER(ServeConfig), ctypes.POINTER(_vp)]
@@ -581,6 +586,42 @@ class Scene:
         _check(lib().irtt_adjoint_trace(self._h, ctypes.byref(ap), lam.shape[0], _ptr(lam), _ptr(x),
                                         _ptr(g), ctypes.byref(st)))
         return g.reshape(-1, 3), st.as_dict()
+
+    def source_gradient(self, lam):
+        """lambda^T db/dvertex (n_vertices, 3) of the material-flux source
+        (irtt_source_gradient)."""
+        lam = np.ascontiguousarray(lam, np.float64)
+        if lam.shape != (self.n_bins,):
+            raise ValueError("lambda must have n_bins entries")
+        g = np.zeros(3 * self.info["n_vertices"])
+        _check(lib().irtt_source_gradient(self._h, _ptr(lam), _ptr(g)))
+        return g.reshape(-1, 3)
+
+    def design_gradient(self, dLdx, params=None, solver=None, edge_ratio=0.05, tiny_eps=1e-2,
+                        flags=ADJ_AREA | ADJ_CONTINUOUS | ADJ_EDGE):
+        """Gradient at the current solved state (irtt_design_gradient).
+        params: (vertex, param, dxdp, n_params) as for project_gradient.
+        Returns (dp or None, grad_x (n_vertices, 3), lambda, stats)."""
+        sp = solver if solver is not None else solver_params()
+        g = np.ascontiguousarray(dLdx, np.float64)
+        if g.shape != (self.n_bins,):
+            raise ValueError("dLdx must have n_bins entries")
+        gx = np.empty(3 * self.info["n_vertices"])
+        lam = np.empty(self.n_bins)
+        ap = AdjointParams(edge_ratio, tiny_eps, flags, 0)
+        st = SolveStats()
+        dp = None
+        args = (0, None, None, None, 0, None)
+        if params is not None:
+            v, p, d, n_params = params
+            v = np.ascontiguousarray(v, np.uint32)
+            p = np.ascontiguousarray(p, np.uint32)
+            d = np.ascontiguousarray(d, np.float32).reshape(-1, 3)
+            dp = np.zeros(n_params)
+            args = (v.size, _ptr(v), _ptr(p), _ptr(d), n_params, _ptr(dp))
+        _check(lib().irtt_design_gradient(self._h, ctypes.byref(sp), _ptr(g), ctypes.byref(ap),
+                                          *args, _ptr(gx), _ptr(lam), ctypes.byref(st)))
+        return dp, gx.reshape(-1, 3), lam, st.as_dict()
 
 
 REQ_QUEUED = 0

@@ -17,7 +17,7 @@ PKG = os.path.dirname(HERE)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libirtt.so")
 
-SOURCES = ["api.cpp", "compat_api.cpp", "serve.cpp", "history.cpp", "lbvh.hip", "trace.hip", "operator.hip", "aca.hip",
+SOURCES = ["api.cpp", "compat_api.cpp", "serve.cpp", "history.cpp", "design.cpp", "lbvh.hip", "trace.hip", "operator.hip", "aca.hip",
            "solver.hip", "batch.hip", "adjoint.hip", "sparse.hip"]
 
 FLAGS = [

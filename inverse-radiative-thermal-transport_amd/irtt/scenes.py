@@ -234,6 +234,182 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
     return sd
 
 
+def _oriented_patch(origin, u, v, nu, nv_, normal):
+    """grid_patch whose normal (u x v) points along `normal` (u, v swapped if not)."""
+    u, v = np.asarray(u, np.float64), np.asarray(v, np.float64)
+    if np.dot(np.cross(u, v), normal) < 0:
+        return grid_patch(origin, v, u, nv_, nu)
+    return grid_patch(origin, u, v, nu, nv_)
+
+
+def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, depth=16.0,
+           hmin=20.0, hmax=45.0, floor=3.0, bay=4.0, slab=0.3, fin_width=0.3, n_fin_buildings=16,
+           depth_range=(0.3, 1.2), balcony_cells=6, depth_cells=2, fin_cell=1.0,
+           eps_range=(0.85, 0.95), alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5),
+           sun_elevation_deg=45.0, sun_azimuth_deg=135.0, sun_flux=800.0, kelvin=293.0):
+    """Procedural facade scene: nx x ny box buildings (footprint width x
+    depth, heights U[hmin, hmax]) on a ground plane of 1 m cells; every
+    facade carries balconies (slabs of thickness `slab`, one per bay and
+    floor, 1 m narrower than the bay) and the -y facades of the first
+    `n_fin_buildings` buildings carry vertical fins at the bay boundaries.
+    Each extrusion is an open box (front, sides, and top/bottom) standing on
+    the wall; its depth along the wall normal is a design parameter shared by
+    a group: the balconies of one height third of one facade (3 x 4 x 20 =
+    240 groups) and the fins of one building (16) = 256 parameters by
+    default, initial depths U[depth_range].  Element subdivision is fixed
+    (balcony_cells along a balcony, depth_cells along the depth, fins in
+    fin_cell rows), so the mesh topology does not depend on the depths; walls
+    and roofs are subdivided to reach about target_tris triangles.
+
+    meta: ``param_vertex``, ``param_index``, ``param_dir`` (d x / d depth,
+    per element vertex), ``depths`` (initial), ``base_xyz`` (positions at
+    depth 0), ``ground_vertex`` (mask), ``n_params``."""
+    rng = np.random.default_rng(seed)
+    nb_ = nx * ny
+    heights = rng.uniform(hmin, hmax, nb_)
+    px, py = width + street, depth + street
+    ncx, ncy = int(round(nx * px)), int(round(ny * py))
+    occupied = np.zeros((ncx, ncy), bool)
+    bld = []
+    for b in range(nb_):
+        i, j = b % nx, b // nx
+        x0, y0 = i * px + street / 2.0, j * py + street / 2.0
+        occupied[int(round(x0)):int(round(x0 + width)), int(round(y0)):int(round(y0 + depth))] = True
+        bld.append((x0, y0, float(heights[b])))
+    xs, ts, off = [], [], 0
+    ground, pv, pi_, pdir = [], [], [], []
+
+    def add(x, t, is_ground=False, param=None, n=None, d0=None, wall_pt=None):
+        nonlocal off
+        xs.append(x)
+        ts.append(t + off)
+        ground.append(np.full(x.shape[0], is_ground))
+        if param is not None:
+            frac = np.clip(((x - wall_pt) @ n) / d0, 0.0, 1.0)
+            pv.append(np.arange(off, off + x.shape[0]))
+            pi_.append(np.full(x.shape[0], param))
+            pdir.append(frac[:, None] * n[None, :])
+        off += x.shape[0]
+
+    # ground: maximal runs of free cells along x in each row of 1 m cells
+    for cy in range(ncy):
+        cx = 0
+        while cx < ncx:
+            if occupied[cx, cy]:
+                cx += 1
+                continue
+            c0 = cx
+            while cx < ncx and not occupied[cx, cy]:
+                cx += 1
+            add(*grid_patch([c0, cy, 0.0], [cx - c0, 0, 0], [0, 1.0, 0], cx - c0, 1), True)
+
+    # elements first (their count fixes the wall subdivision); parameters:
+    # 12 balcony groups per building, then one fin group per fin building
+    n_bal = 12 * nb_
+    depths = rng.uniform(*depth_range, n_bal + min(n_fin_buildings, nb_))
+    ez = np.array([0.0, 0.0, 1.0])
+
+    def extrusion(corner, a_vec, b_vec, n, d, na, nbb, param, bottom=True):
+        # open box on the wall rectangle corner + [0,1] a_vec + [0,1] b_vec,
+        # extruded by d along n
+        corner = np.asarray(corner, np.float64)
+        a_hat = a_vec / np.linalg.norm(a_vec)
+        b_hat = b_vec / np.linalg.norm(b_vec)
+        dn = d * n
+        kw = dict(param=param, n=n, d0=d, wall_pt=corner)
+        add(*_oriented_patch(corner + dn, a_vec, b_vec, na, nbb, n), **kw)              # front
+        add(*_oriented_patch(corner + b_vec, a_vec, dn, na, depth_cells, b_hat), **kw)  # b = 1
+        if bottom:
+            add(*_oriented_patch(corner, a_vec, dn, na, depth_cells, -b_hat), **kw)     # b = 0
+        add(*_oriented_patch(corner, b_vec, dn, nbb, depth_cells, -a_hat), **kw)        # a = 0
+        add(*_oriented_patch(corner + a_vec, b_vec, dn, nbb, depth_cells, a_hat), **kw)  # a = 1
+
+    walls = []
+    for b, (x0, y0, h) in enumerate(bld):
+        x1, y1 = x0 + width, y0 + depth
+        # (start, tangent, length, outward normal) of the 4 facades; q = 0 is the -y facade
+        fac = [(np.array([x0, y0, 0.0]), np.array([1.0, 0, 0]), width, np.array([0, -1.0, 0])),
+               (np.array([x1, y0, 0.0]), np.array([0, 1.0, 0]), depth, np.array([1.0, 0, 0])),
+               (np.array([x1, y1, 0.0]), np.array([-1.0, 0, 0]), width, np.array([0, 1.0, 0])),
+               (np.array([x0, y1, 0.0]), np.array([0, -1.0, 0]), depth, np.array([-1.0, 0, 0]))]
+        walls.append((b, h, fac))
+        nf = int((h - 1.0) // floor)
+        for q, (p0, t, L, n) in enumerate(fac):
+            nbay = max(1, int(L // bay))
+            bw = L / nbay
+            for k in range(nbay):
+                s0 = k * bw + 0.5
+                for f in range(1, nf + 1):
+                    band = min(2, 3 * (f - 1) // max(nf, 1))
+                    param = (b * 4 + q) * 3 + band
+                    corner = p0 + s0 * t + (f * floor - slab) * ez
+                    extrusion(corner, (bw - 1.0) * t, slab * ez, n, depths[param],
+                              balcony_cells, 1, param)
+            if q == 0 and b < n_fin_buildings:
+                param = n_bal + b
+                nh = max(1, int(round(h / fin_cell)))
+                for k in range(1, nbay):
+                    corner = p0 + (k * bw - fin_width / 2.0) * t
+                    extrusion(corner, fin_width * t, h * ez, n, depths[param], 1, nh, param,
+                              bottom=False)
+    elem_tris = sum(t.shape[0] for t in ts) - sum(
+        t.shape[0] for t, g in zip(ts, ground) if g.all())
+    ground_tris = sum(t.shape[0] for t, g in zip(ts, ground) if g.all())
+    wall_area = sum(2 * (width + depth) * h for _, h, _ in walls)
+    roof_area = nb_ * width * depth
+    rest = max(target_tris - ground_tris - elem_tris, 1000)
+    edge = np.sqrt(2.0 * (wall_area + roof_area) / rest)
+    for b, h, fac in walls:
+        nh = max(1, int(round(h / edge)))
+        for p0, t, L, n in fac:
+            add(*_oriented_patch(p0, L * t, h * ez, max(1, int(round(L / edge))), nh, n))
+        x0, y0 = fac[0][0][0], fac[0][0][1]
+        add(*grid_patch([x0, y0, h], [width, 0, 0], [0, depth, 0],
+                        max(1, int(round(width / edge))), max(1, int(round(depth / edge)))))
+    xyz = np.concatenate(xs)
+    idx = np.concatenate(ts)
+    nt = idx.shape[0]
+    nq = nt // 2
+    eps = np.repeat(rng.uniform(*eps_range, nq), 2).astype(np.float32)
+    alpha = np.repeat(rng.uniform(*alpha_range, nq), 2)
+    cap = np.repeat(rng.uniform(*capacity_range, nq), 2).astype(np.float32)
+    el, az = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg)
+    sun = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
+    sd = _uniform_scene("facade", xyz, idx, rho_d=1.0 - eps, heat_capacity=cap, kelvin=kelvin)
+    cosi = np.clip(sd.normals() @ sun, 0.0, None)
+    sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
+    v = np.concatenate(pv).astype(np.uint32)
+    dirs = np.concatenate(pdir)
+    params = np.concatenate(pi_).astype(np.uint32)
+    base = np.asarray(xyz, np.float64).copy()
+    base[v] -= dirs * depths[params][:, None]
+    areas = sd.triangle_areas()
+    sd.meta.update(seed=seed, buildings=nb_, edge=float(edge), sun=sun.tolist(),
+                   mean_triangle_area=float(areas.mean()), total_area=float(areas.sum()),
+                   alpha=alpha.astype(np.float32), heights=heights,
+                   param_vertex=v, param_index=params, param_dir=dirs.astype(np.float32),
+                   depths=depths, base_xyz=base, ground_vertex=np.concatenate(ground),
+                   n_params=depths.size)
+    return sd
+
+
+def facade_vertices(sd, depths):
+    """Vertex positions of a facade scene at the given extrusion depths."""
+    m = sd.meta
+    x = m["base_xyz"].copy()
+    d = np.asarray(depths, np.float64)
+    x[m["param_vertex"]] += m["param_dir"].astype(np.float64) * d[m["param_index"]][:, None]
+    return x.astype(np.float32)
+
+
+def facade_parametrization(sd):
+    """(vertex, param, dxdp, n_params) of the facade depths, for
+    irtt_project_gradient (the reference's vertex-subset parametrizations,
+    thermal_params.hpp:41-130)."""
+    m = sd.meta
+    return m["param_vertex"], m["param_index"], m["param_dir"], m["n_params"]
+
+
 def sun_flux(sd, elevation_deg, azimuth_deg, flux=800.0):
     """Absorbed direct-sun flux per triangle (W/m^2) for a city block:
     alpha * flux * max(0, n.s), unshadowed."""

@@ -81,7 +81,8 @@ def test_adjoint_history_matches_flux_fd(irtt, steady, steps):
 
 def test_adjoint_history_gradient_runs_the_trace(irtt):
     """With grad_x requested the trace consumes the history rows; the steady
-    single-step gradient equals the trace called by hand with the same rows."""
+    single-step gradient equals the trace called by hand with the same rows
+    plus the source term lambda^T db/dv (irtt_source_gradient)."""
     sd = _plates()
     area = sd.triangle_areas()
     w = 1e-2 * area
@@ -93,6 +94,7 @@ def test_adjoint_history_gradient_runs_the_trace(irtt):
         lam, L, g, st = s.adjoint_history(H, w, target, params=p, gradient=True)
         g2, _ = s.adjoint_trace(np.stack([lam[0], np.full(sd.nt, -293.0 ** 4)]),
                                 np.stack([H[0] ** 4, lam[0]]))
+        g2 = g2 + s.source_gradient(lam[0])
     scale = np.abs(g2).max()
     print(f"|grad| max {scale:.3e}, objective {L:.6e}")
     assert np.abs(g - g2).max() <= 1e-12 * scale
