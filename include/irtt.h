@@ -310,6 +310,13 @@ int irtt_adjoint_solve(irtt_scene h, const irtt_solver_params *p, const double *
 #define IRTT_ADJ_EDGE 4u               /* approximate edge sampling */
 #define IRTT_ADJ_REFERENCE_DROPPED 8u  /* dropped photons still contribute with absorber bin 0
                                           (the reference shader's behaviour) */
+#define IRTT_ADJ_EDGE_GAIN 16u         /* extension, off by default: the edge term also credits
+                                          the occluding side (the photon's absorbing bin) with
+                                          the power the moving edge takes from the shadow side
+                                          (or from escape); the reference's edge term counts
+                                          the shadow side only (thermal_tracing_unified.glsl:
+                                          537-567, whose commented-out follow-up names the
+                                          missing term) */
 
 typedef struct irtt_adjoint_params {
     float edge_ratio;   /* ThermalVars_s::gradEdgeRatio (0.05), thermal_common.hpp:124 */

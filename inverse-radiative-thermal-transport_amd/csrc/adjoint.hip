@@ -270,31 +270,47 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
                         to = to - scl3(d, a.eps);
                         Hit oc = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, to, d, 1e-12f,
                                                       1e12f, sn, stt, nvis, ntri);
-                        if (oc.prim >= 0 && oc.t >= a.tiny_dist) {
+                        const bool valid = oc.prim >= 0 && oc.t >= a.tiny_dist;
+                        const uint32_t origin_v =
+                            prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
+                        bool nbr = false;
+                        uint32_t shadow_v = 0;
+                        if (valid) {
                             const uint32_t op = (uint32_t)oc.prim;
                             const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1],
                                            c2 = a.idx[3 * op + 2];
-                            bool nbr = c0 == j0 || c0 == j1 || c0 == j2 || c1 == j0 || c1 == j1 ||
-                                       c1 == j2 || c2 == j0 || c2 == j1 || c2 == j2;
+                            nbr = c0 == j0 || c0 == j1 || c0 == j2 || c1 == j0 || c1 == j1 ||
+                                  c1 == j2 || c2 == j0 || c2 == j1 || c2 == j2;
                             const V3 s0{a.xyz[3 * c0], a.xyz[3 * c0 + 1], a.xyz[3 * c0 + 2]};
                             const V3 s1{a.xyz[3 * c1], a.xyz[3 * c1 + 1], a.xyz[3 * c1 + 2]};
                             const V3 s2{a.xyz[3 * c2], a.xyz[3 * c2 + 1], a.xyz[3 * c2 + 2]};
                             V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
+                            const float ob0 = 1.0f - oc.b1 - oc.b2;
+                            const int sm = max_corner(ob0, oc.b1, oc.b2);
+                            shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
                             if (!nbr && dot3(tn, d) < -1e-3f) {
                                 cnt[A_OCCL]++;
-                                const float ob0 = 1.0f - oc.b1 - oc.b2;
-                                const int sm = max_corner(ob0, oc.b1, oc.b2);
-                                const uint32_t shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
                                 const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
                                 const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
                                 const float d1 = h.t, d2 = oc.t, tot = d1 + d2;
                                 const double fs = hist_factor(a, shadow_bin, src);
-                                const uint32_t origin_v =
-                                    prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
                                 acc3(a.grad, shadow_v, scl3(dE, sa * (d1 / tot)), fs);
                                 acc3(a.grad, origin_v, scl3(dE, sa * (d2 / tot)), fs);
                                 acc3(a.grad, edge_v, scl3(dE, -sa), fs);
                             }
+                        }
+                        // Extension (not in the reference): the power the
+                        // moving edge takes from the shadow side lands on the
+                        // hit triangle and follows this photon's path, so it
+                        // is credited to the path's absorbing bin with the
+                        // opposite sign, also when nothing lies behind the
+                        // edge (the power escaped before).
+                        if ((a.aflags & IRTT_ADJ_EDGE_GAIN) && (oc.prim < 0 || (valid && !nbr))) {
+                            const double fg = hist_factor(a, abs_bin, src);
+                            const float f_or = valid ? oc.t / (h.t + oc.t) : 1.0f;
+                            acc3(a.grad, edge_v, dE, fg);
+                            acc3(a.grad, origin_v, scl3(dE, f_or), -fg);
+                            if (valid) acc3(a.grad, shadow_v, scl3(dE, 1.0f - f_or), -fg);
                         }
                     }
                     prev0 = j0;

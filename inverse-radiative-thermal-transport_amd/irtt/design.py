@@ -31,11 +31,13 @@ class FacadeDesign:
     """A facade scene on the device with its extrusion depths as the design
     parameters (scenes.facade)."""
 
-    def __init__(self, irtt, sd, transport, solver, x0=None):
+    def __init__(self, irtt, sd, transport, solver, x0=None, adjoint_flags=0):
         self.irtt = irtt
         self.sd = sd
         self.tp = transport
         self.sp = solver
+        # 0: the reference's terms (area | continuous | edge)
+        self.flags = adjoint_flags or (irtt.ADJ_AREA | irtt.ADJ_CONTINUOUS | irtt.ADJ_EDGE)
         self.s = irtt.Scene.from_data(sd, irtt.BIN_VERTEX)
         self.x0 = np.asarray(sd.kelvin if x0 is None else x0, np.float32)
         self.w = ground_weights(self.s, sd)
@@ -66,10 +68,11 @@ class FacadeDesign:
                        "nnz": tst["nnz"], "rays": tst["rays"], "converged": sst["converged"]}
         return float(self.w @ x), x
 
-    def gradient(self):
+    def gradient(self, flags=None):
         """dL/d(depths) at the state of the last forward()."""
         t0 = time.perf_counter()
-        dp, gx, lam, st = self.s.design_gradient(self.w, params=self.par, solver=self.sp)
+        dp, gx, lam, st = self.s.design_gradient(self.w, params=self.par, solver=self.sp,
+                                                 flags=flags or self.flags)
         self.timing["gradient_ms"] = (time.perf_counter() - t0) * 1e3
         return dp
 

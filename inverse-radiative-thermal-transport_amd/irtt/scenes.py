@@ -246,7 +246,8 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
            hmin=20.0, hmax=45.0, floor=3.0, bay=4.0, slab=0.3, fin_width=0.3, n_fin_buildings=16,
            depth_range=(0.3, 1.2), balcony_cells=6, depth_cells=2, fin_cell=1.0,
            eps_range=(0.85, 0.95), alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5),
-           sun_elevation_deg=45.0, sun_azimuth_deg=135.0, sun_flux=800.0, kelvin=293.0):
+           sun_elevation_deg=45.0, sun_azimuth_deg=135.0, sun_flux=800.0, kelvin=293.0,
+           weld=False):
     """Procedural facade scene: nx x ny box buildings (footprint width x
     depth, heights U[hmin, hmax]) on a ground plane of 1 m cells; every
     facade carries balconies (slabs of thickness `slab`, one per bay and
@@ -259,7 +260,9 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
     default, initial depths U[depth_range].  Element subdivision is fixed
     (balcony_cells along a balcony, depth_cells along the depth, fins in
     fin_cell rows), so the mesh topology does not depend on the depths; walls
-    and roofs are subdivided to reach about target_tris triangles.
+    and roofs are subdivided to reach about target_tris triangles.  weld:
+    merge coincident vertices (faces meeting at an edge share its vertices;
+    otherwise every planar patch has its own).
 
     meta: ``param_vertex``, ``param_index``, ``param_dir`` (d x / d depth,
     per element vertex), ``depths`` (initial), ``base_xyz`` (positions at
@@ -368,6 +371,22 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
                         max(1, int(round(width / edge))), max(1, int(round(depth / edge)))))
     xyz = np.concatenate(xs)
     idx = np.concatenate(ts)
+    gmask = np.concatenate(ground)
+    v = np.concatenate(pv)
+    dirs = np.concatenate(pdir)
+    params = np.concatenate(pi_)
+    if weld:
+        # one vertex per position: faces of a box share their edge vertices
+        # (coincident vertices of an element share depth fraction and group)
+        uq, inv = np.unique(xyz.astype(np.float32), axis=0, return_inverse=True)
+        inv = inv.ravel()
+        xyz = uq.astype(np.float64)
+        idx = inv[idx]
+        g2 = np.zeros(uq.shape[0], bool)
+        g2[inv[gmask]] = True
+        gmask = g2
+        v, first = np.unique(inv[v], return_index=True)
+        dirs, params = dirs[first], params[first]
     nt = idx.shape[0]
     nq = nt // 2
     eps = np.repeat(rng.uniform(*eps_range, nq), 2).astype(np.float32)
@@ -378,9 +397,8 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
     sd = _uniform_scene("facade", xyz, idx, rho_d=1.0 - eps, heat_capacity=cap, kelvin=kelvin)
     cosi = np.clip(sd.normals() @ sun, 0.0, None)
     sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
-    v = np.concatenate(pv).astype(np.uint32)
-    dirs = np.concatenate(pdir)
-    params = np.concatenate(pi_).astype(np.uint32)
+    v = v.astype(np.uint32)
+    params = params.astype(np.uint32)
     base = np.asarray(xyz, np.float64).copy()
     base[v] -= dirs * depths[params][:, None]
     areas = sd.triangle_areas()
@@ -388,7 +406,7 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
                    mean_triangle_area=float(areas.mean()), total_area=float(areas.sum()),
                    alpha=alpha.astype(np.float32), heights=heights,
                    param_vertex=v, param_index=params, param_dir=dirs.astype(np.float32),
-                   depths=depths, base_xyz=base, ground_vertex=np.concatenate(ground),
+                   depths=depths, base_xyz=base, ground_vertex=gmask,
                    n_params=depths.size)
     return sd
 

@@ -303,6 +303,7 @@ ADJ_AREA = 1
 ADJ_CONTINUOUS = 2
 ADJ_EDGE = 4
 ADJ_REFERENCE_DROPPED = 8
+ADJ_EDGE_GAIN = 16
 
 
 class AdjointParams(ctypes.Structure):

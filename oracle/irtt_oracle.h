@@ -130,6 +130,7 @@ int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
 #define OR_ADJ_CONTINUOUS 2u
 #define OR_ADJ_EDGE 4u
 #define OR_ADJ_REFERENCE_DROPPED 8u /* dropped photons keep absorber bin 0, as the shader does */
+#define OR_ADJ_EDGE_GAIN 16u        /* extension: occluding side of the edge term */
 
 typedef struct {
     float edge_ratio;   /* ubo.gradEdgeRatio, 0.05 */

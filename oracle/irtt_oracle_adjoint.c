@@ -298,29 +298,41 @@ int or_adjoint(const or_scene *sc, const or_params *p, const or_adjoint_params *
                         to = add(to, scl(n_edg, eps + dot(sub(qv[near_], to), n_edg)));
                         to = sub(to, scl(d, eps));
                         hit_t oc = closest(&bvh, to, d);
-                        if (oc.prim >= 0 && oc.t >= p->tiny_dist) {
+                        const int valid = oc.prim >= 0 && oc.t >= p->tiny_dist;
+                        const uint32_t origin_v = prev[prev_max];
+                        int nb_ = 0;
+                        uint32_t shadow_v = 0;
+                        if (valid) {
                             const uint32_t *ci = sc->idx + 3 * (uint32_t)oc.prim;
-                            int nb_ = 0;
                             for (int a = 0; a < 3; a++)
                                 for (int bb = 0; bb < 3; bb++)
                                     if (ci[a] == hi[bb]) nb_ = 1;
                             f3 c0 = ld3(sc->xyz + 3 * ci[0]), c1 = ld3(sc->xyz + 3 * ci[1]),
                                c2 = ld3(sc->xyz + 3 * ci[2]);
                             f3 tn = nrm(cross(sub(c1, c0), sub(c2, c0)));
+                            float ob0 = 1.0f - oc.b1 - oc.b2;
+                            int sm = argmax3(ob0, oc.b1, oc.b2);
+                            shadow_v = ci[sm];
                             if (!nb_ && dot(tn, d) < -1e-3f) {
                                 s_occl++;
-                                float ob0 = 1.0f - oc.b1 - oc.b2;
-                                int sm = argmax3(ob0, oc.b1, oc.b2);
-                                uint32_t shadow_v = ci[sm];
                                 uint32_t shadow_bin = p->bin_mode ? (uint32_t)oc.prim : shadow_v;
                                 float sa = 1.0f - sc->rho_d[oc.prim] - sc->rho_s[oc.prim];
                                 float d1 = h.t, d2 = oc.t, tot = d1 + d2;
                                 double fs = lambda[shadow_bin] * x4[src];
-                                uint32_t origin_v = prev[prev_max];
                                 acc3(grad, shadow_v, scl(dE, sa * (d1 / tot)), fs);
                                 acc3(grad, origin_v, scl(dE, sa * (d2 / tot)), fs);
                                 acc3(grad, edge_v, scl(dE, -sa), fs);
                             }
+                        }
+                        /* extension (OR_ADJ_EDGE_GAIN): the occluding side gains
+                         * what the shadow side (or escape) loses, credited to
+                         * this path's absorbing bin with the opposite sign */
+                        if ((ap->flags & OR_ADJ_EDGE_GAIN) && (oc.prim < 0 || (valid && !nb_))) {
+                            double fg = lambda[abs_bin] * x4[src];
+                            float f_or = valid ? oc.t / (h.t + oc.t) : 1.0f;
+                            acc3(grad, edge_v, dE, fg);
+                            acc3(grad, origin_v, scl(dE, f_or), -fg);
+                            if (valid) acc3(grad, shadow_v, scl(dE, 1.0f - f_or), -fg);
                         }
                     }
                     prev[0] = hi[0]; prev[1] = hi[1]; prev[2] = hi[2];

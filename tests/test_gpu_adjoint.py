@@ -77,10 +77,11 @@ def test_nop_parameter_exact(irtt):
     assert abs(adj - fd) < 1e-4 * abs(fd)
 
 
-@pytest.mark.parametrize("which", ["plates", "city_small"])
-def test_adjoint_trace_matches_oracle(irtt, oracle, which):
+@pytest.mark.parametrize("which,flags", [("plates", 7), ("city_small", 7), ("city_small", 23)])
+def test_adjoint_trace_matches_oracle(irtt, oracle, which, flags):
     """Same photons, same lambda and x^4: GPU and CPU accumulate the same FP32
-    terms (in different orders, in FP64)."""
+    terms (in different orders, in FP64).  flags 23 adds the edge term's
+    occluding side (IRTT_ADJ_EDGE_GAIN, an extension)."""
     from irtt import scenes
     sd = {"plates": lambda: scenes.parallel_plates(n=4, emissivity=0.8),
           "city_small": lambda: scenes.city_block(seed=2, nx=2, ny=2, target_tris=1500)}[which]()
@@ -89,9 +90,10 @@ def test_adjoint_trace_matches_oracle(irtt, oracle, which):
     x4 = (290.0 + 10 * rng.random(sd.nv)) ** 4
     with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
         s.build_transport(threads_per_tri=32, rays_per_thread=4, batch_seed=9)
-        g, st = s.adjoint_trace(lam, x4)
+        g, st = s.adjoint_trace(lam, x4, flags=flags)
     sc = oracle.OracleScene.from_irtt(sd)
-    og, ost = oracle.adjoint(sc, oracle.make_params(32, 4, batch_seed=9, bin_mode=0), lam, x4)
+    og, ost = oracle.adjoint(sc, oracle.make_params(32, 4, batch_seed=9, bin_mode=0), lam, x4,
+                             flags=flags)
     for k in ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded", "rays"):
         assert st[k] == ost[k], (k, st[k], ost[k])
     scale = np.abs(og).max()

@@ -105,3 +105,32 @@ def test_transport_is_deterministic_and_seeded(oracle):
     a3 = _run(oracle, sd, 64, 4, bin_mode=0, seed=8)[1]
     assert np.array_equal(a1, a2)
     assert not np.array_equal(a1, a3)
+
+
+def test_far_field_blocks_do_not_compress(oracle):
+    """Why the large scenes use the sparse layout rather than hierarchical ACA
+    blocks: a well-separated block of the Monte-Carlo operator is a sparse
+    matrix of photon counts, not a smooth kernel.  Its rank at 1e-4 relative
+    Frobenius error exceeds 32 and grows with the photon count (the noise is
+    full rank), while its nonzeros, which the sparse layout stores exactly at
+    8 B each, take less memory than any rank-32 FP16 factor pair."""
+    from irtt import scenes
+    sd = scenes.city_block(seed=2, nx=3, ny=3, target_tris=6000)
+    x = np.asarray(sd.xyz, np.float64)[:, 0]
+    lo, hi = np.quantile(x, [0.25, 0.75])
+    rows, cols = np.where(x < lo)[0], np.where(x > hi)[0]
+    ranks = []
+    for per in (64, 512):
+        sc, acc, st, w, q, Q = _run(oracle, sd, per // 4, 4, bin_mode=0, seed=1234)
+        blk = acc[np.ix_(rows, cols)].astype(np.float64) * Q
+        s = np.linalg.svd(blk, compute_uv=False)
+        tail = np.sqrt(np.cumsum((s ** 2)[::-1])[::-1]) / np.sqrt((s ** 2).sum())
+        rank = int(np.argmax(tail <= 1e-4))
+        nnz = int((blk != 0).sum())
+        lowrank_bytes = 32 * (blk.shape[0] + blk.shape[1]) * 2
+        print(f"{per} photons/tri: block {blk.shape}, nnz {nnz}, rank for 1e-4 {rank}, "
+              f"rank-32 error {tail[32]:.2f}, sparse {8 * nnz} B vs rank-32 FP16 {lowrank_bytes} B")
+        assert rank > 32 and tail[32] > 0.1
+        assert 8 * nnz < lowrank_bytes
+        ranks.append(rank)
+    assert ranks[1] > ranks[0]
