@@ -211,7 +211,7 @@ def facade(irtt, fd_params=8, h=1e-3, gd_steps=10, threads_per_tri=336, rays_per
     sd = scenes.facade()
     gen_s = time.perf_counter() - t0
     tp = irtt.transport_params(threads_per_tri, rays_per_thread, batch_seed=1234,
-                               layout=irtt.LAYOUT_SPARSE)
+                               layout=irtt.LAYOUT_SPARSE, flags=irtt.TRANSPORT_KEEP_ABSORBERS)
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-12,
                             lin_rtol=1e-10)
     out = {"triangles": sd.nt, "bins": sd.nv, "params": int(sd.meta["n_params"]),

@@ -127,8 +127,16 @@ typedef struct irtt_transport_params {
     uint32_t max_depth;         /* 0 -> 512 (MAX_MAX_DEPTH) */
     float tiny_dist;            /* ThermalVars_s::tinyDist (1e-3) */
     uint32_t layout;            /* IRTT_LAYOUT_*: how the operator is stored */
-    uint32_t reserved[2];
+    uint32_t flags;             /* IRTT_TRANSPORT_* */
+    uint32_t reserved;
 } irtt_transport_params;
+
+/* Keep the absorbing bin of every photon (4 B per photon, up to 2^31
+ * photons) so irtt_adjoint_trace replays each photon once instead of twice
+ * (the reference's adjoint shader traces a first pass to find it,
+ * thermal_tracing_unified.glsl:292-599).  Costs the forward kernel a few
+ * percent; set it when an adjoint trace will follow. */
+#define IRTT_TRANSPORT_KEEP_ABSORBERS 1u
 
 /* Storage of the operator built by irtt_build_transport.  DENSE is the
  * reference's n_bins x n_bins FP32 matrix (thermal_transport.cpp:612-660).

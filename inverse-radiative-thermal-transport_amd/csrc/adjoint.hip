@@ -3,7 +3,8 @@
 // Replaces ThermalTransport::computeAdjoint (thermal_transport.cpp:740-832)
 // and the raygen program in TRACING_MODE 2
 // (assets/shader/raytracing_thermal/thermal_tracing_unified.glsl:292-599).
-// Each photon is replayed twice from its seed: pass 0 finds the absorbing bin,
+// Each photon is replayed from its seed: pass 0 finds the absorbing bin (or
+// the forward build's per-photon record supplies it and pass 0 is skipped),
 // pass 1 accumulates, weighted by sum_h lambda_h[bin] * x_h^4[source bin]
 // (sumOverHistory, :227-244, indexed history-major so every history length
 // is handled, not only one step):
@@ -129,6 +130,7 @@ struct AdjArgs {
     const double *lambda;  // n_hist x nb
     const double *x4;      // n_hist x nb
     uint64_t photons;
+    const uint32_t *absorber;  // per photon from the forward build, or nullptr
     double *grad;
     unsigned long long *stats;
 };
@@ -143,8 +145,11 @@ __device__ __forceinline__ double hist_factor(const AdjArgs &a, uint32_t bin_l, 
     return f;
 }
 
+// At least 3 waves per SIMD: the kernel would take 179 VGPRs (2 waves); capped
+// at 168 it spills 80 B per lane and runs 20 % faster on the city block (4
+// waves, 128 VGPRs, spills more and is slower again).
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
+__global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
     __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
@@ -164,7 +169,22 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
         const V3 p2{a.xyz[3 * i2], a.xyz[3 * i2 + 1], a.xyz[3 * i2 + 2]};
         uint32_t abs_bin = 0;
         bool absorbed_any = false;
-        for (int pass = 0; pass <= 1; pass++) {
+        // area terms of this photon: coefficient of dw (the source triangle's
+        // photon-weight gradient), added to the gradient after the photon
+        double c_area = 0.0;
+        V3 dw[3] = {V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
+        // continuous-term gradient of the source triangle's vertices (first segment)
+        double src_acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        // pass 0 finds the absorbing bin; the forward build's record saves it
+        int first_pass = 0;
+        if (a.absorber) {
+            const uint32_t r = a.absorber[g];
+            absorbed_any = r != 0xffffffffu;
+            abs_bin = absorbed_any ? r : 0u;
+            if (absorbed_any) cnt[A_ABS]++;
+            first_pass = 1;
+        }
+        for (int pass = first_pass; pass <= 1; pass++) {
             uint32_t seed = seed0;
             float u0 = rnd(seed), u1 = rnd(seed);
             (void)rnd(seed);
@@ -184,18 +204,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
             int sc = max_corner(b0, b1, b2);
             const uint32_t src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
             const float wt = a.w[t];
-            V3 dw[3];
             if (pass == 1 && (a.aflags & IRTT_ADJ_AREA)) {
                 // emission-side area term, thermal_tracing_unified.glsl:327-351
                 V3 ga[3];
                 area_grad(p0, p1, p2, ga);
                 float s = wt / a.area[t];
-                const double fsrc = hist_factor(a, src, src);
-                const uint32_t tv[3] = {i0, i1, i2};
-                for (int c = 0; c < 3; c++) {
-                    dw[c] = scl3(ga[c], s);
-                    if (finite3(dw[c])) acc3(a.grad, tv[c], dw[c], -fsrc);
-                }
+                for (int c = 0; c < 3; c++) dw[c] = scl3(ga[c], s);
+                c_area -= hist_factor(a, src, src);
             }
             if (pass == 1 && !absorbed_any && !(a.aflags & IRTT_ADJ_REFERENCE_DROPPED)) break;
             const double fpath = pass ? hist_factor(a, abs_bin, src) : 0.0;
@@ -232,7 +247,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
                             const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
                             for (int v = 0; v < 6; v++) {
                                 V3 dv = scl3(gg[v], wt);
-                                if (finite3(dv)) acc3(a.grad, vid[v], dv, fpath);
+                                if (!finite3(dv)) continue;
+                                if (v < 3 && depth == 1) {
+                                    // the first segment starts on the source
+                                    // triangle: summed per wave after the photon
+                                    src_acc[3 * v] += fpath * (double)dv.x;
+                                    src_acc[3 * v + 1] += fpath * (double)dv.y;
+                                    src_acc[3 * v + 2] += fpath * (double)dv.z;
+                                } else {
+                                    acc3(a.grad, vid[v], dv, fpath);
+                                }
                             }
                         }
                     }
@@ -346,13 +370,37 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(AdjArgs a)
                         cnt[A_ABS]++;
                     } else if (a.aflags & IRTT_ADJ_AREA) {
                         // absorption-side area term, :588-592
-                        const uint32_t tv[3] = {i0, i1, i2};
-                        for (int c = 0; c < 3; c++)
-                            if (finite3(dw[c])) acc3(a.grad, tv[c], dw[c], fpath);
+                        c_area += fpath;
                     }
                     break;
                 }
             }
+        }
+        {
+            // What this photon adds to its source triangle's three vertices:
+            // the area terms (all along dw) and the first segment's
+            // continuous term.  The 64 lanes of a wave trace consecutive
+            // photons, nearly always of one triangle: then the wave's sum is
+            // added by one lane (9 atomics per wave instead of 9 per lane, all
+            // on the same 3 vertices).
+            const uint32_t tv[3] = {i0, i1, i2};
+            double sv[9];
+            for (int q = 0; q < 3; q++) {
+                const bool ok = c_area != 0.0 && finite3(dw[q]);
+                sv[3 * q] = src_acc[3 * q] + (ok ? c_area * (double)dw[q].x : 0.0);
+                sv[3 * q + 1] = src_acc[3 * q + 1] + (ok ? c_area * (double)dw[q].y : 0.0);
+                sv[3 * q + 2] = src_acc[3 * q + 2] + (ok ? c_area * (double)dw[q].z : 0.0);
+            }
+            const unsigned long long all = __ballot(1);
+            const uint32_t tf = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+            const bool uniform = all == ~0ull && __ballot(t == tf) == ~0ull;
+            if (uniform) {
+                for (int e = 0; e < 9; e++)
+                    for (int off = 32; off > 0; off >>= 1) sv[e] += __shfl_xor(sv[e], off, 64);
+            }
+            if (!uniform || (threadIdx.x & 63) == 0)
+                for (int e = 0; e < 9; e++)
+                    if (sv[e] != 0.0) unsafeAtomicAdd(a.grad + 3 * (size_t)tv[e / 3] + e % 3, sv[e]);
         }
     }
     for (int s = 0; s < A_COUNT; s++) {
@@ -424,16 +472,20 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     a.lambda = d_lam.p;
     a.x4 = d_x4.p;
     a.photons = (uint64_t)per * s.nt;
+    a.absorber = s.d_absorber.n == a.photons ? s.d_absorber.p : nullptr;
     a.grad = d_grad.p;
     a.stats = d_st.p;
     int dev = 0, ncu = 256;
     IRTT_HIP(hipGetDevice(&dev));
     IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int S = stack_size_for(s.bvh.max_depth);
+    // the forward kernel's stack choice: 24 entries (more waves per CU) for
+    // trees up to 23 levels
+    const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
     uint64_t want = (a.photons + kTraceBlock - 1) / kTraceBlock;
     uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ncu * 64));
     IRTT_HIP(hipEventRecord(s.ev0, stm));
-    if (S == 32) k_adjoint<32><<<grid, kTraceBlock, 0, stm>>>(a);
+    if (S == 24) k_adjoint<24><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 32) k_adjoint<32><<<grid, kTraceBlock, 0, stm>>>(a);
     else if (S == 64) k_adjoint<64><<<grid, kTraceBlock, 0, stm>>>(a);
     else k_adjoint<128><<<grid, kTraceBlock, 0, stm>>>(a);
     IRTT_HIP(hipGetLastError());
@@ -450,7 +502,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     st.continuous = hs[A_CONT];
     st.edge_hits = hs[A_EDGE];
     st.occluded = hs[A_OCCL];
-    st.rays = hs[A_RAYS];
+    st.rays = a.absorber ? s.fwd_rays : hs[A_RAYS];
 }
 
 }  // namespace irtt

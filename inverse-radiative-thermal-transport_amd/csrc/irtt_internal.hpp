@@ -222,6 +222,8 @@ struct Scene {
     std::vector<double> colsum;        // column sums of T (exact)
     std::vector<double> Tdiag;         // diagonal of the FP32 operator as stored
     DBuf<unsigned long long> d_stats;
+    DBuf<uint32_t> d_absorber;        // absorbing bin of every photon (~0: none), for the adjoint
+    uint64_t fwd_rays = 0;            // ray segments of the last transport build
 
     // operator forms (operator.hip, aca.hip)
     uint32_t op_kind = IRTT_OP_DENSE;

@@ -70,6 +70,8 @@ uint64_t footprint(const irtt_request &q, uint32_t nb)
     if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
     b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;
     b += (uint64_t)q.nt * 256 + (uint64_t)q.nv * 64 + (64ull << 20);
+    if (q.target)  // per-photon absorber record for the adjoint trace
+        b += 4ull * q.transport.threads_per_tri * q.transport.rays_per_thread * q.nt;
     return b;
 }
 
@@ -149,7 +151,11 @@ int Server::run(Req &q)
         if (rc) return done(rc);
     }
     irtt_transport_stats ts{};
-    if ((rc = irtt_build_transport(h, &r.transport, &ts))) return done(rc);
+    // an adjoint trace follows when there is an objective: keep each
+    // photon's absorbing bin so it replays every photon once
+    irtt_transport_params tp = r.transport;
+    if (!q.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_ABSORBERS;
+    if ((rc = irtt_build_transport(h, &tp, &ts))) return done(rc);
     if (r.operator_kind != IRTT_OP_DENSE && (rc = irtt_select_operator(h, r.operator_kind)))
         return done(rc);
     irtt_solve_stats ss{};

@@ -62,6 +62,7 @@ struct PhotonArgs {
     float tiny_dist;
     uint64_t photon_begin, photon_end;
     PhotonSink sink;
+    uint32_t *absorber;  // per photon, or nullptr
     unsigned long long *stats;
 };
 
@@ -175,6 +176,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
                                 a.sink.overflow);
                     else
                         atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
+                    if (a.absorber) a.absorber[g] = ab;
                     if (absorbed) cnt[S_ABS]++;
                     else cnt[S_FORCED]++;
                 } else {
@@ -349,6 +351,19 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     }
     IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
     IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
+    // absorbing bin of every photon (4 B each) so the adjoint trace replays
+    // each photon once instead of twice; not kept above 2^31 photons or
+    // when HBM is short (the adjoint then finds absorbers with a pass of its own)
+    s.d_absorber.release();
+    if ((p.flags & IRTT_TRANSPORT_KEEP_ABSORBERS) && total <= (1ull << 31)) {
+        try {
+            s.d_absorber.alloc(total);
+            IRTT_HIP(hipMemsetAsync(s.d_absorber.p, 0xff, s.d_absorber.bytes(), stm));
+        } catch (const Error &e) {
+            if (e.code != IRTT_ERR_NOMEM) throw;
+            s.d_absorber.release();
+        }
+    }
 
     PhotonArgs a{};
     a.xyz = s.d_xyz.p;
@@ -373,6 +388,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     a.photon_begin = 0;
     a.photon_end = total;
     a.sink = sink;
+    a.absorber = s.d_absorber.p;
     a.stats = s.d_stats.p;
 
     int dev = 0, ncu = 256;
@@ -427,6 +443,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.Tdiag.assign(dcol.begin(), dcol.end());
     st.photons = total;
     st.rays = hstats[S_RAYS];
+    s.fwd_rays = st.rays;
     st.absorbed = hstats[S_ABS];
     st.miss = hstats[S_MISS];
     st.too_close = hstats[S_CLOSE];

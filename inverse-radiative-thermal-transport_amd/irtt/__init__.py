@@ -55,7 +55,7 @@ class TransportParams(ctypes.Structure):
     _fields_ = [("threads_per_tri", ctypes.c_uint32), ("rays_per_thread", ctypes.c_uint32),
                 ("batch_seed", ctypes.c_uint32), ("max_depth", ctypes.c_uint32),
                 ("tiny_dist", ctypes.c_float), ("layout", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 2)]
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class TransportStats(ctypes.Structure):
@@ -183,6 +183,7 @@ OP_DENSE_F16 = 2
 OP_SPARSE = 3
 LAYOUT_DENSE = 0
 LAYOUT_SPARSE = 1
+TRANSPORT_KEEP_ABSORBERS = 1
 ACA_FULL = 0
 ACA_PARTIAL = 1
 ACA_PLUS = 2
@@ -282,7 +283,7 @@ def make_materials(nt, rho_d=0.0, rho_s=0.0, thickness=1.0, density=1.0, heat_ca
 
 
 def transport_params(threads_per_tri=32768, rays_per_thread=8, batch_seed=1234, max_depth=512,
-                     tiny_dist=1e-3, layout=0):
+                     tiny_dist=1e-3, layout=0, flags=0):
     p = TransportParams()
     p.threads_per_tri = threads_per_tri
     p.rays_per_thread = rays_per_thread
@@ -290,6 +291,7 @@ def transport_params(threads_per_tri=32768, rays_per_thread=8, batch_seed=1234, 
     p.max_depth = max_depth
     p.tiny_dist = tiny_dist
     p.layout = layout
+    p.flags = flags
     return p
 
 

@@ -149,3 +149,26 @@ def test_project_gradient(irtt):
     # parameter 0 moves vertices 1 and 3 along z, parameter 1 moves vertex 2 along x
     dp = irtt.project_gradient(g, [1, 3, 2], [0, 0, 1], [[0, 0, 1], [0, 0, 1], [1, 0, 0]], 2)
     assert np.array_equal(dp, [g[1, 2] + g[3, 2], g[2, 0]])
+
+
+def test_adjoint_with_absorber_record_equals_two_passes(irtt):
+    """IRTT_TRANSPORT_KEEP_ABSORBERS: the forward build records each photon's
+    absorbing bin and the adjoint skips its first pass; the gradient and every
+    counter equal the two-pass replay's (FP64 atomics reorder: 1e-12)."""
+    from irtt import scenes
+    sd = scenes.city_block(seed=2, nx=2, ny=2, target_tris=1500)
+    rng = np.random.default_rng(3)
+    lam = rng.normal(size=sd.nv)
+    x4 = (290.0 + 10 * rng.random(sd.nv)) ** 4
+    out = []
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        for flags in (0, irtt.TRANSPORT_KEEP_ABSORBERS):
+            s.build_transport(threads_per_tri=32, rays_per_thread=4, batch_seed=9, flags=flags)
+            out.append(s.adjoint_trace(np.stack([lam, -lam]), np.stack([x4, x4 * 0.5]),
+                                       flags=23))
+    (g0, s0), (g1, s1) = out
+    for k in ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded", "rays"):
+        assert s0[k] == s1[k], (k, s0[k], s1[k])
+    err = np.abs(g0 - g1).max() / np.abs(g0).max()
+    print(f"record vs two passes: rel diff {err:.1e}; {s1}")
+    assert err < 1e-12
