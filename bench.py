@@ -192,6 +192,27 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
                         "designs": batch,
                         "linear_iters": fst["linear_iters"] + ast["linear_iters"],
                         "max_dT_vs_dense_fp32_K": float(np.abs(X[0] - x_ref).max())}
+        # adjoint trace of the same photons (dL/dvertex with the rows
+        # (lambda, x^4) and (-Ta^4, lambda)), absorbers kept by the build
+        q.flags = irtt.TRANSPORT_KEEP_ABSORBERS
+        s.build_transport(q)
+        s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+        s.solve(sp)
+        x = s.temperatures()
+        lam, _ = s.adjoint_solve(ea * (x - x.mean()), params=sp)
+        rows_l = np.stack([lam, np.full(len(x), -293.0 ** 4)])
+        rows_x = np.stack([x ** 4, lam])
+        tr = []
+        for _ in range(reps):
+            _, ast = s.adjoint_trace(rows_l, rows_x)
+            tr.append(ast)
+        ms = min(a["trace_ms"] for a in tr)
+        out["adjoint_trace"] = {"ms": round(ms, 2), "photons": tr[-1]["photons"],
+                                "segments": tr[-1]["segments"],
+                                "edge_hits": tr[-1]["edge_hits"],
+                                "occluded_edge_hits": tr[-1]["occluded"],
+                                "segments_per_s_M": round(tr[-1]["segments"] / ms / 1e3, 1),
+                                "forward_photon_kernel_ms": round(st["trace_ms"], 2)}
     out["unit"] = ("sparse layout of the city block operator: build, and forward steady + "
                    "adjoint solves per second (FP32 entries, FP64 sums)")
     return out

@@ -260,8 +260,18 @@ typedef struct irtt_solver_params {
     double ambient_kelvin;      /* < 0: no ambient exchange (reference) */
     double x_tol;               /* stop when max |dx| < x_tol (K) */
     double f_tol;               /* stop when ||r||^2 < f_tol (reference 1e-6) */
-    double lin_rtol;            /* BiCGSTAB relative tolerance */
+    double lin_rtol;            /* Krylov relative tolerance */
+    uint32_t krylov;            /* IRTT_KRYLOV_*: linear solver of the Newton steps */
+    uint32_t gmres_restart;     /* GMRES basis size before a restart; 0 -> 30 */
 } irtt_solver_params;
+
+/* Krylov method of irtt_solve / irtt_adjoint_solve (the batched solves use
+ * BiCGSTAB).  BiCGSTAB is the reference's (Eigen::BiCGSTAB,
+ * thermal_solver.hpp:55-56); GMRES(m) with the same right Jacobi
+ * preconditioner and stop rule is the alternative (the Jacobian is
+ * nonsymmetric, so CG does not apply). */
+#define IRTT_KRYLOV_BICGSTAB 0u
+#define IRTT_KRYLOV_GMRES 1u
 
 typedef struct irtt_solve_stats {
     uint32_t newton_iters, linear_iters, line_search_halvings, converged;

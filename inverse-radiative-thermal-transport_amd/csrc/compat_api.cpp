@@ -353,6 +353,12 @@ int simulate(float _step_size_hours, unsigned int _time_step_count, float *_time
         tp.batch_seed = 1234;  // constRandSeed / randSeed, thermal_common.hpp:126-127
         tp.max_depth = 0;
         tp.tiny_dist = 1e-3f;
+        // the reference's dense matrix overflows its indexing above 46 340
+        // bins (thermal_transport.cpp:143); larger scenes use the sparse
+        // layout, whose entries equal the dense ones
+        irtt_scene_info info{};
+        if (irtt_scene_info_get(g.scene, &info) == IRTT_OK && info.n_bins > 46340u)
+            tp.layout = IRTT_LAYOUT_SPARSE;
         int rc = irtt_build_transport(g.scene, &tp, nullptr);
         if (rc) return fail(rc, irtt_last_error());
         g.transport_built = true;

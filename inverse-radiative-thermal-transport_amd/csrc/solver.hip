@@ -9,7 +9,8 @@
 //   transient: r = M (x - x_prev) + dt (T x^4 + L x - b)
 //   b = external absorbed power + T_amb^4 * colsum(T)   (ambient exchange)
 // Fixed bins hold their value and have r = 0.  Newton steps solve J dx = -r
-// with BiCGSTAB (right Jacobi preconditioning) using matrix-free products;
+// with BiCGSTAB or restarted GMRES (right Jacobi preconditioning) using
+// matrix-free products;
 // the Jacobian is the exact one or the reference's row-scaled form
 // (thermal_solver.cpp:39-96).  A damped line search halves the step while the
 // residual norm grows (thermal_solver.cpp:192-203).  Operator products go
@@ -275,6 +276,7 @@ struct Ctx {
     double *part, *out;
     double *sc;  // device scalars of the Krylov iteration
     double hbuf[4];
+    uint32_t krylov = IRTT_KRYLOV_BICGSTAB, restart = 30;  // linear solver
     explicit Ctx(Scene &sc) : s(sc), st(sc.stream), nb(sc.nb), ld(sc.ld) {}
 
     // operator product of the residual side
@@ -396,6 +398,9 @@ static void setup(Scene &s, Ctx &c, const irtt_solver_params &p, FixedBuf &fb)
     y.cdr = op_diag_outside(s, true).empty() ? nullptr : c.v[V_CDR];
     y.cdj = op_diag_outside(s, false).empty() ? nullptr : c.v[V_CDJ];
     y.c_unscaled = (s.op_kind == IRTT_OP_ACA && y.jmode == J_REFERENCE) ? 1 : 0;
+    if (p.krylov > IRTT_KRYLOV_GMRES) throw Error(IRTT_ERR_ARG, "unknown Krylov method");
+    c.krylov = p.krylov;
+    c.restart = p.gmres_restart ? p.gmres_restart : 30;
 }
 
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r)
@@ -592,6 +597,192 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
     return std::min<uint32_t>((uint32_t)hs[1], max_lin);
 }
 
+// ---- restarted GMRES(m) ---------------------------------------------------
+namespace {
+// part[i * nparts + b] = partial (V_i . w) over block b's rows, i < nv
+__global__ __launch_bounds__(kRedThreads) void k_gm_dots(uint32_t n, const double *__restrict__ V,
+                                                         uint32_t ld, const double *__restrict__ w,
+                                                         double *__restrict__ part)
+{
+    __shared__ double sm[kRedThreads];
+    const double *v = V + (size_t)blockIdx.y * ld;
+    double a = 0.0;
+    for (uint32_t i = blockIdx.x * kRedThreads + threadIdx.x; i < n; i += gridDim.x * kRedThreads)
+        a += v[i] * w[i];
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = sm[0];
+}
+// out[i] = sum_b part[i * nparts + b] in block order (deterministic)
+__global__ void k_gm_sums(const double *__restrict__ part, uint32_t nparts, double *__restrict__ out)
+{
+    __shared__ double sm[kRedThreads];
+    double a = 0.0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += kRedThreads)
+        a += part[(size_t)blockIdx.x * nparts + b];
+    sm[threadIdx.x] = a;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+// w -= sum_{i < nv} h_i V_i
+__global__ void k_gm_update(uint32_t n, const double *__restrict__ V, uint32_t ld, uint32_t nv,
+                            const double *__restrict__ h, double *__restrict__ w)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double a = w[r];
+    for (uint32_t i = 0; i < nv; i++) a -= h[i] * V[(size_t)i * ld + r];
+    w[r] = a;
+}
+// out = a * s
+__global__ void k_gm_scale(uint32_t n, const double *__restrict__ a, double s,
+                           double *__restrict__ out)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = a[r] * s;
+}
+// out = a .* b
+__global__ void k_gm_mul(uint32_t n, const double *__restrict__ a, const double *__restrict__ b,
+                         double *__restrict__ out)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n) out[r] = a[r] * b[r];
+}
+// dx += dinv .* sum_{i < nv} y_i V_i
+__global__ void k_gm_combine(uint32_t n, const double *__restrict__ V, uint32_t ld, uint32_t nv,
+                             const double *__restrict__ y, const double *__restrict__ dinv,
+                             double *__restrict__ dx)
+{
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double a = 0.0;
+    for (uint32_t i = 0; i < nv; i++) a += y[i] * V[(size_t)i * ld + r];
+    dx[r] += dinv[r] * a;
+}
+}  // namespace
+
+// GMRES(m) (Saad & Schultz 1986) on J dx = -r with right Jacobi
+// preconditioning, the same system, preconditioner, start (dx = 0) and stop
+// rule (||r_lin||^2 <= lin_rtol^2 rn) as bicgstab.  Arnoldi by classical
+// Gram-Schmidt applied twice (one batched dot kernel and one update kernel
+// per pass, instead of one pair per basis vector); the small Hessenberg
+// least-squares problem (Givens rotations) runs on the host.  Returns the
+// iteration count (one operator product each).
+static uint32_t gmres(Ctx &c, uint32_t max_lin, double lin_rtol, double rn, uint32_t m)
+{
+    Scene &s = c.s;
+    const uint32_t nb = c.nb, ld = c.ld, g = cdiv(nb, 256);
+    hipStream_t stm = c.st;
+    m = std::max<uint32_t>(1, std::min<uint32_t>(m, 200));
+    DBuf<double> V, part, dev;
+    V.alloc((size_t)(m + 1) * ld);
+    part.alloc((size_t)(m + 1) * kRedBlocks);
+    dev.alloc(2 * (size_t)(m + 1) + 1 + m);
+    double *hA = dev.p, *hB = dev.p + (m + 1), *nrm = dev.p + 2 * (m + 1), *ydev = nrm + 1;
+    k_jdiag<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_TDIAG], c.v[V_LDIAG], c.v[V_DINV]);
+    IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
+    const double goal = lin_rtol * lin_rtol * rn;
+    double *w = c.v[V_T], *z = c.v[V_PH], *rl = c.v[V_XT];
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), gv(m + 1), hbuf(2 * (m + 1) + 1);
+    uint32_t it = 0;
+    // one pass of Gram-Schmidt of w against V_0..V_{nv-1}; coefficients into h
+    auto cgs = [&](uint32_t nv, double *h) {
+        k_gm_dots<<<dim3(kRedBlocks, nv), kRedThreads, 0, stm>>>(nb, V.p, ld, w, part.p);
+        k_gm_sums<<<nv, kRedThreads, 0, stm>>>(part.p, kRedBlocks, h);
+        k_gm_update<<<g, 256, 0, stm>>>(nb, V.p, ld, nv, h, w);
+    };
+    auto norm2 = [&](const double *a, double *dst) {
+        k_gm_dots<<<dim3(kRedBlocks, 1), kRedThreads, 0, stm>>>(nb, a, ld, a, part.p);
+        k_gm_sums<<<1, kRedThreads, 0, stm>>>(part.p, kRedBlocks, dst);
+    };
+    while (it < max_lin) {
+        // linear residual rl = -r - J dx
+        if (it == 0) {
+            k_gm_scale<<<g, 256, 0, stm>>>(nb, c.v[V_R], -1.0, rl);
+        } else {
+            c.jv(c.v[V_DX], w);
+            k_axpy2<<<g, 256, 0, stm>>>(nb, 1.0, c.v[V_R], w, rl);  // r + J dx
+            k_gm_scale<<<g, 256, 0, stm>>>(nb, rl, -1.0, rl);
+        }
+        norm2(rl, nrm);
+        double b2 = 0.0;
+        IRTT_HIP(hipMemcpyAsync(&b2, nrm, sizeof(double), hipMemcpyDeviceToHost, stm));
+        IRTT_HIP(hipStreamSynchronize(stm));
+        if (!(b2 > goal) || !std::isfinite(b2)) break;
+        const double beta = std::sqrt(b2);
+        k_gm_scale<<<g, 256, 0, stm>>>(nb, rl, 1.0 / beta, V.p);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        uint32_t k = 0;
+        bool done = false;
+        for (; k < m && it < max_lin; k++) {
+            k_gm_mul<<<g, 256, 0, stm>>>(nb, c.v[V_DINV], V.p + (size_t)k * ld, z);
+            c.jv(z, w);
+            it++;
+            cgs(k + 1, hA);
+            cgs(k + 1, hB);
+            norm2(w, nrm);
+            IRTT_HIP(hipMemcpyAsync(hbuf.data(), dev.p, (2 * (m + 1) + 1) * sizeof(double),
+                                    hipMemcpyDeviceToHost, stm));
+            IRTT_HIP(hipStreamSynchronize(stm));
+            double *col = H.data() + (size_t)k * (m + 1);
+            for (uint32_t i = 0; i <= k; i++) col[i] = hbuf[i] + hbuf[(m + 1) + i];
+            const double hn = std::sqrt(std::max(hbuf[2 * (m + 1)], 0.0));
+            col[k + 1] = hn;
+            for (uint32_t i = 0; i < k; i++) {
+                const double a = cs[i] * col[i] + sn[i] * col[i + 1];
+                col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+                col[i] = a;
+            }
+            const double den = std::hypot(col[k], col[k + 1]);
+            cs[k] = den > 0.0 ? col[k] / den : 1.0;
+            sn[k] = den > 0.0 ? col[k + 1] / den : 0.0;
+            col[k] = den;
+            col[k + 1] = 0.0;
+            gv[k + 1] = -sn[k] * gv[k];
+            gv[k] = cs[k] * gv[k];
+            const double res2 = gv[k + 1] * gv[k + 1];
+            if (res2 <= goal || !(hn > 0.0) || !std::isfinite(res2)) {
+                k++;
+                done = true;
+                break;
+            }
+            k_gm_scale<<<g, 256, 0, stm>>>(nb, w, 1.0 / hn, V.p + (size_t)(k + 1) * ld);
+        }
+        // y = H^-1 g (upper triangular, k x k), dx += M^-1 V y
+        std::vector<double> y(k, 0.0);
+        for (int i = (int)k - 1; i >= 0; i--) {
+            double a = gv[i];
+            for (uint32_t j = i + 1; j < k; j++) a -= H[(size_t)j * (m + 1) + i] * y[j];
+            const double d = H[(size_t)i * (m + 1) + i];
+            y[i] = d != 0.0 ? a / d : 0.0;
+        }
+        if (k) {
+            IRTT_HIP(hipMemcpyAsync(ydev, y.data(), k * sizeof(double), hipMemcpyHostToDevice, stm));
+            k_gm_combine<<<g, 256, 0, stm>>>(nb, V.p, ld, k, ydev, c.v[V_DINV], c.v[V_DX]);
+        }
+        if (done) break;
+    }
+    IRTT_HIP(hipStreamSynchronize(stm));
+    (void)s;
+    return std::min(it, max_lin);
+}
+
+// the Krylov method of the solver parameters
+static uint32_t linsolve(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
+{
+    if (c.krylov == IRTT_KRYLOV_GMRES) return gmres(c, max_lin, lin_rtol, rn, c.restart);
+    return bicgstab(c, max_lin, lin_rtol, rn);
+}
+
 namespace {
 // r = fixed ? 0 : -rhs
 __global__ void k_neg_masked(Sys s, const double *__restrict__ rhs, double *__restrict__ r)
@@ -639,7 +830,7 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
     const uint32_t max_lin = p.max_linear ? p.max_linear : 200;
     const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
     double *res_lam = c.v[V_DX];
-    if (rn > 0.0) st.linear_iters = bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
+    if (rn > 0.0) st.linear_iters = linsolve(c, max_lin, krylov_rtol(s, lin_rtol), rn);
     else IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
     if (rn > 0.0 && s.op_kind == IRTT_OP_DENSE_F16) {
         // iterative refinement: r = J^T lambda - rhs with the FP32 operator,
@@ -655,7 +846,7 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
             c.dots(1, c.v[V_R], c.v[V_R], nullptr, nullptr, nullptr, nullptr, &r2);
             st.newton_iters = it + 1;
             if (r2 <= goal || !std::isfinite(r2)) break;
-            st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), r2);
+            st.linear_iters += linsolve(c, max_lin, krylov_rtol(s, lin_rtol), r2);
             k_axpy2<<<g, 256, 0, stm>>>(nb, 1.0, c.v[V_LAM], c.v[V_DX], c.v[V_LAM]);
         }
         res_lam = c.v[V_LAM];
@@ -704,7 +895,7 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
         // conduction, a close start otherwise.  (The reference relies on its
         // I/dt regulariser instead, thermal_solver.cpp:150.)
         c.sys.jmode = J_PREDICT;
-        st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
+        st.linear_iters += linsolve(c, max_lin, krylov_rtol(s, lin_rtol), rn);
         k_y_update<<<g, 256, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_DX], c.v[V_X]);
         c.sys.jmode = J_EXACT;
         rn = c.residual(c.v[V_X]);
@@ -715,7 +906,7 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
             st.converged = 1;
             break;
         }
-        st.linear_iters += bicgstab(c, max_lin, krylov_rtol(s, lin_rtol), rn);
+        st.linear_iters += linsolve(c, max_lin, krylov_rtol(s, lin_rtol), rn);
         // damped line search, thermal_solver.cpp:192-203
         double a = 1.0;
         k_axpy2<<<g, 256, 0, stm>>>(nb, a, c.v[V_X], c.v[V_DX], c.v[V_XT]);

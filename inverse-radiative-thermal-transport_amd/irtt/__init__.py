@@ -76,7 +76,8 @@ class SolverParams(ctypes.Structure):
                 ("max_newton", ctypes.c_uint32), ("max_linear", ctypes.c_uint32),
                 ("dt", ctypes.c_double), ("ambient_kelvin", ctypes.c_double),
                 ("x_tol", ctypes.c_double), ("f_tol", ctypes.c_double),
-                ("lin_rtol", ctypes.c_double)]
+                ("lin_rtol", ctypes.c_double), ("krylov", ctypes.c_uint32),
+                ("gmres_restart", ctypes.c_uint32)]
 
 
 class SolveStats(ctypes.Structure):
@@ -295,9 +296,16 @@ def transport_params(threads_per_tri=32768, rays_per_thread=8, batch_seed=1234, 
     return p
 
 
+KRYLOV_BICGSTAB = 0
+KRYLOV_GMRES = 1
+
+
 def solver_params(steady=True, jacobian=JAC_EXACT, max_newton=100, max_linear=300, dt=300.0,
-                  ambient_kelvin=-1.0, x_tol=1e-5, f_tol=1e-12, lin_rtol=1e-10):
+                  ambient_kelvin=-1.0, x_tol=1e-5, f_tol=1e-12, lin_rtol=1e-10, krylov=0,
+                  gmres_restart=0):
     p = SolverParams()
+    p.krylov = krylov
+    p.gmres_restart = gmres_restart
     p.steady = 1 if steady else 0
     p.jacobian = jacobian
     p.max_newton = max_newton

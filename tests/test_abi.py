@@ -31,7 +31,7 @@ def test_struct_layouts(irtt):
     assert ctypes.sizeof(irtt.Material) == 32
     assert irtt.MATERIAL_DTYPE.itemsize == 32
     assert ctypes.sizeof(irtt.TransportParams) == 32
-    assert ctypes.sizeof(irtt.SolverParams) == 56
+    assert ctypes.sizeof(irtt.SolverParams) == 64
     assert ctypes.sizeof(irtt.AcaParams) == 32
     assert ctypes.sizeof(irtt.AcaStats) == 48
 

@@ -118,3 +118,21 @@ def test_view_factor_plates_full_size(irtt):
     print(f"F_AB mc={frac:.6f} closed={F:.6f} sigma={sig:.2e} rays={st['rays']} "
           f"trace_ms={st['trace_ms']:.1f} Mrays/s={st['rays'] / st['trace_ms'] / 1e3:.1f}")
     assert abs(frac - F) < 4 * sig
+    # reciprocity A_A F_AB = A_B F_BA (equal plates: F_AB = F_BA), the two
+    # estimates independent; per triangle pair, A_j F_ij - A_i F_ji against
+    # its Monte-Carlo noise (counts n_ij, n_ji: sigma ~ sqrt(n_ij + n_ji))
+    frac_ba = float(acc[:na, na:].sum()) * Q / 5.670374419e-8
+    rec = abs(frac - frac_ba)
+    print(f"reciprocity: F_AB={frac:.6f} F_BA={frac_ba:.6f} |diff|={rec:.2e} "
+          f"(sigma of the difference {np.sqrt(2) * sig:.2e})")
+    assert rec < 4 * np.sqrt(2) * sig
+    blk = acc[na:, :na]
+    # every triangle has the same area, hence the same fixed-point photon
+    # weight q: the gcd of the entries (counts x q)
+    q = float(np.gcd.reduce(blk[blk > 0].ravel()))
+    n_ij = blk.astype(np.float64) / q
+    n_ji = acc[:na, na:].T.astype(np.float64) / q
+    z = (n_ij - n_ji) / np.sqrt(np.maximum(n_ij + n_ji, 1.0))
+    print(f"pairwise reciprocity z-scores: rms {np.sqrt((z ** 2).mean()):.3f}, "
+          f"max {np.abs(z).max():.2f} over {z.size} pairs")
+    assert 0.8 < np.sqrt((z ** 2).mean()) < 1.2

@@ -112,3 +112,71 @@ def test_transient_step_vs_fp64(irtt):
     err = np.abs(x1 - x64).max()
     print(f"transient step: max|dT| vs FP64 = {err:.3e} K, mean rise {np.mean(x1 - x0):.4f} K; {st}")
     assert st["converged"] and err < TOL_K
+
+
+@pytest.mark.parametrize("restart", [30, 5])
+def test_gmres_steady_conduction_and_adjoint_vs_fp64(irtt, restart):
+    """IRTT_KRYLOV_GMRES: the Newton steps and the adjoint solve with
+    restarted GMRES (same preconditioner and stop rule as BiCGSTAB), on the
+    conduction + fixed-bin case against the FP64 Newton, and the transposed
+    solve against a dense FP64 solve.  restart 5 exercises restarts."""
+    from irtt import scenes
+    sd = scenes.closed_box(4, emissivity=0.85)
+    sd.conductivity[:] = 1.5
+    sd.thickness[:] = 0.1
+    sd.heat_flux[:32] = 300.0
+    sd.kelvin[:] = 290.0
+    sd.fixed[:25] = 1
+    p = irtt.solver_params(ambient_kelvin=-1.0, x_tol=1e-9, f_tol=1e-18, lin_rtol=1e-12,
+                           krylov=irtt.KRYLOV_GMRES, gmres_restart=restart, max_linear=400)
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=256, rays_per_thread=4, batch_seed=5)
+        T = s.transport().astype(np.float64)
+        b = _oracle_rhs(s, sd, 0, -1.0)
+        st = s.solve(p)
+        x = s.temperatures()
+        rng = np.random.default_rng(1)
+        rhs = rng.normal(size=s.n_bins)
+        lam, ast = s.adjoint_solve(rhs, params=p)
+        # BiCGSTAB on the same system for comparison
+        s.set_vertex_state(sd.kelvin, sd.fixed)
+        s.solve(irtt.solver_params(ambient_kelvin=-1.0, x_tol=1e-9, f_tol=1e-18, lin_rtol=1e-12))
+        xb = s.temperatures()
+    L = physics.cotan_laplacian(sd.xyz, sd.idx, sd.conductivity * sd.thickness, sd.nv)
+    fixed = sd.fixed.astype(bool)
+    x64, _ = solver.newton(T, b, fixed, sd.kelvin.astype(np.float64),
+                           sd.kelvin.astype(np.float64), lap=L, tol=1e-12)
+    err = np.abs(x - x64).max()
+    J = T * (4.0 * x ** 3)[None, :] + (L.toarray() if hasattr(L, "toarray") else L)
+    f = ~fixed
+    l64 = np.zeros_like(x)
+    l64[f] = np.linalg.solve(J[np.ix_(f, f)].T, rhs[f])
+    rel = np.abs(lam - l64).max() / np.abs(l64).max()
+    print(f"GMRES({restart}): max |dT| vs FP64 {err:.2e} K, vs BiCGSTAB "
+          f"{np.abs(x - xb).max():.2e} K, adjoint rel {rel:.2e}; {st}; adjoint {ast}")
+    assert st["converged"] and err < 1e-6 and np.abs(x - xb).max() < 1e-6
+    # the Krylov stop rule bounds the residual at 1e-12 relative; the solution
+    # error is that times the conditioning of J (about 1e4 here)
+    assert rel < 1e-7
+
+
+def test_gmres_on_sparse_city(irtt):
+    from irtt import scenes
+    sd = scenes.city_block(seed=6, nx=2, ny=2, target_tris=2500)
+    src = physics.bins(sd, 0)[3]
+    out = []
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=64, rays_per_thread=4, batch_seed=2,
+                          layout=irtt.LAYOUT_SPARSE)
+        T = s.transport().astype(np.float64)
+        cs = s.column_sums()
+        for kr in (irtt.KRYLOV_BICGSTAB, irtt.KRYLOV_GMRES):
+            s.set_temperatures(np.zeros(s.n_bins))
+            st = s.solve(irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-9, f_tol=1e-18,
+                                            lin_rtol=1e-12, krylov=kr))
+            out.append((s.temperatures(), st))
+    n = T.shape[0]
+    x64 = solver.steady_linear(T, solver.rhs(src, cs, 293.0), np.zeros(n, bool), np.zeros(n))
+    for x, st in out:
+        print(f"{st}: max |dT| vs FP64 {np.abs(x - x64).max():.2e} K")
+        assert st["converged"] and np.abs(x - x64).max() < 1e-7
