@@ -71,13 +71,22 @@ __device__ __forceinline__ V3 load_v(const float *xyz, uint32_t v)
     return V3{xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
 }
 
+// waves per SIMD that the LDS stack admits (160 KiB of LDS per CU, four
+// SIMDs), capped at 6: the register budget is held to that occupancy
+constexpr int photon_waves(int stack)
+{
+    return (160 * 1024 / (stack * 4 * kTraceBlock)) / 4 < 6
+               ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
+               : 6;
+}
+
 // Each loop iteration traces one ray segment per lane.  A lane whose photon
 // ends (absorbed, missed, dropped) starts its next photon in the following
 // iteration, so lanes do not wait for the longest path of the wave.  A lane
 // takes photons g, g + stride, ...; lanes of a wave start on consecutive
 // photons of the same source triangle (coherent first-bounce rays).
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock) void k_photons(PhotonArgs a)
+__global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(PhotonArgs a)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
     __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
