@@ -80,11 +80,119 @@ constexpr int photon_waves(int stack)
                : 6;
 }
 
-// Each loop iteration traces one ray segment per lane.  A lane whose photon
-// ends (absorbed, missed, dropped) starts its next photon in the following
-// iteration, so lanes do not wait for the longest path of the wave.  A lane
-// takes photons g, g + stride, ...; lanes of a wave start on consecutive
-// photons of the same source triangle (coherent first-bounce rays).
+// Photon g's first ray (getEmissionRay, thermal_tracing_unified.glsl:125-161,
+// seed at :291): origin, direction, RNG state after it, emitting bin and
+// triangle.  False for a degenerate (zero-area) emitter.
+__device__ __forceinline__ bool emit_photon(const PhotonArgs &a, uint64_t g, uint32_t &seed, V3 &o,
+                                            V3 &d, uint32_t &src, uint32_t &tri)
+{
+    uint32_t t = (uint32_t)(g / a.per_tri);
+    uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+    uint32_t x = k / a.rays_per_thread, nn = k - x * a.rays_per_thread;
+    seed = tea16(a.nt * x + t, a.batch_seed + nn);
+    uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
+    V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
+    float u0 = rnd(seed), u1 = rnd(seed);
+    (void)rnd(seed);
+    float su = sqrt_rn(u0);
+    float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
+    V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
+    float len;
+    V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
+    o = offset_ray(pos, nrm);
+    d = nrm;
+    if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
+        float r0 = rnd(seed), r1 = rnd(seed);
+        (void)rnd(seed);
+        d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
+    }
+    if (!(len > 0.0f)) return false;
+    int sc = max_corner(b0, b1, b2);
+    src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
+    tri = t;
+    return true;
+}
+
+// The answered ray segment `depth` (1 = emission ray) of photon g: either ends
+// the photon and returns the counter slot of how (S_MISS, S_CLOSE, S_BACK,
+// S_ABS, or S_FORCED at max_depth; absorptions add the photon's weight qt to
+// (absorbing bin, src)), or turns o, d, seed into its next ray (getNextRay,
+// thermal_tracing_unified.glsl:163-197) and returns -1.
+__device__ __forceinline__ int shade_segment(const PhotonArgs &a, const Hit &h, uint64_t g,
+                                             uint32_t src, unsigned long long qt, uint32_t depth,
+                                             uint32_t &seed, V3 &o, V3 &d)
+{
+    if (h.prim < 0) return S_MISS;
+    if (h.t < a.tiny_dist) return S_CLOSE;
+    uint32_t hp = (uint32_t)h.prim;
+    uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+    V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
+    V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
+    if (dot3(hn, d) > -1e-3f) return S_BACK;
+    float hb0 = 1.0f - h.b1 - h.b2;
+    V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+    o = offset_ray(hpos, hn);
+    float u = rnd(seed);
+    float rd = a.rho_d[hp], rs = a.rho_s[hp];
+    bool absorbed = false;
+    if (u >= rd + rs) {
+        absorbed = true;
+    } else if (u >= rd) {
+        V3 in = normalize3(d);
+        float dn = 2.0f * dot3(hn, in);
+        d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
+    } else {
+        float r0 = rnd(seed), r1 = rnd(seed);
+        (void)rnd(seed);
+        d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
+    }
+    if (!absorbed && depth != a.max_depth) return -1;
+    // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
+    int hc = max_corner(hb0, h.b1, h.b2);
+    uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
+    if (a.sink.tab)
+        tab_add(a.sink.tab, a.sink.lg, tab_key(ab, src, a.sink.key_bits), qt, a.sink.overflow);
+    else
+        atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
+    if (a.absorber) a.absorber[g] = ab;
+    return absorbed ? S_ABS : S_FORCED;
+}
+
+// counts one answered segment and its outcome (constant indices only, so the
+// counters stay in registers)
+__device__ __forceinline__ void count_segment(uint32_t *cnt, int outcome)
+{
+    cnt[S_RAYS]++;
+#pragma unroll
+    for (int k = S_ABS; k < S_COUNT; k++) cnt[k] += outcome == k ? 1u : 0u;
+}
+
+// wave sums of the photon counters into stats (one wave per block)
+__device__ __forceinline__ void flush_counts(unsigned long long *stats, const uint32_t *cnt,
+                                             uint32_t nvis, uint32_t ntri)
+{
+#pragma unroll
+    for (int s = 0; s < S_COUNT; s++) {
+        uint32_t v = cnt[s];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (threadIdx.x == 0 && v) atomicAdd(stats + s, (unsigned long long)v);
+    }
+    unsigned long long nv64 = nvis, nt64 = ntri;
+    for (int off = 32; off > 0; off >>= 1) {
+        nv64 += __shfl_down(nv64, off, 64);
+        nt64 += __shfl_down(nt64, off, 64);
+    }
+    if (threadIdx.x == 0 && (nv64 || nt64)) {
+        atomicAdd(stats + S_COUNT, nv64);
+        atomicAdd(stats + S_COUNT + 1, nt64);
+    }
+}
+
+// One kernel for whole photon paths.  Each loop iteration traces one ray
+// segment per lane; a lane whose photon ends starts its next photon in the
+// following iteration.  A lane takes photons g, g + stride, ...; lanes of a
+// wave start on consecutive photons of the same source triangle (coherent
+// first-bounce rays).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(PhotonArgs a)
 {
@@ -102,118 +210,31 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
     V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
     for (;;) {
         if (!live) {
-            // start this lane's next photon; degenerate emitters are skipped
+            // this lane's next photon; degenerate emitters are skipped
             while (g < a.photon_end) {
-                uint32_t t = (uint32_t)(g / a.per_tri);
-                uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
-                uint32_t x = k / a.rays_per_thread, nn = k - x * a.rays_per_thread;
-                // thermal_tracing_unified.glsl:291
-                seed = tea16(a.nt * x + t, a.batch_seed + nn);
-                uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
-                V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
-                // getEmissionRay, thermal_tracing_unified.glsl:125-161
-                float u0 = rnd(seed), u1 = rnd(seed);
-                (void)rnd(seed);
-                float su = sqrt_rn(u0);
-                float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
-                V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
-                float len;
-                V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
-                o = offset_ray(pos, nrm);
-                d = nrm;
-                if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
-                    float r0 = rnd(seed), r1 = rnd(seed);
-                    (void)rnd(seed);
-                    d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
+                uint32_t tri;
+                if (emit_photon(a, g, seed, o, d, src, tri)) {
+                    qt = a.q[tri];
+                    depth = 0;
+                    live = true;
+                    break;
                 }
-                if (!(len > 0.0f)) {
-                    cnt[S_DEG]++;
-                    g += stride;
-                    continue;
-                }
-                int sc = max_corner(b0, b1, b2);
-                src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
-                qt = a.q[t];
-                depth = 0;
-                live = true;
-                break;
+                cnt[S_DEG]++;
+                g += stride;
             }
             if (!live) break;
         }
-        // one ray segment of the photon in flight
         ++depth;
-        bool done = true;
         Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn, stt, nvis,
                                      ntri);
-        cnt[S_RAYS]++;
-        if (h.prim < 0) {
-            cnt[S_MISS]++;
-        } else if (h.t < a.tiny_dist) {
-            cnt[S_CLOSE]++;
-        } else {
-            uint32_t hp = (uint32_t)h.prim;
-            uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
-            V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
-            V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
-            if (dot3(hn, d) > -1e-3f) {
-                cnt[S_BACK]++;
-            } else {
-                float hb0 = 1.0f - h.b1 - h.b2;
-                // getNextRay, thermal_tracing_unified.glsl:163-197
-                V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
-                o = offset_ray(hpos, hn);
-                float u = rnd(seed);
-                float rd = a.rho_d[hp], rs = a.rho_s[hp];
-                bool absorbed = false;
-                if (u >= rd + rs) {
-                    absorbed = true;
-                } else if (u >= rd) {
-                    V3 in = normalize3(d);
-                    float dn = 2.0f * dot3(hn, in);
-                    d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
-                } else {
-                    float r0 = rnd(seed), r1 = rnd(seed);
-                    (void)rnd(seed);
-                    d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
-                }
-                if (absorbed || depth == a.max_depth) {
-                    // thermal_tracing_unified.glsl:578-583: row = absorber, column = source
-                    int hc = max_corner(hb0, h.b1, h.b2);
-                    uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
-                    if (a.sink.tab)
-                        tab_add(a.sink.tab, a.sink.lg, tab_key(ab, src, a.sink.key_bits), qt,
-                                a.sink.overflow);
-                    else
-                        atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
-                    if (a.absorber) a.absorber[g] = ab;
-                    if (absorbed) cnt[S_ABS]++;
-                    else cnt[S_FORCED]++;
-                } else {
-                    done = false;
-                }
-            }
-        }
-        if (done) {
+        const int outcome = shade_segment(a, h, g, src, qt, depth, seed, o, d);
+        count_segment(cnt, outcome);
+        if (outcome >= 0) {
             live = false;
             g += stride;
         }
     }
-    // wave-level reduction of the counters (one wave per block)
-    for (int s = 0; s < S_COUNT; s++) {
-        uint32_t v = cnt[s];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (threadIdx.x == 0 && v) atomicAdd(a.stats + s, (unsigned long long)v);
-    }
-    // traversal work: internal nodes fetched and triangles tested
-    unsigned long long nv64 = nvis, nt64 = ntri;
-    for (int off = 32; off > 0; off >>= 1) {
-        nv64 += __shfl_down(nv64, off, 64);
-        nt64 += __shfl_down(nt64, off, 64);
-    }
-    if (threadIdx.x == 0) {
-        atomicAdd(a.stats + S_COUNT, nv64);
-        atomicAdd(a.stats + S_COUNT + 1, nt64);
-    }
+    flush_counts(a.stats, cnt, nvis, ntri);
 }
 
 // T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
