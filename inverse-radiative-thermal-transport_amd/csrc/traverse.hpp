@@ -37,19 +37,37 @@ struct Hit {
     float t, b1, b2;
 };
 
-// conservative slab test; the far distance is widened by 1 + 2^-21 (>= 1 + 2*gamma_3).
-// Pruning against the best hit keeps a 1e-4 relative margin: box entry and
-// triangle distances round differently, and exact ties must still be visited.
+// Conservative slab test, one fused multiply-add per slab end:
+// t = fma(l, inv, -oi_l), where per ray and axis oi_l is o * inv moved by
+// s = 4u |o * inv| (u = 2^-24) so that the slab end the ray meets first comes
+// out no later, and the one it meets last no earlier, than (l - o) * inv
+// exactly (oi_lo for the box's low bound, oi_hi for its high bound; the shift
+// covers the rounding of o * inv and of the shift itself).  What remains is
+// the relative error of inv and of the fma (2u), covered by widening the far
+// distance by 1 + 2^-21.  Pruning against the best hit keeps a 1e-4 relative
+// margin: box entry and triangle distances round differently, and exact ties
+// must still be visited.
 __device__ __forceinline__ bool box_hit(float lx, float ly, float lz, float hx, float hy, float hz,
-                                        V3 o, V3 inv, float tmin, float tbest, float &tnear)
+                                        V3 inv, V3 oi_lo, V3 oi_hi, float tmin, float tbest,
+                                        float &tnear)
 {
-    float tx0 = (lx - o.x) * inv.x, tx1 = (hx - o.x) * inv.x;
-    float ty0 = (ly - o.y) * inv.y, ty1 = (hy - o.y) * inv.y;
-    float tz0 = (lz - o.z) * inv.z, tz1 = (hz - o.z) * inv.z;
+    float tx0 = fmaf(lx, inv.x, -oi_lo.x), tx1 = fmaf(hx, inv.x, -oi_hi.x);
+    float ty0 = fmaf(ly, inv.y, -oi_lo.y), ty1 = fmaf(hy, inv.y, -oi_hi.y);
+    float tz0 = fmaf(lz, inv.z, -oi_lo.z), tz1 = fmaf(hz, inv.z, -oi_hi.z);
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000048f;
     tnear = n;
     return (n <= f) && (n <= tbest * 1.0001f);
+}
+
+// o * inv shifted by 4u |o * inv| towards the near (lo: sign of inv) or far
+// (hi) side of a slab, for box_hit
+__device__ __forceinline__ void slab_offsets(float o, float inv, float &lo, float &hi)
+{
+    const float oi = o * inv;
+    const float sh = fabsf(oi) * 2.38418579e-7f;
+    lo = inv >= 0.0f ? oi + sh : oi - sh;
+    hi = inv >= 0.0f ? oi - sh : oi + sh;
 }
 
 __device__ __forceinline__ void leaf_test(const float4 *__restrict__ tris, uint32_t leaf,
@@ -78,7 +96,8 @@ __device__ __forceinline__ void leaf_test(const float4 *__restrict__ tris, uint3
 // lane's LDS stack, entry k at [k * kTraceBlock]; the stack must exceed the
 // tree depth (checked on the host).
 struct Query {
-    V3 o, inv;
+    V3 o, inv;           // origin, 1 / direction
+    V3 oi_lo, oi_hi;     // o * inv shifted for the low and high slab ends (slab_offsets)
     RayPre rp;
     Hit h;
     uint32_t cur;
@@ -103,6 +122,9 @@ __device__ __forceinline__ bool query_begin(Query &q, const float4 *__restrict__
     }
     q.inv = V3{1.0f / (d.x != 0.0f ? d.x : 1e-30f), 1.0f / (d.y != 0.0f ? d.y : 1e-30f),
                1.0f / (d.z != 0.0f ? d.z : 1e-30f)};
+    slab_offsets(o.x, q.inv.x, q.oi_lo.x, q.oi_hi.x);
+    slab_offsets(o.y, q.inv.y, q.oi_lo.y, q.oi_hi.y);
+    slab_offsets(o.z, q.inv.z, q.oi_lo.z, q.oi_hi.z);
     return true;
 }
 
@@ -128,8 +150,8 @@ __device__ __forceinline__ bool query_node(Query &q, const float4 *__restrict__ 
     const float4 *nd = nodes + 4 * (size_t)q.cur;
     float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
     float t0, t1;
-    bool h0 = box_hit(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, q.o, q.inv, tmin, q.h.t, t0);
-    bool h1 = box_hit(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, q.o, q.inv, tmin, q.h.t, t1);
+    bool h0 = box_hit(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, q.inv, q.oi_lo, q.oi_hi, tmin, q.h.t, t0);
+    bool h1 = box_hit(n1.z, n1.w, n2.x, n2.y, n2.z, n2.w, q.inv, q.oi_lo, q.oi_hi, tmin, q.h.t, t1);
     uint32_t c0 = __float_as_uint(n3.x), c1 = __float_as_uint(n3.y);
     if (h0 && h1) {
         uint32_t nr, fr;

@@ -234,26 +234,41 @@ int or_tri_hit(const float o[3], const float d[3], const float p0[3],
     return 1;
 }
 
-/* conservative slab test: far distance widened by 1 + 2^-21 (>= 1 + 2*gamma_3,
- * Ize 2013), zero direction components replaced by 1e-30 before inversion.
- * Boxes are pruned against the best hit only beyond a 1e-4 relative margin:
- * the box entry distance and the triangle distance round differently, and a
- * triangle that ties the best hit exactly must still be visited. */
-static inline int box_hit(const float *lo, const float *hi, const float o[3],
-                          const float inv[3], float tmin, float tbest, float *tnear)
+/* conservative slab test, one fused multiply-add per slab end:
+ * t = fma(l, inv, -oi_l), where per ray and axis oi_l is o * inv moved by
+ * s = 4u |o * inv| (u = 2^-24) so that the slab end the ray meets first comes
+ * out no later, and the one it meets last no earlier, than (l - o) * inv
+ * exactly; the remaining relative error of inv and of the fma (2u) is covered
+ * by widening the far distance by 1 + 2^-21.  Zero direction components are
+ * replaced by 1e-30 before inversion.  Boxes are pruned against the best hit
+ * only beyond a 1e-4 relative margin: the box entry distance and the triangle
+ * distance round differently, and a triangle that ties the best hit exactly
+ * must still be visited. */
+static inline int box_hit(const float *lo, const float *hi, const float inv[3],
+                          const float oi_lo[3], const float oi_hi[3], float tmin, float tbest,
+                          float *tnear)
 {
-    float tx0 = (lo[0] - o[0]) * inv[0], tx1 = (hi[0] - o[0]) * inv[0];
-    float ty0 = (lo[1] - o[1]) * inv[1], ty1 = (hi[1] - o[1]) * inv[1];
-    float tz0 = (lo[2] - o[2]) * inv[2], tz1 = (hi[2] - o[2]) * inv[2];
+    float tx0 = fmaf(lo[0], inv[0], -oi_lo[0]), tx1 = fmaf(hi[0], inv[0], -oi_hi[0]);
+    float ty0 = fmaf(lo[1], inv[1], -oi_lo[1]), ty1 = fmaf(hi[1], inv[1], -oi_hi[1]);
+    float tz0 = fmaf(lo[2], inv[2], -oi_lo[2]), tz1 = fmaf(hi[2], inv[2], -oi_hi[2]);
     float n = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
     float f = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1)) * 1.00000048f;
     *tnear = n;
     return (n <= f) && (n <= tbest * 1.0001f);
 }
 
-static inline void ray_inv(const float d[3], float inv[3])
+/* 1 / d and o * (1 / d) shifted by 4u |o / d| towards the near (lo: sign of
+ * 1 / d) or far (hi) side of each slab */
+static inline void ray_inv(const float o[3], const float d[3], float inv[3], float oi_lo[3],
+                           float oi_hi[3])
 {
-    for (int k = 0; k < 3; k++) inv[k] = 1.0f / (d[k] != 0.0f ? d[k] : 1e-30f);
+    for (int k = 0; k < 3; k++) {
+        inv[k] = 1.0f / (d[k] != 0.0f ? d[k] : 1e-30f);
+        float oi = o[k] * inv[k];
+        float sh = fabsf(oi) * 2.38418579e-7f;
+        oi_lo[k] = inv[k] >= 0.0f ? oi + sh : oi - sh;
+        oi_hi[k] = inv[k] >= 0.0f ? oi - sh : oi + sh;
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -487,8 +502,8 @@ static void trace_one(const or_bvh *bvh, const float o[3], const float d[3],
         if (h->prim < 0) h->t = tmax;
         return;
     }
-    float inv[3];
-    ray_inv(d, inv);
+    float inv[3], oi_lo[3], oi_hi[3];
+    ray_inv(o, d, inv, oi_lo, oi_hi);
     uint32_t stk[128];
     float stt[128];
     int sp = 0;
@@ -499,8 +514,8 @@ static void trace_one(const or_bvh *bvh, const float o[3], const float d[3],
             const uint32_t *nd = bvh->nodes + 16 * cur;
             const float *bx = (const float *)nd;
             float t0, t1;
-            int h0 = box_hit(bx, bx + 3, o, inv, tmin, h->t, &t0);
-            int h1 = box_hit(bx + 6, bx + 9, o, inv, tmin, h->t, &t1);
+            int h0 = box_hit(bx, bx + 3, inv, oi_lo, oi_hi, tmin, h->t, &t0);
+            int h1 = box_hit(bx + 6, bx + 9, inv, oi_lo, oi_hi, tmin, h->t, &t1);
             if (h0 && h1) {
                 uint32_t nr, fr;
                 float ft;
