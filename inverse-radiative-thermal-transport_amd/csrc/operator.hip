@@ -27,9 +27,31 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 // Rows per wave in the row-major products: the wave reads each input-vector
 // chunk once for kGemvRows rows, so the vector's cache traffic is a quarter
 // of the matrix's.
-constexpr uint32_t kGemvRows = 4;
+#ifndef IRTT_GEMV_ROWS
+#define IRTT_GEMV_ROWS 4
+#endif
+constexpr uint32_t kGemvRows = IRTT_GEMV_ROWS;
+// Loads issued before any of their products are summed: k steps of a wave in
+// the row-major products, rows of a thread in the column products (FP32 and
+// FP16 forms).  The sums keep their order, so results do not depend on these
+// counts.  Measured on the 36 475-bin city block operator (rows / rows
+// unrolled / FP32 column unroll): 4/1/1 977 and 1005 us, 4/2/8 908 and
+// 926 us, 8/2/4 940 and 938 us; FP16 column product 447 us unrolled 1,
+// 453-476 us unrolled 2-8.
+#ifndef IRTT_GEMV_UNROLL
+#define IRTT_GEMV_UNROLL 2
+#endif
+#ifndef IRTT_GEMVT_UNROLL
+#define IRTT_GEMVT_UNROLL 8
+#endif
+#ifndef IRTT_GEMV16T_UNROLL
+#define IRTT_GEMV16T_UNROLL 1
+#endif
+constexpr uint32_t kGemvUnroll = IRTT_GEMV_UNROLL;
+constexpr uint32_t kGemvTUnroll = IRTT_GEMVT_UNROLL;
+constexpr uint32_t kGemv16TUnroll = IRTT_GEMV16T_UNROLL;
 
-// y_i = sum_j T_ij u_j: four rows per wave, float4 loads, FP64 accumulation.
+// y_i = sum_j T_ij u_j: kGemvRows rows per wave, float4 loads, FP64 accumulation.
 __global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, uint32_t nb,
                                                    uint32_t ld, const double *__restrict__ u,
                                                    double *__restrict__ y)
@@ -38,19 +60,38 @@ __global__ __launch_bounds__(256) void k_gemv_rows(const float *__restrict__ T, 
     const uint32_t lane = threadIdx.x & 63;
     if (row0 >= nb) return;
     const float4 *r[kGemvRows];
+#pragma unroll
     for (uint32_t q = 0; q < kGemvRows; q++)
         r[q] = reinterpret_cast<const float4 *>(T + (size_t)min(row0 + q, nb - 1) * ld);
+    const double4 *u4 = reinterpret_cast<const double4 *>(u);
     double acc[kGemvRows] = {};
     const uint32_t n4 = ld >> 2;
-    for (uint32_t k = lane; k < n4; k += 64) {
-        const double4 uu = reinterpret_cast<const double4 *>(u)[k];
-        for (uint32_t q = 0; q < kGemvRows; q++) {
-            float4 v = r[q][k];
-            // the padded tail of each row is zero and u is padded with zeros
-            acc[q] += (double)v.x * uu.x + (double)v.y * uu.y + (double)v.z * uu.z +
-                      (double)v.w * uu.w;
+    for (uint32_t k0 = lane; k0 < n4; k0 += 64 * kGemvUnroll) {
+        float4 v[kGemvUnroll][kGemvRows];
+        double4 uu[kGemvUnroll];
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvUnroll; s++) {
+            const uint32_t k = k0 + 64 * s;
+            if (k < n4) {
+                uu[s] = u4[k];
+#pragma unroll
+                for (uint32_t q = 0; q < kGemvRows; q++) v[s][q] = r[q][k];
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvUnroll; s++) {
+            if (k0 + 64 * s < n4) {
+#pragma unroll
+                for (uint32_t q = 0; q < kGemvRows; q++) {
+                    // the padded tail of each row is zero and u is padded with zeros
+                    const float4 w = v[s][q];
+                    acc[q] += (double)w.x * uu[s].x + (double)w.y * uu[s].y +
+                              (double)w.z * uu[s].z + (double)w.w * uu[s].w;
+                }
+            }
         }
     }
+#pragma unroll
     for (uint32_t q = 0; q < kGemvRows; q++) {
         double a = acc[q];
         for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
@@ -71,13 +112,25 @@ __global__ __launch_bounds__(256) void k_gemv_t_partial(const float *__restrict_
     if (4 * j4 >= ld) return;
     uint32_t r0 = blockIdx.y * kTRows, r1 = min(r0 + kTRows, nb);
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    for (uint32_t i = r0; i < r1; i++) {
-        const float4 v = reinterpret_cast<const float4 *>(T + (size_t)i * ld)[j4];
-        const double w = u[i];
-        a0 += (double)v.x * w;
-        a1 += (double)v.y * w;
-        a2 += (double)v.z * w;
-        a3 += (double)v.w * w;
+    for (uint32_t i0 = r0; i0 < r1; i0 += kGemvTUnroll) {
+        float4 v[kGemvTUnroll];
+        double w[kGemvTUnroll];
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvTUnroll; s++) {
+            if (i0 + s < r1) {
+                v[s] = reinterpret_cast<const float4 *>(T + (size_t)(i0 + s) * ld)[j4];
+                w[s] = u[i0 + s];
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvTUnroll; s++) {
+            if (i0 + s < r1) {
+                a0 += (double)v[s].x * w[s];
+                a1 += (double)v[s].y * w[s];
+                a2 += (double)v[s].z * w[s];
+                a3 += (double)v[s].w * w[s];
+            }
+        }
     }
     double *o = part + (size_t)blockIdx.y * ld + 4 * j4;
     o[0] = a0;
@@ -143,22 +196,40 @@ __global__ __launch_bounds__(256) void k_gemv16_rows(const __half *__restrict__ 
     const uint32_t lane = threadIdx.x & 63;
     if (row0 >= nb) return;
     const uint4 *r[kGemvRows];
+#pragma unroll
     for (uint32_t q = 0; q < kGemvRows; q++)
         r[q] = reinterpret_cast<const uint4 *>(H + (size_t)min(row0 + q, nb - 1) * ld);
     const float4 *u4 = reinterpret_cast<const float4 *>(u);
     float acc[kGemvRows] = {};
     const uint32_t n8 = ld >> 3;
-    for (uint32_t k = lane; k < n8; k += 64) {
-        const float4 ua = u4[2 * k], ub = u4[2 * k + 1];
-        for (uint32_t q = 0; q < kGemvRows; q++) {
-            uint4 h = r[q][k];
-            const __half2 *hp = reinterpret_cast<const __half2 *>(&h);
-            float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
-            float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
-            acc[q] += f0.x * ua.x + f0.y * ua.y + f1.x * ua.z + f1.y * ua.w + f2.x * ub.x +
-                      f2.y * ub.y + f3.x * ub.z + f3.y * ub.w;
+    for (uint32_t k0 = lane; k0 < n8; k0 += 64 * kGemvUnroll) {
+        uint4 hv[kGemvUnroll][kGemvRows];
+        float4 ua[kGemvUnroll], ub[kGemvUnroll];
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvUnroll; s++) {
+            const uint32_t k = k0 + 64 * s;
+            if (k < n8) {
+                ua[s] = u4[2 * k];
+                ub[s] = u4[2 * k + 1];
+#pragma unroll
+                for (uint32_t q = 0; q < kGemvRows; q++) hv[s][q] = r[q][k];
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < kGemvUnroll; s++) {
+            if (k0 + 64 * s < n8) {
+#pragma unroll
+                for (uint32_t q = 0; q < kGemvRows; q++) {
+                    const __half2 *hp = reinterpret_cast<const __half2 *>(&hv[s][q]);
+                    float2 f0 = __half22float2(hp[0]), f1 = __half22float2(hp[1]);
+                    float2 f2 = __half22float2(hp[2]), f3 = __half22float2(hp[3]);
+                    acc[q] += f0.x * ua[s].x + f0.y * ua[s].y + f1.x * ua[s].z + f1.y * ua[s].w +
+                              f2.x * ub[s].x + f2.y * ub[s].y + f3.x * ub[s].z + f3.y * ub[s].w;
+                }
+            }
         }
     }
+#pragma unroll
     for (uint32_t q = 0; q < kGemvRows; q++) {
         double a = acc[q];
         for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
@@ -177,14 +248,27 @@ __global__ __launch_bounds__(256) void k_gemv16_t_partial(const __half *__restri
     if (8 * c8 >= ld) return;
     uint32_t r0 = blockIdx.y * kTRows, r1 = min(r0 + kTRows, nb);
     float acc[8] = {};
-    for (uint32_t i = r0; i < r1; i++) {
-        float w = (float)((double)scale[i] * u[i]);
-        uint4 h = reinterpret_cast<const uint4 *>(H + (size_t)i * ld)[c8];
-        const __half2 *hp = reinterpret_cast<const __half2 *>(&h);
-        for (int q = 0; q < 4; q++) {
-            float2 f = __half22float2(hp[q]);
-            acc[2 * q] += f.x * w;
-            acc[2 * q + 1] += f.y * w;
+    for (uint32_t i0 = r0; i0 < r1; i0 += kGemv16TUnroll) {
+        uint4 hv[kGemv16TUnroll];
+        float w[kGemv16TUnroll];
+#pragma unroll
+        for (uint32_t s = 0; s < kGemv16TUnroll; s++) {
+            if (i0 + s < r1) {
+                w[s] = (float)((double)scale[i0 + s] * u[i0 + s]);
+                hv[s] = reinterpret_cast<const uint4 *>(H + (size_t)(i0 + s) * ld)[c8];
+            }
+        }
+#pragma unroll
+        for (uint32_t s = 0; s < kGemv16TUnroll; s++) {
+            if (i0 + s < r1) {
+                const __half2 *hp = reinterpret_cast<const __half2 *>(&hv[s]);
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float2 f = __half22float2(hp[q]);
+                    acc[2 * q] += f.x * w[s];
+                    acc[2 * q + 1] += f.y * w[s];
+                }
+            }
         }
     }
     double *o = part + (size_t)blockIdx.y * ld + 8 * c8;
