@@ -16,7 +16,9 @@
 //         non-adjacent occluded surface receives +, the path's origin vertex +,
 //         the edge vertex - of the edge-irradiance derivative.
 // Contributions are FP32 (as the shader computes them), scaled and summed in
-// FP64 with hardware atomics on a 3 x n_vertices buffer.
+// FP64 with hardware atomics on a 3 x n_vertices buffer; the source triangle's
+// three vertices are summed in each lane's registers over its photons of that
+// triangle first.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -145,115 +147,304 @@ __device__ __forceinline__ double hist_factor(const AdjArgs &a, uint32_t bin_l, 
     return f;
 }
 
-// At least 3 waves per SIMD: the kernel would take 179 VGPRs (2 waves); capped
-// at 168 it spills 80 B per lane and runs 20 % faster on the city block (4
-// waves, 128 VGPRs, spills more and is slower again).
+// Photons per lane per chunk: a wave takes chunks of 64 * kAdjRun consecutive
+// photons, lane l the run [l * kAdjRun, (l + 1) * kAdjRun) of each, so a
+// lane's photons nearly always share their source triangle and the source
+// vertices' sums stay in the lane's registers between photons.
+constexpr uint32_t kAdjRun = 32;
+
+// what a lane does next: start a photon, trace a path segment, trace an
+// occlusion ray, end the current pass's path, nothing (no photons left)
+enum { M_NONE = 0, M_PATH, M_OCCL, M_END, M_DONE };
+
+// waves per SIMD: IRTT_ADJ_WAVES, or what the LDS stack admits (160 KiB per
+// CU, four SIMDs).  Measured on the city block (2^26 photons, all terms):
+// 2 waves (249 VGPRs) 40.7 ms, 3 (168 VGPRs, 208 B spilled per lane) 40.5 ms,
+// 4 (128 VGPRs, 368 B spilled) 56.5 ms.
+#ifndef IRTT_ADJ_WAVES
+#define IRTT_ADJ_WAVES 3
+#endif
+constexpr int adjoint_waves(int stack)
+{
+    return (160 * 1024 / (stack * 4 * kTraceBlock)) / 4 < IRTT_ADJ_WAVES
+               ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
+               : IRTT_ADJ_WAVES;
+}
+
+// One loop iteration traces one ray per lane: a segment of the photon path
+// being replayed, or the occlusion ray of the previous segment's edge term.
+// A lane whose photon ends starts its next photon in the next iteration, so
+// lanes do not wait for the longest path of the wave, and occlusion rays are
+// traced together with other lanes' path segments instead of by the few
+// lanes that need one.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
+__global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(AdjArgs a)
 {
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
     __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
+    const uint32_t lane = threadIdx.x;
     uint32_t cnt[A_COUNT] = {0, 0, 0, 0, 0, 0};
     uint32_t nvis = 0, ntri = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
-    for (uint64_t g = (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x; g < a.photons; g += stride) {
-        uint32_t t = (uint32_t)(g / a.per_tri);
-        uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
-        uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
-        const uint32_t seed0 = tea16(a.nt * xx + t, a.batch_seed + nn);
-        const uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
-        const V3 p0{a.xyz[3 * i0], a.xyz[3 * i0 + 1], a.xyz[3 * i0 + 2]};
-        const V3 p1{a.xyz[3 * i1], a.xyz[3 * i1 + 1], a.xyz[3 * i1 + 2]};
-        const V3 p2{a.xyz[3 * i2], a.xyz[3 * i2 + 1], a.xyz[3 * i2 + 2]};
-        uint32_t abs_bin = 0;
-        bool absorbed_any = false;
-        // area terms of this photon: coefficient of dw (the source triangle's
-        // photon-weight gradient), added to the gradient after the photon
-        double c_area = 0.0;
-        V3 dw[3] = {V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
-        // continuous-term gradient of the source triangle's vertices (first segment)
-        double src_acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        // pass 0 finds the absorbing bin; the forward build's record saves it
-        int first_pass = 0;
-        if (a.absorber) {
-            const uint32_t r = a.absorber[g];
-            absorbed_any = r != 0xffffffffu;
-            abs_bin = absorbed_any ? r : 0u;
-            if (absorbed_any) cnt[A_ABS]++;
-            first_pass = 1;
-        }
-        for (int pass = first_pass; pass <= 1; pass++) {
-            uint32_t seed = seed0;
-            float u0 = rnd(seed), u1 = rnd(seed);
+    const bool f_area = a.aflags & IRTT_ADJ_AREA, f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
+    const bool f_edge = a.aflags & IRTT_ADJ_EDGE, f_gain = a.aflags & IRTT_ADJ_EDGE_GAIN;
+
+    // this lane's run of photons [run_g, run_end) in chunk `chunk`
+    uint64_t chunk = blockIdx.x;
+    uint64_t run_g = 0, run_end = 0;
+    {
+        const uint64_t b0 = chunk * (uint64_t)kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
+        run_g = b0;
+        run_end = b0 < a.photons ? min(b0 + kAdjRun, a.photons) : b0;
+    }
+    // the source vertices' gradient of this lane's photons of triangle acc_t
+    uint32_t acc_t = 0xffffffffu;
+    double sacc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+
+    // the photon being replayed
+    uint64_t g = 0;
+    uint32_t t = 0, i0 = 0, i1 = 0, i2 = 0, src = 0, abs_bin = 0, seed0 = 0;
+    bool absorbed_any = false;
+    int pass = 0;
+    float wt = 0.0f;
+    double fpath = 0.0, c_area = 0.0;
+    V3 dw[3] = {V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
+    // its path
+    uint32_t seed = 0, depth = 0, prev0 = 0, prev1 = 0, prev2 = 0;
+    int prev_max = 0;
+    float pb1 = 0.0f, pb2 = 0.0f;
+    V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+    bool path_on = false;  // a next path segment follows the pending occlusion ray
+    // the pending occlusion ray's edge term
+    V3 od{0.0f, 0.0f, 1.0f}, dE{0.0f, 0.0f, 0.0f};
+    float oht = 0.0f;
+    uint32_t oj0 = 0, oj1 = 0, oj2 = 0, edge_v = 0, origin_v = 0;
+
+    int mode = M_NONE;
+    Query q;
+    bool tracing = false;
+    for (;;) {
+        // ---- lanes without a ray: end the current pass or photon, start the next ----
+        // ph: 0 = nothing pending, 1 = the current pass's path has ended, 2 = start a photon
+        int ph = mode == M_NONE ? 2 : (mode == M_END ? 1 : 0);
+        while (ph != 0) {
+            if (ph == 1) {
+                // the current pass's path has ended
+                if (pass == 0) {
+                    pass = 1;
+                } else {
+                    // the photon's area terms on its source vertices
+                    if (c_area != 0.0) {
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            if (!finite3(dw[c])) continue;
+                            sacc[3 * c] += c_area * (double)dw[c].x;
+                            sacc[3 * c + 1] += c_area * (double)dw[c].y;
+                            sacc[3 * c + 2] += c_area * (double)dw[c].z;
+                        }
+                    }
+                    ph = 2;
+                    continue;
+                }
+            } else {
+                // next photon of this lane
+                while (run_g >= run_end) {
+                    chunk += gridDim.x;
+                    const uint64_t b0 =
+                        chunk * (uint64_t)kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
+                    if (chunk * (uint64_t)kTraceBlock * kAdjRun >= a.photons) break;
+                    run_g = b0;
+                    run_end = b0 < a.photons ? min(b0 + kAdjRun, a.photons) : b0;
+                }
+                if (run_g >= run_end) {
+                    mode = M_DONE;
+                    break;
+                }
+                g = run_g++;
+                t = (uint32_t)(g / a.per_tri);
+                const uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+                const uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
+                // thermal_tracing_unified.glsl:291
+                seed0 = tea16(a.nt * xx + t, a.batch_seed + nn);
+                if (t != acc_t) {
+                    // another source triangle: flush the sums of the last one
+                    if (acc_t != 0xffffffffu) {
+#pragma unroll
+                        for (int e = 0; e < 9; e++)
+                            if (sacc[e] != 0.0)
+                                unsafeAtomicAdd(a.grad + 3 * (size_t)a.idx[3 * acc_t + e / 3] + e % 3,
+                                                sacc[e]);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 9; e++) sacc[e] = 0.0;
+                    acc_t = t;
+                }
+                i0 = a.idx[3 * t];
+                i1 = a.idx[3 * t + 1];
+                i2 = a.idx[3 * t + 2];
+                wt = a.w[t];
+                c_area = 0.0;
+                dw[0] = dw[1] = dw[2] = V3{0.0f, 0.0f, 0.0f};
+                absorbed_any = false;
+                abs_bin = 0;
+                pass = 0;
+                // pass 0 finds the absorbing bin; the forward build's record saves it
+                if (a.absorber) {
+                    const uint32_t r = a.absorber[g];
+                    absorbed_any = r != 0xffffffffu;
+                    abs_bin = absorbed_any ? r : 0u;
+                    if (absorbed_any) cnt[A_ABS]++;
+                    pass = 1;
+                }
+            }
+            // start pass `pass` of photon g: its emission ray
+            // (getEmissionRay, thermal_tracing_unified.glsl:125-161)
+            const V3 p0{a.xyz[3 * i0], a.xyz[3 * i0 + 1], a.xyz[3 * i0 + 2]};
+            const V3 p1{a.xyz[3 * i1], a.xyz[3 * i1 + 1], a.xyz[3 * i1 + 2]};
+            const V3 p2{a.xyz[3 * i2], a.xyz[3 * i2 + 1], a.xyz[3 * i2 + 2]};
+            seed = seed0;
+            const float u0 = rnd(seed), u1 = rnd(seed);
             (void)rnd(seed);
-            float su = sqrt_rn(u0);
-            float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
-            V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
+            const float su = sqrt_rn(u0);
+            const float b0 = 1.0f - su, b1 = su * (1.0f - u1), b2 = su * u1;
+            const V3 pos = bary_point(p0, p1, p2, b0, b1, b2);
             float len;
-            V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
-            V3 o = offset_ray(pos, nrm);
-            V3 d = nrm;
+            const V3 nrm = normalize3(cross3(p1 - p0, p2 - p0), len);
+            o = offset_ray(pos, nrm);
+            d = nrm;
             if (a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION) {
-                float r0 = rnd(seed), r1 = rnd(seed);
+                const float r0 = rnd(seed), r1 = rnd(seed);
                 (void)rnd(seed);
                 d = tangent_to_world(cosine_hemisphere(r0, r1), nrm);
             }
-            if (!(len > 0.0f)) break;
-            int sc = max_corner(b0, b1, b2);
-            const uint32_t src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
-            const float wt = a.w[t];
-            if (pass == 1 && (a.aflags & IRTT_ADJ_AREA)) {
+            if (!(len > 0.0f)) {
+                // degenerate emitter: no contribution
+                c_area = 0.0;
+                pass = 1;
+                ph = 1;
+                continue;
+            }
+            const int sc = max_corner(b0, b1, b2);
+            src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
+            if (pass == 1 && f_area) {
                 // emission-side area term, thermal_tracing_unified.glsl:327-351
                 V3 ga[3];
                 area_grad(p0, p1, p2, ga);
-                float s = wt / a.area[t];
-                for (int c = 0; c < 3; c++) dw[c] = scl3(ga[c], s);
+                const float sa = wt / a.area[t];
+#pragma unroll
+                for (int c = 0; c < 3; c++) dw[c] = scl3(ga[c], sa);
                 c_area -= hist_factor(a, src, src);
             }
-            if (pass == 1 && !absorbed_any && !(a.aflags & IRTT_ADJ_REFERENCE_DROPPED)) break;
-            const double fpath = pass ? hist_factor(a, abs_bin, src) : 0.0;
-            uint32_t prev0 = i0, prev1 = i1, prev2 = i2;
-            float pb1 = b1, pb2 = b2;
-            int prev_max = sc;
-            uint32_t depth = 0;
-            for (;;) {
-                ++depth;
-                if (depth > a.max_depth) break;
-                Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn,
-                                             stt, nvis, ntri);
-                if (pass == 0) cnt[A_RAYS]++;
-                if (h.prim < 0 || h.t < a.tiny_dist) break;
-                const uint32_t hp = (uint32_t)h.prim;
-                const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
-                const V3 q0{a.xyz[3 * j0], a.xyz[3 * j0 + 1], a.xyz[3 * j0 + 2]};
-                const V3 q1{a.xyz[3 * j1], a.xyz[3 * j1 + 1], a.xyz[3 * j1 + 2]};
-                const V3 q2{a.xyz[3 * j2], a.xyz[3 * j2 + 1], a.xyz[3 * j2 + 2]};
-                V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
-                if (dot3(hn, d) > -1e-3f) break;
+            if (pass == 1 && !absorbed_any && !(a.aflags & IRTT_ADJ_REFERENCE_DROPPED)) {
+                ph = 1;
+                continue;
+            }
+            fpath = pass ? hist_factor(a, abs_bin, src) : 0.0;
+            prev0 = i0;
+            prev1 = i1;
+            prev2 = i2;
+            pb1 = b1;
+            pb2 = b2;
+            prev_max = sc;
+            depth = 1;
+            mode = M_PATH;
+            tracing = query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri);
+            ph = 0;
+        }
+        if (__ballot(mode != M_DONE) == 0) break;
+
+        // ---- every lane with a ray traces it ----
+        query_run(q, tracing && mode != M_DONE, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
+        if (mode == M_DONE) continue;
+        const Hit h = query_end(q, 1e12f);
+        tracing = false;
+        int next = 1;  // 0: the pass's path ends, 1: its next segment, 2: occlusion ray begun
+
+        if (mode == M_OCCL) {
+            // the edge term of the previous segment, thermal_tracing_unified.glsl:454-567
+            const bool valid = h.prim >= 0 && h.t >= a.tiny_dist;
+            bool nbr = false;
+            uint32_t shadow_v = 0;
+            if (valid) {
+                const uint32_t op = (uint32_t)h.prim;
+                const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1], c2 = a.idx[3 * op + 2];
+                nbr = c0 == oj0 || c0 == oj1 || c0 == oj2 || c1 == oj0 || c1 == oj1 || c1 == oj2 ||
+                      c2 == oj0 || c2 == oj1 || c2 == oj2;
+                const V3 s0{a.xyz[3 * c0], a.xyz[3 * c0 + 1], a.xyz[3 * c0 + 2]};
+                const V3 s1{a.xyz[3 * c1], a.xyz[3 * c1 + 1], a.xyz[3 * c1 + 2]};
+                const V3 s2{a.xyz[3 * c2], a.xyz[3 * c2 + 1], a.xyz[3 * c2 + 2]};
+                const V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
+                const float ob0 = 1.0f - h.b1 - h.b2;
+                const int sm = max_corner(ob0, h.b1, h.b2);
+                shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
+                if (!nbr && dot3(tn, od) < -1e-3f) {
+                    cnt[A_OCCL]++;
+                    const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
+                    const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
+                    const float d1 = oht, d2 = h.t, tot = d1 + d2;
+                    const double fs = hist_factor(a, shadow_bin, src);
+                    acc3(a.grad, shadow_v, scl3(dE, sa * (d1 / tot)), fs);
+                    acc3(a.grad, origin_v, scl3(dE, sa * (d2 / tot)), fs);
+                    acc3(a.grad, edge_v, scl3(dE, -sa), fs);
+                }
+            }
+            // Extension (not in the reference): the power the moving edge
+            // takes from the shadow side lands on the hit triangle and follows
+            // this photon's path, so it is credited to the path's absorbing
+            // bin with the opposite sign, also when nothing lies behind the
+            // edge (the power escaped before).
+            if (f_gain && (h.prim < 0 || (valid && !nbr))) {
+                const float f_or = valid ? h.t / (oht + h.t) : 1.0f;
+                acc3(a.grad, edge_v, dE, fpath);
+                acc3(a.grad, origin_v, scl3(dE, f_or), -fpath);
+                if (valid) acc3(a.grad, shadow_v, scl3(dE, 1.0f - f_or), -fpath);
+            }
+            next = path_on ? 1 : 0;
+        } else {
+            // a path segment
+            if (pass == 0) cnt[A_RAYS]++;
+            bool stop = h.prim < 0 || h.t < a.tiny_dist;
+            uint32_t hp = 0, j0 = 0, j1 = 0, j2 = 0;
+            V3 q0{0.0f, 0.0f, 0.0f}, q1 = q0, q2 = q0, hn = q0;
+            if (!stop) {
+                hp = (uint32_t)h.prim;
+                j0 = a.idx[3 * hp];
+                j1 = a.idx[3 * hp + 1];
+                j2 = a.idx[3 * hp + 2];
+                q0 = V3{a.xyz[3 * j0], a.xyz[3 * j0 + 1], a.xyz[3 * j0 + 2]};
+                q1 = V3{a.xyz[3 * j1], a.xyz[3 * j1 + 1], a.xyz[3 * j1 + 2]};
+                q2 = V3{a.xyz[3 * j2], a.xyz[3 * j2 + 1], a.xyz[3 * j2 + 2]};
+                hn = normalize3(cross3(q1 - q0, q2 - q0));
+                stop = dot3(hn, d) > -1e-3f;
+            }
+            bool occl = false, ended = false;
+            if (stop) {
+                ended = true;
+            } else {
                 const float hb0 = 1.0f - h.b1 - h.b2;
                 if (pass == 1) {
                     cnt[A_SEG]++;
-                    if (a.aflags & IRTT_ADJ_CONTINUOUS) {
+                    if (f_cont) {
                         // continuous geometric term, :391-448
                         const V3 r0{a.xyz[3 * prev0], a.xyz[3 * prev0 + 1], a.xyz[3 * prev0 + 2]};
                         const V3 r1{a.xyz[3 * prev1], a.xyz[3 * prev1 + 1], a.xyz[3 * prev1 + 2]};
                         const V3 r2{a.xyz[3 * prev2], a.xyz[3 * prev2 + 1], a.xyz[3 * prev2 + 2]};
                         V3 gg[6];
-                        float G = geom_grad(r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps, gg);
+                        const float G =
+                            geom_grad(r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps, gg);
                         if (fabsf(G) > a.eps * a.eps) {
                             cnt[A_CONT]++;
                             const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
+#pragma unroll
                             for (int v = 0; v < 6; v++) {
-                                V3 dv = scl3(gg[v], wt);
+                                const V3 dv = scl3(gg[v], wt);
                                 if (!finite3(dv)) continue;
                                 if (v < 3 && depth == 1) {
-                                    // the first segment starts on the source
-                                    // triangle: summed per wave after the photon
-                                    src_acc[3 * v] += fpath * (double)dv.x;
-                                    src_acc[3 * v + 1] += fpath * (double)dv.y;
-                                    src_acc[3 * v + 2] += fpath * (double)dv.z;
+                                    // the first segment starts on the source triangle
+                                    sacc[3 * v] += fpath * (double)dv.x;
+                                    sacc[3 * v + 1] += fpath * (double)dv.y;
+                                    sacc[3 * v + 2] += fpath * (double)dv.z;
                                 } else {
                                     acc3(a.grad, vid[v], dv, fpath);
                                 }
@@ -262,8 +453,9 @@ __global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
                     }
                     const float hb[3] = {hb0, h.b1, h.b2};
                     const int opp = min_corner(hb0, h.b1, h.b2);
-                    if ((a.aflags & IRTT_ADJ_EDGE) && hb[opp] <= a.ratio) {
-                        // approximate edge term, :454-567
+                    if (f_edge && hb[opp] <= a.ratio) {
+                        // approximate edge term, :454-567: its occlusion ray
+                        // is traced in the next iteration
                         cnt[A_EDGE]++;
                         const int nr = max_corner(hb0, h.b1, h.b2);
                         int other = 0;
@@ -273,10 +465,10 @@ __global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
                         }
                         const V3 qv[3] = {q0, q1, q2};
                         const uint32_t jv[3] = {j0, j1, j2};
-                        const uint32_t edge_v = jv[nr];
+                        edge_v = jv[nr];
                         V3 edir = qv[nr] - qv[other];
-                        V3 oppe = qv[nr] - qv[opp];
-                        float elen = sqrt_rn(dot3(edir, edir));
+                        const V3 oppe = qv[nr] - qv[opp];
+                        const float elen = sqrt_rn(dot3(edir, edir));
                         edir = scl3(edir, 1.0f / elen);
                         V3 n_edg = normalize3(cross3(edir, hn));
                         float th = dot3(oppe, n_edg);
@@ -284,58 +476,22 @@ __global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
                             th = -th;
                             n_edg = scl3(n_edg, -1.0f);
                         }
-                        float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
+                        const float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
                         V3 da = normalize3(cross3(edir, d));
                         if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
-                        V3 dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * wt / larea);
+                        dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * wt / larea);
                         if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
                         V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
                         to = add3(to, scl3(n_edg, a.eps + dot3(qv[nr] - to, n_edg)));
                         to = to - scl3(d, a.eps);
-                        Hit oc = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, to, d, 1e-12f,
-                                                      1e12f, sn, stt, nvis, ntri);
-                        const bool valid = oc.prim >= 0 && oc.t >= a.tiny_dist;
-                        const uint32_t origin_v =
-                            prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
-                        bool nbr = false;
-                        uint32_t shadow_v = 0;
-                        if (valid) {
-                            const uint32_t op = (uint32_t)oc.prim;
-                            const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1],
-                                           c2 = a.idx[3 * op + 2];
-                            nbr = c0 == j0 || c0 == j1 || c0 == j2 || c1 == j0 || c1 == j1 ||
-                                  c1 == j2 || c2 == j0 || c2 == j1 || c2 == j2;
-                            const V3 s0{a.xyz[3 * c0], a.xyz[3 * c0 + 1], a.xyz[3 * c0 + 2]};
-                            const V3 s1{a.xyz[3 * c1], a.xyz[3 * c1 + 1], a.xyz[3 * c1 + 2]};
-                            const V3 s2{a.xyz[3 * c2], a.xyz[3 * c2 + 1], a.xyz[3 * c2 + 2]};
-                            V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
-                            const float ob0 = 1.0f - oc.b1 - oc.b2;
-                            const int sm = max_corner(ob0, oc.b1, oc.b2);
-                            shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
-                            if (!nbr && dot3(tn, d) < -1e-3f) {
-                                cnt[A_OCCL]++;
-                                const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
-                                const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
-                                const float d1 = h.t, d2 = oc.t, tot = d1 + d2;
-                                const double fs = hist_factor(a, shadow_bin, src);
-                                acc3(a.grad, shadow_v, scl3(dE, sa * (d1 / tot)), fs);
-                                acc3(a.grad, origin_v, scl3(dE, sa * (d2 / tot)), fs);
-                                acc3(a.grad, edge_v, scl3(dE, -sa), fs);
-                            }
-                        }
-                        // Extension (not in the reference): the power the
-                        // moving edge takes from the shadow side lands on the
-                        // hit triangle and follows this photon's path, so it
-                        // is credited to the path's absorbing bin with the
-                        // opposite sign, also when nothing lies behind the
-                        // edge (the power escaped before).
-                        if ((a.aflags & IRTT_ADJ_EDGE_GAIN) && (oc.prim < 0 || (valid && !nbr))) {
-                            const double fg = hist_factor(a, abs_bin, src);
-                            const float f_or = valid ? oc.t / (h.t + oc.t) : 1.0f;
-                            acc3(a.grad, edge_v, dE, fg);
-                            acc3(a.grad, origin_v, scl3(dE, f_or), -fg);
-                            if (valid) acc3(a.grad, shadow_v, scl3(dE, 1.0f - f_or), -fg);
-                        }
+                        origin_v = prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
+                        oj0 = j0;
+                        oj1 = j1;
+                        oj2 = j2;
+                        oht = h.t;
+                        od = d;
+                        occl = true;
+                        tracing = query_begin(q, a.tris, a.n_leaves, to, d, 1e-12f, 1e12f, ntri);
                     }
                     prev0 = j0;
                     prev1 = j1;
@@ -345,19 +501,19 @@ __global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
                     prev_max = max_corner(hb0, h.b1, h.b2);
                 }
                 // getNextRay, :163-197
-                V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
+                const V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
                 o = offset_ray(hpos, hn);
-                float u = rnd(seed);
-                float rd = a.rho_d[hp], rs = a.rho_s[hp];
+                const float u = rnd(seed);
+                const float rd = a.rho_d[hp], rs = a.rho_s[hp];
                 bool absorbed = false;
                 if (u >= rd + rs) {
                     absorbed = true;
                 } else if (u >= rd) {
-                    V3 in = normalize3(d);
-                    float dn = 2.0f * dot3(hn, in);
+                    const V3 in = normalize3(d);
+                    const float dn = 2.0f * dot3(hn, in);
                     d = V3{in.x - dn * hn.x, in.y - dn * hn.y, in.z - dn * hn.z};
                 } else {
-                    float r0 = rnd(seed), r1 = rnd(seed);
+                    const float r0 = rnd(seed), r1 = rnd(seed);
                     (void)rnd(seed);
                     d = tangent_to_world(cosine_hemisphere(r0, r1), hn);
                 }
@@ -368,40 +524,32 @@ __global__ __launch_bounds__(kTraceBlock, 3) void k_adjoint(AdjArgs a)
                         abs_bin = ab;
                         absorbed_any = true;
                         cnt[A_ABS]++;
-                    } else if (a.aflags & IRTT_ADJ_AREA) {
+                    } else if (f_area) {
                         // absorption-side area term, :588-592
                         c_area += fpath;
                     }
-                    break;
+                    ended = true;
                 }
             }
+            path_on = !ended;
+            next = occl ? 2 : (ended ? 0 : 1);
         }
-        {
-            // What this photon adds to its source triangle's three vertices:
-            // the area terms (all along dw) and the first segment's
-            // continuous term.  The 64 lanes of a wave trace consecutive
-            // photons, nearly always of one triangle: then the wave's sum is
-            // added by one lane (9 atomics per wave instead of 9 per lane, all
-            // on the same 3 vertices).
-            const uint32_t tv[3] = {i0, i1, i2};
-            double sv[9];
-            for (int q = 0; q < 3; q++) {
-                const bool ok = c_area != 0.0 && finite3(dw[q]);
-                sv[3 * q] = src_acc[3 * q] + (ok ? c_area * (double)dw[q].x : 0.0);
-                sv[3 * q + 1] = src_acc[3 * q + 1] + (ok ? c_area * (double)dw[q].y : 0.0);
-                sv[3 * q + 2] = src_acc[3 * q + 2] + (ok ? c_area * (double)dw[q].z : 0.0);
-            }
-            const unsigned long long all = __ballot(1);
-            const uint32_t tf = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-            const bool uniform = all == ~0ull && __ballot(t == tf) == ~0ull;
-            if (uniform) {
-                for (int e = 0; e < 9; e++)
-                    for (int off = 32; off > 0; off >>= 1) sv[e] += __shfl_xor(sv[e], off, 64);
-            }
-            if (!uniform || (threadIdx.x & 63) == 0)
-                for (int e = 0; e < 9; e++)
-                    if (sv[e] != 0.0) unsafeAtomicAdd(a.grad + 3 * (size_t)tv[e / 3] + e % 3, sv[e]);
+        if (next == 2) {
+            mode = M_OCCL;
+        } else if (next == 1 && depth < a.max_depth) {
+            ++depth;
+            mode = M_PATH;
+            tracing = query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri);
+        } else {
+            mode = M_END;
         }
+    }
+    // this lane's last source triangle
+    if (acc_t != 0xffffffffu) {
+#pragma unroll
+        for (int e = 0; e < 9; e++)
+            if (sacc[e] != 0.0)
+                unsafeAtomicAdd(a.grad + 3 * (size_t)a.idx[3 * acc_t + e / 3] + e % 3, sacc[e]);
     }
     for (int s = 0; s < A_COUNT; s++) {
         uint32_t v = cnt[s];

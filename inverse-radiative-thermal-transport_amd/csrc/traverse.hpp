@@ -205,19 +205,17 @@ __device__ __forceinline__ Hit query_end(const Query &q, float tmax)
     return h;
 }
 
-// The whole query in one call.  The lanes of a wave walk interior nodes
+// Runs the queries the wave's lanes have begun (tracing: this lane's query is
+// open) until every one is answered.  The lanes of a wave walk interior nodes
 // together; a lane that reaches a leaf waits there until IRTT_LEAF_LANES
 // lanes wait (or no lane is left among the interior nodes), and then the
 // waiting lanes test their triangles together.  Each ray's own order of
 // visits is that of an uninterrupted walk.
-template <int STACK>
-__device__ __forceinline__ Hit trace_closest(const float4 *__restrict__ nodes,
-                                             const float4 *__restrict__ tris, uint32_t n_leaves,
-                                             V3 o, V3 d, float tmin, float tmax, uint32_t *stk_n,
-                                             float *stk_t, uint32_t &nvis, uint32_t &ntri)
+__device__ __forceinline__ void query_run(Query &q, bool tracing, const float4 *__restrict__ nodes,
+                                          const float4 *__restrict__ tris, float tmin,
+                                          uint32_t *stk_n, float *stk_t, uint32_t &nvis,
+                                          uint32_t &ntri)
 {
-    Query q;
-    bool tracing = query_begin(q, tris, n_leaves, o, d, tmin, tmax, ntri);
     for (;;) {
         for (;;) {
             bool inner = tracing && !(q.cur & kLeafBit);
@@ -233,6 +231,18 @@ __device__ __forceinline__ Hit trace_closest(const float4 *__restrict__ nodes,
         if (tracing && (q.cur & kLeafBit)) tracing = query_leaf(q, tris, tmin, stk_n, stk_t, ntri);
         if (__ballot(tracing) == 0) break;
     }
+}
+
+// the whole query in one call
+template <int STACK>
+__device__ __forceinline__ Hit trace_closest(const float4 *__restrict__ nodes,
+                                             const float4 *__restrict__ tris, uint32_t n_leaves,
+                                             V3 o, V3 d, float tmin, float tmax, uint32_t *stk_n,
+                                             float *stk_t, uint32_t &nvis, uint32_t &ntri)
+{
+    Query q;
+    const bool tracing = query_begin(q, tris, n_leaves, o, d, tmin, tmax, ntri);
+    query_run(q, tracing, nodes, tris, tmin, stk_n, stk_t, nvis, ntri);
     return query_end(q, tmax);
 }
 
