@@ -30,6 +30,7 @@ struct Error : std::runtime_error {
 constexpr float kSigma = 5.670374419e-8f;      // thermal_common.hpp:39
 constexpr double kSigmaD = 5.670374419e-8;
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kMaxLeaves = 1u << 26;  // traceable triangles per scene
 constexpr uint32_t kMaxDepthDefault = 512;     // MAX_MAX_DEPTH, thermal_tracing_unified.glsl:7
 
 // Stream on which the calling thread's device buffers are allocated and freed

@@ -294,6 +294,9 @@ void bvh_build(Scene &s)
     Bvh &b = s.bvh;
     b.valid = false;
     const uint32_t n = (uint32_t)s.d_traceable.n;
+    // the traversal addresses nodes (64 B) and leaves (48 B) with 32-bit byte offsets
+    if (n > kMaxLeaves)
+        throw Error(IRTT_ERR_LIMIT, "more than 2^26 traceable triangles (" + std::to_string(n) + ")");
     b.n_leaves = n;
     b.max_depth = 0;
     hipStream_t st = s.stream;

@@ -73,7 +73,9 @@ __device__ __forceinline__ void slab_offsets(float o, float inv, float &lo, floa
 __device__ __forceinline__ void leaf_test(const float4 *__restrict__ tris, uint32_t leaf,
                                           const RayPre &rp, V3 o, float tmin, Hit &h)
 {
-    float4 a = tris[3 * leaf], b = tris[3 * leaf + 1], c = tris[3 * leaf + 2];
+    const float4 *tr =
+        reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(tris) + leaf * 48u);
+    float4 a = tr[0], b = tr[1], c = tr[2];
     int32_t id = (int32_t)__float_as_uint(a.w);
     float t, b1, b2;
     if (tri_hit(rp, o, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z}, V3{c.x, c.y, c.z}, tmin, h.t, t, b1,
@@ -147,7 +149,9 @@ __device__ __forceinline__ bool query_node(Query &q, const float4 *__restrict__ 
                                            uint32_t *stk_n, float *stk_t, uint32_t &nvis)
 {
     nvis++;
-    const float4 *nd = nodes + 4 * (size_t)q.cur;
+    // 32-bit byte offset from the array base (fewer than 2^26 nodes, checked on the host)
+    const float4 *nd =
+        reinterpret_cast<const float4 *>(reinterpret_cast<const char *>(nodes) + (q.cur << 6));
     float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
     float t0, t1;
     bool h0 = box_hit(n0.x, n0.y, n0.z, n0.w, n1.x, n1.y, q.inv, q.oi_lo, q.oi_hi, tmin, q.h.t, t0);
@@ -225,7 +229,7 @@ __device__ __forceinline__ void query_run(Query &q, bool tracing, const float4 *
             }
             if (__ballot(inner) == 0) break;
             if (IRTT_LEAF_LANES < kTraceBlock &&
-                (int)__popcll(__ballot(tracing && !inner)) >= IRTT_LEAF_LANES)
+                __builtin_popcountll(__ballot(tracing && !inner)) >= IRTT_LEAF_LANES)
                 break;
         }
         if (tracing && (q.cur & kLeafBit)) tracing = query_leaf(q, tris, tmin, stk_n, stk_t, ntri);
