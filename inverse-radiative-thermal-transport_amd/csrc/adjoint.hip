@@ -62,9 +62,12 @@ __device__ __forceinline__ void area_grad(V3 A, V3 B, V3 C, V3 g[3])
     g[2] = dc;
 }
 
-// G and d ln|G| / d{a_i,b_i,c_i,a_j,b_j,c_j}
+// G and d ln|G| / d{a_i,b_i,c_i,a_j,b_j,c_j}: returns G and hands each
+// vertex's gradient to emit(v, g) as soon as it is formed, so the six are never
+// live at once (the adjoint kernel's register peak)
+template <typename Emit>
 __device__ __forceinline__ float geom_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3 Cj, float bi1,
-                                           float bi2, float bj1, float bj2, float eps, V3 out[6])
+                                           float bi2, float bj1, float bj2, float eps, Emit emit)
 {
     float wi0 = 1.0f - bi1 - bi2, wj0 = 1.0f - bj1 - bj2;
     V3 pi = bary_point(Ai, Bi, Ci, wi0, bi1, bi2);
@@ -80,32 +83,36 @@ __device__ __forceinline__ float geom_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3
     float re = r + eps;
     float re2 = re * re;
     float G = -(ci * cj) / (li * lj * (re2 * re2));
-    const float kd[6] = {-wi0, -bi1, -bi2, wj0, bj1, bj2};
-    V3 e2xd_i = cross3(e2i, d), dxe1_i = cross3(d, e1i);
-    V3 e2xd_j = cross3(e2j, d), dxe1_j = cross3(d, e1j);
-    V3 ab_i = cross3(e2i, ni), ac_i = cross3(ni, e1i);
-    V3 ab_j = cross3(e2j, nj), ac_j = cross3(nj, e1j);
-    V3 dNi[3] = {scl3(add3(e2xd_i, dxe1_i), -1.0f), e2xd_i, dxe1_i};
-    V3 dNj[3] = {scl3(add3(e2xd_j, dxe1_j), -1.0f), e2xd_j, dxe1_j};
-    V3 dLi[3] = {scl3(add3(ab_i, ac_i), -1.0f), ab_i, ac_i};
-    V3 dLj[3] = {scl3(add3(ab_j, ac_j), -1.0f), ab_j, ac_j};
+    if (!(fabsf(G) > eps * eps)) return G;
+    const float ici = 1.0f / ci, icj = 1.0f / cj, ili = 1.0f / li, ilj = 1.0f / lj,
+                ire = 4.0f / re;
+    // one side at a time: s = 0 the segment's start triangle (i), 1 its end (j)
 #pragma unroll
-    for (int v = 0; v < 6; v++) {
-        V3 g_ci = scl3(Ni, kd[v]);
-        V3 g_cj = scl3(Nj, kd[v]);
-        V3 g_li{0.0f, 0.0f, 0.0f}, g_lj{0.0f, 0.0f, 0.0f};
-        if (v < 3) {
-            g_ci = add3(g_ci, dNi[v]);
-            g_li = dLi[v];
-        } else {
-            g_cj = add3(g_cj, dNj[v - 3]);
-            g_lj = dLj[v - 3];
+    for (int side = 0; side < 2; side++) {
+        const V3 e1 = side ? e1j : e1i, e2 = side ? e2j : e2i, n = side ? nj : ni;
+        const V3 e2xd = cross3(e2, d), dxe1 = cross3(d, e1);
+        const V3 ab = cross3(e2, n), ac = cross3(n, e1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const int v = 3 * side + c;
+            const float kd = side ? (c == 0 ? wj0 : (c == 1 ? bj1 : bj2))
+                                  : -(c == 0 ? wi0 : (c == 1 ? bi1 : bi2));
+            const V3 dN = c == 0 ? scl3(add3(e2xd, dxe1), -1.0f) : (c == 1 ? e2xd : dxe1);
+            const V3 dL = c == 0 ? scl3(add3(ab, ac), -1.0f) : (c == 1 ? ab : ac);
+            V3 g_ci = scl3(Ni, kd);
+            V3 g_cj = scl3(Nj, kd);
+            V3 g_li{0.0f, 0.0f, 0.0f}, g_lj{0.0f, 0.0f, 0.0f};
+            if (side == 0) {
+                g_ci = add3(g_ci, dN);
+                g_li = dL;
+            } else {
+                g_cj = add3(g_cj, dN);
+                g_lj = dL;
+            }
+            V3 g_r = scl3(dh, kd);
+            emit(v, add3(add3(scl3(g_ci, ici), scl3(g_cj, icj)),
+                         scl3(add3(add3(scl3(g_li, ili), scl3(g_lj, ilj)), scl3(g_r, ire)), -1.0f)));
         }
-        V3 g_r = scl3(dh, kd[v]);
-        out[v] = add3(add3(scl3(g_ci, 1.0f / ci), scl3(g_cj, 1.0f / cj)),
-                      scl3(add3(add3(scl3(g_li, 1.0f / li), scl3(g_lj, 1.0f / lj)),
-                                scl3(g_r, 4.0f / re)),
-                           -1.0f));
     }
     return G;
 }
@@ -159,10 +166,13 @@ enum { M_NONE = 0, M_PATH, M_OCCL, M_END, M_DONE };
 
 // waves per SIMD: IRTT_ADJ_WAVES, or what the LDS stack admits (160 KiB per
 // CU, four SIMDs).  Measured on the city block (2^26 photons, all terms):
-// 2 waves (249 VGPRs) 40.7 ms, 3 (168 VGPRs, 208 B spilled per lane) 40.5 ms,
-// 4 (128 VGPRs, 368 B spilled) 56.5 ms.
+// 3 waves (168 VGPRs, 48 B spilled per lane) 36.6 ms, 4 waves (128 VGPRs,
+// 240 B spilled) 35.0-35.3 ms.  The area terms are summed when a photon ends
+// and the counters are kept per wave, so that little stays live across the
+// path; with the area state carried along, 4 waves spilled 368 B and took
+// 56.5 ms.
 #ifndef IRTT_ADJ_WAVES
-#define IRTT_ADJ_WAVES 3
+#define IRTT_ADJ_WAVES 4
 #endif
 constexpr int adjoint_waves(int stack)
 {
@@ -185,31 +195,31 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
     const uint32_t lane = threadIdx.x;
-    uint32_t cnt[A_COUNT] = {0, 0, 0, 0, 0, 0};
+    // per-wave event counts (A_ABS per lane: several photons may start in
+    // one iteration); a lane marks an iteration's events in ev (bit k = slot k)
+    uint32_t wcnt[A_COUNT] = {0, 0, 0, 0, 0, 0};
+    uint32_t n_abs = 0;
     uint32_t nvis = 0, ntri = 0;
     const bool f_area = a.aflags & IRTT_ADJ_AREA, f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
     const bool f_edge = a.aflags & IRTT_ADJ_EDGE, f_gain = a.aflags & IRTT_ADJ_EDGE_GAIN;
 
-    // this lane's run of photons [run_g, run_end) in chunk `chunk`
-    uint64_t chunk = blockIdx.x;
-    uint64_t run_g = 0, run_end = 0;
-    {
-        const uint64_t b0 = chunk * (uint64_t)kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
-        run_g = b0;
-        run_end = b0 < a.photons ? min(b0 + kAdjRun, a.photons) : b0;
-    }
+    // this lane's run of photons: the next is run_g, `left` remain, in chunk `chunk`
+    uint32_t chunk = blockIdx.x;
+    uint64_t run_g = (uint64_t)chunk * kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
+    uint32_t left = run_g < a.photons ? (uint32_t)min<uint64_t>(kAdjRun, a.photons - run_g) : 0u;
     // the source vertices' gradient of this lane's photons of triangle acc_t
     uint32_t acc_t = 0xffffffffu;
     double sacc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 
     // the photon being replayed
     uint64_t g = 0;
-    uint32_t t = 0, i0 = 0, i1 = 0, i2 = 0, src = 0, abs_bin = 0, seed0 = 0;
+    uint32_t t = 0, src = 0, abs_bin = 0;
     bool absorbed_any = false;
     int pass = 0;
-    float wt = 0.0f;
-    double fpath = 0.0, c_area = 0.0;
-    V3 dw[3] = {V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
+    double fpath = 0.0;
+    // area terms: emission side (pass 1 started on a non-degenerate emitter),
+    // absorption side (absorbed in pass 1); summed when the photon ends
+    bool area_on = false, area_abs = false;
     // its path
     uint32_t seed = 0, depth = 0, prev0 = 0, prev1 = 0, prev2 = 0;
     int prev_max = 0;
@@ -219,7 +229,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
     // the pending occlusion ray's edge term
     V3 od{0.0f, 0.0f, 1.0f}, dE{0.0f, 0.0f, 0.0f};
     float oht = 0.0f;
-    uint32_t oj0 = 0, oj1 = 0, oj2 = 0, edge_v = 0, origin_v = 0;
+    uint32_t ohp = 0, edge_v = 0, origin_v = 0;  // ohp: the segment's hit triangle
 
     int mode = M_NONE;
     Query q;
@@ -234,14 +244,24 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                 if (pass == 0) {
                     pass = 1;
                 } else {
-                    // the photon's area terms on its source vertices
+                    // the photon's area terms on its source vertices,
+                    // thermal_tracing_unified.glsl:327-351 and :588-592
+                    const double c_area =
+                        area_on ? -hist_factor(a, src, src) + (area_abs ? fpath : 0.0) : 0.0;
                     if (c_area != 0.0) {
+                        const uint32_t v0 = a.idx[3 * t], v1 = a.idx[3 * t + 1], v2 = a.idx[3 * t + 2];
+                        V3 ga[3];
+                        area_grad(V3{a.xyz[3 * v0], a.xyz[3 * v0 + 1], a.xyz[3 * v0 + 2]},
+                                  V3{a.xyz[3 * v1], a.xyz[3 * v1 + 1], a.xyz[3 * v1 + 2]},
+                                  V3{a.xyz[3 * v2], a.xyz[3 * v2 + 1], a.xyz[3 * v2 + 2]}, ga);
+                        const float sa = a.w[t] / a.area[t];
 #pragma unroll
                         for (int c = 0; c < 3; c++) {
-                            if (!finite3(dw[c])) continue;
-                            sacc[3 * c] += c_area * (double)dw[c].x;
-                            sacc[3 * c + 1] += c_area * (double)dw[c].y;
-                            sacc[3 * c + 2] += c_area * (double)dw[c].z;
+                            const V3 dw = scl3(ga[c], sa);
+                            if (!finite3(dw)) continue;
+                            sacc[3 * c] += c_area * (double)dw.x;
+                            sacc[3 * c + 1] += c_area * (double)dw.y;
+                            sacc[3 * c + 2] += c_area * (double)dw.z;
                         }
                     }
                     ph = 2;
@@ -249,24 +269,21 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                 }
             } else {
                 // next photon of this lane
-                while (run_g >= run_end) {
+                while (left == 0) {
                     chunk += gridDim.x;
-                    const uint64_t b0 =
-                        chunk * (uint64_t)kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
-                    if (chunk * (uint64_t)kTraceBlock * kAdjRun >= a.photons) break;
-                    run_g = b0;
-                    run_end = b0 < a.photons ? min(b0 + kAdjRun, a.photons) : b0;
+                    const uint64_t base = (uint64_t)chunk * kTraceBlock * kAdjRun;
+                    if (base >= a.photons) break;
+                    run_g = base + (uint64_t)lane * kAdjRun;
+                    left = run_g < a.photons ? (uint32_t)min<uint64_t>(kAdjRun, a.photons - run_g)
+                                             : 0u;
                 }
-                if (run_g >= run_end) {
+                if (left == 0) {
                     mode = M_DONE;
                     break;
                 }
                 g = run_g++;
+                left--;
                 t = (uint32_t)(g / a.per_tri);
-                const uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
-                const uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
-                // thermal_tracing_unified.glsl:291
-                seed0 = tea16(a.nt * xx + t, a.batch_seed + nn);
                 if (t != acc_t) {
                     // another source triangle: flush the sums of the last one
                     if (acc_t != 0xffffffffu) {
@@ -280,12 +297,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                     for (int e = 0; e < 9; e++) sacc[e] = 0.0;
                     acc_t = t;
                 }
-                i0 = a.idx[3 * t];
-                i1 = a.idx[3 * t + 1];
-                i2 = a.idx[3 * t + 2];
-                wt = a.w[t];
-                c_area = 0.0;
-                dw[0] = dw[1] = dw[2] = V3{0.0f, 0.0f, 0.0f};
+                area_on = area_abs = false;
                 absorbed_any = false;
                 abs_bin = 0;
                 pass = 0;
@@ -294,16 +306,22 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                     const uint32_t r = a.absorber[g];
                     absorbed_any = r != 0xffffffffu;
                     abs_bin = absorbed_any ? r : 0u;
-                    if (absorbed_any) cnt[A_ABS]++;
+                    if (absorbed_any) n_abs++;
                     pass = 1;
                 }
             }
             // start pass `pass` of photon g: its emission ray
             // (getEmissionRay, thermal_tracing_unified.glsl:125-161)
+            const uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
             const V3 p0{a.xyz[3 * i0], a.xyz[3 * i0 + 1], a.xyz[3 * i0 + 2]};
             const V3 p1{a.xyz[3 * i1], a.xyz[3 * i1 + 1], a.xyz[3 * i1 + 2]};
             const V3 p2{a.xyz[3 * i2], a.xyz[3 * i2 + 1], a.xyz[3 * i2 + 2]};
-            seed = seed0;
+            {
+                // thermal_tracing_unified.glsl:291 (photon g = run_g - 1)
+                const uint32_t k = (uint32_t)(run_g - 1 - (uint64_t)t * a.per_tri);
+                const uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
+                seed = tea16(a.nt * xx + t, a.batch_seed + nn);
+            }
             const float u0 = rnd(seed), u1 = rnd(seed);
             (void)rnd(seed);
             const float su = sqrt_rn(u0);
@@ -320,22 +338,14 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
             }
             if (!(len > 0.0f)) {
                 // degenerate emitter: no contribution
-                c_area = 0.0;
+                area_on = false;
                 pass = 1;
                 ph = 1;
                 continue;
             }
             const int sc = max_corner(b0, b1, b2);
             src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
-            if (pass == 1 && f_area) {
-                // emission-side area term, thermal_tracing_unified.glsl:327-351
-                V3 ga[3];
-                area_grad(p0, p1, p2, ga);
-                const float sa = wt / a.area[t];
-#pragma unroll
-                for (int c = 0; c < 3; c++) dw[c] = scl3(ga[c], sa);
-                c_area -= hist_factor(a, src, src);
-            }
+            if (pass == 1 && f_area) area_on = true;  // emission-side area term
             if (pass == 1 && !absorbed_any && !(a.aflags & IRTT_ADJ_REFERENCE_DROPPED)) {
                 ph = 1;
                 continue;
@@ -356,7 +366,8 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
 
         // ---- every lane with a ray traces it ----
         query_run(q, tracing && mode != M_DONE, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
-        if (mode == M_DONE) continue;
+        uint32_t ev = 0;
+        if (mode != M_DONE) {
         const Hit h = query_end(q, 1e12f);
         tracing = false;
         int next = 1;  // 0: the pass's path ends, 1: its next segment, 2: occlusion ray begun
@@ -369,6 +380,8 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
             if (valid) {
                 const uint32_t op = (uint32_t)h.prim;
                 const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1], c2 = a.idx[3 * op + 2];
+                const uint32_t oj0 = a.idx[3 * ohp], oj1 = a.idx[3 * ohp + 1],
+                               oj2 = a.idx[3 * ohp + 2];
                 nbr = c0 == oj0 || c0 == oj1 || c0 == oj2 || c1 == oj0 || c1 == oj1 || c1 == oj2 ||
                       c2 == oj0 || c2 == oj1 || c2 == oj2;
                 const V3 s0{a.xyz[3 * c0], a.xyz[3 * c0 + 1], a.xyz[3 * c0 + 2]};
@@ -379,7 +392,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                 const int sm = max_corner(ob0, h.b1, h.b2);
                 shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
                 if (!nbr && dot3(tn, od) < -1e-3f) {
-                    cnt[A_OCCL]++;
+                    ev |= 1u << A_OCCL;
                     const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
                     const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
                     const float d1 = oht, d2 = h.t, tot = d1 + d2;
@@ -403,7 +416,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
             next = path_on ? 1 : 0;
         } else {
             // a path segment
-            if (pass == 0) cnt[A_RAYS]++;
+            if (pass == 0) ev |= 1u << A_RAYS;
             bool stop = h.prim < 0 || h.t < a.tiny_dist;
             uint32_t hp = 0, j0 = 0, j1 = 0, j2 = 0;
             V3 q0{0.0f, 0.0f, 0.0f}, q1 = q0, q2 = q0, hn = q0;
@@ -424,23 +437,21 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
             } else {
                 const float hb0 = 1.0f - h.b1 - h.b2;
                 if (pass == 1) {
-                    cnt[A_SEG]++;
+                    ev |= 1u << A_SEG;
                     if (f_cont) {
                         // continuous geometric term, :391-448
                         const V3 r0{a.xyz[3 * prev0], a.xyz[3 * prev0 + 1], a.xyz[3 * prev0 + 2]};
                         const V3 r1{a.xyz[3 * prev1], a.xyz[3 * prev1 + 1], a.xyz[3 * prev1 + 2]};
                         const V3 r2{a.xyz[3 * prev2], a.xyz[3 * prev2 + 1], a.xyz[3 * prev2 + 2]};
-                        V3 gg[6];
-                        const float G =
-                            geom_grad(r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps, gg);
-                        if (fabsf(G) > a.eps * a.eps) {
-                            cnt[A_CONT]++;
-                            const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
-#pragma unroll
-                            for (int v = 0; v < 6; v++) {
-                                const V3 dv = scl3(gg[v], wt);
-                                if (!finite3(dv)) continue;
-                                if (v < 3 && depth == 1) {
+                        const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
+                        const bool first = depth == 1;
+                        const float wt = a.w[t];
+                        const float G = geom_grad(
+                            r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps,
+                            [&](int v, V3 g) {
+                                const V3 dv = scl3(g, wt);
+                                if (!finite3(dv)) return;
+                                if (v < 3 && first) {
                                     // the first segment starts on the source triangle
                                     sacc[3 * v] += fpath * (double)dv.x;
                                     sacc[3 * v + 1] += fpath * (double)dv.y;
@@ -448,15 +459,15 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                                 } else {
                                     acc3(a.grad, vid[v], dv, fpath);
                                 }
-                            }
-                        }
+                            });
+                        if (fabsf(G) > a.eps * a.eps) ev |= 1u << A_CONT;
                     }
                     const float hb[3] = {hb0, h.b1, h.b2};
                     const int opp = min_corner(hb0, h.b1, h.b2);
                     if (f_edge && hb[opp] <= a.ratio) {
                         // approximate edge term, :454-567: its occlusion ray
                         // is traced in the next iteration
-                        cnt[A_EDGE]++;
+                        ev |= 1u << A_EDGE;
                         const int nr = max_corner(hb0, h.b1, h.b2);
                         int other = 0;
                         if (opp == 0 || nr == 0) {
@@ -479,15 +490,13 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                         const float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
                         V3 da = normalize3(cross3(edir, d));
                         if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
-                        dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * wt / larea);
+                        dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * a.w[t] / larea);
                         if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
                         V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
                         to = add3(to, scl3(n_edg, a.eps + dot3(qv[nr] - to, n_edg)));
                         to = to - scl3(d, a.eps);
                         origin_v = prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
-                        oj0 = j0;
-                        oj1 = j1;
-                        oj2 = j2;
+                        ohp = hp;
                         oht = h.t;
                         od = d;
                         occl = true;
@@ -523,10 +532,9 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                     if (pass == 0) {
                         abs_bin = ab;
                         absorbed_any = true;
-                        cnt[A_ABS]++;
+                        n_abs++;
                     } else if (f_area) {
-                        // absorption-side area term, :588-592
-                        c_area += fpath;
+                        area_abs = true;  // absorption-side area term
                     }
                     ended = true;
                 }
@@ -543,6 +551,11 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
         } else {
             mode = M_END;
         }
+        }
+        // the wave's events of this iteration (every lane is here)
+#pragma unroll
+        for (int k = A_SEG; k < A_COUNT; k++)
+            wcnt[k] += (uint32_t)__builtin_popcountll(__ballot((ev >> k) & 1u));
     }
     // this lane's last source triangle
     if (acc_t != 0xffffffffu) {
@@ -551,11 +564,10 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
             if (sacc[e] != 0.0)
                 unsafeAtomicAdd(a.grad + 3 * (size_t)a.idx[3 * acc_t + e / 3] + e % 3, sacc[e]);
     }
-    for (int s = 0; s < A_COUNT; s++) {
-        uint32_t v = cnt[s];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (threadIdx.x == 0 && v) atomicAdd(a.stats + s, (unsigned long long)v);
-    }
+    for (int off = 32; off > 0; off >>= 1) n_abs += __shfl_down(n_abs, off, 64);
+    wcnt[A_ABS] = n_abs;
+    for (int k = 0; k < A_COUNT; k++)
+        if (threadIdx.x == 0 && wcnt[k]) atomicAdd(a.stats + k, (unsigned long long)wcnt[k]);
 }
 
 }  // namespace
