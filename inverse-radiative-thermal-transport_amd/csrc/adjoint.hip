@@ -139,6 +139,7 @@ struct AdjArgs {
     const double *lambda;  // n_hist x nb
     const double *x4;      // n_hist x nb
     uint64_t photons;
+    uint32_t run;              // photons per lane per chunk, 1 .. kAdjRun
     const uint32_t *absorber;  // per photon from the forward build, or nullptr
     double *grad;
     unsigned long long *stats;
@@ -154,10 +155,12 @@ __device__ __forceinline__ double hist_factor(const AdjArgs &a, uint32_t bin_l, 
     return f;
 }
 
-// Photons per lane per chunk: a wave takes chunks of 64 * kAdjRun consecutive
-// photons, lane l the run [l * kAdjRun, (l + 1) * kAdjRun) of each, so a
-// lane's photons nearly always share their source triangle and the source
-// vertices' sums stay in the lane's registers between photons.
+// Photons per lane per chunk (a.run, at most kAdjRun): a wave takes chunks of
+// 64 * a.run consecutive photons, lane l the run [l * a.run, (l + 1) * a.run)
+// of each, so a lane's photons nearly always share their source triangle and
+// the source vertices' sums stay in the lane's registers between photons.
+// The host shortens the runs when there are too few photons to give every
+// wave of the grid a chunk.
 constexpr uint32_t kAdjRun = 32;
 
 // what a lane does next: start a photon, trace a path segment, trace an
@@ -205,8 +208,9 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
 
     // this lane's run of photons: the next is run_g, `left` remain, in chunk `chunk`
     uint32_t chunk = blockIdx.x;
-    uint64_t run_g = (uint64_t)chunk * kTraceBlock * kAdjRun + (uint64_t)lane * kAdjRun;
-    uint32_t left = run_g < a.photons ? (uint32_t)min<uint64_t>(kAdjRun, a.photons - run_g) : 0u;
+    const uint32_t run = a.run;
+    uint64_t run_g = (uint64_t)chunk * kTraceBlock * run + (uint64_t)lane * run;
+    uint32_t left = run_g < a.photons ? (uint32_t)min<uint64_t>(run, a.photons - run_g) : 0u;
     // the source vertices' gradient of this lane's photons of triangle acc_t
     uint32_t acc_t = 0xffffffffu;
     double sacc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -271,10 +275,10 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                 // next photon of this lane
                 while (left == 0) {
                     chunk += gridDim.x;
-                    const uint64_t base = (uint64_t)chunk * kTraceBlock * kAdjRun;
+                    const uint64_t base = (uint64_t)chunk * kTraceBlock * run;
                     if (base >= a.photons) break;
-                    run_g = base + (uint64_t)lane * kAdjRun;
-                    left = run_g < a.photons ? (uint32_t)min<uint64_t>(kAdjRun, a.photons - run_g)
+                    run_g = base + (uint64_t)lane * run;
+                    left = run_g < a.photons ? (uint32_t)min<uint64_t>(run, a.photons - run_g)
                                              : 0u;
                 }
                 if (left == 0) {
@@ -641,8 +645,12 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     // the forward kernel's stack choice: 24 entries (more waves per CU) for
     // trees up to 23 levels
     const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
-    uint64_t want = (a.photons + kTraceBlock - 1) / kTraceBlock;
-    uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, (uint64_t)ncu * 64));
+    // runs as long as kAdjRun while the chunks still cover ncu * 64 waves
+    const uint64_t waves = (uint64_t)ncu * 64;
+    a.run = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(kAdjRun, a.photons / (waves * kTraceBlock)));
+    uint64_t want = (a.photons + (uint64_t)kTraceBlock * a.run - 1) / ((uint64_t)kTraceBlock * a.run);
+    uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, waves));
     IRTT_HIP(hipEventRecord(s.ev0, stm));
     if (S == 24) k_adjoint<24><<<grid, kTraceBlock, 0, stm>>>(a);
     else if (S == 32) k_adjoint<32><<<grid, kTraceBlock, 0, stm>>>(a);

@@ -51,6 +51,35 @@ Scene *S(irtt_scene h)
     return s;
 }
 
+// Scene buffers come from the device's stream-ordered pool (hipMallocAsync on
+// the scene's stream).  By default the pool hands freed memory back to the
+// driver at every synchronisation, so a request that creates, builds and
+// destroys a scene maps and unmaps all its buffers again; the pool is told to
+// keep up to kPoolKeep bytes of freed memory for reuse instead.  Once per device.
+#ifndef IRTT_POOL_KEEP_GIB
+#define IRTT_POOL_KEEP_GIB 8
+#endif
+constexpr uint64_t kPoolKeep = (uint64_t)IRTT_POOL_KEEP_GIB << 30;
+
+void configure_pool()
+{
+    static std::mutex mu;
+    static std::vector<int> done;
+    int dev = 0;
+    IRTT_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    if (kPoolKeep == 0 || std::find(done.begin(), done.end(), dev) != done.end()) return;
+    done.push_back(dev);
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || !pool) {
+        (void)hipGetLastError();  // no pool: allocations fall back to hipMalloc
+        return;
+    }
+    uint64_t keep = kPoolKeep;
+    if (hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess)
+        (void)hipGetLastError();
+}
+
 // cotangent of the angle between a and b, thermal_data.cpp:253-258; a
 // degenerate corner contributes nothing instead of inf/NaN
 double cotan(const double a[3], const double b[3])
@@ -262,6 +291,7 @@ int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32
         if (!xyz || !tri || !mat || nv == 0 || nt == 0)
             throw Error(IRTT_ERR_ARG, "empty geometry or null pointer");
         if (bin_mode > IRTT_BIN_TRIANGLE) throw Error(IRTT_ERR_ARG, "unknown bin mode");
+        configure_pool();
         for (uint64_t i = 0; i < 3ull * nt; i++)
             if (tri[i] >= nv)
                 throw Error(IRTT_ERR_ARG, "triangle index " + std::to_string(tri[i]) +
