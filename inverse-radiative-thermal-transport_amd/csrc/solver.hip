@@ -106,36 +106,35 @@ __device__ __forceinline__ bool d_before(const Sys &s)
 }
 
 // w = fixed ? 0 : (D before the product ? 4 x^3 v : v)
+__device__ __forceinline__ double jv_pre_at(const Sys &s, const double *__restrict__ xe, double v,
+                                            uint32_t i)
+{
+    double vi = s.fixed[i] ? 0.0 : v;
+    if (s.jmode != J_PREDICT && d_before(s)) {
+        double x = xe[i];
+        vi = 4.0 * x * x * x * vi;
+    }
+    return vi;
+}
+
 __global__ void k_jv_pre(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
                          double *__restrict__ w)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= s.nb) return;
-    double vi = s.fixed[i] ? 0.0 : v[i];
-    if (s.jmode != J_PREDICT && d_before(s)) {
-        double x = xe[i];
-        vi = 4.0 * x * x * x * vi;
-    }
-    w[i] = vi;
+    w[i] = jv_pre_at(s, xe, v[i], i);
 }
 
-// out = J v from Tw (= T w).  J_PREDICT is the operator of the steady
-// radiative balance in y = x^4 alone (T), used for the initial guess.
-__global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
-                          const double *__restrict__ w, const double *__restrict__ Tw,
-                          double *__restrict__ out)
+// (J v)_i from Tw (= T w).  J_PREDICT is the operator of the steady radiative
+// balance in y = x^4 alone (T), used for the initial guess.
+__device__ __forceinline__ double jv_post_at(const Sys &s, const double *__restrict__ xe,
+                                             const double *__restrict__ v,
+                                             const double *__restrict__ w,
+                                             const double *__restrict__ Tw, uint32_t i)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= s.nb) return;
-    if (s.fixed[i]) {
-        out[i] = v[i];
-        return;
-    }
+    if (s.fixed[i]) return v[i];
     const double cd = s.cdj ? s.cdj[i] : 0.0;
-    if (s.jmode == J_PREDICT) {
-        out[i] = Tw[i] + cd * w[i];
-        return;
-    }
+    if (s.jmode == J_PREDICT) return Tw[i] + cd * w[i];
     double lv = lap_row(s, i, v);  // the cotangent Laplacian is symmetric
     double rad = Tw[i];
     if (!s.c_unscaled) rad += cd * w[i];
@@ -147,10 +146,18 @@ __global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__
     if (s.steady) {
         double o = rad + lv;
         if (s.jmode == J_REFERENCE) o += v[i] / s.dt;  // I/dt regulariser, thermal_solver.cpp:150
-        out[i] = o;
-    } else {
-        out[i] = s.mass[i] * v[i] + s.dt * (rad + lv);
+        return o;
     }
+    return s.mass[i] * v[i] + s.dt * (rad + lv);
+}
+
+__global__ void k_jv_post(Sys s, const double *__restrict__ xe, const double *__restrict__ v,
+                          const double *__restrict__ w, const double *__restrict__ Tw,
+                          double *__restrict__ out)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.nb) return;
+    out[i] = jv_post_at(s, xe, v, w, Tw, i);
 }
 
 // deterministic multi-dot: part[k*kRedBlocks + block] = sum a_k * b_k
@@ -416,7 +423,7 @@ void residual_host(Scene &s, const irtt_solver_params &p, const double *x, doubl
 
 // ---- BiCGSTAB with its scalars in device memory --------------------------
 // The iteration's scalars (rho, alpha, omega, ...) live in a small device
-// array: reductions write into it, one-thread kernels derive the next
+// array: each reduction's last kernel writes into it and derives the next
 // scalars, vector kernels read them, and a converged or broken-down
 // iteration sets `done`, after which every update is a no-op.  No host round
 // trip is needed inside an iteration, so K iterations are enqueued back to
@@ -434,8 +441,9 @@ __global__ void kd_init(double *sc, double stop)
     sc[B_STOP] = stop;
 }
 
+// the scalar steps of the iteration (run by one thread after a reduction)
 // rho_new from the last reduction; breakdown ends the solve
-__global__ void kd_rho(double *sc)
+__device__ __forceinline__ void sc_rho(double *sc)
 {
     if (sc[B_DONE] != 0.0) return;
     const double rn = sc[B_RHO_NEW];
@@ -446,37 +454,32 @@ __global__ void kd_rho(double *sc)
     sc[B_BETA] = sc[B_FIRST] != 0.0 ? 0.0 : (rn / sc[B_RHO]) * (sc[B_ALPHA] / sc[B_OMEGA]);
     sc[B_FIRST] = 0.0;
 }
-
-__global__ void kd_alpha(double *sc)
+__device__ __forceinline__ void sc_alpha(double *sc)
 {
     if (sc[B_DONE] != 0.0) return;
     const double d = sc[B_TMP];
     if (d == 0.0 || !isfinite(d)) sc[B_DONE] = 1.0;
     else sc[B_ALPHA] = sc[B_RHO_NEW] / d;
 }
-
-// ||s||^2 small enough: finish with dx += alpha ph (applied by kd_half_apply)
-__global__ void kd_half(double *sc)
+// ||s||^2 small enough: finish with dx += alpha ph
+__device__ __forceinline__ void sc_half(double *sc)
 {
     sc[B_HALF] = (sc[B_DONE] == 0.0 && sc[B_SS] <= sc[B_STOP]) ? 1.0 : 0.0;
 }
-
-__global__ void kd_half_done(double *sc)
+__device__ __forceinline__ void sc_half_done(double *sc)
 {
     if (sc[B_HALF] != 0.0) {
         sc[B_DONE] = 1.0;
         sc[B_ITERS] += 1.0;
     }
 }
-
-__global__ void kd_omega(double *sc)
+__device__ __forceinline__ void sc_omega(double *sc)
 {
     if (sc[B_DONE] != 0.0) return;
     if (sc[B_TT] == 0.0) sc[B_DONE] = 1.0;
     else sc[B_OMEGA] = sc[B_TS] / sc[B_TT];
 }
-
-__global__ void kd_end(double *sc)
+__device__ __forceinline__ void sc_end(double *sc)
 {
     if (sc[B_DONE] != 0.0) return;
     sc[B_RHO] = sc[B_RHO_NEW];
@@ -490,50 +493,174 @@ __global__ void kd_end(double *sc)
     }
 }
 
-__global__ void kd_p_update(uint32_t n, const double *__restrict__ sc, const double *r,
-                            const double *v, double *p)
+// which scalar steps follow a reduction
+enum { F_RHO = 0, F_ALPHA, F_HALF, F_OMEGA, F_END_RHO };
+
+// k_final_sum of m partial sums, then the scalar steps of `stage`, which
+// stores the sums in their slots of sc.
+// F_END_RHO's second sum (R . r) is the next iteration's rho: it is stored
+// only after the end step has used the previous one.
+__global__ void kd_final(const double *part, int m, double *sc, int stage)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[B_DONE] != 0.0) return;
-    p[i] = r[i] + sc[B_BETA] * (p[i] - sc[B_OMEGA] * v[i]);
+    __shared__ double sm[kRedThreads];
+    double res[2] = {0.0, 0.0};
+    for (int k = 0; k < m; k++) {
+        double a = 0.0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) a += part[k * kRedBlocks + i];
+        sm[threadIdx.x] = a;
+        __syncthreads();
+        for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) sm[threadIdx.x] += sm[threadIdx.x + off];
+            __syncthreads();
+        }
+        res[k] = sm[0];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    switch (stage) {
+    case F_RHO:
+        sc[B_RHO_NEW] = res[0];
+        sc_rho(sc);
+        break;
+    case F_ALPHA:
+        sc[B_TMP] = res[0];
+        sc_alpha(sc);
+        break;
+    case F_HALF:
+        sc[B_SS] = res[0];
+        sc_half(sc);
+        break;
+    case F_OMEGA:
+        sc_half_done(sc);
+        sc[B_TS] = res[0];
+        sc[B_TT] = res[1];
+        sc_omega(sc);
+        break;
+    default:
+        sc[B_RR] = res[0];
+        sc_end(sc);
+        sc[B_RHO_NEW] = res[1];
+        sc_rho(sc);
+        break;
+    }
 }
-__global__ void kd_scale(uint32_t n, const double *__restrict__ sc, const double *dinv,
-                         const double *x, double *y)
+
+// the reduction skeleton of k_dots: grid-stride over i, per-thread sums in
+// index order, tree within the block, partials per block (the same order as
+// k_dots, so the fused kernels below give the same sums)
+template <int M>
+__device__ __forceinline__ void block_partials(const double (&acc)[M], double *part)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[B_DONE] != 0.0) return;
-    y[i] = dinv[i] * x[i];
+    __shared__ double sm[M][kRedThreads];
+    for (int k = 0; k < M; k++) sm[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off)
+            for (int k = 0; k < M; k++) sm[k][threadIdx.x] += sm[k][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < M; k++) part[k * kRedBlocks + blockIdx.x] = sm[k][0];
 }
-// s = r - alpha v
-__global__ void kd_s(uint32_t n, const double *__restrict__ sc, const double *r, const double *v,
-                     double *s)
+
+// p = r + beta (p - omega v); ph = dinv p; w = jv_pre(ph)
+__global__ void kd_p_pre(Sys sy, const double *__restrict__ sc, const double *__restrict__ xe,
+                         const double *r, const double *v, double *p, const double *dinv,
+                         double *ph, double *w)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[B_DONE] != 0.0) return;
-    s[i] = r[i] + (-sc[B_ALPHA]) * v[i];
+    if (i >= sy.nb || sc[B_DONE] != 0.0) return;
+    const double pi = r[i] + sc[B_BETA] * (p[i] - sc[B_OMEGA] * v[i]);
+    p[i] = pi;
+    const double phi = dinv[i] * pi;
+    ph[i] = phi;
+    w[i] = jv_pre_at(sy, xe, phi, i);
 }
-__global__ void kd_half_apply(uint32_t n, const double *__restrict__ sc, const double *ph,
-                              double *dx)
+
+// out = J vin from Tw; partial sums of a0 . out (and, M = 2, out . out)
+template <int M>
+__global__ void kd_post_dot(Sys sy, const double *__restrict__ xe, const double *vin,
+                            const double *w, const double *Tw, double *out, const double *a0,
+                            double *part)
 {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[B_HALF] == 0.0) return;
-    dx[i] = dx[i] + sc[B_ALPHA] * ph[i];
+    double acc[M];
+    for (int k = 0; k < M; k++) acc[k] = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < sy.nb;
+         i += gridDim.x * blockDim.x) {
+        const double o = jv_post_at(sy, xe, vin, w, Tw, i);
+        out[i] = o;
+        acc[0] += a0[i] * o;
+        if (M > 1) acc[M - 1] += o * o;
+    }
+    block_partials<M>(acc, part);
 }
-__global__ void kd_step(uint32_t n, const double *__restrict__ sc, const double *ph,
-                        const double *sh, const double *s, const double *t, double *x, double *r)
+
+// s = r - alpha v; partial sums of s . s
+__global__ void kd_s_dot(uint32_t n, const double *__restrict__ sc, const double *r,
+                         const double *v, double *s, double *part)
+{
+    const bool live = sc[B_DONE] == 0.0;
+    double acc[1] = {0.0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double si;
+        if (live) {
+            si = r[i] + (-sc[B_ALPHA]) * v[i];
+            s[i] = si;
+        } else {
+            si = s[i];
+        }
+        acc[0] += si * si;
+    }
+    block_partials<1>(acc, part);
+}
+
+// half step: dx += alpha ph; otherwise sh = dinv s; then w = jv_pre(sh)
+__global__ void kd_sh_pre(Sys sy, const double *__restrict__ sc, const double *__restrict__ xe,
+                          const double *ph, double *dx, const double *dinv, const double *s,
+                          double *sh, double *w)
 {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || sc[B_DONE] != 0.0) return;
-    x[i] += sc[B_ALPHA] * ph[i] + sc[B_OMEGA] * sh[i];
-    r[i] = s[i] - sc[B_OMEGA] * t[i];
+    if (i >= sy.nb) return;
+    if (sc[B_HALF] != 0.0) {
+        dx[i] = dx[i] + sc[B_ALPHA] * ph[i];
+        return;
+    }
+    if (sc[B_DONE] != 0.0) return;
+    const double shi = dinv[i] * s[i];
+    sh[i] = shi;
+    w[i] = jv_pre_at(sy, xe, shi, i);
+}
+
+// x += alpha ph + omega sh; r = s - omega t; partial sums of r . r and R . r
+__global__ void kd_step_dot(uint32_t n, const double *__restrict__ sc, const double *ph,
+                            const double *sh, const double *s, const double *t, double *x,
+                            double *r, const double *R, double *part)
+{
+    const bool live = sc[B_DONE] == 0.0;
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double ri;
+        if (live) {
+            x[i] += sc[B_ALPHA] * ph[i] + sc[B_OMEGA] * sh[i];
+            ri = s[i] - sc[B_OMEGA] * t[i];
+            r[i] = ri;
+        } else {
+            ri = r[i];
+        }
+        acc[0] += ri * ri;
+        acc[1] += R[i] * ri;
+    }
+    block_partials<2>(acc, part);
 }
 
 // BiCGSTAB (van der Vorst 1992), right Jacobi preconditioning, on J dx = -r
 // with J the current Jacobian mode at V_XE and r in V_R; dx0 = 0, result in
 // V_DX.  Stops at ||r_lin||^2 <= lin_rtol^2 * rn (rn = ||r||^2).  Returns the
-// iteration count.  Operators above 64 MiB run one iteration per host check
-// (a product costs far more than the check); smaller ones enqueue eight
-// iterations per check.
+// iteration count.  An iteration is 12 launches: each vector update is fused
+// with the preconditioner, the Jacobian's pre-scaling or the dot product that
+// reads it, and each reduction's last step runs the scalar recurrences.
+// Operators above 64 MiB run one iteration per host check (a product costs
+// far more than the check); smaller ones enqueue eight iterations per check.
 static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
 {
     Scene &s = c.s;
@@ -551,33 +678,29 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
     IRTT_HIP(hipMemcpyAsync(R, rl, nb * sizeof(double), hipMemcpyDeviceToDevice, stm));
     kd_init<<<1, 1, 0, stm>>>(sc, lin_rtol * lin_rtol * rn);
     op_prepare(s, c.sys.transposed != 0);
-    auto dot_into = [&](int m, const double *a0, const double *b0, const double *a1,
-                        const double *b1, double *dst) {
-        k_dots<<<kRedBlocks, kRedThreads, 0, stm>>>(nb, m, a0, b0, a1, b1, nullptr, nullptr,
-                                                    c.part);
-        k_final_sum<<<1, kRedThreads, 0, stm>>>(c.part, m, kRedBlocks, dst);
-    };
+    const bool tr = c.sys.transposed != 0;
+    // the first rho; later ones come with each iteration's last reduction
+    k_dots<<<kRedBlocks, kRedThreads, 0, stm>>>(nb, 1, R, rl, nullptr, nullptr, nullptr, nullptr,
+                                                c.part);
+    kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 1, sc, F_RHO);
     auto iteration = [&]() {
-        dot_into(1, R, rl, nullptr, nullptr, sc + B_RHO_NEW);
-        kd_rho<<<1, 1, 0, stm>>>(sc);
-        kd_p_update<<<g, 256, 0, stm>>>(nb, sc, rl, c.v[V_V], c.v[V_P]);
-        kd_scale<<<g, 256, 0, stm>>>(nb, sc, c.v[V_DINV], c.v[V_P], c.v[V_PH]);
-        c.jv(c.v[V_PH], c.v[V_V]);
-        dot_into(1, R, c.v[V_V], nullptr, nullptr, sc + B_TMP);
-        kd_alpha<<<1, 1, 0, stm>>>(sc);
-        kd_s<<<g, 256, 0, stm>>>(nb, sc, rl, c.v[V_V], c.v[V_S]);
-        dot_into(1, c.v[V_S], c.v[V_S], nullptr, nullptr, sc + B_SS);
-        kd_half<<<1, 1, 0, stm>>>(sc);
-        kd_half_apply<<<g, 256, 0, stm>>>(nb, sc, c.v[V_PH], c.v[V_DX]);
-        kd_half_done<<<1, 1, 0, stm>>>(sc);
-        kd_scale<<<g, 256, 0, stm>>>(nb, sc, c.v[V_DINV], c.v[V_S], c.v[V_SH]);
-        c.jv(c.v[V_SH], c.v[V_T]);
-        dot_into(2, c.v[V_T], c.v[V_S], c.v[V_T], c.v[V_T], sc + B_TS);
-        kd_omega<<<1, 1, 0, stm>>>(sc);
-        kd_step<<<g, 256, 0, stm>>>(nb, sc, c.v[V_PH], c.v[V_SH], c.v[V_S], c.v[V_T], c.v[V_DX],
-                                    rl);
-        dot_into(1, rl, rl, nullptr, nullptr, sc + B_RR);
-        kd_end<<<1, 1, 0, stm>>>(sc);
+        kd_p_pre<<<g, 256, 0, stm>>>(c.sys, sc, c.v[V_XE], rl, c.v[V_V], c.v[V_P], c.v[V_DINV],
+                                     c.v[V_PH], c.v[V_W]);
+        op_apply(s, c.v[V_W], c.v[V_TW], tr, false);
+        kd_post_dot<1><<<kRedBlocks, kRedThreads, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_PH], c.v[V_W],
+                                                           c.v[V_TW], c.v[V_V], R, c.part);
+        kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 1, sc, F_ALPHA);
+        kd_s_dot<<<kRedBlocks, kRedThreads, 0, stm>>>(nb, sc, rl, c.v[V_V], c.v[V_S], c.part);
+        kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 1, sc, F_HALF);
+        kd_sh_pre<<<g, 256, 0, stm>>>(c.sys, sc, c.v[V_XE], c.v[V_PH], c.v[V_DX], c.v[V_DINV],
+                                      c.v[V_S], c.v[V_SH], c.v[V_W]);
+        op_apply(s, c.v[V_W], c.v[V_TW], tr, false);
+        kd_post_dot<2><<<kRedBlocks, kRedThreads, 0, stm>>>(c.sys, c.v[V_XE], c.v[V_SH], c.v[V_W],
+                                                           c.v[V_TW], c.v[V_T], c.v[V_S], c.part);
+        kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 2, sc, F_OMEGA);
+        kd_step_dot<<<kRedBlocks, kRedThreads, 0, stm>>>(nb, sc, c.v[V_PH], c.v[V_SH], c.v[V_S],
+                                                         c.v[V_T], c.v[V_DX], rl, R, c.part);
+        kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 2, sc, F_END_RHO);
     };
     const size_t op_bytes = s.op_kind == IRTT_OP_SPARSE
                                 ? (size_t)s.sp.nnz * 8
