@@ -235,6 +235,7 @@ struct Scene {
     DBuf<float> d_u32;                // FP32 copy of a product's input vector
     DBuf<double> d_cd;                // diagonal outside the stored product
     DBuf<float> d_xs;                 // per-vector FP16 scales of a batched product
+    DBuf<double> d_xsm;               // their per-slice maxima
     DBuf<uint16_t> d_xh;              // FP16 vectors of a batched product (bit patterns)
     DBuf<float> d_gpart;              // split-K partial sums of the MFMA GEMM
     DBuf<double> d_x64;               // padded vectors of the FP64 MFMA GEMM

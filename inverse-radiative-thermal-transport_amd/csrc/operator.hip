@@ -293,15 +293,21 @@ __device__ __forceinline__ half8_t as_half8(uint4 v)
     return __builtin_bit_cast(half8_t, v);
 }
 
-// xs[r] = max_i |rs_i x_r[i]| / 2^14 (rs = nullptr: 1)
+// xs[r] = max_i |rs_i x_r[i]| / 2^14 (rs = nullptr: 1), in two steps: block
+// (r, s) takes the maximum over slice s of kRhsSlices of vector r, then one
+// block takes the maximum of the slices (a maximum does not depend on the
+// grouping).  One block per vector would leave most CUs idle.
+constexpr uint32_t kRhsSlices = 16;
 __global__ __launch_bounds__(256) void k_rhs_scale(const double *__restrict__ X, uint32_t n,
                                                    uint32_t ld, const float *__restrict__ rs,
-                                                   float *__restrict__ xs)
+                                                   double *__restrict__ smax)
 {
     __shared__ double sm[256];
     const double *x = X + (size_t)blockIdx.x * ld;
+    const uint32_t per = (n + kRhsSlices - 1) / kRhsSlices;
+    const uint32_t i0 = blockIdx.y * per, i1 = min(i0 + per, n);
     double m = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += 256)
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += 256)
         m = fmax(m, fabs(x[i] * (rs ? (double)rs[i] : 1.0)));
     sm[threadIdx.x] = m;
     __syncthreads();
@@ -309,7 +315,17 @@ __global__ __launch_bounds__(256) void k_rhs_scale(const double *__restrict__ X,
         if ((int)threadIdx.x < off) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + off]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) xs[blockIdx.x] = (float)(sm[0] * 0x1p-14);
+    if (threadIdx.x == 0) smax[blockIdx.x * kRhsSlices + blockIdx.y] = sm[0];
+}
+
+__global__ void k_rhs_scale_fin(const double *__restrict__ smax, uint32_t R,
+                                float *__restrict__ xs)
+{
+    const uint32_t r = threadIdx.x;
+    if (r >= R) return;
+    double m = 0.0;
+    for (uint32_t k = 0; k < kRhsSlices; k++) m = fmax(m, smax[r * kRhsSlices + k]);
+    xs[r] = (float)(m * 0x1p-14);
 }
 
 // Xh = FP16(rs_i x_r[i] / xs[r]), zero for r >= R and i >= n, stored in MFMA
@@ -739,6 +755,7 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
     const uint32_t kbchunk = cdiv(KB, ksplit);
     ksplit = cdiv(KB, kbchunk);
     s.d_xs.alloc(64);
+    s.d_xsm.alloc(64 * kRhsSlices);
     s.d_xh.alloc((size_t)64 * ld);
     s.d_gpart.alloc((size_t)ksplit * 64 * ld);
     _Float16 *Xh = reinterpret_cast<_Float16 *>(s.d_xh.p);
@@ -746,7 +763,9 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
     for (uint32_t r0 = 0; r0 < R; r0 += 64) {
         const uint32_t rc = std::min<uint32_t>(64, R - r0), nt = cdiv(rc, 32), rpad = 32 * nt;
         const double *Xc = X + (size_t)r0 * ld;
-        k_rhs_scale<<<rc, 256, 0, st>>>(Xc, nb, ld, transposed ? rs : nullptr, s.d_xs.p);
+        k_rhs_scale<<<dim3(rc, kRhsSlices), 256, 0, st>>>(Xc, nb, ld, transposed ? rs : nullptr,
+                                                           s.d_xsm.p);
+        k_rhs_scale_fin<<<1, 64, 0, st>>>(s.d_xsm.p, rc, s.d_xs.p);
         k_rhs_quant<<<cdiv((size_t)rpad * ld, 256), 256, 0, st>>>(
             Xc, nb, ld, rc, rpad, transposed ? rs : nullptr, s.d_xs.p, Xh);
         dim3 grid(cdiv(tiles, 4), ksplit);
