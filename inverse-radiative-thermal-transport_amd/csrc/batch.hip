@@ -256,6 +256,103 @@ __global__ void kb_sub(BSys s, const double *__restrict__ a, const double *__res
 
 #undef COL
 
+// ---- per-column scalars of the lockstep BiCGSTAB in device memory ----
+// bsc[c * kBS + field] per column, g[] wave-uniform fields; coef / coefh are
+// the {live, alpha, beta, omega} rows the vector kernels read (coefh: the
+// half step, live = the columns finishing with dx += alpha ph).  The
+// recurrences are those of a host loop over the columns, in the same order
+// and FP64 arithmetic.
+enum { BS_RHO = 0, BS_RHO_NEW, BS_ALPHA, BS_OMEGA, BS_BETA, BS_LIVE, BS_STOP, kBS };
+enum { BG_ITERS = 0, BG_FIRST, BG_ANY, kBG };
+
+__device__ __forceinline__ void bs_coef(const double *b, double *coef, uint32_t c)
+{
+    coef[4 * c] = b[BS_LIVE];
+    coef[4 * c + 1] = b[BS_ALPHA];
+    coef[4 * c + 2] = b[BS_BETA];
+    coef[4 * c + 3] = b[BS_OMEGA];
+}
+
+__global__ void kbs_init(uint32_t R, const double *__restrict__ stop, const uint8_t *__restrict__ live,
+                         double *bsc, double *gs, double *coef)
+{
+    for (uint32_t c = threadIdx.x; c < R; c += blockDim.x) {
+        double *b = bsc + (size_t)c * kBS;
+        b[BS_RHO] = b[BS_ALPHA] = b[BS_OMEGA] = 1.0;
+        b[BS_RHO_NEW] = b[BS_BETA] = 0.0;
+        b[BS_STOP] = stop[c];
+        b[BS_LIVE] = live[c] ? 1.0 : 0.0;
+        bs_coef(b, coef, c);
+    }
+    if (threadIdx.x == 0) {
+        gs[BG_ITERS] = 0.0;
+        gs[BG_FIRST] = 1.0;
+        gs[BG_ANY] = 1.0;
+    }
+}
+
+// a stage's per-column step after its reduction (sums[k * R + c]);
+// one block, threads over the columns
+enum { BST_RHO = 0, BST_ALPHA, BST_HALF, BST_OMEGA, BST_END };
+__global__ void kbs_step(uint32_t R, int stage, const double *__restrict__ sums, double *bsc,
+                         double *gs, double *coef, double *coefh)
+{
+    __shared__ int any_live;
+    if (threadIdx.x == 0) any_live = 0;
+    __syncthreads();
+    const bool first = gs[BG_FIRST] != 0.0;
+    for (uint32_t c = threadIdx.x; c < R; c += blockDim.x) {
+        double *b = bsc + (size_t)c * kBS;
+        const bool live = b[BS_LIVE] != 0.0;
+        if (stage == BST_RHO) {
+            if (live) {
+                atomicOr(&any_live, 1);
+                const double rn = sums[c];
+                b[BS_RHO_NEW] = rn;
+                if (rn == 0.0 || !isfinite(rn)) b[BS_LIVE] = 0.0;
+                b[BS_BETA] = first ? 0.0 : (rn / b[BS_RHO]) * (b[BS_ALPHA] / b[BS_OMEGA]);
+            }
+        } else if (stage == BST_ALPHA) {
+            if (live) {
+                const double d = sums[c];
+                if (d == 0.0 || !isfinite(d)) b[BS_LIVE] = 0.0;
+                else b[BS_ALPHA] = b[BS_RHO_NEW] / d;
+            }
+        } else if (stage == BST_HALF) {
+            // columns whose s is small enough finish with dx += alpha ph
+            const bool fin = live && sums[c] <= b[BS_STOP];
+            coefh[4 * c] = fin ? 1.0 : 0.0;
+            coefh[4 * c + 1] = b[BS_ALPHA];
+            coefh[4 * c + 2] = b[BS_BETA];
+            coefh[4 * c + 3] = b[BS_OMEGA];
+            if (fin) b[BS_LIVE] = 0.0;
+        } else if (stage == BST_OMEGA) {
+            if (live) {
+                const double tt = sums[R + c];
+                if (tt == 0.0) b[BS_LIVE] = 0.0;
+                else b[BS_OMEGA] = sums[c] / tt;
+            }
+        } else {
+            if (live) {
+                b[BS_RHO] = b[BS_RHO_NEW];
+                if (sums[c] <= b[BS_STOP] || b[BS_OMEGA] == 0.0) b[BS_LIVE] = 0.0;
+            }
+            if (b[BS_LIVE] != 0.0) atomicOr(&any_live, 1);
+        }
+        bs_coef(b, coef, c);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (stage == BST_RHO) {
+            // an iteration counts when some column was live at its start
+            if (any_live) gs[BG_ITERS] += 1.0;
+            gs[BG_FIRST] = 0.0;
+        } else if (stage == BST_END) {
+            gs[BG_ANY] = any_live ? 1.0 : 0.0;
+        }
+    }
+}
+
 enum {
     B_X = 0, B_XE, B_Y, B_TY, B_R, B_DX, B_XT, B_P, B_V, B_S, B_T, B_PH, B_SH, B_RHAT, B_W, B_TW,
     B_DINV, B_B, B_RL, B_LAM, B_AUX, B_COUNT
@@ -266,7 +363,9 @@ struct BCtx {
     uint32_t nb, ld, R;
     hipStream_t st;
     BSys sys{};
-    DBuf<double> vec, part, sums, coef;
+    DBuf<double> vec, part, sums, coef, coefh, bsc, gs;
+    DBuf<double> dstop;
+    DBuf<uint8_t> dlive;
     DBuf<double> kfix, tdiag, ldiag;
     DBuf<uint8_t> fixed;
     std::vector<double> hsum, hcoef;
@@ -290,6 +389,13 @@ struct BCtx {
         IRTT_HIP(hipMemcpyAsync(hsum.data(), sums.p, (size_t)m * R * sizeof(double),
                                 hipMemcpyDeviceToHost, st));
         IRTT_HIP(hipStreamSynchronize(st));
+    }
+    // per-column sums of m dot products into sums[k * R + c], on the device
+    void dots_dev(int m, const double *a0, const double *b0, const double *a1 = nullptr,
+                  const double *b1 = nullptr)
+    {
+        kb_dots<<<gr, 256, 0, st>>>(nb, ld, m, a0, b0, a1, b1, part.p);
+        kb_sum<<<cdiv((size_t)m * R, 256), 256, 0, st>>>(part.p, m * R, sums.p);
     }
     void upload_coef()
     {
@@ -334,6 +440,11 @@ void setup_batch(BCtx &c, const irtt_solver_params &p)
     c.part.alloc((size_t)2 * R * kBRed);
     c.sums.alloc((size_t)2 * R);
     c.coef.alloc((size_t)4 * R);
+    c.coefh.alloc((size_t)4 * R);
+    c.bsc.alloc((size_t)kBS * R);
+    c.gs.alloc(kBG);
+    c.dstop.alloc(R);
+    c.dlive.alloc(R);
     c.hcoef.assign((size_t)4 * R, 0.0);
     auto up = [&](DBuf<double> &d, const std::vector<double> &h) {
         d.alloc(ld);
@@ -381,7 +492,11 @@ void setup_batch(BCtx &c, const irtt_solver_params &p)
 }
 
 // BiCGSTAB on all live columns: J dx = -r (B_R), dx in B_DX.  live[c] = 0
-// columns keep dx = 0.  Returns the largest per-column iteration count.
+// columns keep dx = 0.  Returns the number of iterations in which some column
+// was live.  The per-column scalars stay in device memory (kbs_step after each
+// reduction), so the host checks once per iteration, or once per eight for
+// operators up to 64 MiB (above that, the products of iterations run after
+// every column has stopped would cost more than the checks they save).
 uint32_t bicgstab_batch(BCtx &c, std::vector<uint8_t> live, uint32_t max_lin, double rtol,
                         const std::vector<double> &rn)
 {
@@ -393,81 +508,54 @@ uint32_t bicgstab_batch(BCtx &c, std::vector<uint8_t> live, uint32_t max_lin, do
     kb_neg<<<c.g, 256, 0, st>>>(c.sys, c.v[B_R], c.v[B_RL]);
     IRTT_HIP(hipMemcpyAsync(c.v[B_RHAT], c.v[B_RL], (size_t)R * ld * sizeof(double),
                             hipMemcpyDeviceToDevice, st));
-    std::vector<double> rho(R, 1.0), alpha(R, 1.0), omega(R, 1.0), stop(R);
+    std::vector<double> stop(R);
     for (uint32_t k = 0; k < R; k++) {
         stop[k] = rtol * rtol * rn[k];
         if (!(rn[k] > 0.0)) live[k] = 0;
     }
-    auto set_coef = [&](const std::vector<double> &a, const std::vector<double> &b,
-                        const std::vector<double> &o) {
-        for (uint32_t k = 0; k < R; k++) {
-            c.hcoef[4 * k] = live[k] ? 1.0 : 0.0;
-            c.hcoef[4 * k + 1] = a[k];
-            c.hcoef[4 * k + 2] = b[k];
-            c.hcoef[4 * k + 3] = o[k];
-        }
-        c.upload_coef();
+    bool any = false;
+    for (uint32_t k = 0; k < R; k++) any |= live[k] != 0;
+    if (!any) return 0;
+    IRTT_HIP(hipMemcpyAsync(c.dstop.p, stop.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
+    IRTT_HIP(hipMemcpyAsync(c.dlive.p, live.data(), R, hipMemcpyHostToDevice, st));
+    const uint32_t sthreads = std::min<uint32_t>(1024, (R + 63) / 64 * 64);
+    kbs_init<<<1, sthreads, 0, st>>>(R, c.dstop.p, c.dlive.p, c.bsc.p, c.gs.p, c.coef.p);
+    auto step = [&](int stage) {
+        kbs_step<<<1, sthreads, 0, st>>>(R, stage, c.sums.p, c.bsc.p, c.gs.p, c.coef.p, c.coefh.p);
     };
-    std::vector<double> beta(R, 0.0), zero(R, 0.0);
-    uint32_t li = 0;
-    for (; li < max_lin; li++) {
-        bool any = false;
-        for (uint32_t k = 0; k < R; k++) any |= live[k] != 0;
-        if (!any) break;
-        c.dots(1, c.v[B_RHAT], c.v[B_RL]);
-        std::vector<double> rho_new(c.hsum.begin(), c.hsum.begin() + R);
-        for (uint32_t k = 0; k < R; k++) {
-            if (!live[k]) continue;
-            if (rho_new[k] == 0.0 || !std::isfinite(rho_new[k])) live[k] = 0;
-            beta[k] = li == 0 ? 0.0 : (rho_new[k] / rho[k]) * (alpha[k] / omega[k]);
-        }
-        set_coef(alpha, beta, omega);
+    auto iteration = [&]() {
+        c.dots_dev(1, c.v[B_RHAT], c.v[B_RL]);
+        step(BST_RHO);
         kb_p_update<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, c.v[B_RL], c.v[B_V], c.v[B_P]);
         kb_scale<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, c.v[B_DINV], c.v[B_P], c.v[B_PH]);
         c.jv(c.v[B_PH], c.v[B_V]);
-        c.dots(1, c.v[B_RHAT], c.v[B_V]);
-        for (uint32_t k = 0; k < R; k++) {
-            if (!live[k]) continue;
-            double d = c.hsum[k];
-            if (d == 0.0 || !std::isfinite(d)) live[k] = 0;
-            else alpha[k] = rho_new[k] / d;
-        }
-        set_coef(alpha, beta, omega);
+        c.dots_dev(1, c.v[B_RHAT], c.v[B_V]);
+        step(BST_ALPHA);
         kb_axpy<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, -1.0, c.v[B_RL], c.v[B_V], c.v[B_S]);
-        c.dots(1, c.v[B_S], c.v[B_S]);
-        // columns whose s is small enough finish with dx += alpha ph
-        std::vector<uint8_t> fin(R, 0);
-        for (uint32_t k = 0; k < R; k++)
-            if (live[k] && c.hsum[k] <= stop[k]) fin[k] = 1;
-        {
-            std::vector<uint8_t> keep = live;
-            for (uint32_t k = 0; k < R; k++) live[k] = fin[k];
-            set_coef(alpha, beta, omega);
-            kb_axpy<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, 1.0, c.v[B_DX], c.v[B_PH], c.v[B_DX]);
-            for (uint32_t k = 0; k < R; k++) live[k] = keep[k] && !fin[k];
-        }
-        set_coef(alpha, beta, omega);
+        c.dots_dev(1, c.v[B_S], c.v[B_S]);
+        step(BST_HALF);
+        kb_axpy<<<c.g, 256, 0, st>>>(nb, ld, c.coefh.p, 1.0, c.v[B_DX], c.v[B_PH], c.v[B_DX]);
         kb_scale<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, c.v[B_DINV], c.v[B_S], c.v[B_SH]);
         c.jv(c.v[B_SH], c.v[B_T]);
-        c.dots(2, c.v[B_T], c.v[B_S], c.v[B_T], c.v[B_T]);
-        for (uint32_t k = 0; k < R; k++) {
-            if (!live[k]) continue;
-            double tt = c.hsum[R + k];
-            if (tt == 0.0) live[k] = 0;
-            else omega[k] = c.hsum[k] / tt;
-        }
-        set_coef(alpha, beta, omega);
+        c.dots_dev(2, c.v[B_T], c.v[B_S], c.v[B_T], c.v[B_T]);
+        step(BST_OMEGA);
         kb_step<<<c.g, 256, 0, st>>>(nb, ld, c.coef.p, c.v[B_PH], c.v[B_SH], c.v[B_S], c.v[B_T],
                                      c.v[B_DX], c.v[B_RL]);
-        c.dots(1, c.v[B_RL], c.v[B_RL]);
-        for (uint32_t k = 0; k < R; k++) {
-            if (!live[k]) continue;
-            rho[k] = rho_new[k];
-            if (c.hsum[k] <= stop[k] || omega[k] == 0.0) live[k] = 0;
-        }
+        c.dots_dev(1, c.v[B_RL], c.v[B_RL]);
+        step(BST_END);
+    };
+    const uint32_t K = c.s.op_bytes() > (64u << 20) ? 1u : 8u;
+    double hg[kBG] = {0.0, 0.0, 0.0};
+    uint32_t launched = 0;
+    while (any && launched < max_lin) {
+        const uint32_t kk = std::min<uint32_t>(K, max_lin - launched);
+        for (uint32_t k = 0; k < kk; k++) iteration();
+        launched += kk;
+        IRTT_HIP(hipMemcpyAsync(hg, c.gs.p, sizeof(hg), hipMemcpyDeviceToHost, st));
+        IRTT_HIP(hipStreamSynchronize(st));
+        any = hg[BG_ANY] != 0.0;
     }
-    (void)zero;
-    return li;
+    return (uint32_t)hg[BG_ITERS];
 }
 
 }  // namespace

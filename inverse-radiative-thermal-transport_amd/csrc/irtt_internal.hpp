@@ -250,6 +250,12 @@ struct Scene {
     DBuf<double> d_lap_val;
 
     uint64_t device_bytes() const;
+    // bytes one product with the selected operator form reads
+    size_t op_bytes() const
+    {
+        return op_kind == IRTT_OP_SPARSE ? (size_t)sp.nnz * 8
+                                         : (size_t)nb * ld * (op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
+    }
 };
 
 // api.cpp: thread-local message returned by irtt_last_error()

@@ -702,10 +702,7 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
                                                          c.v[V_T], c.v[V_DX], rl, R, c.part);
         kd_final<<<1, kRedThreads, 0, stm>>>(c.part, 2, sc, F_END_RHO);
     };
-    const size_t op_bytes = s.op_kind == IRTT_OP_SPARSE
-                                ? (size_t)s.sp.nnz * 8
-                                : (size_t)s.nb * s.ld * (s.op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
-    const uint32_t K = op_bytes > (64u << 20) ? 1u : 8u;
+    const uint32_t K = s.op_bytes() > (64u << 20) ? 1u : 8u;
     double hs[2] = {0.0, 0.0};
     uint32_t launched = 0;
     while (launched < max_lin) {
