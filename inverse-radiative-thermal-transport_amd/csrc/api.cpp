@@ -312,13 +312,16 @@ int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32
         IRTT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         IRTT_HIP(hipEventCreate(&s->ev0));
         IRTT_HIP(hipEventCreate(&s->ev1));
+        IRTT_HIP(hipEventCreate(&s->ev2));
         t_alloc_stream = s->stream;
         try {
             upload_geometry(*s);
         } catch (...) {
             hipStream_t st = s->stream;
+            hipEvent_t e[3] = {s->ev0, s->ev1, s->ev2};
             s.reset();
             (void)hipStreamSynchronize(st);
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
             (void)hipStreamDestroy(st);
             t_alloc_stream = nullptr;
             throw;
@@ -335,11 +338,11 @@ int irtt_scene_destroy(irtt_scene h)
         t_alloc_stream = nullptr;
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         hipStream_t st = s->stream;
-        hipEvent_t e0 = s->ev0, e1 = s->ev1;
+        hipEvent_t e[3] = {s->ev0, s->ev1, s->ev2};
         delete s;  // buffers go back to the pool in the scene's stream order
         if (st) (void)hipStreamSynchronize(st);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
+        for (hipEvent_t x : e)
+            if (x) (void)hipEventDestroy(x);
         if (st) (void)hipStreamDestroy(st);
     });
 }

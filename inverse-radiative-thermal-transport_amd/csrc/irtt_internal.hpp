@@ -178,11 +178,37 @@ struct Aca {
     }
 };
 
+// Page-locked host memory for a scene's small read-backs (hipHostMalloc),
+// grown on demand and freed with the scene.  A caller synchronizes the
+// stream that last copied into it before asking for a larger block.
+struct HostPin {
+    void *p = nullptr;
+    size_t n = 0;
+    HostPin() = default;
+    HostPin(const HostPin &) = delete;
+    HostPin &operator=(const HostPin &) = delete;
+    ~HostPin()
+    {
+        if (p) (void)hipHostFree(p);
+    }
+    void *get(size_t bytes)
+    {
+        if (bytes <= n) return p;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        IRTT_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+        n = bytes;
+        return p;
+    }
+};
+
 struct Scene {
     uint32_t nv = 0, nt = 0, nb = 0, bin_mode = 0;
     uint32_t ld = 0;  // row pitch of the dense operator: nb rounded up to 64
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    HostPin h_pin;                     // transport build read-backs
 
     // host copies of the inputs
     std::vector<float> xyz;
@@ -215,9 +241,11 @@ struct Scene {
     Sparse sp;                        // IRTT_LAYOUT_SPARSE
     double quantum = 0.0;
     DBuf<uint64_t> d_q;               // per-triangle fixed-point weight
+    std::vector<uint64_t> h_q;        // what d_q holds (uploads of the same weights are skipped)
     DBuf<unsigned long long> d_acc;   // nb*ld fixed-point accumulators
     DBuf<float> d_T;                  // nb*ld FP32 operator, row-major
     DBuf<double> d_diag;              // sigma*eps*A per bin
+    std::vector<double> h_diag;       // what d_diag holds
     DBuf<unsigned long long> d_colacc;  // column sums of d_acc
     DBuf<float> d_tdiag;              // diagonal of d_T
     std::vector<double> colsum;        // column sums of T (exact)

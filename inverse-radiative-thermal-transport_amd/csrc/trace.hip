@@ -360,18 +360,32 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.tparams = p;
 
     hipStream_t stm = s.stream;
+    // weights and diagonals are uploaded only when they differ from what the
+    // device buffers already hold (repeated builds of one scene)
+    const uint64_t *prev_q = s.d_q.p;
     s.d_q.alloc(s.nt);
-    IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice, stm));
+    if (s.d_q.p != prev_q || s.h_q != q) {
+        s.h_q.clear();
+        IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                stm));
+        s.h_q = q;
+    }
     if (!sparse) {
         s.d_acc.alloc(nb2);
         s.d_T.alloc(nb2);
     }
+    const double *prev_diag = s.d_diag.p;
     s.d_diag.alloc(nb);
     s.d_colacc.alloc(nb);
     s.d_stats.alloc(16);
     std::vector<double> diag(nb);
     for (uint32_t i = 0; i < nb; i++) diag[i] = kSigmaD * s.bin_eps_area[i];
-    IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice, stm));
+    if (s.d_diag.p != prev_diag || s.h_diag != diag) {
+        s.h_diag.clear();
+        IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice,
+                                stm));
+        s.h_diag = diag;
+    }
     PhotonSink sink;
     if (sparse) {
         sparse_table_begin(s, total, sink);
@@ -442,10 +456,8 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
     IRTT_HIP(hipGetLastError());
     IRTT_HIP(hipEventRecord(s.ev1, stm));
-    IRTT_HIP(hipStreamSynchronize(stm));
-    IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
 
-    IRTT_HIP(hipEventRecord(s.ev0, stm));
+    // finalize timed from ev1 to ev2; both times are read after the last sync
     if (sparse) {
         sparse_table_finish(s, sink);
     } else {
@@ -455,22 +467,27 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
                                         s.d_tdiag.p);
         IRTT_HIP(hipGetLastError());
     }
-    IRTT_HIP(hipEventRecord(s.ev1, stm));
-    std::vector<unsigned long long> colacc(nb), hstats(16);
-    std::vector<float> dcol(nb);
-    IRTT_HIP(hipMemcpyAsync(colacc.data(), s.d_colacc.p, nb * sizeof(unsigned long long),
+    IRTT_HIP(hipEventRecord(s.ev2, stm));
+    // read-backs into the scene's page-locked block: column sums, counters,
+    // stored diagonal
+    auto *colacc = static_cast<unsigned long long *>(
+        s.h_pin.get((size_t)nb * sizeof(unsigned long long) + 16 * sizeof(unsigned long long) +
+                    (size_t)nb * sizeof(float)));
+    unsigned long long *hstats = colacc + nb;
+    float *dcol = reinterpret_cast<float *>(hstats + 16);
+    IRTT_HIP(hipMemcpyAsync(colacc, s.d_colacc.p, nb * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
-    IRTT_HIP(hipMemcpyAsync(hstats.data(), s.d_stats.p, 16 * sizeof(unsigned long long),
+    IRTT_HIP(hipMemcpyAsync(hstats, s.d_stats.p, 16 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
-    IRTT_HIP(hipMemcpyAsync(dcol.data(), s.d_tdiag.p, nb * sizeof(float), hipMemcpyDeviceToHost,
-                            stm));
+    IRTT_HIP(hipMemcpyAsync(dcol, s.d_tdiag.p, nb * sizeof(float), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
-    IRTT_HIP(hipEventElapsedTime(&st.finalize_ms, s.ev0, s.ev1));
+    IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
+    IRTT_HIP(hipEventElapsedTime(&st.finalize_ms, s.ev1, s.ev2));
 
     s.colsum.assign(nb, 0.0);
     for (uint32_t j = 0; j < nb; j++) s.colsum[j] = diag[j] - (double)colacc[j] * Q;
     // diagonal of the stored operator, for the Jacobi preconditioner
-    s.Tdiag.assign(dcol.begin(), dcol.end());
+    s.Tdiag.assign(dcol, dcol + nb);
     st.photons = total;
     st.rays = hstats[S_RAYS];
     s.fwd_rays = st.rays;
