@@ -337,6 +337,20 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         s.d_T.release();
     }
 
+    hipStream_t stm = s.stream;
+    const uint64_t total = per64 * s.nt;
+    // the accumulator (or hash table) clear runs while the host derives the
+    // photon weights
+    PhotonSink sink;
+    if (sparse) {
+        sparse_table_begin(s, total, sink);
+    } else {
+        s.d_acc.alloc(nb2);
+        s.d_T.alloc(nb2);
+        IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+        sink.acc = s.d_acc.p;
+    }
+
     // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
     // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s
     std::vector<float> w(s.nt);
@@ -347,7 +361,6 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         w[t] = wt;
         wmax = std::max(wmax, wt);
     }
-    const uint64_t total = per64 * s.nt;
     int sh = 62 - (int)ceil_log2_u64(total ? total : 1);
     if (sh > 40) sh = 40;
     const double Q = wmax > 0.0f ? std::ldexp((double)wmax, -sh) : 1.0;
@@ -359,7 +372,6 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.quantum = Q;
     s.tparams = p;
 
-    hipStream_t stm = s.stream;
     // weights and diagonals are uploaded only when they differ from what the
     // device buffers already hold (repeated builds of one scene)
     const uint64_t *prev_q = s.d_q.p;
@@ -369,10 +381,6 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice,
                                 stm));
         s.h_q = q;
-    }
-    if (!sparse) {
-        s.d_acc.alloc(nb2);
-        s.d_T.alloc(nb2);
     }
     const double *prev_diag = s.d_diag.p;
     s.d_diag.alloc(nb);
@@ -385,13 +393,6 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice,
                                 stm));
         s.h_diag = diag;
-    }
-    PhotonSink sink;
-    if (sparse) {
-        sparse_table_begin(s, total, sink);
-    } else {
-        IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
-        sink.acc = s.d_acc.p;
     }
     IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
     IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
