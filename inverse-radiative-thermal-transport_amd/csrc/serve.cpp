@@ -113,31 +113,44 @@ struct Server {
                 cv_work.wait(lk, [&] { return stop || (q = admit_locked()) != nullptr; });
                 if (!q) return;  // stop with nothing admissible left
             }
-            int rc = run(*q);
+            // a request is done once its results are in the caller's buffers;
+            // its memory stays counted until its scene has been released
+            auto publish = [&] {
+                {
+                    std::lock_guard<std::mutex> lk(m);
+                    q->t_end = Clock::now();
+                    q->state = IRTT_REQ_DONE;
+                }
+                cv_done.notify_all();
+            };
+            int rc = run(*q, publish);
             {
                 std::lock_guard<std::mutex> lk(m);
                 in_use -= q->footprint;
-                q->t_end = Clock::now();
-                q->state = rc == IRTT_OK ? IRTT_REQ_DONE : rc;
+                if (q->state == IRTT_REQ_RUNNING) {
+                    q->t_end = Clock::now();
+                    q->state = rc == IRTT_OK ? IRTT_REQ_DONE : rc;
+                }
             }
             cv_done.notify_all();
             cv_work.notify_all();
         }
     }
 
-    static int run(Req &q);
+    static int run(Req &q, const std::function<void()> &publish);
 };
 
 // One design: scene -> transport -> solve -> adjoint solve -> adjoint trace
 // -> projection (the steps of ThermalRenderer::runForwardSim / runAdjointSim
 // for one state, thermal_renderer.cpp:958-1133).
-int Server::run(Req &q)
+int Server::run(Req &q, const std::function<void()> &publish)
 {
     const irtt_request &r = q.r;
     irtt_scene h = nullptr;
     int rc = irtt_scene_create(q.xyz.data(), r.nv, q.tri.data(), r.nt, q.mat.data(), r.bin_mode, &h);
     if (rc) return rc;
     auto done = [&](int code) {
+        if (code == IRTT_OK) publish();
         irtt_scene_destroy(h);
         return code;
     };
