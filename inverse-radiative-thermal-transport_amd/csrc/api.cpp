@@ -11,7 +11,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
-#include <map>
+
 #include <memory>
 #include <mutex>
 #include <new>
@@ -92,6 +92,74 @@ double cotan(const double a[3], const double b[3])
     return ab / std::sqrt(den);
 }
 
+// Cotangent Laplacian scaled by conductivity * thickness (thermal_data.cpp:259-313)
+// into s.lap (CSR, columns ascending).  Each triangle adds nine terms to its
+// vertices' rows.  The terms are bucketed by row in triangle order (counting
+// sort), sorted by column within a row (stable) and summed in that order from
+// 0.0, which gives every entry the same sum as accumulating the triangles one
+// after another.
+static void build_laplacian(Scene &s)
+{
+    const uint32_t nt = s.nt, nv = s.nv;
+    std::vector<uint32_t> start(nv + 1, 0);
+    for (uint32_t t = 0; t < nt; t++) {
+        if ((double)s.mat[t].heat_conductivity * (double)s.mat[t].thickness == 0.0) continue;
+        for (int c = 0; c < 3; c++) start[s.idx[3 * t + c] + 1] += 3;
+    }
+    for (uint32_t i = 0; i < nv; i++) start[i + 1] += start[i];
+    std::vector<std::pair<uint32_t, double>> ent(start[nv]);
+    std::vector<uint32_t> fill(start.begin(), start.end() - 1);
+    for (uint32_t t = 0; t < nt; t++) {
+        const irtt_material &m = s.mat[t];
+        double k = (double)m.heat_conductivity * (double)m.thickness;
+        if (k == 0.0) continue;
+        const uint32_t v[3] = {s.idx[3 * t], s.idx[3 * t + 1], s.idx[3 * t + 2]};
+        double a[3], b[3], c[3];
+        for (int q = 0; q < 3; q++) {
+            a[q] = s.xyz[3 * v[0] + q];
+            b[q] = s.xyz[3 * v[1] + q];
+            c[q] = s.xyz[3 * v[2] + q];
+        }
+        double ac[3], bc[3], ba[3], ca[3], cb[3], ab[3];
+        for (int q = 0; q < 3; q++) {
+            ac[q] = a[q] - c[q]; bc[q] = b[q] - c[q];
+            ba[q] = b[q] - a[q]; ca[q] = c[q] - a[q];
+            cb[q] = c[q] - b[q]; ab[q] = a[q] - b[q];
+        }
+        const double cot1 = cotan(ac, bc), cot2 = cotan(ba, ca), cot3 = cotan(cb, ab);
+        // w[r][c]: the term of vertex r's row in vertex c's column
+        const double w[3][3] = {{k * 0.5 * (cot1 + cot3), k * 0.5 * (-cot1), k * 0.5 * (-cot3)},
+                                {k * 0.5 * (-cot1), k * 0.5 * (cot1 + cot2), k * 0.5 * (-cot2)},
+                                {k * 0.5 * (-cot3), k * 0.5 * (-cot2), k * 0.5 * (cot2 + cot3)}};
+        for (int r = 0; r < 3; r++)
+            for (int cc = 0; cc < 3; cc++) ent[fill[v[r]]++] = {v[cc], w[r][cc]};
+    }
+    s.lap.n = nv;
+    s.lap.rowptr.assign(nv + 1, 0);
+    s.lap.col.clear();
+    s.lap.val.clear();
+    s.lap.col.reserve(ent.size());
+    s.lap.val.reserve(ent.size());
+    for (uint32_t i = 0; i < nv; i++) {
+        auto b0 = ent.begin() + start[i], b1 = ent.begin() + start[i + 1];
+        // insertion sort by column: stable, and a row holds a few dozen terms
+        for (auto it = b0 + (b0 != b1); it < b1; ++it) {
+            const auto e = *it;
+            auto j = it;
+            for (; j != b0 && (j - 1)->first > e.first; --j) *j = *(j - 1);
+            *j = e;
+        }
+        for (auto it = b0; it != b1;) {
+            const uint32_t col = it->first;
+            double sum = 0.0;
+            for (; it != b1 && it->first == col; ++it) sum += it->second;
+            s.lap.col.push_back(col);
+            s.lap.val.push_back(sum);
+        }
+        s.lap.rowptr[i + 1] = (uint32_t)s.lap.col.size();
+    }
+}
+
 void assemble(Scene &s)
 {
     const uint32_t nt = s.nt, nv = s.nv;
@@ -167,44 +235,7 @@ void assemble(Scene &s)
         if (s.bin_mode == IRTT_BIN_TRIANGLE)
             throw Error(IRTT_ERR_ARG, "heat conduction needs vertex bins (IRTT_BIN_VERTEX)");
         // cotangent Laplacian scaled by conductivity * thickness, thermal_data.cpp:259-313
-        std::vector<std::map<uint32_t, double>> rows(nv);
-        for (uint32_t t = 0; t < nt; t++) {
-            const irtt_material &m = s.mat[t];
-            double k = (double)m.heat_conductivity * (double)m.thickness;
-            if (k == 0.0) continue;
-            uint32_t i0 = s.idx[3 * t], i1 = s.idx[3 * t + 1], i2 = s.idx[3 * t + 2];
-            double a[3], b[3], c[3];
-            for (int q = 0; q < 3; q++) {
-                a[q] = s.xyz[3 * i0 + q];
-                b[q] = s.xyz[3 * i1 + q];
-                c[q] = s.xyz[3 * i2 + q];
-            }
-            double ac[3], bc[3], ba[3], ca[3], cb[3], ab[3];
-            for (int q = 0; q < 3; q++) {
-                ac[q] = a[q] - c[q]; bc[q] = b[q] - c[q];
-                ba[q] = b[q] - a[q]; ca[q] = c[q] - a[q];
-                cb[q] = c[q] - b[q]; ab[q] = a[q] - b[q];
-            }
-            double cot1 = cotan(ac, bc), cot2 = cotan(ba, ca), cot3 = cotan(cb, ab);
-            rows[i0][i0] += k * 0.5 * (cot1 + cot3);
-            rows[i0][i1] += k * 0.5 * (-cot1);
-            rows[i0][i2] += k * 0.5 * (-cot3);
-            rows[i1][i0] += k * 0.5 * (-cot1);
-            rows[i1][i1] += k * 0.5 * (cot1 + cot2);
-            rows[i1][i2] += k * 0.5 * (-cot2);
-            rows[i2][i0] += k * 0.5 * (-cot3);
-            rows[i2][i1] += k * 0.5 * (-cot2);
-            rows[i2][i2] += k * 0.5 * (cot2 + cot3);
-        }
-        s.lap.n = nv;
-        s.lap.rowptr.assign(nv + 1, 0);
-        for (uint32_t i = 0; i < nv; i++) {
-            for (auto &kv : rows[i]) {
-                s.lap.col.push_back(kv.first);
-                s.lap.val.push_back(kv.second);
-            }
-            s.lap.rowptr[i + 1] = (uint32_t)s.lap.col.size();
-        }
+        build_laplacian(s);
     }
     s.d_lap_rowptr.release();
     s.d_lap_col.release();
