@@ -515,7 +515,7 @@ uint32_t bicgstab_batch(BCtx &c, std::vector<uint8_t> live, uint32_t max_lin, do
     }
     bool any = false;
     for (uint32_t k = 0; k < R; k++) any |= live[k] != 0;
-    if (!any) return 0;
+    if (!any || max_lin == 0) return 0;
     IRTT_HIP(hipMemcpyAsync(c.dstop.p, stop.data(), R * sizeof(double), hipMemcpyHostToDevice, st));
     IRTT_HIP(hipMemcpyAsync(c.dlive.p, live.data(), R, hipMemcpyHostToDevice, st));
     const uint32_t sthreads = std::min<uint32_t>(1024, (R + 63) / 64 * 64);
