@@ -161,7 +161,11 @@ __device__ __forceinline__ double hist_factor(const AdjArgs &a, uint32_t bin_l, 
 // the source vertices' sums stay in the lane's registers between photons.
 // The host shortens the runs when there are too few photons to give every
 // wave of the grid a chunk.
-constexpr uint32_t kAdjRun = 32;
+// (city block, 2^26 photons: 16 -> 37.5-38.3 ms, 32 -> 35.0-36.0, 64 -> 32.7-33.4)
+#ifndef IRTT_ADJ_RUN
+#define IRTT_ADJ_RUN 64
+#endif
+constexpr uint32_t kAdjRun = IRTT_ADJ_RUN;
 
 // what a lane does next: start a photon, trace a path segment, trace an
 // occlusion ray, end the current pass's path, nothing (no photons left)
