@@ -467,6 +467,8 @@ typedef struct irtt_request_status {
 int irtt_serve_create(const irtt_serve_config *cfg, irtt_server *out);
 int irtt_serve_submit(irtt_server sv, const irtt_request *req, uint64_t *ticket);
 /* non-blocking status; irtt_serve_wait blocks until the request has ended.
+ * A request has ended once its outputs are written; its scene may still be
+ * releasing device memory, which stays counted against the budget until then.
  * The status of an ended request stays readable until irtt_serve_destroy. */
 int irtt_serve_poll(irtt_server sv, uint64_t ticket, irtt_request_status *st);
 int irtt_serve_wait(irtt_server sv, uint64_t ticket, irtt_request_status *st);
