@@ -31,7 +31,7 @@ BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
 # DRAM bytes per k_photons launch on this workload, from rocprofv3 FETCH_SIZE
 # (x 2.0) + WRITE_SIZE (x 1.0) scaled by a 1 GiB copy calibration; the
 # commands and raw values are in profiles/k_photons_city_block_2p26.md
-TRAFFIC_BYTES_CITY = (1227200 * 2.0 + 1162026 * 1.0) * 1024
+TRAFFIC_BYTES_CITY = (1077598 * 2.0 + 1161716 * 1.0) * 1024
 
 
 def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):

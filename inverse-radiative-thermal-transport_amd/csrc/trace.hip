@@ -240,6 +240,14 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
 // T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
 // integer column sums of acc are accumulated per 256-row tile, and the stored
 // diagonal is gathered into tdiag.
+// photon kernel grid: at most this many waves per CU, lanes striding over
+// the photons.  Smaller per-wave shares even out the end of the launch
+// (city block, 2^26 photons: 24 -> 13.3 ms, 64 -> 13.0-13.1, 96 -> 12.7,
+// 256 -> 12.09-12.14, 512 -> 12.10-12.13, 1024 -> 14.4)
+#ifndef IRTT_PHOTON_WAVES_PER_CU
+#define IRTT_PHOTON_WAVES_PER_CU 256
+#endif
+
 constexpr int kFinRows = 256;
 __global__ void k_finalize(const unsigned long long *__restrict__ acc, const double *__restrict__ diag,
                            uint32_t nb, uint32_t ld, double Q, float *__restrict__ T,
@@ -448,7 +456,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
     // enough waves to cover every CU several times; lanes loop over photons
     uint64_t want = cdiv(total, kTraceBlock);
-    uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * 64);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU);
     if (grid == 0) grid = 1;
     IRTT_HIP(hipEventRecord(s.ev0, stm));
     if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
