@@ -706,8 +706,9 @@ static uint32_t bicgstab(Ctx &c, uint32_t max_lin, double lin_rtol, double rn)
     double hs[2] = {0.0, 0.0};
     uint32_t launched = 0;
     while (launched < max_lin) {
-        for (uint32_t k = 0; k < K; k++) iteration();
-        launched += K;
+        const uint32_t kk = std::min(K, max_lin - launched);
+        for (uint32_t k = 0; k < kk; k++) iteration();
+        launched += kk;
         IRTT_HIP(hipMemcpyAsync(hs, sc + B_DONE, sizeof(double), hipMemcpyDeviceToHost, stm));
         IRTT_HIP(hipMemcpyAsync(hs + 1, sc + B_ITERS, sizeof(double), hipMemcpyDeviceToHost,
                                 stm));

@@ -35,6 +35,60 @@ def build():
                    stdout=subprocess.DEVNULL)
 
 
+# The reference's generated gradient headers compiled into a checker of the
+# oracle's gradient formulas (``make ref``; see ``ref_grad_harness.c``).  Built
+# only where the reference tree is present; the library lands in ``_ref/``.
+REF_SHADERS = "/root/reference/assets/shader/raytracing_thermal"
+_REF_PATH = os.path.join(_HERE, "_ref", "libref_grad.so")
+_ref_lib = None
+
+
+def build_ref():
+    """Compile ``_ref/libref_grad.so`` from the reference's headers where they
+    lie; returns False when the reference tree is absent."""
+    if not os.path.isdir(REF_SHADERS):
+        return False
+    subprocess.run(["make", "-B", "-C", _HERE, "ref", f"REF_SHADERS={REF_SHADERS}"],
+                   check=True, stdout=subprocess.DEVNULL)
+    return True
+
+
+def ref_lib():
+    """The compiled reference gradient headers, or None when neither the
+    library nor the reference tree is here."""
+    global _ref_lib
+    if _ref_lib is None:
+        if not os.path.exists(_REF_PATH) and not build_ref():
+            return None
+        L = ctypes.CDLL(_REF_PATH)
+        d = ctypes.POINTER(ctypes.c_double)
+        L.ref_area_gradient.argtypes = [d] * 6
+        L.ref_geom_term_grad.restype = ctypes.c_double
+        L.ref_geom_term_grad.argtypes = [d, d, ctypes.c_double, d]
+        _ref_lib = L
+    return _ref_lib
+
+
+def ref_area_gradient(a, b, c):
+    """d area / d{a, b, c} as the reference's symGradArea3D.h computes it (FP64)."""
+    arrs = [np.ascontiguousarray(v, np.float64) for v in (a, b, c)]
+    out = [np.empty(3) for _ in range(3)]
+    d = ctypes.POINTER(ctypes.c_double)
+    ref_lib().ref_area_gradient(*[v.ctypes.data_as(d) for v in arrs + out])
+    return out
+
+
+def ref_geom_term_grad(verts6, bi, bj, eps=1e-2):
+    """(G, dG/dv (6, 3)) as the reference's symGradGeomTerm3D.h computes them (FP64)."""
+    v = np.ascontiguousarray(np.asarray(verts6, np.float64).reshape(18))
+    beta = np.array([bi[0], bi[1], bj[0], bj[1]], np.float64)
+    out = np.empty(18)
+    d = ctypes.POINTER(ctypes.c_double)
+    G = ref_lib().ref_geom_term_grad(v.ctypes.data_as(d), beta.ctypes.data_as(d), float(eps),
+                                     out.ctypes.data_as(d))
+    return G, out.reshape(6, 3)
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -180,3 +180,54 @@ def test_gmres_on_sparse_city(irtt):
     for x, st in out:
         print(f"{st}: max |dT| vs FP64 {np.abs(x - x64).max():.2e} K")
         assert st["converged"] and np.abs(x - x64).max() < 1e-7
+
+
+def _bicgstab_np(A, b, iters):
+    """Right-Jacobi-preconditioned BiCGSTAB (van der Vorst 1992) from x = 0,
+    exactly `iters` iterations (no stop test): the iteration the device runs."""
+    dinv = 1.0 / np.diag(A)
+    x = np.zeros_like(b)
+    r = b.copy()
+    R = r.copy()
+    rho = alpha = omega = 1.0
+    p = np.zeros_like(b)
+    v = np.zeros_like(b)
+    for k in range(iters):
+        rho_new = R @ r
+        beta = 0.0 if k == 0 else (rho_new / rho) * (alpha / omega)
+        p = r + beta * (p - omega * v)
+        ph = dinv * p
+        v = A @ ph
+        alpha = rho_new / (R @ v)
+        s = r - alpha * v
+        sh = dinv * s
+        t = A @ sh
+        omega = (t @ s) / (t @ t)
+        x = x + alpha * ph + omega * sh
+        r = s - omega * t
+        rho = rho_new
+    return x
+
+
+@pytest.mark.parametrize("max_linear", [3, 11])
+def test_bicgstab_honours_max_linear(irtt, max_linear):
+    """A linear solve stops after exactly max_linear iterations, also when
+    max_linear is not a multiple of the iterations enqueued per host check:
+    the adjoint lambda equals max_linear restated iterations on J^T."""
+    from irtt import scenes
+    sd = scenes.parallel_plates(n=8, emissivity=0.9)
+    rng = np.random.default_rng(7)
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=128, rays_per_thread=4, batch_seed=2)
+        T = s.transport().astype(np.float64)
+        x = 300.0 + 20.0 * rng.random(s.n_bins)
+        rhs = rng.normal(size=s.n_bins)
+        lam, st = s.adjoint_solve(rhs, x=x, params=irtt.solver_params(
+            max_linear=max_linear, lin_rtol=1e-300))
+    JT = (T * (4.0 * x ** 3)[None, :]).T
+    want = _bicgstab_np(JT, rhs, max_linear)
+    err = np.abs(lam - want).max() / np.abs(want).max()
+    print(f"max_linear={max_linear}: iterations {st['linear_iters']}, rel. diff vs restated "
+          f"{err:.2e}")
+    assert st["linear_iters"] == max_linear
+    assert err < 1e-9
