@@ -335,6 +335,20 @@ int irtt_adjoint_solve(irtt_scene h, const irtt_solver_params *p, const double *
                                           the shadow side only (thermal_tracing_unified.glsl:
                                           537-567, whose commented-out follow-up names the
                                           missing term) */
+#define IRTT_ADJ_HIT_AREA 32u          /* extension, off by default: every hit point is a fixed
+                                          barycentric point of its triangle, so a path's weight
+                                          also carries w dA_j/dv / A_j of each hit triangle j
+                                          (the reference keeps this factor for the emitting
+                                          triangle only, thermal_tracing_unified.glsl:327-351,
+                                          588-592): the occluder's own gain when its boundary
+                                          triangles stretch */
+#define IRTT_ADJ_EDGE_PROJECTED 64u    /* extension, off by default: the edge term's irradiance
+                                          derivative divided by |da . n_edge| instead of
+                                          multiplied (thermal_tracing_unified.glsl:486): the
+                                          area an edge displacement sweeps on the occluder's
+                                          plane, seen along the ray; equal at normal incidence,
+                                          the reference's factor is (da . n_edge)^2 too small
+                                          at grazing incidence */
 
 typedef struct irtt_adjoint_params {
     float edge_ratio;   /* ThermalVars_s::gradEdgeRatio (0.05), thermal_common.hpp:124 */

@@ -209,6 +209,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
     uint32_t nvis = 0, ntri = 0;
     const bool f_area = a.aflags & IRTT_ADJ_AREA, f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
     const bool f_edge = a.aflags & IRTT_ADJ_EDGE, f_gain = a.aflags & IRTT_ADJ_EDGE_GAIN;
+    const bool f_hit = a.aflags & IRTT_ADJ_HIT_AREA, f_proj = a.aflags & IRTT_ADJ_EDGE_PROJECTED;
 
     // this lane's run of photons: the next is run_g, `left` remain, in chunk `chunk`
     uint32_t chunk = blockIdx.x;
@@ -470,6 +471,19 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                             });
                         if (fabsf(G) > a.eps * a.eps) ev |= 1u << A_CONT;
                     }
+                    if (f_hit) {
+                        // extension: the hit triangle's area Jacobian (the hit
+                        // point is a fixed barycentric point of it)
+                        V3 ga[3];
+                        area_grad(q0, q1, q2, ga);
+                        const float sh = a.w[t] / a.area[hp];
+                        const uint32_t jv[3] = {j0, j1, j2};
+#pragma unroll
+                        for (int c = 0; c < 3; c++) {
+                            const V3 dh = scl3(ga[c], sh);
+                            if (finite3(dh)) acc3(a.grad, jv[c], dh, fpath);
+                        }
+                    }
                     const float hb[3] = {hb0, h.b1, h.b2};
                     const int opp = min_corner(hb0, h.b1, h.b2);
                     if (f_edge && hb[opp] <= a.ratio) {
@@ -498,7 +512,13 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adjoint(A
                         const float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
                         V3 da = normalize3(cross3(edir, d));
                         if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
-                        dE = scl3(da, fabsf(dot3(da, n_edg)) * elen * a.w[t] / larea);
+                        // the reference scales by |da.n_edg| (:486); the
+                        // extension divides by it: an edge moved by u sweeps
+                        // (u.da)/|da.n_edg| of the occluder's plane per unit
+                        // length, seen along the ray
+                        const float cpl = fabsf(dot3(da, n_edg));
+                        dE = f_proj ? scl3(da, elen * a.w[t] / (larea * cpl))
+                                    : scl3(da, cpl * elen * a.w[t] / larea);
                         if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
                         V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
                         to = add3(to, scl3(n_edg, a.eps + dot3(qv[nr] - to, n_edg)));

@@ -8,6 +8,10 @@
   absorptivity and areal heat capacity; direct sun at a given elevation.
 * ``closed_box`` -- the inside of a closed unit cube (every photon is absorbed
   somewhere), used for conservation checks.
+* ``blocker_plates`` -- an emitter plate, an opaque strip and a receiver
+  plate: a straight shadow edge, for checking the edge-sampling adjoint.
+* ``fin_plates`` -- the same with a vertical fin that the emitter's rays
+  reach at grazing incidence.
 
 Each returns a ``SceneData`` with per-triangle materials and per-vertex
 initial / fixed temperatures, the per-object inputs of the reference's
@@ -122,6 +126,66 @@ def parallel_plates(n=16, gap=1.0, emissivity=0.9, heat_flux_a=500.0, kelvin=0.0
                         kelvin=kelvin)
     sd.meta.update(n=n, gap=gap, emissivity=emissivity, plate_a_tris=ta.shape[0],
                    plate_a_verts=xa.shape[0])
+    return sd
+
+
+def blocker_plates(n=16, gap=1.0, blocker_z=0.5, blocker_x=(0.3, 0.6), blocker_y=(-0.5, 1.5),
+                   blocker_cells=(6, 20), receiver=True, emissivity=1.0):
+    """Emitter plate [0,1]^2 at z=0 facing +z, an opaque strip at z=blocker_z
+    spanning blocker_x x blocker_y facing the emitter (-z), and (receiver)
+    plate [0,1]^2 at z=gap facing -z.  The strip's edge at x = blocker_x[1]
+    casts a straight shadow across the receiver: moving that edge is a pure
+    visibility change, the case the adjoint's edge term is for.
+    meta: ``emitter_verts``, ``blocker_verts`` (vertex ranges), ``edge_vertex``
+    (the strip's vertices on x = blocker_x[1])."""
+    xe, te = grid_patch([0, 0, 0], [1, 0, 0], [0, 1, 0], n, n)
+    x0, x1 = blocker_x
+    y0, y1 = blocker_y
+    # normal = u x v = (0, y, 0) x (x, 0, 0) = -z
+    xs_, ts_ = grid_patch([x0, y0, blocker_z], [0, y1 - y0, 0], [x1 - x0, 0, 0],
+                          blocker_cells[1], blocker_cells[0])
+    parts = [(xe, te), (xs_, ts_)]
+    if receiver:
+        parts.append(grid_patch([0, 0, gap], [0, 1, 0], [1, 0, 0], n, n))
+    xyz, idx, off = [], [], 0
+    for x, t in parts:
+        xyz.append(x)
+        idx.append(t + off)
+        off += x.shape[0]
+    xyz = np.concatenate(xyz)
+    sd = _uniform_scene("blocker_plates", xyz, np.concatenate(idx), rho_d=1.0 - emissivity)
+    nve, nvb = xe.shape[0], xs_.shape[0]
+    edge = nve + np.nonzero(np.isclose(xs_[:, 0], x1))[0]
+    sd.meta.update(n=n, emitter_verts=(0, nve), blocker_verts=(nve, nve + nvb),
+                   emitter_tris=te.shape[0], edge_vertex=edge.astype(np.uint32),
+                   blocker_x=blocker_x, receiver=receiver)
+    return sd
+
+
+def fin_plates(n=16, fin_x=0.5, fin_z=(0.05, 0.3), fin_y=(-0.5, 1.5), fin_cells=(4, 20),
+               emitter_x=(0.0, 0.5), gap=1.0, emissivity=1.0):
+    """Emitter plate emitter_x x [0,1] at z=0 facing +z, a vertical opaque fin
+    in the plane x = fin_x spanning fin_y x fin_z facing -x (towards the
+    emitter), and a receiver plate [0,1]^2 at z=gap facing -z.  Rays reach
+    the fin's top edge at grazing incidence: the case where the edge term's
+    projection factor matters.  meta: ``emitter_verts``, ``fin_verts``,
+    ``edge_vertex`` (the fin's vertices on z = fin_z[1])."""
+    e0, e1 = emitter_x
+    ne = max(1, int(round(n * (e1 - e0))))
+    xe, te = grid_patch([e0, 0, 0], [e1 - e0, 0, 0], [0, 1, 0], ne, n)
+    z0, z1 = fin_z
+    y0, y1 = fin_y
+    # normal = u x v = (0, 0, z) x (0, y, 0) = -x
+    xf, tf = grid_patch([fin_x, y0, z0], [0, 0, z1 - z0], [0, y1 - y0, 0], fin_cells[0],
+                        fin_cells[1])
+    xr, tr = grid_patch([0, 0, gap], [0, 1, 0], [1, 0, 0], n, n)
+    xyz = np.concatenate([xe, xf, xr])
+    idx = np.concatenate([te, tf + xe.shape[0], tr + xe.shape[0] + xf.shape[0]])
+    sd = _uniform_scene("fin_plates", xyz, idx, rho_d=1.0 - emissivity)
+    nve, nvf = xe.shape[0], xf.shape[0]
+    edge = nve + np.nonzero(np.isclose(xf[:, 2], z1))[0]
+    sd.meta.update(n=n, emitter_verts=(0, nve), fin_verts=(nve, nve + nvf),
+                   edge_vertex=edge.astype(np.uint32), fin_z=fin_z)
     return sd
 
 

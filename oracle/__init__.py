@@ -358,6 +358,8 @@ ADJ_CONTINUOUS = 2
 ADJ_EDGE = 4
 ADJ_REFERENCE_DROPPED = 8
 ADJ_EDGE_GAIN = 16
+ADJ_HIT_AREA = 32
+ADJ_EDGE_PROJECTED = 64
 
 
 class AdjointParams(ctypes.Structure):

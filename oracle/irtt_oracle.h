@@ -131,6 +131,8 @@ int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
 #define OR_ADJ_EDGE 4u
 #define OR_ADJ_REFERENCE_DROPPED 8u /* dropped photons keep absorber bin 0, as the shader does */
 #define OR_ADJ_EDGE_GAIN 16u        /* extension: occluding side of the edge term */
+#define OR_ADJ_HIT_AREA 32u         /* extension: area Jacobian of every hit triangle */
+#define OR_ADJ_EDGE_PROJECTED 64u   /* extension: edge displacement projected along the ray */
 
 typedef struct {
     float edge_ratio;   /* ubo.gradEdgeRatio, 0.05 */

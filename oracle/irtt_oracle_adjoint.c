@@ -269,6 +269,22 @@ int or_adjoint(const or_scene *sc, const or_params *p, const or_adjoint_params *
                             }
                         }
                     }
+                    /* extension (OR_ADJ_HIT_AREA): the hit point is a fixed
+                     * barycentric point of the hit triangle, whose area
+                     * element scales the path density there, so the path's
+                     * weight gains w * dA_j/dv / A_j on the hit triangle's
+                     * corners (the reference keeps this factor for the
+                     * emitting triangle only, :327-351,588-592) */
+                    if (ap->flags & OR_ADJ_HIT_AREA) {
+                        float ga[3], gb[3], gc[3];
+                        float pa[3] = {q0.x, q0.y, q0.z}, pb[3] = {q1.x, q1.y, q1.z},
+                              pc[3] = {q2.x, q2.y, q2.z};
+                        or_area_gradient(pa, pb, pc, ga, gb, gc);
+                        const float s = wt / area[h.prim];
+                        const f3 dh[3] = {scl(ld3(ga), s), scl(ld3(gb), s), scl(ld3(gc), s)};
+                        for (int cc = 0; cc < 3; cc++)
+                            if (finite3(dh[cc])) acc3(grad, hi[cc], dh[cc], fpath);
+                    }
                     /* (iii) edge term, :454-567 */
                     int opp = argmin3(hb0, h.b1, h.b2);
                     float hb[3] = {hb0, h.b1, h.b2};
@@ -292,7 +308,14 @@ int or_adjoint(const or_scene *sc, const or_params *p, const or_adjoint_params *
                         float larea = ratio * th * (1.0f - 0.5f * ratio) * elen;
                         f3 da = nrm(cross(edir, d));
                         if (dot(da, n_edg) < 0.0f) da = scl(da, -1.0f);
-                        f3 dE = scl(da, fabsf(dot(da, n_edg)) * elen * wt / larea);
+                        /* the reference scales by |da.n_edg| (:486); moving the
+                         * edge by u sweeps (u.da)/|da.n_edg| of the occluder's
+                         * plane per unit length, seen along the ray, so
+                         * OR_ADJ_EDGE_PROJECTED divides by it instead */
+                        const float cpl = fabsf(dot(da, n_edg));
+                        f3 dE = (ap->flags & OR_ADJ_EDGE_PROJECTED)
+                                    ? scl(da, elen * wt / (larea * cpl))
+                                    : scl(da, cpl * elen * wt / larea);
                         if (!finite3(dE)) dE = mk(0, 0, 0);
                         f3 to = bary(q0, q1, q2, hb0, h.b1, h.b2);
                         to = add(to, scl(n_edg, eps + dot(sub(qv[near_], to), n_edg)));

@@ -102,6 +102,35 @@ def test_adjoint_trace_matches_oracle(irtt, oracle, which, flags):
     assert err < 1e-9
 
 
+@pytest.mark.parametrize("which", ["blocker", "blocker_open", "fin"])
+@pytest.mark.parametrize("flags", [7, 23, 39, 71, 103])
+def test_adjoint_trace_edge_scenes_match_oracle(irtt, oracle, which, flags):
+    """The scenes that pin the edge term against finite differences
+    (tests/test_oracle_edge_term.py): an opaque strip between two plates,
+    the same without the receiver (occlusion rays escape), a fin reached at
+    grazing incidence.  Every term and extension (16 edge gain, 32 hit-area
+    Jacobian, 64 projected edge factor), GPU = oracle on the same photons."""
+    from irtt import scenes
+    sd = {"blocker": lambda: scenes.blocker_plates(),
+          "blocker_open": lambda: scenes.blocker_plates(receiver=False),
+          "fin": lambda: scenes.fin_plates()}[which]()
+    rng = np.random.default_rng(5)
+    lam = 1.0 + rng.random(sd.nv)
+    x4 = rng.random(sd.nv)
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=64, rays_per_thread=8, batch_seed=5)
+        g, st = s.adjoint_trace(lam, x4, flags=flags)
+    og, ost = oracle.adjoint(oracle.OracleScene.from_irtt(sd),
+                             oracle.make_params(64, 8, batch_seed=5, bin_mode=0), lam, x4,
+                             flags=flags)
+    for k in ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded", "rays"):
+        assert st[k] == ost[k], (k, st[k], ost[k])
+    assert ost["occluded"] > 0 or which == "blocker_open"
+    err = np.abs(g - og).max() / np.abs(og).max()
+    print(f"{which} flags {flags}: rel err {err:.2e}; {st}")
+    assert err < 1e-9
+
+
 def _mean_b(irtt, sd, dz, nva, seed, photons):
     s2 = type(sd)(**{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in sd.__dict__.items()})
     s2.xyz[nva:, 2] += dz
