@@ -166,7 +166,7 @@ def _declare(L):
     L.or_transport.restype = ctypes.c_int
     L.or_transport.argtypes = [ctypes.POINTER(Scene), ctypes.POINTER(Params),
                                ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, _u64p, _u32p,
-                               _u32p, ctypes.c_uint64, _u64p, ctypes.c_int,
+                               _u32p, _u32p, ctypes.c_uint64, _u64p, ctypes.c_int,
                                ctypes.POINTER(Stats)]
 
 
@@ -329,27 +329,36 @@ def photon_weights(sc, params):
 
 
 def transport(sc, params, tri_begin=0, tri_end=None, dense=True, records=0,
-              n_threads=0, tri_stride=1):
+              n_threads=0, tri_stride=1, with_tri=False):
     """Runs the oracle photon tracer.  Returns (acc or None, (abs, src) records
-    or None, stats dict)."""
+    -- plus each record's source triangle with ``with_tri`` -- or None, stats
+    dict)."""
     if tri_end is None:
         tri_end = sc.nt
     nb = sc.nt if params.bin_mode else sc.nv
     acc = np.zeros((nb, nb), np.uint64) if dense else None
-    ra = rs = None
+    ra = rs = rt = None
+    with_tri = bool(with_tri and records)
     if records:
         ra = np.empty(records, np.uint32)
         rs = np.empty(records, np.uint32)
+        if with_tri:
+            rt = np.empty(records, np.uint32)
     cnt = ctypes.c_uint64()
     st = Stats()
     rc = lib().or_transport(ctypes.byref(sc.c), ctypes.byref(params), tri_begin,
                             tri_end, tri_stride, _p(acc, _u64p) if dense else None,
                             _p(ra, _u32p) if records else None,
-                            _p(rs, _u32p) if records else None, records,
+                            _p(rs, _u32p) if records else None,
+                            _p(rt, _u32p) if with_tri else None, records,
                             ctypes.byref(cnt), n_threads, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("or_transport failed")
-    recs = (ra[: cnt.value], rs[: cnt.value]) if records else None
+    recs = None
+    if records:
+        recs = (ra[: cnt.value], rs[: cnt.value])
+        if with_tri:
+            recs += (rt[: cnt.value],)
     return acc, recs, st.as_dict()
 
 

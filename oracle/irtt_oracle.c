@@ -623,7 +623,7 @@ double or_photon_weights(const or_scene *sc, const float *area,
 
 int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
                  uint32_t tri_end, uint32_t tri_stride, uint64_t *acc, uint32_t *rec_abs,
-                 uint32_t *rec_src, uint64_t rec_cap, uint64_t *rec_count,
+                 uint32_t *rec_src, uint32_t *rec_tri, uint64_t rec_cap, uint64_t *rec_count,
                  int n_threads, or_stats *st_out)
 {
     if (tri_end > sc->nt) tri_end = sc->nt;
@@ -731,7 +731,11 @@ int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
 #pragma omp atomic capture
 #endif
                     slot = nrec++;
-                    if (slot < rec_cap) { rec_abs[slot] = ab; rec_src[slot] = src; }
+                    if (slot < rec_cap) {
+                        rec_abs[slot] = ab;
+                        rec_src[slot] = src;
+                        if (rec_tri) rec_tri[slot] = t;
+                    }
                 }
                 break;
             }

@@ -118,11 +118,13 @@ double or_photon_weights(const or_scene *sc, const float *area,
  * ... below tri_end (tri_stride 0 means 1).
  * acc (may be NULL): dense nb*nb uint64 row-major, acc[abs*nb + src] += q_t.
  * rec_abs/rec_src (may be NULL): one record per absorbed photon, up to rec_cap
- * (records beyond the cap are dropped from the list, not from acc).
+ * (records beyond the cap are dropped from the list, not from acc), in no
+ * particular order; rec_tri (may be NULL) receives each record's source
+ * triangle, so that sum of q[rec_tri] per (rec_abs, rec_src) is acc.
  * n_threads: OpenMP threads (0 = default). */
 int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
                  uint32_t tri_end, uint32_t tri_stride, uint64_t *acc, uint32_t *rec_abs,
-                 uint32_t *rec_src, uint64_t rec_cap, uint64_t *rec_count,
+                 uint32_t *rec_src, uint32_t *rec_tri, uint64_t rec_cap, uint64_t *rec_count,
                  int n_threads, or_stats *st);
 
 /* ---- adjoint photon trace (thermal_tracing_unified.glsl:292-599), irtt_oracle_adjoint.c ---- */

@@ -134,3 +134,22 @@ def test_far_field_blocks_do_not_compress(oracle):
         assert 8 * nnz < lowrank_bytes
         ranks.append(rank)
     assert ranks[1] > ranks[0]
+
+
+def test_records_aggregate_to_accumulators(oracle):
+    """The per-photon records (absorbing bin, source bin, source triangle)
+    summed with the photon quanta give the dense accumulators exactly: the
+    form the full-size parity test aggregates the headline workload in."""
+    from irtt import scenes
+    sd = scenes.city_block(seed=2, nx=2, ny=2, target_tris=1500)
+    sc = oracle.OracleScene.from_irtt(sd)
+    p = oracle.make_params(16, 4, batch_seed=3, bin_mode=0)
+    acc, _, st = oracle.transport(sc, p)
+    _, (ra, rs, rt), st2 = oracle.transport(sc, p, dense=False, records=st["photons"],
+                                            with_tri=True)
+    _, q, _ = oracle.photon_weights(sc, p)
+    assert ra.size == st["absorbed"] + st["forced_absorb"]
+    agg = np.zeros_like(acc)
+    np.add.at(agg, (ra, rs), q[rt])
+    assert np.array_equal(agg, acc)
+    assert st == st2
