@@ -28,18 +28,51 @@ NODE_BYTES = 64      # two child boxes + two child refs
 TRI_BYTES = 40       # 3 x float3 vertices + id
 RAY_BYTES = 48       # 32 B ray in + 16 B hit out
 BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
-# DRAM bytes per k_photons launch on this workload, from rocprofv3 FETCH_SIZE
-# (x 2.0) + WRITE_SIZE (x 1.0) scaled by a 1 GiB copy calibration; the
-# commands and raw values are in profiles/k_photons_city_block_2p26.md
-TRAFFIC_BYTES_CITY = (1077598 * 2.0 + 1161716 * 1.0) * 1024
+# Per k_photons launch on the default workload (city block seed 0, 162 x 8
+# photons per triangle, dense layout), from rocprofv3 counters; commands and
+# raw values in profiles/k_photons_city_block_2p26.md.
+# DRAM bytes: FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a
+# 1 GiB device copy (bench.py --calibrate-copy 1024).
+TRAFFIC_BYTES_CITY = (1091398 * 2.0 + 1161716 * 1.0) * 1024
+# VALU wave-instructions issued (SQ_INSTS_VALU) and lane utilisation
+# (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)); the work is the same
+# every launch (fixed seeds), so the count is a property of the workload.
+VALU_INSTS_CITY = 5921752115
+VALU_LANE_UTIL_CITY = 164314725944 / (64.0 * 5985472421)
+# VALU issue peak: one wave64 instruction per cycle per CU (four SIMD16 each
+# taking four cycles per instruction), 256 CUs at 2.4 GHz
+VALU_PEAK_G = 256 * 2.4
 
 
-def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):
+def _host_cpu():
+    """(nproc, cores this process may use, CPU model string)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return os.cpu_count() or 1, usable, model
+
+
+def cpu_baseline(sd, threads_per_tri, rays_per_thread, check=None, target_s=12.0):
     """Times the oracle tracer (all photons of a strided subset of source
-    triangles) on the host cores; returns a dict for the JSON line."""
+    triangles) on the host cores, and, with `check` (the stored operator and
+    the GPU solutions), an FP64 CPU solve of the same operator: the
+    reference's CPU solve, and the independent FP64 answer each GPU form's
+    max |dT| is measured against.  Returns a dict for the JSON line."""
     import numpy as np
     import oracle
-    cores = min(16, os.cpu_count() or 1)
+    nproc, usable, model = _host_cpu()
+    # the GPU box allots 16 cores to a one-GPU job, whatever nproc shows
+    cores = max(1, min(16, usable))
     sc = oracle.OracleScene.from_irtt(sd)
     p = oracle.make_params(threads_per_tri, rays_per_thread, batch_seed=1234, bin_mode=0)
     stride = 512
@@ -54,11 +87,68 @@ def cpu_baseline(sd, threads_per_tri, rays_per_thread, target_s=12.0):
             break
         stride = max(1, int(stride * dt / target_s))
     rays, photons = st["rays"], st["photons"]
-    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{photons} photons ({rays} ray segments): all "
-                      f"{threads_per_tri * rays_per_thread} photons of every {stride}-th source "
-                      f"triangle of the same city block, CPU oracle (C + OpenMP, LBVH build "
-                      f"included), {dt:.1f} s"}
+    out = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+           "nproc": nproc, "usable_cores": usable, "cpu_model": model,
+           "sample": f"{photons} photons ({rays} ray segments): all "
+                     f"{threads_per_tri * rays_per_thread} photons of every {stride}-th source "
+                     f"triangle of the same city block, CPU oracle (C + OpenMP, {cores} "
+                     f"threads, LBVH build included), {dt:.1f} s"}
+    if check is not None:
+        out["fp64"] = fp64_check(sd, check)
+    return out
+
+
+def fp64_check(sd, c):
+    """max |dT| of every GPU solve form against an FP64 solve of the same
+    stored operator on the CPU (scipy, Jacobi-preconditioned GMRES to a
+    relative residual of 1e-14; oracle/sparse_fp64.py)."""
+    import numpy as np
+    from irtt import scenes
+    from oracle import physics, sparse_fp64
+    rp, col, val, colsum = c["operator"]
+    T = sparse_fp64.to_scipy(rp, col, val, len(colsum))
+    out = {}
+    t0 = time.perf_counter()
+    x64, res = sparse_fp64.steady_linear(T, sparse_fp64.rhs(physics.bins(sd, 0)[3], colsum, 293.0))
+    out["steady_solve_s"] = round(time.perf_counter() - t0, 2)
+    out["steady_residual"] = float(res)
+    for k in ("fp32", "fp16", "sparse"):
+        if k in c:
+            out[f"max_dT_{k}_K"] = float(np.abs(c[k] - x64).max())
+    if "batch" in c:
+        fluxes, X = c["batch"]
+        worst = 0.0
+        for k in (0, 1, len(fluxes) // 2, len(fluxes) - 1):
+            s2 = _with_flux(sd, fluxes[k])
+            xk, _ = sparse_fp64.steady_linear(T, sparse_fp64.rhs(physics.bins(s2, 0)[3], colsum,
+                                                                 293.0))
+            worst = max(worst, float(np.abs(X[k] - xk).max()))
+        out["max_dT_fp16_batch_K"] = worst
+    if "transient" in c:
+        fluxes, H, dt = c["transient"]
+        _, _, mass, _ = physics.bins(sd, 0)
+        x = np.full(len(colsum), 293.0)
+        d = []
+        t0 = time.perf_counter()
+        for k in range(H.shape[0]):
+            src = physics.bins(_with_flux(sd, fluxes[k]), 0)[3]
+            x, _ = sparse_fp64.implicit_euler_step(T, sparse_fp64.rhs(src, colsum, 293.0), mass,
+                                                   x, dt)
+            d.append(float(np.abs(H[k] - x).max()))
+        out["transient_steps"] = H.shape[0]
+        out["transient_fp64_s_per_step"] = round((time.perf_counter() - t0) / H.shape[0], 2)
+        out["max_dT_transient_fp16_per_step_K"] = [float(f"{v:.3e}") for v in d]
+    out["note"] = ("FP64 CPU solve of the GPU's stored operator (every product and sum FP64); "
+                   "bar 0.05 K")
+    return out
+
+
+def _with_flux(sd, flux):
+    import copy
+    import numpy as np
+    s2 = copy.copy(sd)
+    s2.heat_flux = np.asarray(flux, np.float32)
+    return s2
 
 
 def solve_rates(irtt, s, sd, reps=3, batch=32):
@@ -99,10 +189,8 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
     # FP32-operator refinement
     from irtt import scenes
     R = batch
-    srcs = [scenes.bin_sources(sd, sd.heat_flux)]
-    for k in range(1, R):
-        srcs.append(scenes.bin_sources(sd, scenes.sun_flux(sd, 45.0, 360.0 * k / R)))
-    srcs = np.stack(srcs)
+    fluxes = [sd.heat_flux] + [scenes.sun_flux(sd, 45.0, 360.0 * k / R) for k in range(1, R)]
+    srcs = np.stack([scenes.bin_sources(sd, f) for f in fluxes])
     s.select_operator(irtt.OP_DENSE_F16)
     s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
     times = []
@@ -121,6 +209,8 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
                          "linear_iters": fst["linear_iters"] + ast["linear_iters"],
                          "residual_sq_max": fst["residual_sq"]}
     out["x_fp32"] = xs["fp32"]
+    out["x_fp16"] = xs["fp16"]
+    out["batch_solutions"] = (fluxes, X)
     out["max_dT_fp16_vs_fp32_K"] = float(np.abs(xs["fp16"] - xs["fp32"]).max())
     out["max_dT_fp16_batch_vs_fp32_K"] = float(np.abs(X[0] - xs["fp32"]).max())
     out["speedup_fp16"] = round(out["fp16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
@@ -172,6 +262,7 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
             lam, ast = s.adjoint_solve(ea * (x - x.mean()), params=sp)
             times.append(time.perf_counter() - t0)
         t = min(times)
+        out["x"] = x
         out["single"] = {"solves_per_s": round(1.0 / t, 2), "ms": round(t * 1e3, 2),
                          "linear_iters": fst["linear_iters"] + ast["linear_iters"],
                          "max_dT_vs_dense_fp32_K": float(np.abs(x - x_ref).max())}
@@ -272,7 +363,7 @@ def facade(irtt, fd_params=8, h=1e-3, gd_steps=10, threads_per_tri=336, rays_per
     return out
 
 
-def transient(irtt, s, sd, steps=288, dt=300.0):
+def transient(irtt, s, sd, steps=288, dt=300.0, check_steps=6):
     """The city block over 24 h: `steps` implicit-Euler steps of dt with the
     sun sweeping east to west (elevation 60 sin(pi (t - 6 h) / 12 h), no sun
     at night), per-patch areal heat capacity U[1e4, 1e5] J/(m^2 K).  FP16 form
@@ -280,12 +371,13 @@ def transient(irtt, s, sd, steps=288, dt=300.0):
     against the FP32 form; max |dT| between them per step."""
     import numpy as np
     from irtt import scenes
-    srcs = []
+    srcs, fluxes = [], []
     for k in range(steps):
         th = (k + 1) * dt / 3600.0
         el = 60.0 * np.sin(np.pi * (th - 6.0) / 12.0)
         az = 90.0 + 15.0 * (th - 6.0)
         flux = scenes.sun_flux(sd, el, az) if el > 0 else np.zeros(sd.nt, np.float32)
+        fluxes.append(flux)
         srcs.append(scenes.bin_sources(sd, flux))
     p = irtt.solver_params(steady=False, dt=dt, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-14,
                            lin_rtol=1e-8)
@@ -305,6 +397,7 @@ def transient(irtt, s, sd, steps=288, dt=300.0):
         hist[name] = H
         out[name] = {"ms_per_step": round(t * 1e3 / steps, 2), "linear_iters": iters}
     s.select_operator(irtt.OP_DENSE)
+    out["first_steps"] = (fluxes[:check_steps], hist["fp16"][:check_steps], dt)
     d = np.abs(hist["fp16"] - hist["fp32"]).max(axis=1)
     out.update({"steps": steps, "dt_s": dt,
                 "max_dT_fp16_vs_fp32_K": float(d.max()),
@@ -365,6 +458,27 @@ def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=200000, seed=7):
                     "building heights"}
 
 
+def calibrate_copy(mib, reps=3):
+    """Copies mib MiB device to device `reps` times (hipMemcpy's copy kernel):
+    rocprofv3's FETCH_SIZE / WRITE_SIZE per dispatch of this copy, divided by
+    mib MiB, give the factors applied to the kernels' counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, ctypes.c_size_t(n))
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3) != 0:  # device to device
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"copied {mib} MiB x {reps}", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,7 +499,13 @@ def main():
                     help="run only the facade section and print it")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device three times (a plain copy kernel "
+                         "of known size, for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by source triangles across GPUs by design only)",
@@ -410,6 +530,8 @@ def main():
     for _ in range(args.steps):
         stats.append(s.build_transport(p))
     t1 = time.perf_counter()
+    # the stored operator, for the FP64 check of the solves below
+    check = {"operator": s.transport_sparse() + (s.column_sums(),)}
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     st = stats[-1]
     rays = st["rays"]
@@ -420,17 +542,47 @@ def main():
     b_launch = rays * (RAY_BYTES + n_node * NODE_BYTES + n_tri * TRI_BYTES) + \
         (st["absorbed"] + st["forced_absorb"]) * BIN_BYTES
     achieved = b_launch / (trace_ms * 1e-3) / 1e9
+    default = args.seed == 0 and args.threads_per_tri == 162 and args.rays_per_thread == 8
+    # the photon kernel is bound by VALU issue (lanes of a wave diverge
+    # between node visits, leaf tests and shading): achieved = its VALU
+    # wave-instructions per second against the issue peak; the HBM figures
+    # beside it show that DRAM is not the bound (the 5.8 MB tree is served
+    # from L2 / MALL)
+    valu = VALU_INSTS_CITY / (trace_ms * 1e-3) / 1e9 if default else None
+    traffic = TRAFFIC_BYTES_CITY if default else None
+    roofline = {
+        "bound": "valu", "achieved": round(valu, 1) if valu else None, "peak": VALU_PEAK_G,
+        "unit": "G VALU wave-instructions/s", "frac": round(valu / VALU_PEAK_G, 4) if valu else None,
+        "traffic": round(traffic / 1e9, 3) if traffic else None,
+        "traffic_unit": "GB DRAM per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
+        "kernel": "k_photons", "kernel_ms": round(trace_ms, 3),
+        "valu_lane_utilisation": round(VALU_LANE_UTIL_CITY, 4) if default else None,
+        "valu_insts_per_launch": VALU_INSTS_CITY if default else None,
+        "hbm": {"dram_GBs": round(traffic / (trace_ms * 1e-3) / 1e9, 1) if traffic else None,
+                "dram_frac": round(traffic / (trace_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+                if traffic else None,
+                "algorithmic_GBs": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "algorithmic_bytes_per_ray": round(b_launch / rays, 1),
+                "node_visits_per_ray": round(n_node, 2), "tri_tests_per_ray": round(n_tri, 2),
+                "note": "algorithmic bytes (48 + 64 n_node + 40 n_tri per segment) exceed what "
+                        "DRAM carries: node fetches are L2 / MALL hits"}}
     # steady solve on the built operator (reported beside the metric)
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-10)
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
     solves = solve_rates(irtt, s, sd)
     x_fp32 = solves.pop("x_fp32")
+    check["fp32"] = x_fp32
+    check["fp16"] = solves.pop("x_fp16")
+    check["batch"] = solves.pop("batch_solutions")
     trans = None if args.no_transient else transient(irtt, s, sd)
+    if trans is not None:
+        check["transient"] = trans.pop("first_steps")
     sparse = None
     if not args.no_sparse:
         s.close()  # the dense operator (16 GB with its accumulators) is no longer needed
         sparse = sparse_layout(irtt, sd, p, x_fp32)
+        check["sparse"] = sparse.pop("x")
     line = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -450,16 +602,7 @@ def main():
                                   args.threads_per_tri * args.rays_per_thread),
                    "bins": s.n_bins, "triangles": sd.nt, "photons": st["photons"],
                    "ray_segments": rays},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (round(TRAFFIC_BYTES_CITY / 1e9, 3)
-                                 if (args.seed == 0 and args.threads_per_tri == 162
-                                     and args.rays_per_thread == 8) else None),
-                     "traffic_unit": "GB per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE)",
-                     "kernel": "k_photons", "kernel_ms": round(trace_ms, 3),
-                     "bytes_per_ray": round(b_launch / rays, 1),
-                     "node_visits_per_ray": round(n_node, 2),
-                     "tri_tests_per_ray": round(n_tri, 2)},
+        "roofline": roofline,
         "breakdown_ms": {"photon_kernel": round(trace_ms, 3),
                          "finalize": round(sum(x["finalize_ms"] for x in stats) / len(stats), 3),
                          "bvh_build": round(st["bvh_ms"], 3)},
@@ -476,7 +619,11 @@ def main():
     if not args.no_serving:
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread)
+        line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread,
+                                            check=check)
+        fp = line["cpu_baseline"]["fp64"]
+        line["solves"]["max_dT_vs_fp64_K"] = {k[7:-2]: v for k, v in fp.items()
+                                             if k.startswith("max_dT_") and k.endswith("_K")}
     s.close()
     print(json.dumps(line), flush=True)
 
