@@ -337,7 +337,11 @@ void sp_download(Scene &s, uint32_t *rowptr, uint32_t *col, float *val);
 void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st);
 void aca_apply(Scene &s, const double *w, double *y, bool transposed);
 void aca_get(Scene &s, double *U, double *V, double *cdiag, int32_t *prow, int32_t *pcol);
+// trace.hip: a forward pass that only records each photon's absorbing bin
+// (absorber: one word per photon, ~0 where none); returns the ray segments
+void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays);
 // adjoint.hip
+uint64_t edge_record_capacity(uint64_t rays);
 void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
                    const double *x, double *grad, irtt_adjoint_stats &st);
 

@@ -152,7 +152,7 @@ __device__ __forceinline__ int shade_segment(const PhotonArgs &a, const Hit &h, 
     uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
     if (a.sink.tab)
         tab_add(a.sink.tab, a.sink.lg, tab_key(ab, src, a.sink.key_bits), qt, a.sink.overflow);
-    else
+    else if (a.sink.acc)
         atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
     if (a.absorber) a.absorber[g] = ab;
     return absorbed ? S_ABS : S_FORCED;
@@ -320,6 +320,84 @@ static uint32_t ceil_log2_u64(uint64_t x)
     return l;
 }
 
+// Launches the photon kernel over every photon of the scene on the scene's
+// stream (timed: between events ev0 and ev1): bins into sink (none: absorbers
+// only), records absorbing bins into absorber (or not), counts into stats.
+static void launch_photons(Scene &s, const irtt_transport_params &p, uint32_t per,
+                           uint32_t max_depth, uint64_t total, const PhotonSink &sink,
+                           uint32_t *absorber, unsigned long long *stats, bool timed)
+{
+    hipStream_t stm = s.stream;
+    PhotonArgs a{};
+    a.xyz = s.d_xyz.p;
+    a.idx = s.d_idx.p;
+    a.rho_d = s.d_rho_d.p;
+    a.rho_s = s.d_rho_s.p;
+    a.flags = s.d_flags.p;
+    a.q = reinterpret_cast<const unsigned long long *>(s.d_q.p);
+    a.nodes = s.bvh.nodes.p;
+    a.tris = s.bvh.tris.p;
+    a.n_leaves = s.bvh.n_leaves;
+    a.nt = s.nt;
+    a.nb = s.nb;
+    a.ld = s.ld;
+    a.bin_mode = s.bin_mode;
+    a.threads_per_tri = p.threads_per_tri;
+    a.rays_per_thread = p.rays_per_thread;
+    a.per_tri = per;
+    a.batch_seed = p.batch_seed;
+    a.max_depth = max_depth;
+    a.tiny_dist = p.tiny_dist;
+    a.photon_begin = 0;
+    a.photon_end = total;
+    a.sink = sink;
+    a.absorber = absorber;
+    a.stats = stats;
+
+    int dev = 0, ncu = 256;
+    IRTT_HIP(hipGetDevice(&dev));
+    IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    // The photon kernel's occupancy is bound by its LDS stack: 24 entries
+    // (6 KiB per wave) admit 24 waves per CU against 20 for 32 entries, for
+    // trees up to 23 levels (the 51.8k-triangle city block has 21).  Measured
+    // on that block: 17.2 against 18.9 ms.  (A 16-entry LDS stack spilling
+    // deeper entries to global memory, with 64 VGPRs for 32 waves, measured
+    // 17.4 ms: the register spills cost what the occupancy gains.)
+    const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
+    // enough waves to cover every CU several times; lanes loop over photons
+    uint64_t want = cdiv(total, kTraceBlock);
+    uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU);
+    if (grid == 0) grid = 1;
+    if (timed) IRTT_HIP(hipEventRecord(s.ev0, stm));
+    if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
+    else if (S == 64) k_photons<64><<<grid, kTraceBlock, 0, stm>>>(a);
+    else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
+    IRTT_HIP(hipGetLastError());
+    if (timed) IRTT_HIP(hipEventRecord(s.ev1, stm));
+}
+
+// A forward pass that records each photon's absorbing bin and bins nothing:
+// the adjoint trace's pass 0 when the build did not keep its record.
+void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays)
+{
+    const irtt_transport_params &p = s.tparams;
+    const uint32_t per = p.threads_per_tri * p.rays_per_thread;
+    const uint32_t max_depth =
+        (p.max_depth == 0 || p.max_depth > kMaxDepthDefault) ? kMaxDepthDefault : p.max_depth;
+    const uint64_t total = (uint64_t)per * s.nt;
+    hipStream_t stm = s.stream;
+    DBuf<unsigned long long> d_st;
+    d_st.alloc(16);
+    IRTT_HIP(hipMemsetAsync(d_st.p, 0, d_st.bytes(), stm));
+    IRTT_HIP(hipMemsetAsync(absorber, 0xff, total * sizeof(uint32_t), stm));
+    launch_photons(s, p, per, max_depth, total, PhotonSink{}, absorber, d_st.p, false);
+    unsigned long long hs[16];
+    IRTT_HIP(hipMemcpyAsync(hs, d_st.p, sizeof(hs), hipMemcpyDeviceToHost, stm));
+    IRTT_HIP(hipStreamSynchronize(stm));
+    rays = hs[S_RAYS];
+}
+
 void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_stats &st)
 {
     if (p.threads_per_tri == 0 || p.rays_per_thread == 0)
@@ -418,53 +496,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         }
     }
 
-    PhotonArgs a{};
-    a.xyz = s.d_xyz.p;
-    a.idx = s.d_idx.p;
-    a.rho_d = s.d_rho_d.p;
-    a.rho_s = s.d_rho_s.p;
-    a.flags = s.d_flags.p;
-    a.q = reinterpret_cast<const unsigned long long *>(s.d_q.p);
-    a.nodes = s.bvh.nodes.p;
-    a.tris = s.bvh.tris.p;
-    a.n_leaves = s.bvh.n_leaves;
-    a.nt = s.nt;
-    a.nb = nb;
-    a.ld = ld;
-    a.bin_mode = s.bin_mode;
-    a.threads_per_tri = p.threads_per_tri;
-    a.rays_per_thread = p.rays_per_thread;
-    a.per_tri = per;
-    a.batch_seed = p.batch_seed;
-    a.max_depth = max_depth;
-    a.tiny_dist = p.tiny_dist;
-    a.photon_begin = 0;
-    a.photon_end = total;
-    a.sink = sink;
-    a.absorber = s.d_absorber.p;
-    a.stats = s.d_stats.p;
-
-    int dev = 0, ncu = 256;
-    IRTT_HIP(hipGetDevice(&dev));
-    IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    // The photon kernel's occupancy is bound by its LDS stack: 24 entries
-    // (6 KiB per wave) admit 24 waves per CU against 20 for 32 entries, for
-    // trees up to 23 levels (the 51.8k-triangle city block has 21).  Measured
-    // on that block: 17.2 against 18.9 ms.  (A 16-entry LDS stack spilling
-    // deeper entries to global memory, with 64 VGPRs for 32 waves, measured
-    // 17.4 ms: the register spills cost what the occupancy gains.)
-    const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
-    // enough waves to cover every CU several times; lanes loop over photons
-    uint64_t want = cdiv(total, kTraceBlock);
-    uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU);
-    if (grid == 0) grid = 1;
-    IRTT_HIP(hipEventRecord(s.ev0, stm));
-    if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
-    else if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
-    else if (S == 64) k_photons<64><<<grid, kTraceBlock, 0, stm>>>(a);
-    else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
-    IRTT_HIP(hipGetLastError());
-    IRTT_HIP(hipEventRecord(s.ev1, stm));
+    launch_photons(s, p, per, max_depth, total, sink, s.d_absorber.p, s.d_stats.p, true);
 
     // finalize timed from ev1 to ev2; both times are read after the last sync
     if (sparse) {
