@@ -38,6 +38,8 @@
 #include "irtt_internal.hpp"
 #include "traverse.hpp"
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 namespace irtt {
 namespace {
 
@@ -138,11 +140,11 @@ __device__ __forceinline__ float geom_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3
 // 2^63 q (all but rare outliers) take one 64-bit atomic add: lo wraps modulo
 // 2^64, and the carry out of lo (seen in the value the atomic returns) goes to
 // hi with the sign extension of v.
-__device__ __forceinline__ void fx_add(unsigned long long *pair, double v, double inv_q)
+// v in units of q, rounded once, as the 128-bit pair (lo, hi)
+__device__ __forceinline__ void fx_quant(double v, double inv_q, unsigned long long &lo,
+                                         long long &hi)
 {
     const double y = rint(v * inv_q);
-    unsigned long long lo;
-    long long hi;
     if (fabs(y) < 9.223372036854775808e18) {
         const long long s = (long long)y;
         lo = (unsigned long long)s;
@@ -155,12 +157,37 @@ __device__ __forceinline__ void fx_add(unsigned long long *pair, double v, doubl
         lo = (unsigned long long)ld;
         hi = (long long)hd;
     } else {
-        return;
+        lo = 0;
+        hi = 0;
     }
+}
+
+// adds the 128-bit (lo, hi) to the pair at p
+__device__ __forceinline__ void fx_add_q(unsigned long long *pair, unsigned long long lo,
+                                         long long hi)
+{
     if (lo == 0 && hi == 0) return;
     const unsigned long long old = atomicAdd(pair, lo);
     if (old + lo < old) hi += 1;
     if (hi != 0) atomicAdd(pair + 1, (unsigned long long)hi);
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long *pair, double v, double inv_q)
+{
+    unsigned long long lo;
+    long long hi;
+    fx_quant(v, inv_q, lo, hi);
+    fx_add_q(pair, lo, hi);
+}
+
+// The eight XCDs each sum into their own copy of the gradient (3 n_vertices
+// {lo, hi} pairs apiece), so that atomics to one component from different
+// XCDs do not queue on one address; the host adds the copies.  The copy is
+// the XCD's id (hardware register XCC_ID, bits 3:0).
+constexpr uint32_t kGradCopies = 8;
+__device__ __forceinline__ uint32_t xcc_id()
+{
+    return (uint32_t)__builtin_amdgcn_s_getreg((3 << 11) | 20) & (kGradCopies - 1);
 }
 
 __device__ __forceinline__ void fx_add3(unsigned long long *g, uint32_t v, V3 d, double f,
@@ -189,12 +216,19 @@ struct AdjCommon {
     const double *x4;          // n_hist x nb
     const uint32_t *absorber;  // per photon (~0: none)
     uint64_t photons;
-    unsigned long long *grad;  // 3 n_vertices components, {lo, hi} each
+    unsigned long long *grad;  // kGradCopies x 3 n_vertices components, {lo, hi} each
+    uint64_t grad_stride;      // words per copy (6 n_vertices)
     double inv_q;
     unsigned long long *stats;
 };
 
-enum { A_ABS = 0, A_SEG, A_CONT, A_EDGE, A_OCCL, A_DROPPED_EDGES, A_COUNT };
+enum { A_ABS = 0, A_SEG, A_CONT, A_EDGE, A_OCCL, A_COUNT };
+
+// photon kernel grid of the path replay: waves per CU, lanes striding over
+// the photons (the forward kernel's setting)
+#ifndef IRTT_PHOTON_WAVES_PER_CU_ADJ
+#define IRTT_PHOTON_WAVES_PER_CU_ADJ 128
+#endif
 
 __device__ __forceinline__ double hist_factor(const AdjCommon &a, uint32_t bin_l, uint32_t bin_x)
 {
@@ -214,30 +248,42 @@ struct EdgeRec {
     uint4 r3;
 };
 
-struct PathArgs {
-    AdjCommon c;
-    uint32_t run;          // photons per lane per chunk
-    uint32_t edges_only;   // re-run after an edge-record overflow: records only
-    uint64_t g_start;      // photons below g_start are skipped (re-runs)
-    EdgeRec *edges;
-    uint64_t edge_cap;
-    unsigned long long *edge_count;
-    unsigned long long *first_dropped;  // smallest photon whose edge record did not fit
+// Segment record: one path segment of a replayed photon, its terms still to
+// compute.
+//   r0: start triangle (the emitter on the first segment), hit triangle,
+//       source bin, absorbing bin
+//   r1: barycentric weights of corners 1 and 2 at the start and at the hit
+//   r2: photon index, source triangle, first segment (1) or not (0)
+struct SegRec {
+    uint4 r0;
+    float4 r1;
+    uint4 r2;
 };
 
-// Photons per lane per chunk (a.run, at most kAdjRun): a wave takes chunks of
-// 64 * a.run consecutive photons, lane l the run [l * a.run, (l + 1) * a.run)
-// of each, so a lane's photons nearly always share their source triangle and
-// the first segments' source-vertex terms stay in the lane's registers.
-#ifndef IRTT_ADJ_RUN
-#define IRTT_ADJ_RUN 64
-#endif
-constexpr uint32_t kAdjRun = IRTT_ADJ_RUN;
+// Records are written in chunks of kChunk consecutive slots, each chunk
+// owned by one wave of the replay kernel (one atomic per chunk, not per
+// write); fills[c] is the number of records chunk c holds.
+constexpr uint32_t kChunk = 256;
+constexpr uint32_t kNoChunk = 0xffffffffu;
+
+struct RecBuf {
+    void *p;                       // SegRec or EdgeRec slots
+    uint32_t n_chunks;             // capacity in chunks
+    uint32_t *fills;               // per chunk
+    unsigned long long *next;      // chunks taken (may run past n_chunks: dropped records)
+};
+
+struct PathArgs {
+    AdjCommon c;
+    uint64_t g_begin, g_end;       // photons replayed
+    RecBuf segs, edges;
+    unsigned long long *first_dropped;  // smallest photon with a record that did not fit
+};
 
 // waves per SIMD the LDS stack admits (160 KiB per CU, four SIMDs), at most
-// IRTT_ADJ_WAVES
+// IRTT_ADJ_WAVES (the replay kernel needs 94 VGPRs: 5 waves without spills)
 #ifndef IRTT_ADJ_WAVES
-#define IRTT_ADJ_WAVES 4
+#define IRTT_ADJ_WAVES 5
 #endif
 constexpr int adjoint_waves(int stack)
 {
@@ -246,23 +292,54 @@ constexpr int adjoint_waves(int stack)
                : IRTT_ADJ_WAVES;
 }
 
-// wave-aggregated slot allocation: the active lanes with `want` get
-// consecutive slots (one atomic per wave)
-__device__ __forceinline__ unsigned long long wave_slot(bool want, unsigned long long *counter)
+// The wave's next slots for the lanes with `want`, called by all 64 lanes
+// together: chunk and fill are the wave's current chunk (the same in every
+// lane).  The current chunk is filled to the end before a new one is taken.
+// Returns the slot; ok is false when the buffer is full.
+__device__ __forceinline__ uint64_t chunk_slot(bool want, uint32_t &chunk, uint32_t &fill,
+                                               const RecBuf &b, bool &ok)
 {
     const unsigned long long m = __ballot(want);
-    unsigned long long base = 0;
-    const int leader = m ? __builtin_ctzll(m) : 0;
-    if (want && (int)__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-    base = __shfl(base, leader, 64);
-    const unsigned long long below = m & ((1ull << __lane_id()) - 1ull);
-    return base + (unsigned long long)__popcll(below);
+    const uint32_t k = (uint32_t)__popcll(m);
+    ok = false;
+    if (k == 0) return 0;
+    const uint32_t prefix = (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
+    const uint32_t rem = chunk == kNoChunk ? 0u : kChunk - fill;
+    if (k <= rem) {
+        ok = chunk < b.n_chunks;
+        const uint64_t slot = (uint64_t)chunk * kChunk + fill + prefix;
+        fill += k;
+        return slot;
+    }
+    unsigned long long nc = 0;
+    if (__lane_id() == 0) nc = atomicAdd(b.next, 1ull);
+    nc = __shfl(nc, 0, 64);
+    const uint32_t ncu = nc < (unsigned long long)(kNoChunk - 1) ? (uint32_t)nc : kNoChunk - 1;
+    uint64_t slot;
+    if (prefix < rem) {
+        ok = chunk < b.n_chunks;
+        slot = (uint64_t)chunk * kChunk + fill + prefix;
+    } else {
+        ok = ncu < b.n_chunks;
+        slot = (uint64_t)ncu * kChunk + (prefix - rem);
+    }
+    if (chunk != kNoChunk && chunk < b.n_chunks && __lane_id() == 0) b.fills[chunk] = kChunk;
+    chunk = ncu;
+    fill = k - rem;
+    return slot;
 }
 
-// One loop iteration traces one path segment per lane; a lane whose photon
-// ends starts its next photon in the next iteration (the forward kernel's
-// scheduling).  Photons without an absorbing bin have no path terms (unless
-// IRTT_ADJ_REFERENCE_DROPPED replays them with bin 0, as the shader does).
+__device__ __forceinline__ void chunk_close(uint32_t chunk, uint32_t fill, const RecBuf &b)
+{
+    if (chunk != kNoChunk && chunk < b.n_chunks && __lane_id() == 0) b.fills[chunk] = fill;
+}
+
+// Replays the photon paths (one segment per lane per iteration, lanes of a
+// wave on consecutive photons, a lane whose photon ends takes its next
+// photon) and writes one segment record per accepted segment and one edge
+// record per hit near an edge.  Photons without an absorbing bin are not
+// replayed (unless IRTT_ADJ_REFERENCE_DROPPED replays them with bin 0, as the
+// shader does).  A record that does not fit is dropped and its photon noted.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths(PathArgs pa)
 {
@@ -271,59 +348,25 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
     __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
-    const uint32_t lane = threadIdx.x;
-    const bool f_cont = !pa.edges_only && (a.aflags & IRTT_ADJ_CONTINUOUS);
-    const bool f_hit = !pa.edges_only && (a.aflags & IRTT_ADJ_HIT_AREA);
     const bool f_edge = a.aflags & IRTT_ADJ_EDGE, f_proj = a.aflags & IRTT_ADJ_EDGE_PROJECTED;
     const bool dropped_too = a.aflags & IRTT_ADJ_REFERENCE_DROPPED;
-    uint32_t n_seg = 0, n_cont = 0, n_edge = 0, n_drop = 0, nvis = 0, ntri = 0;
-
-    uint32_t chunk = blockIdx.x;
-    const uint32_t run = pa.run;
-    uint64_t run_g = (uint64_t)chunk * kTraceBlock * run + (uint64_t)lane * run;
-    uint32_t left = run_g < a.photons ? (uint32_t)min<uint64_t>(run, a.photons - run_g) : 0u;
-    // the first segments' source-vertex terms of this lane's photons of triangle acc_t
-    uint32_t acc_t = 0xffffffffu;
-    double sacc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-
-    uint64_t g = 0;
-    uint32_t t = 0, src = 0, abs_bin = 0, seed = 0, depth = 0;
-    uint32_t prev0 = 0, prev1 = 0, prev2 = 0;
+    uint32_t nvis = 0, ntri = 0;
+    uint32_t s_chunk = kNoChunk, s_fill = 0, e_chunk = kNoChunk, e_fill = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    // g: this lane's next photon to look at; gp: the photon being replayed
+    uint64_t g = pa.g_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x, gp = 0;
+    uint32_t t = 0, src = 0, abs_bin = 0, seed = 0, depth = 0, prev_tri = 0;
     int prev_max = 0;
     float pb1 = 0.0f, pb2 = 0.0f;
-    double fpath = 0.0;
     V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
     bool live = false;
     for (;;) {
         if (!live) {
             // this lane's next photon with a path to replay
-            for (;;) {
-                while (left == 0) {
-                    chunk += gridDim.x;
-                    const uint64_t base = (uint64_t)chunk * kTraceBlock * run;
-                    if (base >= a.photons) break;
-                    run_g = base + (uint64_t)lane * run;
-                    left = run_g < a.photons ? (uint32_t)min<uint64_t>(run, a.photons - run_g) : 0u;
-                }
-                if (left == 0) break;
-                g = run_g++;
-                left--;
-                if (g < pa.g_start) continue;
+            for (; g < pa.g_end; g += stride) {
                 const uint32_t r = a.absorber[g];
                 if (r == 0xffffffffu && !dropped_too) continue;
                 t = (uint32_t)(g / a.per_tri);
-                if (t != acc_t) {
-                    if (acc_t != 0xffffffffu) {
-#pragma unroll
-                        for (int e = 0; e < 9; e++)
-                            if (sacc[e] != 0.0)
-                                fx_add(a.grad + 2 * (3 * (size_t)a.idx[3 * acc_t + e / 3] + e % 3),
-                                       sacc[e], a.inv_q);
-                    }
-#pragma unroll
-                    for (int e = 0; e < 9; e++) sacc[e] = 0.0;
-                    acc_t = t;
-                }
                 // the emission ray (getEmissionRay, thermal_tracing_unified.glsl:125-161)
                 const uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
                 const V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
@@ -351,77 +394,72 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
                 const int sc = max_corner(b0, b1, b2);
                 src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
                 abs_bin = r == 0xffffffffu ? 0u : r;
-                fpath = hist_factor(a, abs_bin, src);
-                prev0 = i0;
-                prev1 = i1;
-                prev2 = i2;
+                prev_tri = t;
                 pb1 = b1;
                 pb2 = b2;
                 prev_max = sc;
                 depth = 0;
+                gp = g;
+                g += stride;
                 live = true;
                 break;
             }
-            if (!live) break;
         }
-        ++depth;
-        const Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn,
-                                           stt, nvis, ntri);
+        // every lane stays in the loop until the wave is done: the record
+        // slots are allocated by the whole wave together
+        if (__ballot(live) == 0) break;
+        Query q{};
+        bool tracing = false;
+        if (live) {
+            ++depth;
+            tracing = query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri);
+        }
+        query_run(q, live && tracing, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
         // the reference's reject rules: miss, too close, (near) backface
-        if (h.prim < 0 || h.t < a.tiny_dist) {
-            live = false;
-            continue;
-        }
-        const uint32_t hp = (uint32_t)h.prim;
-        const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
-        const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
-        const V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
-        if (dot3(hn, d) > -1e-3f) {
-            live = false;
-        } else {
-            const float hb0 = 1.0f - h.b1 - h.b2;
-            if (!pa.edges_only) n_seg++;
-            if (f_cont) {
-                // continuous geometric term, :391-448
-                const V3 r0 = load_v(a.xyz, prev0), r1 = load_v(a.xyz, prev1),
-                         r2 = load_v(a.xyz, prev2);
-                const uint32_t vid[6] = {prev0, prev1, prev2, j0, j1, j2};
-                const bool first = depth == 1;
-                const float wt = a.w[t];
-                const float G = geom_grad(
-                    r0, r1, r2, q0, q1, q2, pb1, pb2, h.b1, h.b2, a.eps, [&](int v, V3 gv) {
-                        const V3 dv = scl3(gv, wt);
-                        if (!finite3(dv)) return;
-                        if (v < 3 && first) {
-                            // the first segment starts on the source triangle
-                            sacc[3 * v] += fpath * (double)dv.x;
-                            sacc[3 * v + 1] += fpath * (double)dv.y;
-                            sacc[3 * v + 2] += fpath * (double)dv.z;
-                        } else {
-                            fx_add3(a.grad, vid[v], dv, fpath, a.inv_q);
-                        }
-                    });
-                if (fabsf(G) > a.eps * a.eps) n_cont++;
+        bool seg = false;
+        Hit h{-1, 0.0f, 0.0f, 0.0f};
+        uint32_t hp = 0, j0 = 0, j1 = 0, j2 = 0;
+        V3 q0{0.0f, 0.0f, 0.0f}, q1 = q0, q2 = q0, hn = q0;
+        if (live) {
+            h = query_end(q, 1e12f);
+            if (h.prim >= 0 && h.t >= a.tiny_dist) {
+                hp = (uint32_t)h.prim;
+                j0 = a.idx[3 * hp];
+                j1 = a.idx[3 * hp + 1];
+                j2 = a.idx[3 * hp + 2];
+                q0 = load_v(a.xyz, j0);
+                q1 = load_v(a.xyz, j1);
+                q2 = load_v(a.xyz, j2);
+                hn = normalize3(cross3(q1 - q0, q2 - q0));
+                seg = dot3(hn, d) <= -1e-3f;
             }
-            if (f_hit) {
-                // extension: the hit triangle's area Jacobian (the hit point is
-                // a fixed barycentric point of it)
-                V3 ga[3];
-                area_grad(q0, q1, q2, ga);
-                const float sh = a.w[t] / a.area[hp];
-                const uint32_t jv[3] = {j0, j1, j2};
-#pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const V3 dh = scl3(ga[c], sh);
-                    if (finite3(dh)) fx_add3(a.grad, jv[c], dh, fpath, a.inv_q);
+            if (!seg) live = false;
+        }
+        {
+            bool ok;
+            const uint64_t slot = chunk_slot(seg, s_chunk, s_fill, pa.segs, ok);
+            if (seg) {
+                if (ok) {
+                    SegRec *dst = static_cast<SegRec *>(pa.segs.p) + slot;
+                    dst->r0 = make_uint4(prev_tri, hp, src, abs_bin);
+                    dst->r1 = make_float4(pb1, pb2, h.b1, h.b2);
+                    dst->r2 = make_uint4((uint32_t)gp, t, depth == 1 ? 1u : 0u, 0u);
+                } else {
+                    atomicMin(pa.first_dropped, (unsigned long long)gp);
                 }
             }
+        }
+        // the approximate edge term, :454-567: a hit near an edge becomes an
+        // edge record with the occlusion ray
+        const float hb0 = 1.0f - h.b1 - h.b2;
+        bool edge = false;
+        float4 e0, e1, e2;
+        uint4 e3;
+        if (seg && f_edge) {
             const int opp = min_corner(hb0, h.b1, h.b2);
             const float hb_opp = opp == 0 ? hb0 : (opp == 1 ? h.b1 : h.b2);
-            if (f_edge && hb_opp <= a.ratio) {
-                // approximate edge term, :454-567: the occlusion ray goes into
-                // an edge record
-                if (!pa.edges_only) n_edge++;
+            if (hb_opp <= a.ratio) {
+                edge = true;
                 const int nr = max_corner(hb0, h.b1, h.b2);
                 int other = 0;
                 if (opp == 0 || nr == 0) {
@@ -455,23 +493,30 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
                 V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
                 to = add3(to, scl3(n_edg, a.eps + dot3(qn - to, n_edg)));
                 to = to - scl3(d, a.eps);
-                const uint32_t origin_v = prev_max == 0 ? prev0 : (prev_max == 1 ? prev1 : prev2);
-                // one slot range per wave for the lanes at an edge
-                const unsigned long long slot = wave_slot(true, pa.edge_count);
-                if (slot < pa.edge_cap) {
-                    EdgeRec *dst = pa.edges + slot;
-                    dst->r0 = make_float4(to.x, to.y, to.z, h.t);
-                    dst->r1 = make_float4(d.x, d.y, d.z, __uint_as_float(hp));
-                    dst->r2 = make_float4(dE.x, dE.y, dE.z, __uint_as_float(jn));
-                    dst->r3 = make_uint4(origin_v, src, abs_bin, (uint32_t)g);
+                const uint32_t origin_v = a.idx[3 * prev_tri + prev_max];
+                e0 = make_float4(to.x, to.y, to.z, h.t);
+                e1 = make_float4(d.x, d.y, d.z, __uint_as_float(hp));
+                e2 = make_float4(dE.x, dE.y, dE.z, __uint_as_float(jn));
+                e3 = make_uint4(origin_v, src, abs_bin, (uint32_t)gp);
+            }
+        }
+        if (f_edge) {
+            bool ok;
+            const uint64_t slot = chunk_slot(edge, e_chunk, e_fill, pa.edges, ok);
+            if (edge) {
+                if (ok) {
+                    EdgeRec *dst = static_cast<EdgeRec *>(pa.edges.p) + slot;
+                    dst->r0 = e0;
+                    dst->r1 = e1;
+                    dst->r2 = e2;
+                    dst->r3 = e3;
                 } else {
-                    n_drop++;
-                    atomicMin(pa.first_dropped, (unsigned long long)g);
+                    atomicMin(pa.first_dropped, (unsigned long long)gp);
                 }
             }
-            prev0 = j0;
-            prev1 = j1;
-            prev2 = j2;
+        }
+        if (seg) {
+            prev_tri = hp;
             pb1 = h.b1;
             pb2 = h.b2;
             prev_max = max_corner(hb0, h.b1, h.b2);
@@ -495,54 +540,203 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
             if (absorbed || depth == a.max_depth) live = false;
         }
     }
-    if (acc_t != 0xffffffffu) {
+    chunk_close(s_chunk, s_fill, pa.segs);
+    chunk_close(e_chunk, e_fill, pa.edges);
+}
+
+// sort keys of the segment slots: the start (side 0) or hit (side 1)
+// triangle, nt for empty slots
+__global__ void k_seg_keys(const SegRec *__restrict__ segs, const uint32_t *__restrict__ fills,
+                           uint32_t n_chunks, uint32_t nt, uint32_t side,
+                           uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_chunks * kChunk) return;
+    const uint32_t c = (uint32_t)(i / kChunk), k = (uint32_t)(i % kChunk);
+    keys[i] = k < fills[c] ? (side ? segs[i].r0.y : segs[i].r0.x) : nt;
+    vals[i] = (uint32_t)i;
+}
+
+struct SegArgs {
+    AdjCommon c;
+    const SegRec *segs;
+    const uint32_t *order;      // slots sorted by the side's triangle
+    const uint32_t *keys;       // their triangles (nt: empty slot)
+    uint64_t n;                 // slots
+    uint32_t side;              // 0: start triangle's vertices, 1: hit triangle's
+    uint64_t g_limit;           // records of photons at or above this are skipped
+};
+
+// 128-bit (lo, hi) sum
+__device__ __forceinline__ void add128(unsigned long long &lo, long long &hi, unsigned long long lo2,
+                                       long long hi2)
+{
+    const unsigned long long s = lo + lo2;
+    hi = hi + hi2 + (s < lo ? 1 : 0);
+    lo = s;
+}
+
+// The segment terms with a sort-reduce scatter: the continuous geometric
+// term (thermal_tracing_unified.glsl:391-448) on one side of each segment --
+// side 0 its start triangle's vertices, side 1 its hit triangle's vertices
+// and the hit-area extension -- over the records sorted by that triangle.  A
+// window of 64 records sums the terms of equal triangles in consecutive
+// lanes exactly (128-bit integers, segmented scan) and adds each run once:
+// about one add per triangle and window instead of one per record.
+__global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
+{
+    const AdjCommon &a = sa.c;
+    unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
+    const bool f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
+    const bool f_hit = sa.side == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
+    const uint32_t lane = __lane_id();
+    uint32_t n_seg = 0, n_cont = 0;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
+    const uint32_t n_waves = gridDim.x * blockDim.x / 64;
+    const uint64_t n_windows = sa.n / 64;
+    for (uint64_t wi = wave; wi < n_windows; wi += n_waves) {
+        const uint64_t pos = wi * 64 + lane;
+        bool have = sa.keys[pos] != a.nt;
+        const uint64_t slot = sa.order[pos];
+        if (__ballot(have) == 0) continue;
+        SegRec r{};
+        if (have) {
+            r = sa.segs[slot];
+            have = (uint64_t)r.r2.x < sa.g_limit;
+        }
+        const uint32_t key = have ? (sa.side == 0 ? r.r0.x : r.r0.y) : 0xffffffffu;
+        unsigned long long lo[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        long long hi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (have) {
+            if (sa.side == 0) n_seg++;
+            const uint32_t pt = r.r0.x, hp = r.r0.y, t = r.r2.y;
+            const double fpath = hist_factor(a, r.r0.w, r.r0.z);
+            const uint32_t i0 = a.idx[3 * pt], i1 = a.idx[3 * pt + 1], i2 = a.idx[3 * pt + 2];
+            const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+            const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
+            const float wt = a.w[t];
+            if (f_cont) {
+                const int lo_v = sa.side == 0 ? 0 : 3;
+                const float G = geom_grad(
+                    load_v(a.xyz, i0), load_v(a.xyz, i1), load_v(a.xyz, i2), q0, q1, q2, r.r1.x,
+                    r.r1.y, r.r1.z, r.r1.w, a.eps, [&](int v, V3 gv) {
+                        if (v < lo_v || v >= lo_v + 3) return;
+                        const V3 dv = scl3(gv, wt);
+                        if (!finite3(dv)) return;
+                        const int c = v - lo_v;
+                        fx_quant(fpath * (double)dv.x, a.inv_q, lo[3 * c], hi[3 * c]);
+                        fx_quant(fpath * (double)dv.y, a.inv_q, lo[3 * c + 1], hi[3 * c + 1]);
+                        fx_quant(fpath * (double)dv.z, a.inv_q, lo[3 * c + 2], hi[3 * c + 2]);
+                    });
+                if (sa.side == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
+            }
+            if (f_hit) {
+                // extension: the hit triangle's area Jacobian (the hit point is
+                // a fixed barycentric point of it)
+                V3 ga[3];
+                area_grad(q0, q1, q2, ga);
+                const float sh = wt / a.area[hp];
 #pragma unroll
-        for (int e = 0; e < 9; e++)
-            if (sacc[e] != 0.0)
-                fx_add(a.grad + 2 * (3 * (size_t)a.idx[3 * acc_t + e / 3] + e % 3), sacc[e],
-                       a.inv_q);
+                for (int c = 0; c < 3; c++) {
+                    const V3 dh = scl3(ga[c], sh);
+                    if (!finite3(dh)) continue;
+                    const double comp[3] = {fpath * (double)dh.x, fpath * (double)dh.y,
+                                            fpath * (double)dh.z};
+#pragma unroll
+                    for (int e = 0; e < 3; e++) {
+                        unsigned long long l2;
+                        long long h2;
+                        fx_quant(comp[e], a.inv_q, l2, h2);
+                        add128(lo[3 * c + e], hi[3 * c + e], l2, h2);
+                    }
+                }
+            }
+        }
+        // segmented inclusive scan over the window: runs of equal keys (every
+        // lane takes part in each shuffle: a lane outside the active set
+        // reads as zero)
+        const uint32_t key_prev = __shfl_up(key, 1, 64);
+        bool head = lane == 0 || key_prev != key;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const bool head_up = __shfl_up(head ? 1 : 0, off, 64) != 0;
+            const bool take = lane >= (uint32_t)off && !head;
+#pragma unroll
+            for (int e = 0; e < 9; e++) {
+                const unsigned long long l2 = __shfl_up(lo[e], off, 64);
+                const long long h2 = __shfl_up(hi[e], off, 64);
+                if (take) add128(lo[e], hi[e], l2, h2);
+            }
+            if (lane >= (uint32_t)off) head = head || head_up;
+        }
+        // the run's last lane adds the run's sums
+        const uint32_t key_next = __shfl_down(key, 1, 64);
+        const bool last = lane == 63 || key_next != key;
+        if (last && key != 0xffffffffu) {
+            const uint32_t tri = key;
+#pragma unroll
+            for (int e = 0; e < 9; e++)
+                fx_add_q(gx + 2 * (3 * (size_t)a.idx[3 * tri + e / 3] + e % 3), lo[e], hi[e]);
+        }
     }
-    uint32_t cnt[4] = {n_seg, n_cont, n_edge, n_drop};
-    const int slot_of[4] = {A_SEG, A_CONT, A_EDGE, A_DROPPED_EDGES};
+    if (sa.side == 0) {
+        uint32_t cnt[2] = {n_seg, n_cont};
+        const int slot_of[2] = {A_SEG, A_CONT};
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint32_t v = cnt[k];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-        if (lane == 0 && v) atomicAdd(a.stats + slot_of[k], (unsigned long long)v);
+        for (int k = 0; k < 2; k++) {
+            uint32_t v = cnt[k];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+            if (lane == 0 && v) atomicAdd(a.stats + slot_of[k], (unsigned long long)v);
+        }
     }
 }
 
 struct OcclArgs {
     AdjCommon c;
     const EdgeRec *edges;
-    uint64_t n;            // records written (at most the capacity)
+    const uint32_t *fills;
+    uint32_t n_chunks;     // chunks written
     uint64_t g_limit;      // records of photons at or above this are skipped
 };
 
-// The edge records' occlusion rays, one per lane at a time (the edge term of
+// The edge records' occlusion rays, chunk by chunk (the edge term of
 // thermal_tracing_unified.glsl:495-567).
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(OcclArgs oa)
 {
     const AdjCommon &a = oa.c;
+    unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
     __shared__ uint32_t stk_n[STACK * kTraceBlock];
     __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
     const bool f_gain = a.aflags & IRTT_ADJ_EDGE_GAIN;
-    uint32_t n_occl = 0, nvis = 0, ntri = 0;
-    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x; i < oa.n; i += stride) {
-        const EdgeRec er = oa.edges[i];
-        const uint32_t g = er.r3.w;
-        if ((uint64_t)g >= oa.g_limit) continue;
+    uint32_t n_edge = 0, n_occl = 0, nvis = 0, ntri = 0;
+    const uint64_t n_windows = (uint64_t)oa.n_chunks * (kChunk / 64);
+    for (uint64_t wi = blockIdx.x; wi < n_windows; wi += gridDim.x) {
+        const uint32_t c = (uint32_t)(wi / (kChunk / 64));
+        const uint32_t k = (uint32_t)(wi % (kChunk / 64)) * 64 + threadIdx.x;
+        bool have = k < oa.fills[c];
+        if (__ballot(have) == 0) continue;
+        EdgeRec er{};
+        if (have) {
+            er = oa.edges[(uint64_t)c * kChunk + k];
+            have = (uint64_t)er.r3.w < oa.g_limit;
+        }
         const V3 to{er.r0.x, er.r0.y, er.r0.z}, od{er.r1.x, er.r1.y, er.r1.z};
+        Query q{};
+        bool tracing = false;
+        if (have) {
+            n_edge++;
+            tracing = query_begin(q, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, ntri);
+        }
+        query_run(q, have && tracing, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
+        if (!have) continue;
+        const Hit h = query_end(q, 1e12f);
         const V3 dE{er.r2.x, er.r2.y, er.r2.z};
         const float oht = er.r0.w;
         const uint32_t ohp = __float_as_uint(er.r1.w), edge_v = __float_as_uint(er.r2.w);
         const uint32_t origin_v = er.r3.x, src = er.r3.y, abs_bin = er.r3.z;
-        const Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, sn,
-                                           stt, nvis, ntri);
         const bool valid = h.prim >= 0 && h.t >= a.tiny_dist;
         bool nbr = false;
         uint32_t shadow_v = 0;
@@ -563,9 +757,9 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(
                 const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
                 const float d1 = oht, d2 = h.t, tot = d1 + d2;
                 const double fs = hist_factor(a, shadow_bin, src);
-                fx_add3(a.grad, shadow_v, scl3(dE, sa * (d1 / tot)), fs, a.inv_q);
-                fx_add3(a.grad, origin_v, scl3(dE, sa * (d2 / tot)), fs, a.inv_q);
-                fx_add3(a.grad, edge_v, scl3(dE, -sa), fs, a.inv_q);
+                fx_add3(gx, shadow_v, scl3(dE, sa * (d1 / tot)), fs, a.inv_q);
+                fx_add3(gx, origin_v, scl3(dE, sa * (d2 / tot)), fs, a.inv_q);
+                fx_add3(gx, edge_v, scl3(dE, -sa), fs, a.inv_q);
             }
         }
         // Extension (not in the reference): the power the moving edge takes
@@ -575,14 +769,19 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(
         if (f_gain && (h.prim < 0 || (valid && !nbr))) {
             const double fpath = hist_factor(a, abs_bin, src);
             const float f_or = valid ? h.t / (oht + h.t) : 1.0f;
-            fx_add3(a.grad, edge_v, dE, fpath, a.inv_q);
-            fx_add3(a.grad, origin_v, scl3(dE, f_or), -fpath, a.inv_q);
-            if (valid) fx_add3(a.grad, shadow_v, scl3(dE, 1.0f - f_or), -fpath, a.inv_q);
+            fx_add3(gx, edge_v, dE, fpath, a.inv_q);
+            fx_add3(gx, origin_v, scl3(dE, f_or), -fpath, a.inv_q);
+            if (valid) fx_add3(gx, shadow_v, scl3(dE, 1.0f - f_or), -fpath, a.inv_q);
         }
     }
-    uint32_t v = n_occl;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if (threadIdx.x == 0 && v) atomicAdd(a.stats + A_OCCL, (unsigned long long)v);
+    uint32_t cnt[2] = {n_edge, n_occl};
+    const int slot_of[2] = {A_EDGE, A_OCCL};
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        uint32_t v = cnt[k];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (threadIdx.x == 0 && v) atomicAdd(a.stats + slot_of[k], (unsigned long long)v);
+    }
 }
 
 // The source-area terms (thermal_tracing_unified.glsl:327-351 and :588-592),
@@ -594,6 +793,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(
 __global__ __launch_bounds__(256) void k_adj_area(AdjCommon a, uint32_t do_area)
 {
     const uint32_t t = blockIdx.x;
+    unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
     __shared__ double sm[256];
     __shared__ unsigned int sabs;
     if (threadIdx.x == 0) sabs = 0;
@@ -638,7 +838,7 @@ __global__ __launch_bounds__(256) void k_adj_area(AdjCommon a, uint32_t do_area)
             const uint32_t iv[3] = {i0, i1, i2};
             for (int cc = 0; cc < 3; cc++) {
                 const V3 dw = scl3(ga[cc], sa);
-                if (finite3(dw)) fx_add3(a.grad, iv[cc], dw, sm[0], a.inv_q);
+                if (finite3(dw)) fx_add3(gx, iv[cc], dw, sm[0], a.inv_q);
             }
         }
     }
@@ -648,13 +848,14 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 
 }  // namespace
 
-// Edge records the trace holds at once (64 B each): half the forward ray
-// segments (hits near an edge are a fraction of them: 28 % of the city
-// block's), at most 2^25 (2 GiB); more events re-run the paths from the first
-// photon whose record did not fit.
-uint64_t edge_record_capacity(uint64_t rays)
+// Device bytes the adjoint trace's records take for a replay of `segments`
+// accepted segments (segment records with their sort keys, edge records for
+// up to half of them, per-wave partial chunks), for serving's admission.
+uint64_t adjoint_record_bytes(uint64_t photons, uint64_t segments)
 {
-    return std::min<uint64_t>(rays / 2 + 4096, 1ull << 25);
+    const uint64_t waves = std::min<uint64_t>((photons + 1023) / 1024, 256ull * 128);
+    return (segments + (waves + 1) * kChunk) * (sizeof(SegRec) + 16) +
+           (segments / 2 + (waves + 1) * kChunk) * sizeof(EdgeRec) + 4 * photons;
 }
 
 void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
@@ -707,7 +908,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     d_area.alloc(s.nt);
     d_lam.alloc((size_t)n_hist * nb);
     d_x4.alloc((size_t)n_hist * nb);
-    d_grad.alloc(6 * (size_t)s.nv);
+    d_grad.alloc(kGradCopies * 6 * (size_t)s.nv);
     d_st.alloc(16);
     IRTT_HIP(hipMemcpyAsync(d_w.p, w.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
     IRTT_HIP(hipMemcpyAsync(d_area.p, s.tri_area.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
@@ -720,12 +921,12 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     // absorbing bins: the forward build's record, or a forward pass of our own
     DBuf<uint32_t> d_abs_own;
     const uint32_t *absorber = nullptr;
-    uint64_t rays = s.fwd_rays;
+    uint64_t rays = s.fwd_rays, segments = s.fwd_segments;
     if (s.d_absorber.n == photons) {
         absorber = s.d_absorber.p;
     } else {
         d_abs_own.alloc(photons);
-        absorber_pass(s, d_abs_own.p, rays);
+        absorber_pass(s, d_abs_own.p, rays, segments);
         absorber = d_abs_own.p;
     }
 
@@ -757,6 +958,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     c.absorber = absorber;
     c.photons = photons;
     c.grad = d_grad.p;
+    c.grad_stride = 6 * (uint64_t)s.nv;
     c.inv_q = inv_q;
     c.stats = d_st.p;
 
@@ -769,73 +971,143 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     k_adj_area<<<s.nt, 256, 0, stm>>>(c, (c.aflags & IRTT_ADJ_AREA) ? 1u : 0u);
     IRTT_HIP(hipGetLastError());
 
-    // path segments, edge records; an edge-record buffer that fills up makes
-    // the occlusion kernel stop at the first photon with a dropped record and
-    // the paths from there on run again for their records only
+    // Path replay into segment and edge records, then their terms.  The
+    // records of all photons fit when the device has room (the forward
+    // build's accepted segments bound them; edge records are taken as at most
+    // half of those); otherwise, and whenever a buffer overflows, the photons
+    // are replayed in ranges, each range's terms computed up to its first
+    // photon with a dropped record.
     const bool want_edges = c.aflags & IRTT_ADJ_EDGE;
-    const uint64_t cap = want_edges ? edge_record_capacity(rays) : 1;
+    // replay grid: at most IRTT_PHOTON_WAVES_PER_CU_ADJ waves per CU, and at
+    // least 16 photons per lane (each wave's last chunk is partly empty)
+    const uint64_t waves = std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU_ADJ,
+                              (photons + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
+    const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
+    size_t free_b = 0, total_b = 0;
+    IRTT_HIP(hipMemGetInfo(&free_b, &total_b));
+    const double budget = std::max(2.0 * (1u << 30), 0.3 * (double)free_b);
+    uint64_t seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
+    uint64_t edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
+    const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
+    uint64_t n_ranges = 1;
+    if (need > budget) {
+        const double f = budget / need;
+        seg_chunks = std::max<uint64_t>(2 * waves + 1, (uint64_t)(seg_chunks * f));
+        edge_chunks = want_edges ? std::max<uint64_t>(2 * waves + 1, (uint64_t)(edge_chunks * f))
+                                 : 1;
+        n_ranges = (uint64_t)std::ceil(1.0 / f);
+    }
+    DBuf<SegRec> d_segs;
     DBuf<EdgeRec> d_edges;
-    DBuf<unsigned long long> d_ctl;  // edge count, first dropped photon
-    d_edges.alloc(cap);
-    d_ctl.alloc(2);
-    const uint64_t waves = (uint64_t)ncu * 64;
-    PathArgs pa{};
-    pa.c = c;
-    pa.run = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(kAdjRun, photons /
-                                                                              (waves * kTraceBlock)));
-    pa.edges = d_edges.p;
-    pa.edge_cap = cap;
-    pa.edge_count = d_ctl.p;
-    pa.first_dropped = d_ctl.p + 1;
-    const uint64_t want = (photons + (uint64_t)kTraceBlock * pa.run - 1) /
-                          ((uint64_t)kTraceBlock * pa.run);
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, waves));
-    uint64_t g_start = 0;
-    const unsigned long long ctl0[2] = {0ull, ~0ull};
-    for (int pass = 0;; pass++) {
-        IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
-        pa.edges_only = pass > 0 ? 1u : 0u;
-        pa.g_start = g_start;
-        if (S == 24) k_adj_paths<24><<<grid, kTraceBlock, 0, stm>>>(pa);
-        else if (S == 32) k_adj_paths<32><<<grid, kTraceBlock, 0, stm>>>(pa);
-        else if (S == 64) k_adj_paths<64><<<grid, kTraceBlock, 0, stm>>>(pa);
-        else k_adj_paths<128><<<grid, kTraceBlock, 0, stm>>>(pa);
-        IRTT_HIP(hipGetLastError());
-        unsigned long long ctl[2];
-        IRTT_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stm));
-        IRTT_HIP(hipStreamSynchronize(stm));
-        if (!want_edges) break;
-        OcclArgs oa{};
-        oa.c = c;
-        oa.edges = d_edges.p;
-        oa.n = std::min<uint64_t>(ctl[0], cap);
-        oa.g_limit = ctl[1];
-        if (oa.n > 0) {
-            const uint32_t og = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(cdiv(oa.n, kTraceBlock), waves));
-            if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm>>>(oa);
-            else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm>>>(oa);
-            else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm>>>(oa);
-            else k_adj_occl<128><<<og, kTraceBlock, 0, stm>>>(oa);
+    DBuf<uint32_t> d_fills, d_keys, d_keys2, d_order, d_order2;
+    DBuf<unsigned long long> d_ctl;  // chunks taken (segments, edges), first dropped photon
+    d_segs.alloc(seg_chunks * kChunk);
+    d_edges.alloc(edge_chunks * kChunk);
+    d_fills.alloc(seg_chunks + edge_chunks);
+    d_keys.alloc(seg_chunks * kChunk);
+    d_keys2.alloc(seg_chunks * kChunk);
+    d_order.alloc(seg_chunks * kChunk);
+    d_order2.alloc(seg_chunks * kChunk);
+    d_ctl.alloc(3);
+    uint32_t key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) <= s.nt) key_bits++;
+    size_t sort_tmp_b = 0;
+    {
+        rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+        IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb,
+                                           (size_t)(seg_chunks * kChunk), 0, key_bits));
+    }
+    DBuf<uint8_t> d_sort_tmp;
+    d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
+    const uint64_t range = (photons + n_ranges - 1) / n_ranges;
+    const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
+    for (uint64_t gb = 0; gb < photons; gb += range) {
+        const uint64_t ge = std::min<uint64_t>(photons, gb + range);
+        uint64_t g_start = gb;
+        for (;;) {
+            PathArgs pa{};
+            pa.c = c;
+            pa.g_begin = g_start;
+            pa.g_end = ge;
+            pa.segs = RecBuf{d_segs.p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
+            pa.edges = RecBuf{d_edges.p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
+            pa.first_dropped = d_ctl.p + 2;
+            IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
+            const uint32_t grid = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(cdiv(ge - g_start, kTraceBlock), waves));
+            if (S == 24) k_adj_paths<24><<<grid, kTraceBlock, 0, stm>>>(pa);
+            else if (S == 32) k_adj_paths<32><<<grid, kTraceBlock, 0, stm>>>(pa);
+            else if (S == 64) k_adj_paths<64><<<grid, kTraceBlock, 0, stm>>>(pa);
+            else k_adj_paths<128><<<grid, kTraceBlock, 0, stm>>>(pa);
             IRTT_HIP(hipGetLastError());
+            unsigned long long ctl[3];
+            IRTT_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stm));
+            IRTT_HIP(hipStreamSynchronize(stm));
+            const uint32_t sc = (uint32_t)std::min<uint64_t>(ctl[0], seg_chunks);
+            const uint32_t ec = want_edges ? (uint32_t)std::min<uint64_t>(ctl[1], edge_chunks) : 0;
+            if (sc > 0) {
+                // each side over the records sorted by its triangle
+                const uint64_t n = (uint64_t)sc * kChunk;
+                for (uint32_t side = 0; side < 2; side++) {
+                    k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(d_segs.p, d_fills.p, sc, s.nt, side,
+                                                              d_keys.p, d_order.p);
+                    IRTT_HIP(hipGetLastError());
+                    rocprim::double_buffer<uint32_t> kd(d_keys.p, d_keys2.p),
+                        vd(d_order.p, d_order2.p);
+                    size_t tb = sort_tmp_b;
+                    IRTT_HIP(rocprim::radix_sort_pairs(d_sort_tmp.p, tb, kd, vd, (size_t)n, 0,
+                                                       key_bits, stm));
+                    SegArgs sa{};
+                    sa.c = c;
+                    sa.segs = d_segs.p;
+                    sa.keys = kd.current();
+                    sa.order = vd.current();
+                    sa.n = n;
+                    sa.side = side;
+                    sa.g_limit = ctl[2];
+                    const uint32_t sg = (uint32_t)std::max<uint64_t>(
+                        1, std::min<uint64_t>(cdiv(n / 64, 4), (uint64_t)ncu * 16));
+                    k_adj_segments<<<sg, 256, 0, stm>>>(sa);
+                    IRTT_HIP(hipGetLastError());
+                }
+            }
+            if (ec > 0) {
+                OcclArgs oa{};
+                oa.c = c;
+                oa.edges = d_edges.p;
+                oa.fills = d_fills.p + seg_chunks;
+                oa.n_chunks = ec;
+                oa.g_limit = ctl[2];
+                const uint32_t og = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
+                if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm>>>(oa);
+                else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm>>>(oa);
+                else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm>>>(oa);
+                else k_adj_occl<128><<<og, kTraceBlock, 0, stm>>>(oa);
+                IRTT_HIP(hipGetLastError());
+            }
+            if (ctl[2] == ~0ull) break;  // every record of the range fitted
+            if (ctl[2] <= g_start)
+                throw Error(IRTT_ERR_LIMIT, "adjoint trace: the records of one photon exceed the "
+                                            "record buffers");
+            g_start = ctl[2];
         }
-        if (ctl[1] == ~0ull) break;  // every record fitted
-        if (ctl[1] <= g_start)
-            throw Error(IRTT_ERR_LIMIT, "adjoint trace: edge records of one photon range exceed "
-                                        "the record buffer");
-        g_start = ctl[1];
     }
     IRTT_HIP(hipEventRecord(s.ev1, stm));
-    std::vector<unsigned long long> gx(6 * (size_t)s.nv), hs(16);
+    std::vector<unsigned long long> gx(kGradCopies * 6 * (size_t)s.nv), hs(16);
     IRTT_HIP(hipMemcpyAsync(gx.data(), d_grad.p, d_grad.bytes(), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(hs.data(), d_st.p, 16 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
     const double q = 1.0 / inv_q;
+    const size_t words = 6 * (size_t)s.nv;
     for (size_t k = 0; k < 3 * (size_t)s.nv; k++) {
-        const __int128 v = (__int128)(((unsigned __int128)gx[2 * k + 1] << 64) | gx[2 * k]);
-        grad[k] = (double)v * q;
+        unsigned __int128 sum = 0;  // exact: the copies' integers added modulo 2^128
+        for (uint32_t c = 0; c < kGradCopies; c++)
+            sum += ((unsigned __int128)gx[c * words + 2 * k + 1] << 64) | gx[c * words + 2 * k];
+        grad[k] = (double)(__int128)sum * q;
     }
     st.photons = photons;
     st.absorbed = hs[A_ABS];

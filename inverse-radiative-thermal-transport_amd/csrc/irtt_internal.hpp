@@ -253,6 +253,7 @@ struct Scene {
     DBuf<unsigned long long> d_stats;
     DBuf<uint32_t> d_absorber;        // absorbing bin of every photon (~0: none), for the adjoint
     uint64_t fwd_rays = 0;            // ray segments of the last transport build
+    uint64_t fwd_segments = 0;        // ... of them accepted (not a miss, too close or backface)
 
     // operator forms (operator.hip, aca.hip)
     uint32_t op_kind = IRTT_OP_DENSE;
@@ -339,9 +340,10 @@ void aca_apply(Scene &s, const double *w, double *y, bool transposed);
 void aca_get(Scene &s, double *U, double *V, double *cdiag, int32_t *prow, int32_t *pcol);
 // trace.hip: a forward pass that only records each photon's absorbing bin
 // (absorber: one word per photon, ~0 where none); returns the ray segments
-void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays);
+// traced and of them the accepted ones
+void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays, uint64_t &segments);
 // adjoint.hip
-uint64_t edge_record_capacity(uint64_t rays);
+uint64_t adjoint_record_bytes(uint64_t photons, uint64_t segments);
 void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,
                    const double *x, double *grad, irtt_adjoint_stats &st);
 

@@ -70,8 +70,11 @@ uint64_t footprint(const irtt_request &q, uint32_t nb)
     if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
     b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;
     b += (uint64_t)q.nt * 256 + (uint64_t)q.nv * 64 + (64ull << 20);
-    if (q.target)  // per-photon absorber record for the adjoint trace
-        b += 4ull * q.transport.threads_per_tri * q.transport.rays_per_thread * q.nt;
+    if (q.target) {  // the adjoint trace's absorber record and path records
+        const uint64_t photons = (uint64_t)q.transport.threads_per_tri *
+                                 q.transport.rays_per_thread * q.nt;
+        b += adjoint_record_bytes(photons, photons);
+    }
     return b;
 }
 

@@ -379,7 +379,7 @@ static void launch_photons(Scene &s, const irtt_transport_params &p, uint32_t pe
 
 // A forward pass that records each photon's absorbing bin and bins nothing:
 // the adjoint trace's pass 0 when the build did not keep its record.
-void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays)
+void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays, uint64_t &segments)
 {
     const irtt_transport_params &p = s.tparams;
     const uint32_t per = p.threads_per_tri * p.rays_per_thread;
@@ -396,6 +396,7 @@ void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays)
     IRTT_HIP(hipMemcpyAsync(hs, d_st.p, sizeof(hs), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     rays = hs[S_RAYS];
+    segments = hs[S_RAYS] - hs[S_MISS] - hs[S_CLOSE] - hs[S_BACK];
 }
 
 void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_stats &st)
@@ -532,6 +533,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     st.photons = total;
     st.rays = hstats[S_RAYS];
     s.fwd_rays = st.rays;
+    s.fwd_segments = st.rays - st.miss - st.too_close - st.backface;
     st.absorbed = hstats[S_ABS];
     st.miss = hstats[S_MISS];
     st.too_close = hstats[S_CLOSE];
