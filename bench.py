@@ -567,7 +567,7 @@ def main():
                 "note": "algorithmic bytes (48 + 64 n_node + 40 n_tri per segment) exceed what "
                         "DRAM carries: node fetches are L2 / MALL hits"}}
     # steady solve on the built operator (reported beside the metric)
-    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-10)
+    sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14)
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
     solves = solve_rates(irtt, s, sd)
@@ -607,7 +607,17 @@ def main():
                          "finalize": round(sum(x["finalize_ms"] for x in stats) / len(stats), 3),
                          "bvh_build": round(st["bvh_ms"], 3)},
         "steady_solve": {"ms": round(sol["solve_ms"], 1), "newton_iters": sol["newton_iters"],
-                         "linear_iters": sol["linear_iters"], "converged": sol["converged"]},
+                         "linear_iters": sol["linear_iters"], "converged": sol["converged"],
+                         "power_in_W": round(sol["power_in"], 3),
+                         "power_out_W": round(sol["power_out"], 3),
+                         "balance_rel": float(f"{sol['balance_rel']:.3e}")},
+        "energy": {"emitted_W_per_K4": st["emitted_w"], "absorbed_W_per_K4": st["absorbed_w"],
+                   "escaped_W_per_K4": st["escaped_w"], "dropped_W_per_K4": st["dropped_w"],
+                   "rel_err": float(f"{st['energy_rel']:.3e}"),
+                   "integer_identity": bool(st["energy_exact"]),
+                   "note": "FP64 on the device: photons x weights = absorbed + escaped + dropped "
+                           "(exact in fixed point); rel_err is the FP32 rounding of the photon "
+                           "weights"},
         "solves": solves,
     }
     if trans is not None:

@@ -162,6 +162,17 @@ typedef struct irtt_transport_stats {
     uint64_t node_visits;       /* internal BVH nodes fetched over all rays */
     uint64_t tri_tests;         /* leaf triangles tested over all rays */
     uint64_t nnz;               /* stored entries (diagonal included) */
+    /* Energy conservation, FP64 on the device, in W per unit emission factor
+     * (x^4 = 1 K^4 in every bin): emitted = sum sigma eps A; absorbed (the
+     * operator's off-diagonal sums), escaped (misses) and dropped (too close,
+     * backface) from the photons' fixed-point weights.  energy_rel =
+     * |emitted - absorbed - escaped - dropped| / emitted: the FP32 rounding of
+     * the photon weights sigma eps A / photons (below 3e-7); energy_exact = 1
+     * when the integer identity
+     * photons * weights = absorbed + escaped + dropped holds exactly (no
+     * photon lost). */
+    double emitted_w, absorbed_w, escaped_w, dropped_w, energy_rel;
+    uint32_t energy_exact, reserved2;
 } irtt_transport_stats;
 
 /* Replaces ThermalTransport::compute (thermal_transport.cpp:599-738):
@@ -278,6 +289,13 @@ typedef struct irtt_solve_stats {
     double residual_sq;         /* final ||r||^2 (W^2) */
     double max_dx;              /* last Newton update, K */
     float solve_ms;
+    uint32_t reserved;
+    /* Global power balance at the solution over the free bins, FP64 on the
+     * device (irtt_solve only): power_in = external and ambient power
+     * absorbed, power_out = power radiated and conducted away plus, for an
+     * implicit-Euler step, the rate of heat stored; balance_rel =
+     * |power_in - power_out| / power_in. */
+    double power_in, power_out, balance_rel;
 } irtt_solve_stats;
 
 /* Replaces ThermalSolver::solve (thermal_solver.cpp:119-229) driven by

@@ -251,6 +251,7 @@ struct Scene {
     std::vector<double> colsum;        // column sums of T (exact)
     std::vector<double> Tdiag;         // diagonal of the FP32 operator as stored
     DBuf<unsigned long long> d_stats;
+    DBuf<double> d_energy;            // energy figures of the last build (k_energy)
     DBuf<uint32_t> d_absorber;        // absorbing bin of every photon (~0: none), for the adjoint
     uint64_t fwd_rays = 0;            // ray segments of the last transport build
     uint64_t fwd_segments = 0;        // ... of them accepted (not a miss, too close or backface)

@@ -94,6 +94,38 @@ __global__ void k_residual(Sys s, const double *__restrict__ Ty, const double *_
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
 }
 
+// Power balance at x over the free bins (from the last residual's Ty, y,
+// xe): part[0..] = sum b (in), part[kRedBlocks..] = sum (T x^4 + L x) plus
+// the storage rate M (x - x_prev) / dt of a transient step (out).
+__global__ void k_balance(Sys s, const double *__restrict__ Ty, const double *__restrict__ y,
+                          const double *__restrict__ xe, double *part)
+{
+    __shared__ double sm[2][kRedThreads];
+    double a_in = 0.0, a_out = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < s.nb;
+         i += gridDim.x * blockDim.x) {
+        if (s.fixed[i]) continue;
+        const double ty = Ty[i] + (s.cdr ? s.cdr[i] * y[i] : 0.0);
+        a_in += s.b[i];
+        a_out += ty + lap_row(s, i, xe) +
+                 (s.steady ? 0.0 : s.mass[i] * (xe[i] - s.xprev[i]) / s.dt);
+    }
+    sm[0][threadIdx.x] = a_in;
+    sm[1][threadIdx.x] = a_out;
+    __syncthreads();
+    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + off];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = sm[0][0];
+        part[kRedBlocks + blockIdx.x] = sm[1][0];
+    }
+}
+
 // Jacobian-vector product modes
 enum { J_EXACT = 0, J_REFERENCE = 1, J_PREDICT = 2 };
 
@@ -1058,6 +1090,21 @@ void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st)
         }
     }
     st.residual_sq = rn;
+    // the global power balance at the solution, FP64 on the device (V_TY,
+    // V_Y and V_XE are those of the last residual, i.e. of the final x)
+    {
+        double *part = s.d_red.p;
+        double *out = part + 2 * kRedBlocks;
+        k_balance<<<kRedBlocks, kRedThreads, 0, stm>>>(c.sys, c.v[V_TY], c.v[V_Y], c.v[V_XE],
+                                                       part);
+        k_final_sum<<<1, kRedThreads, 0, stm>>>(part, 2, kRedBlocks, out);
+        double pb[2];
+        IRTT_HIP(hipMemcpyAsync(pb, out, sizeof(pb), hipMemcpyDeviceToHost, stm));
+        IRTT_HIP(hipStreamSynchronize(stm));
+        st.power_in = pb[0];
+        st.power_out = pb[1];
+        st.balance_rel = pb[0] != 0.0 ? std::fabs(pb[0] - pb[1]) / std::fabs(pb[0]) : 0.0;
+    }
     std::vector<double> xr(nb);
     // V_XE holds the final x with fixed values applied (set by the last residual call)
     IRTT_HIP(hipMemcpyAsync(xr.data(), c.v[V_XE], nb * sizeof(double), hipMemcpyDeviceToHost, stm));

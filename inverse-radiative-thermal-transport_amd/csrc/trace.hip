@@ -167,10 +167,18 @@ __device__ __forceinline__ void count_segment(uint32_t *cnt, int outcome)
     for (int k = S_ABS; k < S_COUNT; k++) cnt[k] += outcome == k ? 1u : 0u;
 }
 
-// wave sums of the photon counters into stats (one wave per block)
+// wave sums of the photon counters into stats (one wave per block); esc /
+// drop: fixed-point weight of the photons that escaped / were dropped
 __device__ __forceinline__ void flush_counts(unsigned long long *stats, const uint32_t *cnt,
-                                             uint32_t nvis, uint32_t ntri)
+                                             uint32_t nvis, uint32_t ntri, unsigned long long esc,
+                                             unsigned long long drop)
 {
+    for (int off = 32; off > 0; off >>= 1) {
+        esc += __shfl_down(esc, off, 64);
+        drop += __shfl_down(drop, off, 64);
+    }
+    if (threadIdx.x == 0 && esc) atomicAdd(stats + S_COUNT + 2, esc);
+    if (threadIdx.x == 0 && drop) atomicAdd(stats + S_COUNT + 3, drop);
 #pragma unroll
     for (int s = 0; s < S_COUNT; s++) {
         uint32_t v = cnt[s];
@@ -201,6 +209,7 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
     uint32_t *sn = stk_n + threadIdx.x;
     float *stt = stk_t + threadIdx.x;
     uint32_t cnt[S_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned long long q_esc = 0, q_drop = 0;  // energy accounting
     uint32_t nvis = 0, ntri = 0;
     const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
     uint64_t g = a.photon_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
@@ -229,12 +238,14 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
                                      ntri);
         const int outcome = shade_segment(a, h, g, src, qt, depth, seed, o, d);
         count_segment(cnt, outcome);
+        q_esc += outcome == S_MISS ? qt : 0ull;
+        q_drop += (outcome == S_CLOSE || outcome == S_BACK) ? qt : 0ull;
         if (outcome >= 0) {
             live = false;
             g += stride;
         }
     }
-    flush_counts(a.stats, cnt, nvis, ntri);
+    flush_counts(a.stats, cnt, nvis, ntri, q_esc, q_drop);
 }
 
 // T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
@@ -270,6 +281,54 @@ __global__ void k_finalize(const unsigned long long *__restrict__ acc, const dou
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// Energy conservation of a build, one block: every photon of triangle t
+// carries q_t, and ends absorbed (in the operator's column sums), escaped or
+// dropped, so  per * sum_t q_t = sum_j colacc_j + escaped + dropped  exactly
+// in integers; in FP64 (W per unit emission factor): emitted
+// sum_i sigma eps_i A_i against Q times each part.  out: emitted, absorbed,
+// escaped, dropped (W), |emitted - parts| / emitted, integer identity (1/0).
+__global__ __launch_bounds__(256) void k_energy(const unsigned long long *__restrict__ q,
+                                                uint32_t nt, uint32_t per,
+                                                const unsigned long long *__restrict__ colacc,
+                                                const double *__restrict__ diag, uint32_t nb,
+                                                const unsigned long long *__restrict__ stats,
+                                                double Q, double *__restrict__ out)
+{
+    __shared__ unsigned long long su[256], sa[256];
+    __shared__ double sd[256];
+    unsigned long long uq = 0, ua = 0;
+    double em = 0.0;
+    for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) uq += q[t];
+    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
+        ua += colacc[j];
+        em += diag[j];
+    }
+    su[threadIdx.x] = uq;
+    sa[threadIdx.x] = ua;
+    sd[threadIdx.x] = em;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            su[threadIdx.x] += su[threadIdx.x + off];
+            sa[threadIdx.x] += sa[threadIdx.x + off];
+            sd[threadIdx.x] += sd[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const unsigned long long emitted = su[0] * (unsigned long long)per;
+        const unsigned long long esc = stats[S_COUNT + 2], drop = stats[S_COUNT + 3];
+        const double e = sd[0], ab = (double)sa[0] * Q, es = (double)esc * Q,
+                     dr = (double)drop * Q;
+        out[0] = e;
+        out[1] = ab;
+        out[2] = es;
+        out[3] = dr;
+        out[4] = e > 0.0 ? fabs(e - (ab + es + dr)) / e : 0.0;
+        out[5] = emitted == sa[0] + esc + drop ? 1.0 : 0.0;
+    }
+}
 
 }  // namespace
 
@@ -510,13 +569,21 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         IRTT_HIP(hipGetLastError());
     }
     IRTT_HIP(hipEventRecord(s.ev2, stm));
+    // energy conservation in FP64 on the device (after the column sums)
+    s.d_energy.alloc(8);
+    k_energy<<<1, 256, 0, stm>>>(reinterpret_cast<const unsigned long long *>(s.d_q.p), s.nt, per,
+                                 s.d_colacc.p, s.d_diag.p, nb, s.d_stats.p, Q, s.d_energy.p);
+    IRTT_HIP(hipGetLastError());
     // read-backs into the scene's page-locked block: column sums, counters,
-    // stored diagonal
+    // stored diagonal, energy figures
     auto *colacc = static_cast<unsigned long long *>(
         s.h_pin.get((size_t)nb * sizeof(unsigned long long) + 16 * sizeof(unsigned long long) +
-                    (size_t)nb * sizeof(float)));
+                    8 * sizeof(double) + (size_t)nb * sizeof(float)));
     unsigned long long *hstats = colacc + nb;
-    float *dcol = reinterpret_cast<float *>(hstats + 16);
+    double *henergy = reinterpret_cast<double *>(hstats + 16);
+    float *dcol = reinterpret_cast<float *>(henergy + 8);
+    IRTT_HIP(hipMemcpyAsync(henergy, s.d_energy.p, 6 * sizeof(double), hipMemcpyDeviceToHost,
+                            stm));
     IRTT_HIP(hipMemcpyAsync(colacc, s.d_colacc.p, nb * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(hstats, s.d_stats.p, 16 * sizeof(unsigned long long),
@@ -543,6 +610,12 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     st.node_visits = hstats[S_COUNT];
     st.tri_tests = hstats[S_COUNT + 1];
     st.quantum = Q;
+    st.emitted_w = henergy[0];
+    st.absorbed_w = henergy[1];
+    st.escaped_w = henergy[2];
+    st.dropped_w = henergy[3];
+    st.energy_rel = henergy[4];
+    st.energy_exact = henergy[5] != 0.0 ? 1u : 0u;
     st.bvh_ms = s.bvh.build_ms;
     st.layout = p.layout;
     st.nnz = sparse ? s.sp.nnz : (uint64_t)nb * nb;

@@ -65,7 +65,10 @@ class TransportStats(ctypes.Structure):
         ("quantum", ctypes.c_double), ("trace_ms", ctypes.c_float),
         ("finalize_ms", ctypes.c_float), ("bvh_ms", ctypes.c_float),
         ("layout", ctypes.c_uint32), ("node_visits", ctypes.c_uint64),
-        ("tri_tests", ctypes.c_uint64), ("nnz", ctypes.c_uint64)]
+        ("tri_tests", ctypes.c_uint64), ("nnz", ctypes.c_uint64)] + [
+        (n, ctypes.c_double) for n in
+        ("emitted_w", "absorbed_w", "escaped_w", "dropped_w", "energy_rel")] + [
+        ("energy_exact", ctypes.c_uint32), ("reserved2", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: (getattr(self, n)) for n, _ in self._fields_}
@@ -84,7 +87,9 @@ class SolveStats(ctypes.Structure):
     _fields_ = [("newton_iters", ctypes.c_uint32), ("linear_iters", ctypes.c_uint32),
                 ("line_search_halvings", ctypes.c_uint32), ("converged", ctypes.c_uint32),
                 ("residual_sq", ctypes.c_double), ("max_dx", ctypes.c_double),
-                ("solve_ms", ctypes.c_float)]
+                ("solve_ms", ctypes.c_float), ("reserved", ctypes.c_uint32),
+                ("power_in", ctypes.c_double), ("power_out", ctypes.c_double),
+                ("balance_rel", ctypes.c_double)]
 
     def as_dict(self):
         return {n: (getattr(self, n)) for n, _ in self._fields_}
