@@ -119,6 +119,19 @@ int irtt_get_bvh(irtt_scene h, uint32_t *nodes, uint32_t *tris,
 int irtt_trace_rays(irtt_scene h, const float *rays, uint32_t n, float tmin,
                     float tmax, int32_t *prim, float *t, float *b1, float *b2);
 
+/* irtt_trace_rays without any device allocation, for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The rays and answers
+ * are staged through one statically sized device array per device (up to
+ * irtt_trace_rays_noalloc_max() = 262144 rays per call, 10 MiB, held until
+ * the process ends); more rays fail with IRTT_ERR_LIMIT.  The BVH must have
+ * been built (irtt_build_bvh or a transport build): this call does not build
+ * it.  Results equal irtt_trace_rays's bit for bit.  Calls that overlap on
+ * one device share the array and race: serialise them. */
+int irtt_trace_rays_noalloc(irtt_scene h, const float *rays, uint32_t n, float tmin,
+                            float tmax, int32_t *prim, float *t, float *b1, float *b2);
+uint32_t irtt_trace_rays_noalloc_max(void);
+
 /* -------------------------------------------------------------- transport */
 typedef struct irtt_transport_params {
     uint32_t threads_per_tri;   /* ThermalVars_s::threadsPerTriangle (32768) */

@@ -497,6 +497,18 @@ int irtt_trace_rays(irtt_scene h, const float *rays, uint32_t n, float tmin, flo
     });
 }
 
+int irtt_trace_rays_noalloc(irtt_scene h, const float *rays, uint32_t n, float tmin, float tmax,
+                            int32_t *prim, float *t, float *b1, float *b2)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (n && (!rays || !prim || !t || !b1 || !b2)) throw Error(IRTT_ERR_ARG, "null pointer");
+        trace_rays_noalloc(*s, rays, n, tmin, tmax, prim, t, b1, b2);
+    });
+}
+
+uint32_t irtt_trace_rays_noalloc_max(void) { return trace_rays_noalloc_max(); }
+
 int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_transport_stats *st)
 {
     return guard([&] {

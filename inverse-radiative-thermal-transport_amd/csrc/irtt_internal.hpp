@@ -297,6 +297,9 @@ void bvh_build(Scene &s);
 void trace_rays(Scene &s, const float *rays, uint32_t n, float tmin, float tmax, int32_t *prim,
                 float *t, float *b1, float *b2);
 void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_stats &st);
+void trace_rays_noalloc(Scene &s, const float *rays, uint32_t n, float tmin, float tmax,
+                        int32_t *prim, float *t, float *b1, float *b2);
+uint32_t trace_rays_noalloc_max();
 // solver.hip
 void solve(Scene &s, const irtt_solver_params &p, irtt_solve_stats &st);
 void residual_host(Scene &s, const irtt_solver_params &p, const double *x, double *r);

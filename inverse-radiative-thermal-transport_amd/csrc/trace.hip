@@ -16,6 +16,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "device_math.hpp"
 #include "irtt_internal.hpp"
@@ -369,6 +371,84 @@ void trace_rays(Scene &s, const float *rays, uint32_t n, float tmin, float tmax,
     IRTT_HIP(hipMemcpyAsync(t, d_t.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
     IRTT_HIP(hipMemcpyAsync(b1, d_b1.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
     IRTT_HIP(hipMemcpyAsync(b2, d_b2.p, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipStreamSynchronize(st));
+}
+
+// ---- ray queries without allocation -------------------------------------------
+// Staging for irtt_trace_rays_noalloc: the rays of one call (6 floats each)
+// and their answers (prim, t, b1, b2: 4 words each), for up to
+// kNoAllocRays rays, in one statically sized device array (10 MiB; each
+// device holds its own copy until the process ends).  Host code never uses
+// the array's name as an address: it asks the runtime for the device address
+// of this device's copy once and keeps it.
+constexpr uint32_t kNoAllocRays = 1u << 18;
+static __device__ __align__(256) float scratch[10 * (size_t)kNoAllocRays];
+
+namespace {
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// device address of the current device's scratch (looked up on its first use)
+float *scratch_address()
+{
+    int dev = 0;
+    IRTT_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw Error(IRTT_ERR_LIMIT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw Error(IRTT_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(e));
+        g_scratch_dev[dev] = p;
+    }
+    return static_cast<float *>(g_scratch_dev[dev]);
+}
+}  // namespace
+
+uint32_t trace_rays_noalloc_max() { return kNoAllocRays; }
+
+// irtt_trace_rays through the static staging array: no device allocation.
+// The BVH must exist (its build allocates).  Calls that overlap on one device
+// share the array and race.
+void trace_rays_noalloc(Scene &s, const float *rays, uint32_t n, float tmin, float tmax,
+                        int32_t *prim, float *t, float *b1, float *b2)
+{
+    if (!s.bvh.valid)
+        throw Error(IRTT_ERR_STATE, "irtt_trace_rays_noalloc: build the BVH first "
+                                    "(irtt_build_bvh); this entry point allocates nothing");
+    if (n > kNoAllocRays)
+        throw Error(IRTT_ERR_LIMIT, "irtt_trace_rays_noalloc: " + std::to_string(n) +
+                                        " rays exceed the maximum of " +
+                                        std::to_string(kNoAllocRays) + " per call");
+    if (n == 0) return;
+    float *base = scratch_address();
+    float *d_rays = base;
+    int32_t *d_prim = reinterpret_cast<int32_t *>(base + 6 * (size_t)kNoAllocRays);
+    float *d_t = base + 7 * (size_t)kNoAllocRays;
+    float *d_b1 = base + 8 * (size_t)kNoAllocRays;
+    float *d_b2 = base + 9 * (size_t)kNoAllocRays;
+    hipStream_t st = s.stream;
+    IRTT_HIP(hipMemcpyAsync(d_rays, rays, 6 * (size_t)n * sizeof(float), hipMemcpyHostToDevice, st));
+    const int S = stack_size_for(s.bvh.max_depth);
+    dim3 g(cdiv(n, kTraceBlock)), b(kTraceBlock);
+    const float4 *nd = s.bvh.nodes.p, *tr = s.bvh.tris.p;
+    if (S == 32)
+        k_trace_rays<32><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays, n, tmin, tmax, d_prim, d_t,
+                                          d_b1, d_b2);
+    else if (S == 64)
+        k_trace_rays<64><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays, n, tmin, tmax, d_prim, d_t,
+                                          d_b1, d_b2);
+    else
+        k_trace_rays<128><<<g, b, 0, st>>>(nd, tr, s.bvh.n_leaves, d_rays, n, tmin, tmax, d_prim,
+                                           d_t, d_b1, d_b2);
+    IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipMemcpyAsync(prim, d_prim, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(t, d_t, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(b1, d_b1, n * sizeof(float), hipMemcpyDeviceToHost, st));
+    IRTT_HIP(hipMemcpyAsync(b2, d_b2, n * sizeof(float), hipMemcpyDeviceToHost, st));
     IRTT_HIP(hipStreamSynchronize(st));
 }
 

@@ -23,6 +23,16 @@ JAC_EXACT = 0
 JAC_REFERENCE = 1
 
 
+# status codes of the C ABI (include/irtt.h)
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_STATE = -3
+ERR_NOMEM = -4
+ERR_LIMIT = -5
+ERR_NOCONV = -6
+
+
 class IrttError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"irtt error {code}: {msg}")
