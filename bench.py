@@ -44,6 +44,11 @@ VALU_LANE_UTIL_CITY = 164314725944 / (64.0 * 5985472421)
 VALU_PEAK_G = 256 * 2.4
 
 
+def _log(msg):
+    """Progress on stderr (the result line alone goes to stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def _host_cpu():
     """(nproc, cores this process may use, CPU model string)."""
     model = "unknown"
@@ -101,40 +106,52 @@ def cpu_baseline(sd, threads_per_tri, rays_per_thread, check=None, target_s=12.0
 def fp64_check(sd, c):
     """max |dT| of every GPU solve form against an FP64 solve of the same
     stored operator on the CPU (scipy, Jacobi-preconditioned GMRES to a
-    relative residual of 1e-14; oracle/sparse_fp64.py)."""
+    relative residual of 1e-14; oracle/sparse_fp64.py), over the scene's
+    free bins (the sun quads are fixed)."""
     import numpy as np
-    from irtt import scenes
     from oracle import physics, sparse_fp64
     rp, col, val, colsum = c["operator"]
     T = sparse_fp64.to_scipy(rp, col, val, len(colsum))
+    fixed = sd.fixed.astype(bool)
+    free = ~fixed
+    kfix = sd.kelvin.astype(np.float64)
+    src = physics.bins(sd, 0)[3]
     out = {}
     t0 = time.perf_counter()
-    x64, res = sparse_fp64.steady_linear(T, sparse_fp64.rhs(physics.bins(sd, 0)[3], colsum, 293.0))
+    x64, res = sparse_fp64.steady_linear(T, sparse_fp64.rhs(src, colsum, 293.0), fixed, kfix)
     out["steady_solve_s"] = round(time.perf_counter() - t0, 2)
     out["steady_residual"] = float(res)
     for k in ("fp32", "fp16", "sparse"):
         if k in c:
-            out[f"max_dT_{k}_K"] = float(np.abs(c[k] - x64).max())
+            out[f"max_dT_{k}_K"] = float(np.abs(c[k] - x64)[free].max())
     if "batch" in c:
-        fluxes, X = c["batch"]
+        scales, internal, X = c["batch"]
+        sv = np.concatenate([q["vertices"] for q in sd.meta["suns"]])
+        y = np.zeros(len(colsum))
+        y[sv] = kfix[sv] ** 4
+        col64 = -(T @ y)
+        col64[sv] = 0.0
+        k0 = kfix.copy()
+        k0[sv] = 0.0
         worst = 0.0
-        for k in (0, 1, len(fluxes) // 2, len(fluxes) - 1):
-            s2 = _with_flux(sd, fluxes[k])
-            xk, _ = sparse_fp64.steady_linear(T, sparse_fp64.rhs(physics.bins(s2, 0)[3], colsum,
-                                                                 293.0))
-            worst = max(worst, float(np.abs(X[k] - xk).max()))
+        for k in sorted({0, 1, len(scales) // 2, len(scales) - 1}):
+            b = sparse_fp64.rhs(scales[k] * col64 + internal[k], colsum, 293.0)
+            xk, _ = sparse_fp64.steady_linear(T, b, fixed, k0)
+            worst = max(worst, float(np.abs(X[k] - xk)[free].max()))
         out["max_dT_fp16_batch_K"] = worst
     if "transient" in c:
-        fluxes, H, dt = c["transient"]
-        _, _, mass, _ = physics.bins(sd, 0)
-        x = np.full(len(colsum), 293.0)
+        (trp, tcol, tval, tcs), sdt, kel, sv, H, dt = c["transient"]
+        Tt = sparse_fp64.to_scipy(trp, tcol, tval, len(tcs))
+        _, _, mass, srct = physics.bins(sdt, 0)
+        tfixed = sdt.fixed.astype(bool)
+        x = np.where(tfixed, sdt.kelvin.astype(np.float64), 293.0)
         d = []
         t0 = time.perf_counter()
         for k in range(H.shape[0]):
-            src = physics.bins(_with_flux(sd, fluxes[k]), 0)[3]
-            x, _ = sparse_fp64.implicit_euler_step(T, sparse_fp64.rhs(src, colsum, 293.0), mass,
-                                                   x, dt)
-            d.append(float(np.abs(H[k] - x).max()))
+            x[sv] = kel[k]
+            x, _ = sparse_fp64.implicit_euler_step(Tt, sparse_fp64.rhs(srct, tcs, 293.0), mass,
+                                                   x, dt, tfixed)
+            d.append(float(np.abs(H[k] - x)[~tfixed].max()))
         out["transient_steps"] = H.shape[0]
         out["transient_fp64_s_per_step"] = round((time.perf_counter() - t0) / H.shape[0], 2)
         out["max_dT_transient_fp16_per_step_K"] = [float(f"{v:.3e}") for v in d]
@@ -143,12 +160,37 @@ def fp64_check(sd, c):
     return out
 
 
-def _with_flux(sd, flux):
-    import copy
+def _state0(sd, suns=True):
+    """Initial temperatures: fixed bins (the sun quads) at their own values
+    (or 0 K with suns=False, when the sun enters as a per-design source),
+    every other bin 0 K."""
     import numpy as np
-    s2 = copy.copy(sd)
-    s2.heat_flux = np.asarray(flux, np.float32)
-    return s2
+    k = np.where(sd.fixed.astype(bool), sd.kelvin, 0.0).astype(np.float32)
+    if not suns:
+        for q in sd.meta.get("suns", []):
+            k[q["vertices"]] = 0.0
+    return k
+
+
+def design_sources(irtt, s, sd, R, seed=11):
+    """R steady-state designs sharing one operator: design r has the sun at
+    scale_r x its flux (scale_0 = 1, the rest U[0.5, 1.25]) and internal heat
+    gains of U[0, 20] W/m^2 per bin area (none for design 0).  The sun enters
+    as the source -T[:, sun] y_sun it produces (one operator product), with
+    the sun bins held at 0 K.  Returns (sources R x n, scales, internal R x n)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    area, _, _ = s.bin_data()
+    sv = np.concatenate([q["vertices"] for q in sd.meta["suns"]])
+    y = np.zeros(s.n_bins)
+    y[sv] = sd.kelvin[sv].astype(np.float64) ** 4
+    col = -s.apply_operator(y)[0]
+    col[sv] = 0.0
+    scales = np.concatenate([[1.0], rng.uniform(0.5, 1.25, R - 1)])
+    internal = rng.uniform(0.0, 20.0, (R, s.n_bins)) * area[None, :]
+    internal[0] = 0.0
+    internal[:, sv] = 0.0
+    return scales[:, None] * col[None, :] + internal, scales, internal
 
 
 def solve_rates(irtt, s, sd, reps=3, batch=32):
@@ -162,18 +204,20 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14,
                             lin_rtol=1e-10)
     _, ea, _ = s.bin_data()
+    free = ~sd.fixed.astype(bool)
     out = {}
     xs = {}
     for name, op in (("fp32", irtt.OP_DENSE), ("fp16", irtt.OP_DENSE_F16)):
         s.select_operator(op)
         times = []
         for _ in range(reps):
-            s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+            s.set_vertex_state(_state0(sd), sd.fixed)
             t0 = time.perf_counter()
             fst = s.solve(sp)
             x = s.temperatures()
-            # objective gradient of sum_i A_i (x_i - mean x)^2 (thermal_objective.cpp:47-57)
-            rhs = ea * (x - x.mean())
+            # objective gradient of sum_i A_i (x_i - mean x)^2 over the scene's
+            # bins (thermal_objective.cpp:47-57)
+            rhs = np.where(free, ea * (x - x[free].mean()), 0.0)
             lam, ast = s.adjoint_solve(rhs, params=sp)
             times.append(time.perf_counter() - t0)
         xs[name] = x
@@ -184,20 +228,20 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
                      "newton_iters": fst["newton_iters"],
                      "linear_iters": fst["linear_iters"] + ast["linear_iters"],
                      "residual_sq": fst["residual_sq"]}
-    # served form: R designs sharing the operator (the scene's sun plus a sun
-    # sweep) solved in lockstep, forward and adjoint, FP16 MFMA products with
+    # served form: R designs sharing the operator (sun intensity and internal
+    # gains) solved in lockstep, forward and adjoint, FP16 MFMA products with
     # FP32-operator refinement
-    from irtt import scenes
     R = batch
-    fluxes = [sd.heat_flux] + [scenes.sun_flux(sd, 45.0, 360.0 * k / R) for k in range(1, R)]
-    srcs = np.stack([scenes.bin_sources(sd, f) for f in fluxes])
+    s.select_operator(irtt.OP_DENSE)
+    srcs, scales, internal = design_sources(irtt, s, sd, R)
     s.select_operator(irtt.OP_DENSE_F16)
-    s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+    s.set_vertex_state(_state0(sd, suns=False), sd.fixed)
     times = []
     for _ in range(2):
         t0 = time.perf_counter()
         X, fst = s.solve_batch(srcs, params=sp)
-        rhs = ea[None, :] * (X - X.mean(axis=1, keepdims=True))
+        rhs = np.where(free[None, :], ea[None, :] * (X - X[:, free].mean(axis=1, keepdims=True)),
+                       0.0)
         lam, ast = s.adjoint_solve_batch(X, rhs, params=sp)
         times.append(time.perf_counter() - t0)
     t = min(times)
@@ -210,9 +254,9 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
                          "residual_sq_max": fst["residual_sq"]}
     out["x_fp32"] = xs["fp32"]
     out["x_fp16"] = xs["fp16"]
-    out["batch_solutions"] = (fluxes, X)
-    out["max_dT_fp16_vs_fp32_K"] = float(np.abs(xs["fp16"] - xs["fp32"]).max())
-    out["max_dT_fp16_batch_vs_fp32_K"] = float(np.abs(X[0] - xs["fp32"]).max())
+    out["batch_solutions"] = (scales, internal, X)
+    out["max_dT_fp16_vs_fp32_K"] = float(np.abs(xs["fp16"] - xs["fp32"])[free].max())
+    out["max_dT_fp16_batch_vs_fp32_K"] = float(np.abs(X[0] - xs["fp32"])[free].max())
     out["speedup_fp16"] = round(out["fp16"]["solves_per_s"] / out["fp32"]["solves_per_s"], 3)
     out["speedup_fp16_batch"] = round(out["fp16_batch"]["solves_per_s"] /
                                       out["fp32"]["solves_per_s"], 3)
@@ -228,7 +272,6 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
     dense FP32-form solution x_ref."""
     import ctypes
     import numpy as np
-    from irtt import scenes
     q = irtt.TransportParams()
     ctypes.pointer(q)[0] = p
     q.layout = irtt.LAYOUT_SPARSE
@@ -253,44 +296,43 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
                         "operator_MB": round(st["nnz"] * 16 / 1e6, 1),
                         "dense_operator_MB": round(n * ((n + 63) // 64 * 64) * 4 / 1e6, 1)}
         _, ea, _ = s.bin_data()
+        free = ~sd.fixed.astype(bool)
         times = []
         for _ in range(reps):
-            s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+            s.set_vertex_state(_state0(sd), sd.fixed)
             t0 = time.perf_counter()
             fst = s.solve(sp)
             x = s.temperatures()
-            lam, ast = s.adjoint_solve(ea * (x - x.mean()), params=sp)
+            lam, ast = s.adjoint_solve(np.where(free, ea * (x - x[free].mean()), 0.0), params=sp)
             times.append(time.perf_counter() - t0)
         t = min(times)
         out["x"] = x
         out["single"] = {"solves_per_s": round(1.0 / t, 2), "ms": round(t * 1e3, 2),
                          "linear_iters": fst["linear_iters"] + ast["linear_iters"],
-                         "max_dT_vs_dense_fp32_K": float(np.abs(x - x_ref).max())}
-        srcs = [scenes.bin_sources(sd, sd.heat_flux)]
-        for k in range(1, batch):
-            srcs.append(scenes.bin_sources(sd, scenes.sun_flux(sd, 45.0, 360.0 * k / batch)))
-        srcs = np.stack(srcs)
-        s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+                         "max_dT_vs_dense_fp32_K": float(np.abs(x - x_ref)[free].max())}
+        srcs, _, _ = design_sources(irtt, s, sd, batch)
+        s.set_vertex_state(_state0(sd, suns=False), sd.fixed)
         times = []
         for _ in range(2):
             t0 = time.perf_counter()
             X, fst = s.solve_batch(srcs, params=sp)
-            rhs = ea[None, :] * (X - X.mean(axis=1, keepdims=True))
+            rhs = np.where(free[None, :],
+                           ea[None, :] * (X - X[:, free].mean(axis=1, keepdims=True)), 0.0)
             lam, ast = s.adjoint_solve_batch(X, rhs, params=sp)
             times.append(time.perf_counter() - t0)
         t = min(times)
         out["batch"] = {"solves_per_s": round(batch / t, 2), "ms": round(t * 1e3, 2),
                         "designs": batch,
                         "linear_iters": fst["linear_iters"] + ast["linear_iters"],
-                        "max_dT_vs_dense_fp32_K": float(np.abs(X[0] - x_ref).max())}
+                        "max_dT_vs_dense_fp32_K": float(np.abs(X[0] - x_ref)[free].max())}
         # adjoint trace of the same photons (dL/dvertex with the rows
         # (lambda, x^4) and (-Ta^4, lambda)), absorbers kept by the build
         q.flags = irtt.TRANSPORT_KEEP_ABSORBERS
         s.build_transport(q)
-        s.set_vertex_state(np.zeros_like(sd.kelvin), sd.fixed)
+        s.set_vertex_state(_state0(sd), sd.fixed)
         s.solve(sp)
         x = s.temperatures()
-        lam, _ = s.adjoint_solve(ea * (x - x.mean()), params=sp)
+        lam, _ = s.adjoint_solve(np.where(free, ea * (x - x[free].mean()), 0.0), params=sp)
         rows_l = np.stack([lam, np.full(len(x), -293.0 ** 4)])
         rows_x = np.stack([x ** 4, lam])
         tr = []
@@ -309,103 +351,177 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
     return out
 
 
-def facade(irtt, fd_params=8, h=1e-3, gd_steps=10, threads_per_tri=336, rays_per_thread=8):
-    """Procedural facade: 20 buildings, 401k triangles, 363k vertex bins,
-    256 balcony / fin extrusion depths; 336 x 8 = 2688 photons per triangle
-    (1.08e9 ~ 2^30) into the sparse layout; steady solve; design gradient of
-    the mean ground temperature (adjoint solve, adjoint trace, source term,
-    projection); central differences (h, common random numbers) on
-    `fd_params` depths; `gd_steps` steps of normalised gradient descent."""
+def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thread=8,
+           step_size=0.05, fd_params=(6, 93, 162, 167, 0, 109, 240, 255)):
+    """Procedural facade (config 3): 20 buildings, 401k triangles plus the
+    collimated sun quad (45 deg elevation, 135 deg azimuth, 800 W/m^2) so
+    that balconies and fins shade walls and ground; 256 balcony / fin
+    extrusion depths; about 2^30 photons into the sparse layout; steady
+    solve; design gradient of the mean ground temperature (adjoint solve,
+    adjoint trace, source term, projection) with the reference's terms and
+    with the extensions (hit-area Jacobian, projected edge factor, beam
+    transfer); central differences with common random numbers at h (the
+    configured step) and h_wide on `fd_params` depths (balconies of sunlit
+    +y / -x facades, of shaded -y facades, fins); then `gd_steps` Adam steps
+    on the depths (projected onto [0, 1.5] m)."""
     import numpy as np
-    from irtt import scenes
+    from irtt import params as P, scenes
     from irtt.design import FacadeDesign
     t0 = time.perf_counter()
-    sd = scenes.facade()
+    _log("facade: generating the scene")
+    sd = scenes.facade(sun_model="emitter")
     gen_s = time.perf_counter() - t0
-    tp = irtt.transport_params(threads_per_tri, rays_per_thread, batch_seed=1234,
+    tpt = max(1, int(round(photons / (sd.nt * rays_per_thread))))
+    tp = irtt.transport_params(tpt, rays_per_thread, batch_seed=1234,
                                layout=irtt.LAYOUT_SPARSE, flags=irtt.TRANSPORT_KEEP_ABSORBERS)
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-12,
                             lin_rtol=1e-10)
-    out = {"triangles": sd.nt, "bins": sd.nv, "params": int(sd.meta["n_params"]),
-           "photons": threads_per_tri * rays_per_thread * sd.nt,
+    ref_flags = irtt.ADJ_AREA | irtt.ADJ_CONTINUOUS | irtt.ADJ_EDGE
+    ext_flags = ref_flags | irtt.ADJ_HIT_AREA | irtt.ADJ_EDGE_PROJECTED | irtt.ADJ_COLLIMATED
+    out = {"triangles": sd.nt, "scene_triangles": sd.meta["scene_triangles"], "bins": sd.nv,
+           "params": int(sd.meta["n_params"]), "photons": tpt * rays_per_thread * sd.nt,
            "generate_s": round(gen_s, 2)}
-    gain = irtt.ADJ_AREA | irtt.ADJ_CONTINUOUS | irtt.ADJ_EDGE | irtt.ADJ_EDGE_GAIN
-    with FacadeDesign(irtt, sd, tp, sp, adjoint_flags=gain) as fz:
+    with FacadeDesign(irtt, sd, tp, sp, adjoint_flags=ref_flags) as fz:
         d0 = fz.depths.copy()
         fz.forward(d0)
-        L0, x = fz.forward(d0)
+        L0 = fz.forward(d0)
         fwd = dict(fz.timing)
-        dp_ref = fz.gradient(flags=irtt.ADJ_AREA | irtt.ADJ_CONTINUOUS | irtt.ADJ_EDGE)
-        dp = fz.gradient()
+        dp_ext = fz.gradient(flags=ext_flags)
+        dp_ref = fz.gradient(flags=ref_flags)
         out.update({"objective_K": round(L0, 6), "forward": {k: (round(v, 2) if isinstance(
             v, float) else v) for k, v in fwd.items()},
             "gradient_ms": round(fz.timing["gradient_ms"], 1)})
-        ks = np.unique(np.linspace(0, sd.meta["n_params"] - 1, fd_params).astype(int))
-        rows = []
-        for k in ks:
-            fd = fz.central_difference(d0, int(k), h)
-            rows.append({"param": int(k), "fd": round(float(fd), 5),
-                         "adjoint_reference_terms": round(float(dp_ref[k]), 5),
-                         "adjoint_with_edge_gain": round(float(dp[k]), 5)})
-        fdv = np.array([r["fd"] for r in rows])
+        ks = [int(k) for k in fd_params]
+        _log(f"facade: objective {L0:.4f} K, gradient {fz.timing['gradient_ms']:.0f} ms; "
+             f"FD on {len(ks)} depths")
+        fd = fz.fd_gradient(d0, h, dims=ks)
+        fdw = fz.fd_gradient(d0, h_wide, dims=ks)
+        kind = {k: ("fin" if k >= 240 else
+                    ("sunlit balcony" if (k // 3) % 4 in (2, 3) else "shaded balcony"))
+                for k in ks}
+        rows = [{"param": k, "kind": kind[k], "fd": round(float(fd[k]), 5),
+                 "fd_wide": round(float(fdw[k]), 5),
+                 "adjoint_reference_terms": round(float(dp_ref[k]), 5),
+                 "adjoint_extended": round(float(dp_ext[k]), 5)} for k in ks]
+
+        def agree(g, f):
+            return int(np.sum(np.sign(g[ks]) == np.sign(f[ks])))
         out["fd_check"] = {
-            "h_m": h, "rows": rows,
-            "sign_agreement_reference_terms": int(np.sum(np.sign(dp_ref[ks]) == np.sign(fdv))),
-            "sign_agreement_with_edge_gain": int(np.sum(np.sign(dp[ks]) == np.sign(fdv))),
-            "note": "central differences with common random numbers (same seed and photon "
-                    "counts); the adjoint is the reference's approximate edge-sampling "
-                    "estimator, with and without the occluding-side edge term"}
+            "h_m": h, "h_wide_m": h_wide, "rows": rows,
+            "sign_agreement_reference_terms": {"fd": agree(dp_ref, fd),
+                                               "fd_wide": agree(dp_ref, fdw)},
+            "sign_agreement_extended": {"fd": agree(dp_ext, fd), "fd_wide": agree(dp_ext, fdw)},
+            "note": "extended = reference terms + hit-triangle area Jacobian + projected edge "
+                    "factor + beam transfer of the sun's first segments (IRTT_ADJ_HIT_AREA, "
+                    "IRTT_ADJ_EDGE_PROJECTED, IRTT_ADJ_COLLIMATED); FD with common random "
+                    "numbers (same seed and photon counts)"}
+        # the net derivative is a small difference (a deeper sunlit balcony
+        # gains what the wall below loses); the descent uses the reference's
+        # terms, whose omissions both point the same way (no occluder gain,
+        # too small grazing edge factor)
         t1 = time.perf_counter()
-        hist, best = fz.descend(gd_steps)
-        out["descent"] = {"steps": gd_steps, "objective_K": [round(float(v), 5) for v in hist],
-                          "s_per_step": round((time.perf_counter() - t1) / max(gd_steps, 1), 2),
-                          "gradient": "with edge gain", "max_step_m": 0.05}
+        opt = P.Adam(step_size=step_size)
+        ms = {"transport_ms": [], "solve_ms": [], "gradient_ms": []}
+
+        def log(k, L, timing):
+            for key in ms:
+                ms[key].append(timing[key])
+            if k % 10 == 0:
+                _log(f"facade descent step {k}: objective {L:.4f} K")
+        hist = fz.run(d0, opt, gd_steps, log=log, lo=0.0, hi=1.5)
+        dt = time.perf_counter() - t1
+        L = hist["objective"]
+        out["descent"] = {"optimizer": "Adam", "step_size_m": step_size, "steps": gd_steps,
+                          "gradient": "reference terms",
+                          "objective_K_every_5": [round(float(v), 5) for v in L[::5]],
+                          "start_K": round(float(L[0]), 5), "last_K": round(float(L[-1]), 5),
+                          "best_K": round(float(opt.best_value), 5),
+                          "s_per_step": round(dt / max(gd_steps, 1), 2),
+                          "mean_ms": {k: round(float(np.mean(v)), 1) for k, v in ms.items()},
+                          "depth_bounds_m": [0.0, 1.5]}
     return out
 
 
-def transient(irtt, s, sd, steps=288, dt=300.0, check_steps=6):
-    """The city block over 24 h: `steps` implicit-Euler steps of dt with the
-    sun sweeping east to west (elevation 60 sin(pi (t - 6 h) / 12 h), no sun
-    at night), per-patch areal heat capacity U[1e4, 1e5] J/(m^2 K).  FP16 form
-    (FP16 Krylov products, FP32-operator residuals with FP64 accumulation)
-    against the FP32 form; max |dT| between them per step."""
+def transient(irtt, seed, p, steps=288, dt=300.0, check_steps=6, site=(48.21, 16.37, 2),
+              date=(2024, 6, 21)):
+    """The city block over 24 h: `steps` implicit-Euler steps of dt, per-patch
+    areal heat capacity U[1e4, 1e5] J/(m^2 K), ambient 293 K.  The sun moves
+    over the day: one collimated sun quad per daylight hour at the solar
+    position of `site` (latitude, east longitude, UTC offset) on `date`
+    (irtt.sun.solar_position), all in one operator; at time t the quads of
+    the two enclosing hours emit the clear-sky direct irradiance DNI(t)
+    split linearly between them (their fixed temperatures change between
+    steps, the operator does not).  FP16 form (FP16 Krylov products,
+    FP32-operator residuals with FP64 accumulation) against the FP32 form,
+    max |dT| between them per step."""
     import numpy as np
-    from irtt import scenes
-    srcs, fluxes = [], []
+    from irtt import scenes, sun
+    lat, lon, utc = site
+    base = scenes.city_block(seed=seed, sun_model=None)
+    hours, suns = [], []
+    for h in range(24):
+        el, az, zen = sun.solar_position(lat, lon, utc, *date, h)
+        if el > 1.0:
+            hours.append(h)
+            suns.append((sun.sun_direction(el, az, compass=True), sun.clear_sky_dni(zen)))
+    per_sun = 2 * int(round(base.nt / (4 * len(hours))))
+    sd = sun.add_suns(base, suns, triangles_per_sun=per_sun)
+    hidx = {h: i for i, h in enumerate(hours)}
+    fluxes = np.zeros((steps, len(hours)))
     for k in range(steps):
-        th = (k + 1) * dt / 3600.0
-        el = 60.0 * np.sin(np.pi * (th - 6.0) / 12.0)
-        az = 90.0 + 15.0 * (th - 6.0)
-        flux = scenes.sun_flux(sd, el, az) if el > 0 else np.zeros(sd.nt, np.float32)
-        fluxes.append(flux)
-        srcs.append(scenes.bin_sources(sd, flux))
-    p = irtt.solver_params(steady=False, dt=dt, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-14,
-                           lin_rtol=1e-8)
+        t = (k + 1) * dt / 3600.0
+        h0 = int(np.floor(t))
+        el, _, zen = sun.solar_position(lat, lon, utc, *date, h0 % 24, (t - h0) * 60.0)
+        if el <= 0.0:
+            continue
+        dni, w = sun.clear_sky_dni(zen), t - h0
+        for h, wt in ((h0, 1.0 - w), (h0 + 1, w)):
+            if h in hidx:
+                fluxes[k, hidx[h]] += wt * dni
+    sv = np.concatenate([q["vertices"] for q in sd.meta["suns"]])
+    kel = np.stack([sun.sun_vertex_kelvin(sd, f)[1] for f in fluxes])
+    sp = irtt.solver_params(steady=False, dt=dt, ambient_kelvin=293.0, x_tol=1e-6, f_tol=1e-14,
+                            lin_rtol=1e-8)
     out, hist = {}, {}
-    for name, op in (("fp32", irtt.OP_DENSE), ("fp16", irtt.OP_DENSE_F16)):
-        s.select_operator(op)
-        s.set_vertex_state(np.full(sd.nv, 293.0, np.float32), sd.fixed)
-        H = np.empty((steps, s.n_bins))
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
         t0 = time.perf_counter()
-        iters = 0
-        for k in range(steps):
-            s.set_source(srcs[k])
-            st = s.solve(p)
-            iters += st["linear_iters"]
-            H[k] = s.temperatures()
-        t = time.perf_counter() - t0
-        hist[name] = H
-        out[name] = {"ms_per_step": round(t * 1e3 / steps, 2), "linear_iters": iters}
-    s.select_operator(irtt.OP_DENSE)
-    out["first_steps"] = (fluxes[:check_steps], hist["fp16"][:check_steps], dt)
-    d = np.abs(hist["fp16"] - hist["fp32"]).max(axis=1)
-    out.update({"steps": steps, "dt_s": dt,
+        st = s.build_transport(p)
+        out["build_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        operator = s.transport_sparse() + (s.column_sums(),)
+        for name, op in (("fp32", irtt.OP_DENSE), ("fp16", irtt.OP_DENSE_F16)):
+            s.select_operator(op)
+            s.set_vertex_state(np.where(sd.fixed.astype(bool), sd.kelvin, 293.0)
+                               .astype(np.float32), sd.fixed)
+            H = np.empty((steps, s.n_bins))
+            t0 = time.perf_counter()
+            iters = 0
+            for k in range(steps):
+                # this step's sun temperatures (fixed bins), the rest of the
+                # state carried over exactly
+                x = s.temperatures()
+                x[sv] = kel[k]
+                s.set_vertex_state(x.astype(np.float32), sd.fixed)
+                s.set_temperatures(x)
+                r = s.solve(sp)
+                iters += r["linear_iters"]
+                H[k] = s.temperatures()
+            t = time.perf_counter() - t0
+            hist[name] = H
+            out[name] = {"ms_per_step": round(t * 1e3 / steps, 2), "linear_iters": iters}
+    free = ~sd.fixed.astype(bool)
+    out["first_steps"] = (operator, sd, kel[:check_steps], sv, hist["fp16"][:check_steps], dt)
+    d = np.abs(hist["fp16"] - hist["fp32"])[:, free].max(axis=1)
+    out.update({"steps": steps, "dt_s": dt, "site": {"lat": lat, "lon": lon, "utc": utc,
+                                                      "date": "%04d-%02d-%02d" % date},
+                "sun_hours": hours, "triangles": sd.nt, "sun_triangles": per_sun * len(hours),
+                "photons": st["photons"],
                 "max_dT_fp16_vs_fp32_K": float(d.max()),
                 "max_dT_per_step_K": [float(f"{v:.3e}") for v in d[::24]],
-                "T_range_K": [round(float(hist["fp32"].min()), 2),
-                              round(float(hist["fp32"].max()), 2)],
-                "unit": "implicit-Euler steps of the city block, FP32-form solution = FP64-"
-                        "accumulated residual of the FP32 operator (max_dT_per_step every 2 h)"})
+                "T_range_K": [round(float(hist["fp32"][:, free].min()), 2),
+                              round(float(hist["fp32"][:, free].max()), 2)],
+                "unit": "implicit-Euler steps of the city block with a moving sun, FP32-form "
+                        "solution = FP64-accumulated residual of the FP32 operator "
+                        "(max_dT_per_step every 2 h)"})
     return out
 
 
@@ -486,15 +602,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--threads-per-tri", type=int, default=162)
+    ap.add_argument("--threads-per-tri", type=int, default=130)
     ap.add_argument("--rays-per-thread", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-transient", action="store_true")
     ap.add_argument("--no-sparse", action="store_true")
-    ap.add_argument("--facade", action="store_true",
-                    help="add the 400k-triangle facade design section (about a minute)")
-    ap.add_argument("--facade-fd", type=int, default=8)
-    ap.add_argument("--facade-gd", type=int, default=10)
+    ap.add_argument("--no-facade", action="store_true",
+                    help="skip the 400k-triangle facade design section (about two minutes)")
+    ap.add_argument("--facade-gd", type=int, default=50)
     ap.add_argument("--facade-only", action="store_true",
                     help="run only the facade section and print it")
     ap.add_argument("--serve-requests", type=int, default=16)
@@ -516,10 +631,10 @@ def main():
     from irtt import scenes
 
     if args.facade_only:
-        print(json.dumps({"facade": facade(irtt, fd_params=args.facade_fd,
-                                           gd_steps=args.facade_gd)}), flush=True)
+        print(json.dumps({"facade": facade(irtt, gd_steps=args.facade_gd)}), flush=True)
         return
-    sd = scenes.city_block(seed=args.seed)
+    _log("city block: scene and warm-up builds")
+    sd = scenes.city_block(seed=args.seed, sun_model="emitter")
     s = irtt.Scene.from_data(sd, irtt.BIN_VERTEX)
     s.build_bvh()
     p = irtt.transport_params(args.threads_per_tri, args.rays_per_thread, batch_seed=1234)
@@ -542,7 +657,7 @@ def main():
     b_launch = rays * (RAY_BYTES + n_node * NODE_BYTES + n_tri * TRI_BYTES) + \
         (st["absorbed"] + st["forced_absorb"]) * BIN_BYTES
     achieved = b_launch / (trace_ms * 1e-3) / 1e9
-    default = args.seed == 0 and args.threads_per_tri == 162 and args.rays_per_thread == 8
+    default = args.seed == 0 and args.threads_per_tri == 130 and args.rays_per_thread == 8
     # the photon kernel is bound by VALU issue (lanes of a wave diverge
     # between node visits, leaf tests and shading): achieved = its VALU
     # wave-instructions per second against the issue peak; the HBM figures
@@ -570,17 +685,20 @@ def main():
     sp = irtt.solver_params(steady=True, ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14)
     s.set_vertex_state(sd.kelvin, sd.fixed)
     sol = s.solve(sp)
+    _log(f"transport {ms_per_step:.2f} ms per build; solves")
     solves = solve_rates(irtt, s, sd)
     x_fp32 = solves.pop("x_fp32")
     check["fp32"] = x_fp32
     check["fp16"] = solves.pop("x_fp16")
     check["batch"] = solves.pop("batch_solutions")
-    trans = None if args.no_transient else transient(irtt, s, sd)
+    _log("transient")
+    trans = None if args.no_transient else transient(irtt, args.seed, p)
     if trans is not None:
         check["transient"] = trans.pop("first_steps")
     sparse = None
     if not args.no_sparse:
         s.close()  # the dense operator (16 GB with its accumulators) is no longer needed
+        _log("sparse layout")
         sparse = sparse_layout(irtt, sd, p, x_fp32)
         check["sparse"] = sparse.pop("x")
     line = {
@@ -596,10 +714,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "city block 8x8 buildings, seed %d, %d triangles, %d vertex bins, "
-                               "%d photons (%d per triangle), forward transport build"
-                               % (args.seed, sd.nt, s.n_bins, st["photons"],
-                                  args.threads_per_tri * args.rays_per_thread),
+        "config": {"workload": "city block 8x8 buildings, seed %d, %d triangles (%d scene + %d "
+                               "of the collimated sun quad at 45 deg elevation, 800 W/m^2), "
+                               "%d vertex bins, %d photons (%d per triangle), forward transport "
+                               "build" % (args.seed, sd.nt, sd.meta["scene_triangles"],
+                                          sd.nt - sd.meta["scene_triangles"], s.n_bins,
+                                          st["photons"],
+                                          args.threads_per_tri * args.rays_per_thread),
                    "bins": s.n_bins, "triangles": sd.nt, "photons": st["photons"],
                    "ray_segments": rays},
         "roofline": roofline,
@@ -624,11 +745,14 @@ def main():
         line["transient"] = trans
     if sparse is not None:
         line["sparse"] = sparse
-    if args.facade:
-        line["facade"] = facade(irtt, fd_params=args.facade_fd, gd_steps=args.facade_gd)
+    if not args.no_facade:
+        s.close()
+        line["facade"] = facade(irtt, gd_steps=args.facade_gd)
     if not args.no_serving:
+        _log("serving")
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
     if not args.no_cpu_baseline:
+        _log("CPU baseline and FP64 checks")
         line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread,
                                             check=check)
         fp = line["cpu_baseline"]["fp64"]

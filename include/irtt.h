@@ -380,6 +380,15 @@ int irtt_adjoint_solve(irtt_scene h, const irtt_solver_params *p, const double *
                                           plane, seen along the ray; equal at normal incidence,
                                           the reference's factor is (da . n_edge)^2 too small
                                           at grazing incidence */
+#define IRTT_ADJ_COLLIMATED 128u       /* extension, off by default: on the first segment of a
+                                          photon from a collimated (non-diffuse) emitter, such
+                                          as the sun quad, the continuous term differentiates
+                                          the beam's transfer, the cosine at the receiver
+                                          -(N_j . n_i) / |N_j|, instead of the point-to-point
+                                          geometric term (cos cos / r^2,
+                                          thermal_tracing_unified.glsl:391-448), whose distance
+                                          and emitter-cosine factors a parallel beam does not
+                                          have */
 
 typedef struct irtt_adjoint_params {
     float edge_ratio;   /* ThermalVars_s::gradEdgeRatio (0.05), thermal_common.hpp:124 */

@@ -132,6 +132,40 @@ __device__ __forceinline__ float geom_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3
     return G;
 }
 
+// The collimated beam's transfer from emitter triangle i (photons leave along
+// its unit normal n_i) to receiver triangle j: B = -(N_j . n_i) / |N_j|, the
+// cosine at the receiver, and d ln|B| / d{six vertices} through n_i on i's
+// corners and through N_j, |N_j| on j's (a parallel beam has no distance or
+// emitter-cosine factor).  Same emit() and gate as geom_grad.
+template <class Emit>
+__device__ __forceinline__ float beam_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3 Cj, float eps,
+                                           Emit emit)
+{
+    V3 e1i = Bi - Ai, e2i = Ci - Ai, e1j = Bj - Aj, e2j = Cj - Aj;
+    V3 Ni = cross3(e1i, e2i), Nj = cross3(e1j, e2j);
+    float li = sqrt_rn(dot3(Ni, Ni)), lj = sqrt_rn(dot3(Nj, Nj));
+    V3 ni = scl3(Ni, 1.0f / li), nj = scl3(Nj, 1.0f / lj);
+    float cij = dot3(Ni, Nj), cn = dot3(Nj, ni);
+    float B = -cn / lj;
+    if (!(fabsf(B) > eps * eps)) return B;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+        // side 0: d(N_i . N_j) with N_j held; side 1: d(N_j . n_i) with n_i held
+        const V3 e1 = side ? e1j : e1i, e2 = side ? e2j : e2i, n = side ? nj : ni;
+        const V3 av = side ? ni : Nj;
+        const float ic = 1.0f / (side ? cn : cij), il = 1.0f / (side ? lj : li);
+        const V3 e2xa = cross3(e2, av), axe1 = cross3(av, e1);
+        const V3 ab = cross3(e2, n), ac = cross3(n, e1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const V3 dN = c == 0 ? scl3(add3(e2xa, axe1), -1.0f) : (c == 1 ? e2xa : axe1);
+            const V3 dL = c == 0 ? scl3(add3(ab, ac), -1.0f) : (c == 1 ? ab : ac);
+            emit(3 * side + c, scl3(dN, ic) - scl3(dL, il));
+        }
+    }
+    return B;
+}
+
 // ---- exact gradient sums ---------------------------------------------------
 // One gradient component is a signed 128-bit integer in units of q (a power
 // of two): the pair {lo, hi} of 64-bit words, value hi * 2^64 + lo.  v is
@@ -617,17 +651,23 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             const float wt = a.w[t];
             if (f_cont) {
                 const int lo_v = sa.side == 0 ? 0 : 3;
-                const float G = geom_grad(
-                    load_v(a.xyz, i0), load_v(a.xyz, i1), load_v(a.xyz, i2), q0, q1, q2, r.r1.x,
-                    r.r1.y, r.r1.z, r.r1.w, a.eps, [&](int v, V3 gv) {
-                        if (v < lo_v || v >= lo_v + 3) return;
-                        const V3 dv = scl3(gv, wt);
-                        if (!finite3(dv)) return;
-                        const int c = v - lo_v;
-                        fx_quant(fpath * (double)dv.x, a.inv_q, lo[3 * c], hi[3 * c]);
-                        fx_quant(fpath * (double)dv.y, a.inv_q, lo[3 * c + 1], hi[3 * c + 1]);
-                        fx_quant(fpath * (double)dv.z, a.inv_q, lo[3 * c + 2], hi[3 * c + 2]);
-                    });
+                auto emit = [&](int v, V3 gv) {
+                    if (v < lo_v || v >= lo_v + 3) return;
+                    const V3 dv = scl3(gv, wt);
+                    if (!finite3(dv)) return;
+                    const int c = v - lo_v;
+                    fx_quant(fpath * (double)dv.x, a.inv_q, lo[3 * c], hi[3 * c]);
+                    fx_quant(fpath * (double)dv.y, a.inv_q, lo[3 * c + 1], hi[3 * c + 1]);
+                    fx_quant(fpath * (double)dv.z, a.inv_q, lo[3 * c + 2], hi[3 * c + 2]);
+                };
+                // extension: a collimated emitter's first segment carries the
+                // beam's transfer
+                const bool beam = (a.aflags & IRTT_ADJ_COLLIMATED) && r.r2.z != 0 &&
+                                  !(a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION);
+                const V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
+                const float G = beam ? beam_grad(p0, p1, p2, q0, q1, q2, a.eps, emit)
+                                     : geom_grad(p0, p1, p2, q0, q1, q2, r.r1.x, r.r1.y, r.r1.z,
+                                                 r.r1.w, a.eps, emit);
                 if (sa.side == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
             }
             if (f_hit) {

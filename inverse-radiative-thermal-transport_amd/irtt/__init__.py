@@ -230,6 +230,7 @@ ADJ_REFERENCE_DROPPED = 8
 ADJ_EDGE_GAIN = 16
 ADJ_HIT_AREA = 32
 ADJ_EDGE_PROJECTED = 64
+ADJ_COLLIMATED = 128
 
 
 class AdjointParams(ctypes.Structure):

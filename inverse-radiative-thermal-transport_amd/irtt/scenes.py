@@ -12,6 +12,8 @@
   plate: a straight shadow edge, for checking the edge-sampling adjoint.
 * ``fin_plates`` -- the same with a vertical fin that the emitter's rays
   reach at grazing incidence.
+* ``balcony_wall`` -- a wall with one balcony under a collimated sun: the
+  balcony's shadow on the wall, for the adjoint of collimated emitters.
 
 Each returns a ``SceneData`` with per-triangle materials and per-vertex
 initial / fixed temperatures, the per-object inputs of the reference's
@@ -189,6 +191,48 @@ def fin_plates(n=16, fin_x=0.5, fin_z=(0.05, 0.3), fin_y=(-0.5, 1.5), fin_cells=
     return sd
 
 
+def balcony_wall(depth=1.0, sun_elevation_deg=45.0, sun_azimuth_deg=0.0, sun_flux=800.0,
+                 sun_triangles=1024, n=16):
+    """A sunlit wall with one balcony: ground [0,4] x [-2,2] at z=0 facing
+    +z, a wall in the plane x=0 facing +x (4 m wide, 4 m high), and a
+    balcony slab (z 2-2.3 m, y -1..1 m, x 0..depth) as an open box of top,
+    bottom, front and side faces, under a collimated sun quad (azimuth
+    counter-clockwise from +x, so 0 faces the wall).  Emissivity 1.  Growing
+    the depth moves the balcony's shadow down the wall: the balcony gains what
+    the wall loses.  meta: ``owner`` per vertex (0 ground, 1 wall, 2
+    balcony, -1 sun), ``depth_dir`` (d x / d depth per vertex)."""
+    from .sun import add_suns, sun_direction
+    z0, z1, y0, y1 = 2.0, 2.3, -1.0, 1.0
+    ex, ez = np.array([1.0, 0, 0]), np.array([0, 0, 1.0])
+    parts = [(grid_patch([0, -2, 0], [4, 0, 0], [0, 4, 0], n, n), 0),
+             (_oriented_patch([0, -2, 0], [0, 4, 0], [0, 0, 4], n, n, ex), 1),
+             (_oriented_patch([0, y0, z1], [depth, 0, 0], [0, y1 - y0, 0], 4, 8, ez), 2),
+             (_oriented_patch([0, y0, z0], [depth, 0, 0], [0, y1 - y0, 0], 4, 8, -ez), 2),
+             (_oriented_patch([depth, y0, z0], [0, y1 - y0, 0], [0, 0, z1 - z0], 8, 2, ex), 2),
+             (_oriented_patch([0, y0, z0], [depth, 0, 0], [0, 0, z1 - z0], 4, 2,
+                              np.array([0, -1.0, 0])), 2),
+             (_oriented_patch([0, y1, z0], [depth, 0, 0], [0, 0, z1 - z0], 4, 2,
+                              np.array([0, 1.0, 0])), 2)]
+    xyz, idx, owner, off = [], [], [], 0
+    for (x, t), o in parts:
+        xyz.append(x)
+        idx.append(t + off)
+        owner.append(np.full(x.shape[0], o, np.int32))
+        off += x.shape[0]
+    xyz = np.concatenate(xyz)
+    owner = np.concatenate(owner)
+    sd = _uniform_scene("balcony_wall", xyz, np.concatenate(idx))
+    ddir = np.zeros((xyz.shape[0], 3), np.float32)
+    bal = owner == 2
+    ddir[bal, 0] = xyz[bal, 0] / depth
+    sd.meta.update(owner=owner, depth_dir=ddir, depth=depth)
+    sd = add_suns(sd, [(sun_direction(sun_elevation_deg, sun_azimuth_deg), sun_flux)],
+                  triangles_per_sun=sun_triangles)
+    sd.meta["owner"] = np.concatenate([owner, np.full(4, -1, np.int32)])
+    sd.meta["depth_dir"] = np.concatenate([ddir, np.zeros((4, 3), np.float32)])
+    return sd
+
+
 def closed_box(n=4, emissivity=1.0):
     """Inside of the unit cube, every face subdivided n x n, normals inward."""
     faces = [  # normal = u x v
@@ -212,12 +256,16 @@ def closed_box(n=4, emissivity=1.0):
 def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fmax=20.0,
                hmin=6.0, hmax=40.0, ground_cell=2.5, sun_elevation_deg=45.0,
                sun_azimuth_deg=135.0, sun_flux=800.0, eps_range=(0.85, 0.95),
-               alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5), kelvin=293.0):
+               alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5), kelvin=293.0,
+               sun_model="flux", sun_triangles=None):
     """Synthetic city block; returns SceneData with meta on the layout.
 
     Footprints are multiples of ``ground_cell`` so walls meet the ground grid
-    exactly.  Patch = one quad (two triangles).  The direct sun term is the
-    unshadowed absorbed flux alpha * flux * max(0, n.s) per patch."""
+    exactly.  Patch = one quad (two triangles).  Direct sun (sun_model):
+    "emitter" -- a collimated sun quad traced through the scene (sun.add_suns,
+    sun_triangles repeated triangles; buildings shade each other and the
+    ground); "flux" -- the unshadowed absorbed flux alpha * flux * max(0, n.s)
+    per patch as an external source; None -- no sun."""
     rng = np.random.default_rng(seed)
     pitch = fmax + street
     gcells_per_pitch = int(round(pitch / ground_cell))
@@ -288,13 +336,23 @@ def city_block(seed=0, nx=8, ny=8, target_tris=50000, street=5.0, fmin=10.0, fma
     sun = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
     sd = _uniform_scene("city_block", xyz, idx, rho_d=1.0 - eps, heat_capacity=cap,
                         kelvin=kelvin)
-    cosi = np.clip(sd.normals() @ sun, 0.0, None)
-    sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
+    if sun_model == "flux":
+        cosi = np.clip(sd.normals() @ sun, 0.0, None)
+        sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
     areas = sd.triangle_areas()
     sd.meta.update(seed=seed, buildings=len(bld), edge=float(edge), sun=sun.tolist(),
                    mean_triangle_area=float(areas.mean()), total_area=float(areas.sum()),
                    alpha=alpha.astype(np.float32), vertex_building=np.concatenate(owner),
-                   heights=np.array([b[4] for b in bld]))
+                   heights=np.array([b[4] for b in bld]), sun_model=sun_model)
+    return _with_sun(sd, sun_model, sun, sun_flux, sun_triangles)
+
+
+def _with_sun(sd, sun_model, direction, flux, triangles):
+    if sun_model == "emitter":
+        from .sun import add_suns
+        return add_suns(sd, [(direction, flux)], triangles_per_sun=triangles)
+    if sun_model not in ("flux", None):
+        raise ValueError(f"unknown sun model {sun_model!r}")
     return sd
 
 
@@ -311,7 +369,7 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
            depth_range=(0.3, 1.2), balcony_cells=6, depth_cells=2, fin_cell=1.0,
            eps_range=(0.85, 0.95), alpha_range=(0.3, 0.8), capacity_range=(1e4, 1e5),
            sun_elevation_deg=45.0, sun_azimuth_deg=135.0, sun_flux=800.0, kelvin=293.0,
-           weld=False):
+           weld=False, sun_model="flux", sun_triangles=None):
     """Procedural facade scene: nx x ny box buildings (footprint width x
     depth, heights U[hmin, hmax]) on a ground plane of 1 m cells; every
     facade carries balconies (slabs of thickness `slab`, one per bay and
@@ -326,7 +384,9 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
     fin_cell rows), so the mesh topology does not depend on the depths; walls
     and roofs are subdivided to reach about target_tris triangles.  weld:
     merge coincident vertices (faces meeting at an edge share its vertices;
-    otherwise every planar patch has its own).
+    otherwise every planar patch has its own).  sun_model, sun_triangles: as
+    for city_block (with "emitter" the balconies and fins shade the walls
+    and the ground in the forward solve and in the adjoint trace).
 
     meta: ``param_vertex``, ``param_index``, ``param_dir`` (d x / d depth,
     per element vertex), ``depths`` (initial), ``base_xyz`` (positions at
@@ -459,8 +519,9 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
     el, az = np.radians(sun_elevation_deg), np.radians(sun_azimuth_deg)
     sun = np.array([np.cos(el) * np.cos(az), np.cos(el) * np.sin(az), np.sin(el)])
     sd = _uniform_scene("facade", xyz, idx, rho_d=1.0 - eps, heat_capacity=cap, kelvin=kelvin)
-    cosi = np.clip(sd.normals() @ sun, 0.0, None)
-    sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
+    if sun_model == "flux":
+        cosi = np.clip(sd.normals() @ sun, 0.0, None)
+        sd.heat_flux = (alpha * sun_flux * cosi).astype(np.float32)
     v = v.astype(np.uint32)
     params = params.astype(np.uint32)
     base = np.asarray(xyz, np.float64).copy()
@@ -471,8 +532,8 @@ def facade(seed=0, nx=5, ny=4, target_tris=400_000, street=12.0, width=24.0, dep
                    alpha=alpha.astype(np.float32), heights=heights,
                    param_vertex=v, param_index=params, param_dir=dirs.astype(np.float32),
                    depths=depths, base_xyz=base, ground_vertex=gmask,
-                   n_params=depths.size)
-    return sd
+                   n_params=depths.size, sun_model=sun_model)
+    return _with_sun(sd, sun_model, sun, sun_flux, sun_triangles)
 
 
 def facade_vertices(sd, depths):

@@ -39,18 +39,36 @@ def build():
 # oracle's gradient formulas (``make ref``; see ``ref_grad_harness.c``).  Built
 # only where the reference tree is present; the library lands in ``_ref/``.
 REF_SHADERS = "/root/reference/assets/shader/raytracing_thermal"
+REF_GPL = "/root/reference/src/implementations/thermal_renderer/gpl"
 _REF_PATH = os.path.join(_HERE, "_ref", "libref_grad.so")
+_REF_SOLAR_PATH = os.path.join(_HERE, "_ref", "libref_solar.so")
+_ref_solar = None
 _ref_lib = None
 
 
 def build_ref():
-    """Compile ``_ref/libref_grad.so`` from the reference's headers where they
-    lie; returns False when the reference tree is absent."""
-    if not os.path.isdir(REF_SHADERS):
+    """Compile ``_ref/libref_grad.so`` (the reference's gradient headers) and
+    ``_ref/libref_solar.so`` (its solar position routine) from the sources
+    where they lie; returns False when the reference tree is absent."""
+    if not os.path.isdir(REF_SHADERS) or not os.path.isdir(REF_GPL):
         return False
-    subprocess.run(["make", "-B", "-C", _HERE, "ref", f"REF_SHADERS={REF_SHADERS}"],
-                   check=True, stdout=subprocess.DEVNULL)
+    subprocess.run(["make", "-B", "-C", _HERE, "ref", f"REF_SHADERS={REF_SHADERS}",
+                    f"REF_GPL={REF_GPL}"], check=True, stdout=subprocess.DEVNULL)
     return True
+
+
+def ref_sun_zenith(latitude, longitude, utc, year, month, day, hour, minute=0):
+    """The reference's sunAngle (radians; west-positive longitude and time
+    zone), or None when neither the library nor the reference tree is here."""
+    global _ref_solar
+    if _ref_solar is None:
+        if not os.path.exists(_REF_SOLAR_PATH) and not build_ref():
+            return None
+        L = ctypes.CDLL(_REF_SOLAR_PATH)
+        L.ref_sun_zenith.restype = ctypes.c_double
+        L.ref_sun_zenith.argtypes = [ctypes.c_double, ctypes.c_double] + [ctypes.c_int] * 6
+        _ref_solar = L
+    return _ref_solar.ref_sun_zenith(latitude, longitude, utc, year, month, day, hour, minute)
 
 
 def ref_lib():
@@ -369,6 +387,7 @@ ADJ_REFERENCE_DROPPED = 8
 ADJ_EDGE_GAIN = 16
 ADJ_HIT_AREA = 32
 ADJ_EDGE_PROJECTED = 64
+ADJ_COLLIMATED = 128
 
 
 class AdjointParams(ctypes.Structure):

@@ -135,6 +135,7 @@ int or_transport(const or_scene *sc, const or_params *p, uint32_t tri_begin,
 #define OR_ADJ_EDGE_GAIN 16u        /* extension: occluding side of the edge term */
 #define OR_ADJ_HIT_AREA 32u         /* extension: area Jacobian of every hit triangle */
 #define OR_ADJ_EDGE_PROJECTED 64u   /* extension: edge displacement projected along the ray */
+#define OR_ADJ_COLLIMATED 128u      /* extension: beam transfer on a collimated first segment */
 
 typedef struct {
     float edge_ratio;   /* ubo.gradEdgeRatio, 0.05 */

@@ -137,6 +137,40 @@ float or_geom_term_grad(const float *ai, const float *bi, const float *ci, const
     return G;
 }
 
+/* The collimated beam's transfer to a receiver triangle j from an emitter
+ * triangle i whose photons leave along its unit normal n_i:
+ * B = -(N_j . n_i) / |N_j| (the cosine at the receiver), and d ln|B| / d{six
+ * vertices} (18 floats in out): on i's corners through n_i = N_i / |N_i|, on
+ * j's through N_j and |N_j|; the hit point's motion along a parallel beam
+ * changes nothing.  Returns B. */
+float or_beam_term_grad(const float *ai, const float *bi, const float *ci, const float *aj,
+                        const float *bj, const float *cj, float *out)
+{
+    f3 Ai = ld3(ai), Bi = ld3(bi), Ci = ld3(ci), Aj = ld3(aj), Bj = ld3(bj), Cj = ld3(cj);
+    f3 e1i = sub(Bi, Ai), e2i = sub(Ci, Ai), e1j = sub(Bj, Aj), e2j = sub(Cj, Aj);
+    f3 Ni = cross(e1i, e2i), Nj = cross(e1j, e2j);
+    float li = len3(Ni), lj = len3(Nj);
+    f3 ni = scl(Ni, 1.0f / li), nj = scl(Nj, 1.0f / lj);
+    float cij = dot(Ni, Nj);     /* N_i . N_j */
+    float cn = dot(Nj, ni);      /* N_j . n_i */
+    float B = -cn / lj;
+    /* d(N_i . N_j)/dv_i with N_j fixed, d(N_j . n_i)/dv_j with n_i fixed */
+    f3 e2xa_i = cross(e2i, Nj), axe1_i = cross(Nj, e1i);
+    f3 e2xa_j = cross(e2j, ni), axe1_j = cross(ni, e1j);
+    f3 dNi[3] = {scl(add(e2xa_i, axe1_i), -1.0f), e2xa_i, axe1_i};
+    f3 dNj[3] = {scl(add(e2xa_j, axe1_j), -1.0f), e2xa_j, axe1_j};
+    f3 ab_i = cross(e2i, ni), ac_i = cross(ni, e1i);
+    f3 ab_j = cross(e2j, nj), ac_j = cross(nj, e1j);
+    f3 dLi[3] = {scl(add(ab_i, ac_i), -1.0f), ab_i, ac_i};
+    f3 dLj[3] = {scl(add(ab_j, ac_j), -1.0f), ab_j, ac_j};
+    for (int v = 0; v < 6; v++) {
+        f3 g = v < 3 ? sub(scl(dNi[v], 1.0f / cij), scl(dLi[v], 1.0f / li))
+                     : sub(scl(dNj[v - 3], 1.0f / cn), scl(dLj[v - 3], 1.0f / lj));
+        out[3 * v] = g.x; out[3 * v + 1] = g.y; out[3 * v + 2] = g.z;
+    }
+    return B;
+}
+
 typedef struct {
     int32_t prim;
     float t, b1, b2;
@@ -258,8 +292,14 @@ int or_adjoint(const or_scene *sc, const or_params *p, const or_adjoint_params *
                         const float *pv[6] = {sc->xyz + 3 * prev[0], sc->xyz + 3 * prev[1],
                                               sc->xyz + 3 * prev[2], sc->xyz + 3 * hi[0],
                                               sc->xyz + 3 * hi[1], sc->xyz + 3 * hi[2]};
-                        float G = or_geom_term_grad(pv[0], pv[1], pv[2], pv[3], pv[4], pv[5], pb1,
-                                                    pb2, h.b1, h.b2, eps, gg);
+                        /* extension (OR_ADJ_COLLIMATED): a collimated first
+                         * segment carries the beam's transfer */
+                        const int beam = (ap->flags & OR_ADJ_COLLIMATED) && depth == 1 &&
+                                         !(sc->flags[t] & OR_FLAG_DIFFUSE_EMISSION);
+                        float G = beam ? or_beam_term_grad(pv[0], pv[1], pv[2], pv[3], pv[4],
+                                                           pv[5], gg)
+                                       : or_geom_term_grad(pv[0], pv[1], pv[2], pv[3], pv[4],
+                                                           pv[5], pb1, pb2, h.b1, h.b2, eps, gg);
                         if (fabsf(G) > eps * eps) {
                             s_cont++;
                             uint32_t vid[6] = {prev[0], prev[1], prev[2], hi[0], hi[1], hi[2]};

@@ -102,18 +102,21 @@ def test_adjoint_trace_matches_oracle(irtt, oracle, which, flags):
     assert err < 1e-9
 
 
-@pytest.mark.parametrize("which", ["blocker", "blocker_open", "fin"])
-@pytest.mark.parametrize("flags", [7, 23, 39, 71, 103])
+@pytest.mark.parametrize("which", ["blocker", "blocker_open", "fin", "balcony_sun"])
+@pytest.mark.parametrize("flags", [7, 23, 39, 71, 103, 231])
 def test_adjoint_trace_edge_scenes_match_oracle(irtt, oracle, which, flags):
     """The scenes that pin the edge term against finite differences
     (tests/test_oracle_edge_term.py): an opaque strip between two plates,
     the same without the receiver (occlusion rays escape), a fin reached at
-    grazing incidence.  Every term and extension (16 edge gain, 32 hit-area
-    Jacobian, 64 projected edge factor), GPU = oracle on the same photons."""
+    grazing incidence, a balcony's shadow on a wall under a collimated sun.
+    Every term and extension (16 edge gain, 32 hit-area Jacobian, 64
+    projected edge factor, 128 beam transfer of collimated first segments),
+    GPU = oracle on the same photons."""
     from irtt import scenes
     sd = {"blocker": lambda: scenes.blocker_plates(),
           "blocker_open": lambda: scenes.blocker_plates(receiver=False),
-          "fin": lambda: scenes.fin_plates()}[which]()
+          "fin": lambda: scenes.fin_plates(),
+          "balcony_sun": lambda: scenes.balcony_wall(sun_triangles=128)}[which]()
     rng = np.random.default_rng(5)
     lam = 1.0 + rng.random(sd.nv)
     x4 = rng.random(sd.nv)
@@ -128,6 +131,33 @@ def test_adjoint_trace_edge_scenes_match_oracle(irtt, oracle, which, flags):
     assert ost["occluded"] > 0 or which == "blocker_open"
     err = np.abs(g - og).max() / np.abs(og).max()
     print(f"{which} flags {flags}: rel err {err:.2e}; {st}")
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("flags", [7, 231])
+def test_adjoint_trace_sunlit_facade_matches_oracle(irtt, oracle, flags):
+    """The design scene (a facade with balconies and fins, here 2 buildings
+    and 25k triangles) under the collimated sun quad: the GPU trace equals
+    the oracle's on the same photons, with the reference's terms and with
+    every extension for this case (hit area, projected edge, beam transfer);
+    lambda and x4 as the design gradient's first row (lambda on the ground,
+    x^4 of the scene and the sun)."""
+    from irtt import scenes
+    sd = scenes.facade(nx=2, ny=1, hmin=10.0, hmax=12.0, target_tris=20000,
+                       n_fin_buildings=2, seed=3, sun_model="emitter")
+    lam = np.asarray(sd.meta["ground_vertex"], np.float64) + 0.1
+    x4 = sd.kelvin.astype(np.float64) ** 4
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        s.build_transport(threads_per_tri=16, rays_per_thread=8, batch_seed=5,
+                          layout=irtt.LAYOUT_SPARSE)
+        g, st = s.adjoint_trace(lam, x4, flags=flags)
+    og, ost = oracle.adjoint(oracle.OracleScene.from_irtt(sd),
+                             oracle.make_params(16, 8, batch_seed=5, bin_mode=0), lam, x4,
+                             flags=flags)
+    for k in ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded", "rays"):
+        assert st[k] == ost[k], (k, st[k], ost[k])
+    err = np.abs(g - og).max() / np.abs(og).max()
+    print(f"sunlit facade flags {flags}: rel err {err:.2e}; {st}")
     assert err < 1e-9
 
 

@@ -23,7 +23,14 @@ bands allow for the trace's Monte-Carlo noise at 2.6 M photons):
 * its irradiance derivative is too small at grazing incidence: 0.18 on the
   fin, 1.02 with the projection factor (ADJ_EDGE_PROJECTED);
 * every edge event's split (shadow, origin, edge vertices) sums to zero,
-  so a rigid motion of the whole scene has no edge-term gradient.
+  so a rigid motion of the whole scene has no edge-term gradient;
+* under a collimated sun (``balcony_wall``: a balcony's shadow on a wall),
+  the reference's continuous term differentiates a point-to-point geometric
+  term a parallel beam does not have: 0.26 (sun at 45 degrees) and 0.10 (30
+  degrees, 60 degrees azimuth) of the balcony's FD gain where the true
+  continuous part is zero; the beam transfer (ADJ_COLLIMATED) removes it,
+  leaving the hit-area term at 0.99 and 0.98; the wall's loss is carried by
+  the projected edge term (0.85 and 1.17).
 """
 import numpy as np
 import pytest
@@ -33,6 +40,7 @@ REF = 7            # area | continuous | edge: the reference's terms
 GAIN = REF | 16    # + edge gain (extension)
 HIT = REF | 32     # + hit-triangle area Jacobian (extension)
 PROJ = REF | 64    # + projected edge factor (extension)
+BEAM = 128         # collimated first segments: beam transfer (extension)
 
 
 def _phi(oracle, sd, params, lam, x4):
@@ -155,3 +163,44 @@ def test_edge_split_sums_to_zero(oracle, flags):
     assert st["occluded"] > 0
     # the FP32 fractions d1/(d1+d2) and d2/(d1+d2) sum to one within a few ulp
     assert np.all(np.abs(g.sum(0)) <= 1e-6 * np.abs(g).sum(0))
+
+
+@pytest.mark.parametrize("sun", [(45.0, 0.0), (30.0, 60.0)])
+def test_collimated_sun_needs_the_beam_transfer(oracle, sun):
+    """A balcony's depth under a collimated sun: its top gains what the wall
+    below loses.  With lambda on the balcony the derivative is the hit-area
+    term (its triangles stretch; a parallel beam's transfer has no distance
+    factor); with lambda on the wall it is the projected edge term."""
+    from irtt import scenes
+    el, az = sun
+
+    def make(depth=1.0):
+        return scenes.balcony_wall(depth=depth, sun_elevation_deg=el, sun_azimuth_deg=az)
+    sd = make()
+    own = sd.meta["owner"]
+    ddir = sd.meta["depth_dir"].astype(np.float64)
+    sv = sd.meta["suns"][0]["vertices"]
+    x4 = np.zeros(sd.nv)
+    x4[sv] = sd.kelvin[sv].astype(np.float64) ** 4
+    p = oracle.make_params(64, 8, batch_seed=5, bin_mode=0)
+    h = 2e-2
+    out = {}
+    for name, sel in (("balcony", own == 2), ("wall", own == 1)):
+        lam = sel.astype(np.float64)
+        fd = (_phi(oracle, make(1.0 + h), p, lam, x4) - _phi(oracle, make(1.0 - h), p, lam, x4)) \
+            / (2 * h)
+        r = {}
+        for k, f in (("ref", REF), ("ext", REF | 32 | 64), ("beam", REF | 32 | 64 | BEAM),
+                     ("cont", 2), ("cont_beam", 2 | BEAM)):
+            g, _ = oracle.adjoint(oracle.OracleScene.from_irtt(sd), p, lam, x4, flags=f)
+            r[k] = -(g * ddir).sum() / fd
+        out[name] = r
+        print(f"sun {sun}, lambda on the {name}: FD {fd:.4e}, ratios "
+              + ", ".join(f"{k} {v:.3f}" for k, v in r.items()))
+    b, w = out["balcony"], out["wall"]
+    # translated and stretched planar faces keep their cosine: zero to rounding
+    assert abs(b["cont_beam"]) < 1e-6 and abs(w["cont_beam"]) < 1e-6
+    assert b["cont"] > 0.05             # the point-to-point term's spurious part
+    assert 0.93 < b["beam"] < 1.07
+    assert not 0.93 < b["ext"] < 1.07
+    assert 0.8 < w["beam"] < 1.2 and w["beam"] == w["ext"]
