@@ -941,9 +941,10 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     }
 
     hipStream_t stm = s.stream;
-    DBuf<float> d_w, d_area;
-    DBuf<double> d_lam, d_x4;
-    DBuf<unsigned long long> d_grad, d_st;
+    AdjWork &ws = s.adj;
+    DBuf<float> &d_w = ws.w, &d_area = ws.area;
+    DBuf<double> &d_lam = ws.lam, &d_x4 = ws.x4;
+    DBuf<unsigned long long> &d_grad = ws.grad, &d_st = ws.st;
     d_w.alloc(s.nt);
     d_area.alloc(s.nt);
     d_lam.alloc((size_t)n_hist * nb);
@@ -959,11 +960,12 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
 
     IRTT_HIP(hipEventRecord(s.ev0, stm));
     // absorbing bins: the forward build's record, or a forward pass of our own
-    DBuf<uint32_t> d_abs_own;
+    DBuf<uint32_t> &d_abs_own = ws.abs_own;
     const uint32_t *absorber = nullptr;
     uint64_t rays = s.fwd_rays, segments = s.fwd_segments;
     if (s.d_absorber.n == photons) {
         absorber = s.d_absorber.p;
+        d_abs_own.release();
     } else {
         d_abs_own.alloc(photons);
         absorber_pass(s, d_abs_own.p, rays, segments);
@@ -1026,7 +1028,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
     size_t free_b = 0, total_b = 0;
     IRTT_HIP(hipMemGetInfo(&free_b, &total_b));
-    const double budget = std::max(2.0 * (1u << 30), 0.3 * (double)free_b);
+    // the record buffers this scene already holds count as available
+    const double budget =
+        std::max(2.0 * (1u << 30), 0.3 * ((double)free_b + (double)ws.record_bytes()));
     uint64_t seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
     uint64_t edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
     const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
@@ -1038,12 +1042,15 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                                  : 1;
         n_ranges = (uint64_t)std::ceil(1.0 / f);
     }
-    DBuf<SegRec> d_segs;
-    DBuf<EdgeRec> d_edges;
-    DBuf<uint32_t> d_fills, d_keys, d_keys2, d_order, d_order2;
-    DBuf<unsigned long long> d_ctl;  // chunks taken (segments, edges), first dropped photon
-    d_segs.alloc(seg_chunks * kChunk);
-    d_edges.alloc(edge_chunks * kChunk);
+    static_assert(sizeof(SegRec) == 3 * sizeof(uint4) && sizeof(EdgeRec) == 4 * sizeof(uint4),
+                  "record layouts");
+    DBuf<uint32_t> &d_fills = ws.fills, &d_keys = ws.keys, &d_keys2 = ws.keys2,
+                   &d_order = ws.order, &d_order2 = ws.order2;
+    DBuf<unsigned long long> &d_ctl = ws.ctl;  // chunks taken (segments, edges), first dropped
+    ws.segs.alloc(seg_chunks * kChunk * 3);
+    ws.edges.alloc(edge_chunks * kChunk * 4);
+    SegRec *const segs_p = reinterpret_cast<SegRec *>(ws.segs.p);
+    EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
     d_fills.alloc(seg_chunks + edge_chunks);
     d_keys.alloc(seg_chunks * kChunk);
     d_keys2.alloc(seg_chunks * kChunk);
@@ -1058,7 +1065,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb,
                                            (size_t)(seg_chunks * kChunk), 0, key_bits));
     }
-    DBuf<uint8_t> d_sort_tmp;
+    DBuf<uint8_t> &d_sort_tmp = ws.sort_tmp;
     d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
     const uint64_t range = (photons + n_ranges - 1) / n_ranges;
     const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
@@ -1070,8 +1077,8 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             pa.c = c;
             pa.g_begin = g_start;
             pa.g_end = ge;
-            pa.segs = RecBuf{d_segs.p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
-            pa.edges = RecBuf{d_edges.p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
+            pa.segs = RecBuf{segs_p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
+            pa.edges = RecBuf{edges_p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
             pa.first_dropped = d_ctl.p + 2;
             IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
             const uint32_t grid = (uint32_t)std::max<uint64_t>(
@@ -1090,7 +1097,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                 // each side over the records sorted by its triangle
                 const uint64_t n = (uint64_t)sc * kChunk;
                 for (uint32_t side = 0; side < 2; side++) {
-                    k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(d_segs.p, d_fills.p, sc, s.nt, side,
+                    k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(segs_p, d_fills.p, sc, s.nt, side,
                                                               d_keys.p, d_order.p);
                     IRTT_HIP(hipGetLastError());
                     rocprim::double_buffer<uint32_t> kd(d_keys.p, d_keys2.p),
@@ -1100,7 +1107,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                                                        key_bits, stm));
                     SegArgs sa{};
                     sa.c = c;
-                    sa.segs = d_segs.p;
+                    sa.segs = segs_p;
                     sa.keys = kd.current();
                     sa.order = vd.current();
                     sa.n = n;
@@ -1115,7 +1122,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             if (ec > 0) {
                 OcclArgs oa{};
                 oa.c = c;
-                oa.edges = d_edges.p;
+                oa.edges = edges_p;
                 oa.fills = d_fills.p + seg_chunks;
                 oa.n_chunks = ec;
                 oa.g_limit = ctl[2];

@@ -40,11 +40,15 @@ constexpr uint32_t kMaxDepthDefault = 512;     // MAX_MAX_DEPTH, thermal_tracing
 // nullptr: plain hipMalloc / hipFree.
 extern thread_local hipStream_t t_alloc_stream;
 
-// Device buffer with RAII release.
+// Device buffer with RAII release.  alloc() keeps the allocation when the new
+// count fits in what it holds (grow-only): repeated builds and traces of one
+// scene whose sizes drift (a design loop) reuse their buffers instead of
+// returning tens of GB to the pool and mapping them again.
 template <typename T>
 struct DBuf {
     T *p = nullptr;
-    size_t n = 0;
+    size_t n = 0;              // elements in use
+    size_t cap = 0;            // elements allocated
     hipStream_t st = nullptr;  // stream the buffer was allocated on (nullptr: hipMalloc)
     DBuf() = default;
     DBuf(const DBuf &) = delete;
@@ -58,11 +62,15 @@ struct DBuf {
         }
         p = nullptr;
         n = 0;
+        cap = 0;
         st = nullptr;
     }
     void alloc(size_t count)
     {
-        if (count == n && p) return;
+        if (p && count <= cap) {
+            n = count;
+            return;
+        }
         release();
         if (count == 0) return;
         const size_t b = count * sizeof(T);
@@ -83,8 +91,10 @@ struct DBuf {
                                             " bytes failed: " + hipGetErrorString(e));
         }
         n = count;
+        cap = count;
     }
     size_t bytes() const { return n * sizeof(T); }
+    size_t held_bytes() const { return cap * sizeof(T); }
 };
 
 // Sparse matrix in CSR, FP64 (conduction Laplacian).
@@ -128,18 +138,50 @@ struct Sparse {
     DBuf<uint32_t> colptr, row;       // the same entries by column, rows ascending
     DBuf<float> cval;
     DBuf<double> d_xt, d_yt;          // vector-interleaved operands of a batched product
-    DBuf<TabSlot> tab;                // hash table, alive during a build only
+    DBuf<TabSlot> tab;                // hash table of a build (kept for the next)
     DBuf<uint32_t> flag;
+    // table -> rows / columns work space (kept between builds)
+    DBuf<uint32_t> ccount;
+    DBuf<unsigned long long> coff, ka, kb, va, vb;
+    DBuf<uint8_t> sort_tmp;
+    // drops the operator, keeps the buffers for the next build
+    void invalidate()
+    {
+        valid = false;
+        nnz = 0;
+    }
     void reset()
     {
         valid = false;
         nnz = 0;
+        for (DBuf<unsigned long long> *b : {&coff, &ka, &kb, &va, &vb}) b->release();
+        ccount.release();
+        sort_tmp.release();
         for (DBuf<uint32_t> *b : {&rowptr, &col, &colptr, &row, &flag}) b->release();
         val.release();
         cval.release();
         d_xt.release();
         d_yt.release();
         tab.release();
+    }
+};
+
+// Adjoint trace work space (adjoint.hip), kept between traces of a scene:
+// path records (uint4 words: 3 per segment record, 4 per edge record), their
+// sort keys and orders, per-call inputs and the exact gradient sums.
+struct AdjWork {
+    DBuf<float> w, area;
+    DBuf<double> lam, x4;
+    DBuf<unsigned long long> grad, st, ctl;
+    DBuf<uint32_t> abs_own, fills, keys, keys2, order, order2;
+    DBuf<uint4> segs, edges;
+    DBuf<uint8_t> sort_tmp;
+    // bytes held by the record buffers (the part sized against free HBM)
+    size_t record_bytes() const
+    {
+        return segs.held_bytes() + edges.held_bytes() + fills.held_bytes() +
+               keys.held_bytes() + keys2.held_bytes() + order.held_bytes() +
+               order2.held_bytes() + sort_tmp.held_bytes();
     }
 };
 
@@ -239,6 +281,7 @@ struct Scene {
     uint32_t layout = IRTT_LAYOUT_DENSE;  // of the built operator
     irtt_transport_params tparams{};
     Sparse sp;                        // IRTT_LAYOUT_SPARSE
+    AdjWork adj;                      // adjoint trace work space
     double quantum = 0.0;
     DBuf<uint64_t> d_q;               // per-triangle fixed-point weight
     std::vector<uint64_t> h_q;        // what d_q holds (uploads of the same weights are skipped)

@@ -568,12 +568,13 @@ __global__ void k_add_diag(uint32_t n, const double *__restrict__ cd, const doub
 
 }  // namespace
 
-// back to the dense FP32 form; the other forms and the sparse layout are
-// released (a new operator, or new geometry)
+// back to the dense FP32 form; the other forms are released and the sparse
+// layout dropped, its buffers kept for the next sparse build (a new
+// operator, or new geometry)
 void op_reset(Scene &s)
 {
     s.op_kind = IRTT_OP_DENSE;
-    s.sp.reset();
+    s.sp.invalidate();
     s.f16_valid = false;
     s.d_h16.release();
     s.d_h16s.release();

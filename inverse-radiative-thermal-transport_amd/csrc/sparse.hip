@@ -276,7 +276,7 @@ void sparse_table_begin(Scene &s, uint64_t photons, PhotonSink &sink)
         throw Error(IRTT_ERR_LIMIT, "sparse transport: " + std::to_string(photons) +
                                         " photons need a hash table above 2^36 slots");
     hipStream_t stm = s.stream;
-    s.sp.reset();
+    s.sp.invalidate();
     const uint64_t cap = 1ull << lg;
     s.sp.tab.alloc(cap);
     s.sp.flag.alloc(1);
@@ -302,8 +302,9 @@ void sparse_table_finish(Scene &s, const PhotonSink &sink)
     const uint64_t nchunk64 = (cap + kChunk - 1) / kChunk;
     if (nchunk64 > 0x7fffffffull) throw Error(IRTT_ERR_LIMIT, "sparse transport: table too large");
     const uint32_t nchunk = (uint32_t)nchunk64;
-    DBuf<uint32_t> ccount;
-    DBuf<unsigned long long> coff;
+    Sparse &p = s.sp;
+    DBuf<uint32_t> &ccount = p.ccount;
+    DBuf<unsigned long long> &coff = p.coff;
     ccount.alloc(nchunk);
     coff.alloc((size_t)nchunk + 1);
     k_tab_count<<<nchunk, 256, 0, stm>>>(s.sp.tab.p, cap, ccount.p);
@@ -322,24 +323,22 @@ void sparse_table_finish(Scene &s, const PhotonSink &sink)
 
     // ka/va: compacted, kb/vb: the sort's other halves; the column sort
     // reuses all four
-    DBuf<unsigned long long> ka, kb, va, vb;
+    DBuf<unsigned long long> &ka = p.ka, &kb = p.kb, &va = p.va, &vb = p.vb;
     ka.alloc(nnz);
     kb.alloc(nnz);
     va.alloc(nnz);
     vb.alloc(nnz);
     k_tab_compact<<<nchunk, 256, 0, stm>>>(s.sp.tab.p, cap, coff.p, ka.p, va.p);
     IRTT_HIP(hipGetLastError());
-    s.sp.tab.release();
 
     const size_t t1 = sort_temp_bytes<unsigned long long, unsigned long long>(nnz, end_bit);
-    DBuf<uint8_t> tmp;
+    DBuf<uint8_t> &tmp = p.sort_tmp;
     tmp.alloc(std::max<size_t>(t1, 1));
     size_t tb = t1;
     rocprim::double_buffer<unsigned long long> kd(ka.p, kb.p);
     rocprim::double_buffer<unsigned long long> vd(va.p, vb.p);
     IRTT_HIP(rocprim::radix_sort_pairs(tmp.p, tb, kd, vd, (size_t)nnz, 0, end_bit, stm));
 
-    Sparse &p = s.sp;
     p.rowptr.alloc((size_t)nb + 1);
     p.col.alloc(nnz);
     p.val.alloc(nnz);

@@ -561,6 +561,8 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     if (sparse) {
         s.d_acc.release();
         s.d_T.release();
+    } else {
+        s.sp.reset();
     }
 
     hipStream_t stm = s.stream;
@@ -625,8 +627,9 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     // absorbing bin of every photon (4 B each) so the adjoint trace replays
     // each photon once instead of twice; not kept above 2^31 photons or
     // when HBM is short (the adjoint then finds absorbers with a pass of its own)
-    s.d_absorber.release();
-    if ((p.flags & IRTT_TRANSPORT_KEEP_ABSORBERS) && total <= (1ull << 31)) {
+    if (!(p.flags & IRTT_TRANSPORT_KEEP_ABSORBERS) || total > (1ull << 31)) {
+        s.d_absorber.release();
+    } else {
         try {
             s.d_absorber.alloc(total);
             IRTT_HIP(hipMemsetAsync(s.d_absorber.p, 0xff, s.d_absorber.bytes(), stm));
