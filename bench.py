@@ -525,29 +525,27 @@ def transient(irtt, seed, p, steps=288, dt=300.0, check_steps=6, site=(48.21, 16
     return out
 
 
-def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=200000, seed=7):
-    """Poisson stream of design variants through irtt_serve_*: city blocks of
-    log-uniform size (tmin..tmax triangles) with random heights and seeds,
-    each a forward steady solve plus adjoint over up to 64 building-height
-    parameters, operators in the sparse layout (a dense operator of the
-    largest designs would take 40-120 GB).  Returns latency percentiles and
-    throughput."""
+def _city_designs(n, tmin, tmax, seed):
+    """n city-block design variants with the traced sun, log-uniform
+    tmin..tmax scene triangles, random heights and seeds, each with its 64
+    building-height parameters."""
     import numpy as np
     from irtt import scenes
     rng = np.random.default_rng(seed)
-    designs = []
-    for k in range(n_req):
+    out = []
+    for k in range(n):
         t = float(np.exp(rng.uniform(np.log(tmin), np.log(tmax))))
         g = int(np.clip(round(np.sqrt(t / 800.0)), 2, 8))
-        sd = scenes.city_block(seed=1000 + k, nx=g, ny=g, target_tris=int(t))
-        designs.append((sd, scenes.height_parametrization(sd, 64)))
-    tp = irtt.transport_params(threads_per_tri=32, rays_per_thread=8, batch_seed=1234,
-                               layout=irtt.LAYOUT_SPARSE)
-    sp = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-8)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n_req))
+        sd = scenes.city_block(seed=1000 + k, nx=g, ny=g, target_tris=int(t), sun_model="emitter")
+        out.append((sd, scenes.height_parametrization(sd, 64)))
+    return out
+
+
+def _stream(irtt, designs, arrivals, tp, sp, slots):
+    """Submits design k at arrivals[k] seconds (Poisson, or all at once) to
+    a server and waits for all; returns (statuses, wall seconds)."""
     with irtt.Server(slots=slots) as sv:
-        # one warm request so the first arrival does not pay for code loading
-        sd, par = designs[0]
+        sd, par = designs[0]   # one warm request: code loading is not a request's latency
         sv.wait(sv.submit(sd, transport=tp, solver=sp, kelvin0=sd.kelvin, fixed=sd.fixed,
                           target=300.0, objective_weight=1e-3, params=par))
         tickets = []
@@ -560,18 +558,98 @@ def serving(irtt, n_req=16, rate=20.0, slots=4, tmin=2000, tmax=200000, seed=7):
                                      fixed=sd.fixed, target=300.0, objective_weight=1e-3,
                                      params=par))
         stats = [sv.wait(tk)[0] for tk in tickets]
-        wall = time.perf_counter() - t0
-    lat = np.array([st["latency_ms"] for st in stats])
-    return {"requests": n_req, "arrival_rate_per_s": rate, "slots": slots,
-            "triangles": [int(d[0].nt) for d in designs],
-            "p50_ms": round(float(np.percentile(lat, 50)), 1),
-            "p99_ms": round(float(np.percentile(lat, 99)), 1),
-            "mean_ms": round(float(lat.mean()), 1),
-            "mean_queue_ms": round(float(np.mean([st["queued_ms"] for st in stats])), 1),
-            "designs_per_s": round(n_req / wall, 2),
-            "work": "per request: LBVH, 256 photons per triangle into the sparse layout, "
-                    "steady solve, adjoint solve, adjoint trace, dL/dp over up to 64 "
-                    "building heights"}
+        return stats, time.perf_counter() - t0
+
+
+def serving(irtt, n_req=200, rate=8.0, slots=4, tmin=2000, tmax=200000, seed=7, tpt=64, rpt=8):
+    """Poisson stream of design variants through irtt_serve_*: city blocks
+    with the traced sun, log-uniform tmin..tmax scene triangles, random
+    heights and seeds, each a forward steady solve plus adjoint over 64
+    building-height parameters, operators in the sparse layout.  The same
+    n_req designs at `rate` requests/s and at saturation (all submitted at
+    once).  Returns latency percentiles and throughput of each."""
+    import numpy as np
+    t0 = time.perf_counter()
+    designs = _city_designs(n_req, tmin, tmax, seed)
+    gen_s = time.perf_counter() - t0
+    tp = irtt.transport_params(threads_per_tri=tpt, rays_per_thread=rpt, batch_seed=1234,
+                               layout=irtt.LAYOUT_SPARSE)
+    sp = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-4, f_tol=1e-8)
+    rng = np.random.default_rng(seed + 1)
+    out = {"requests": n_req, "slots": slots, "generate_s": round(gen_s, 1),
+           "triangles": {"min": int(min(d[0].nt for d in designs)),
+                         "median": int(np.median([d[0].nt for d in designs])),
+                         "max": int(max(d[0].nt for d in designs))},
+           "photons_per_triangle": tpt * rpt}
+    for name, arrivals in (("at_rate", np.cumsum(rng.exponential(1.0 / rate, n_req))),
+                           ("saturation", np.zeros(n_req))):
+        _log(f"serving: {n_req} requests, {name}")
+        stats, wall = _stream(irtt, designs, arrivals, tp, sp, slots)
+        lat = np.array([st["latency_ms"] for st in stats])
+        run = np.array([st["run_ms"] for st in stats])
+        out[name] = {"p50_ms": round(float(np.percentile(lat, 50)), 1),
+                     "p99_ms": round(float(np.percentile(lat, 99)), 1),
+                     "mean_ms": round(float(lat.mean()), 1),
+                     "mean_run_ms": round(float(run.mean()), 1),
+                     "mean_queue_ms": round(float(np.mean([st["queued_ms"] for st in stats])), 1),
+                     "designs_per_s": round(n_req / wall, 2), "wall_s": round(wall, 2)}
+        if name == "at_rate":
+            out[name]["arrival_rate_per_s"] = rate
+    out["work"] = ("per request: LBVH, %d photons per triangle into the sparse layout, steady "
+                   "solve, adjoint solve, adjoint trace, dL/dp over 64 building heights"
+                   % (tpt * rpt))
+    return out
+
+
+def served_vs_plain(irtt, n=12, tpt=130, rpt=8, slots=4):
+    """n geometry variants of the 52k-triangle city block (random heights
+    and seeds, traced sun), each a transport build, forward steady solve and
+    design gradient (adjoint solve, trace, projection onto 64 heights).
+    Plain: the FP32 GPU path one design after another (dense FP32 operator,
+    scene API).  Served: the same requests through irtt_serve_* (sparse
+    layout, FP32 entries with FP64 sums, `slots` designs on the device at
+    once).  Returns designs/s of each, their ratio and the largest
+    temperature difference between the two answers."""
+    import numpy as np
+    from irtt import scenes
+    designs = []
+    for k in range(n):
+        sd = scenes.city_block(seed=2000 + k, sun_model="emitter")
+        designs.append((sd, scenes.height_parametrization(sd, 64)))
+    sp = irtt.solver_params(ambient_kelvin=293.0, x_tol=1e-7, f_tol=1e-14, lin_rtol=1e-10)
+    tpd = irtt.transport_params(tpt, rpt, batch_seed=1234, flags=irtt.TRANSPORT_KEEP_ABSORBERS)
+    _log(f"served vs plain: {n} city-block variants, plain FP32 path")
+    xs = []
+    t0 = time.perf_counter()
+    for sd, par in designs:
+        with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+            s.set_vertex_state(sd.kelvin, sd.fixed)
+            s.build_transport(tpd)
+            s.solve(sp)
+            x = s.temperatures()
+            area = s.bin_data()[0]
+            s.design_gradient(1e-3 * area * (x - 300.0), params=par, solver=sp)
+            xs.append(x)
+    plain = n / (time.perf_counter() - t0)
+    _log("served vs plain: served path")
+    tps = irtt.transport_params(tpt, rpt, batch_seed=1234, layout=irtt.LAYOUT_SPARSE)
+    stats, wall = _stream(irtt, designs, np.zeros(n), tps, sp, slots)
+    served = n / wall
+    dmax = 0.0
+    with irtt.Server(slots=slots) as sv:
+        for (sd, par), x in zip(designs[:2], xs[:2]):
+            _, o = sv.wait(sv.submit(sd, transport=tps, solver=sp, kelvin0=sd.kelvin,
+                                     fixed=sd.fixed, target=300.0, objective_weight=1e-3,
+                                     params=par))
+            dmax = max(dmax, float(np.abs(o["kelvin"] - x).max()))
+    return {"designs": n, "triangles": designs[0][0].nt, "photons_per_triangle": tpt * rpt,
+            "plain_fp32_designs_per_s": round(plain, 2), "served_designs_per_s": round(served, 2),
+            "ratio": round(served / plain, 2), "served_p99_ms": round(float(np.percentile(
+                [st["latency_ms"] for st in stats], 99)), 1),
+            "max_dT_served_vs_plain_K": dmax,
+            "work": "per design: transport build, forward steady solve, adjoint solve, adjoint "
+                    "trace, dL/dp over 64 heights; plain = dense FP32 operator one design at a "
+                    "time, served = sparse layout, %d concurrent" % slots}
 
 
 def calibrate_copy(mib, reps=3):
@@ -612,8 +690,10 @@ def main():
     ap.add_argument("--facade-gd", type=int, default=50)
     ap.add_argument("--facade-only", action="store_true",
                     help="run only the facade section and print it")
-    ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=8.0)
+    ap.add_argument("--serving-only", action="store_true",
+                    help="run only the serving sections and print them")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device three times (a plain copy kernel "
                          "of known size, for scaling rocprofv3's FETCH_SIZE / WRITE_SIZE)")
@@ -632,6 +712,11 @@ def main():
 
     if args.facade_only:
         print(json.dumps({"facade": facade(irtt, gd_steps=args.facade_gd)}), flush=True)
+        return
+    if args.serving_only:
+        print(json.dumps({"serving": serving(irtt, n_req=args.serve_requests,
+                                             rate=args.serve_rate),
+                          "served_vs_plain": served_vs_plain(irtt)}), flush=True)
         return
     _log("city block: scene and warm-up builds")
     sd = scenes.city_block(seed=args.seed, sun_model="emitter")
@@ -751,6 +836,7 @@ def main():
     if not args.no_serving:
         _log("serving")
         line["serving"] = serving(irtt, n_req=args.serve_requests, rate=args.serve_rate)
+        line["served_vs_plain"] = served_vs_plain(irtt)
     if not args.no_cpu_baseline:
         _log("CPU baseline and FP64 checks")
         line["cpu_baseline"] = cpu_baseline(sd, args.threads_per_tri, args.rays_per_thread,

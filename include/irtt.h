@@ -503,6 +503,17 @@ typedef struct irtt_request {
     double *kelvin;             /* n_bins, or NULL */
     double *dp;                 /* n_params, or NULL */
     double *objective;          /* 1, or NULL */
+    /* zero-initialised: the defaults */
+    irtt_adjoint_params adjoint;  /* terms of the adjoint trace (flags 0: area | continuous |
+                                     edge; edge_ratio, tiny_eps 0: 0.05, 1e-2) */
+    const double *source;       /* n_bins external power per bin (W), replacing the materials'
+                                   heat_flux; NULL: from the materials */
+    uint64_t group;             /* nonzero: requests queued with the same group whose
+                                   geometry, materials, bin mode, kelvin0, fixed, transport,
+                                   solver and operator are identical share one scene and
+                                   transport build; their steady forward and adjoint solves
+                                   run in lockstep (batched products) and each gets its own
+                                   adjoint trace.  Such requests differ in source and target. */
 } irtt_request;
 
 #define IRTT_REQ_QUEUED 0

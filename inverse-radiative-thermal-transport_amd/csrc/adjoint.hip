@@ -1029,8 +1029,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     size_t free_b = 0, total_b = 0;
     IRTT_HIP(hipMemGetInfo(&free_b, &total_b));
     // the record buffers this scene already holds count as available
-    const double budget =
-        std::max(2.0 * (1u << 30), 0.3 * ((double)free_b + (double)ws.record_bytes()));
+    const double budget = std::max(
+        2.0 * (1u << 30),
+        0.3 * ((double)free_b + (double)ws.record_bytes() + (double)dev_cache_bytes()));
     uint64_t seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
     uint64_t edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
     const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));

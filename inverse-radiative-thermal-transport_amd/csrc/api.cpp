@@ -272,6 +272,202 @@ void upload_geometry(Scene &s)
 
 }  // namespace
 
+// ---- process-wide device memory cache and stream pool ----------------------
+#ifndef IRTT_CACHE_GIB
+#define IRTT_CACHE_GIB 96
+#endif
+namespace {
+struct CachedBlock {
+    void *p;
+    size_t bytes;
+    hipEvent_t ev;   // recorded on the freeing stream
+    int dev;
+    uint64_t age;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;
+std::vector<std::pair<int, hipEvent_t>> g_events;      // idle events per device
+std::vector<std::pair<int, hipStream_t>> g_streams;    // idle streams per device
+size_t g_cache_bytes = 0;
+uint64_t g_cache_clock = 0;
+constexpr size_t kCacheMax = (size_t)IRTT_CACHE_GIB << 30;
+
+size_t round_bytes(size_t b)
+{
+    constexpr size_t g = 2u << 20;
+    return b < (1u << 20) ? (b + 255) / 256 * 256 : (b + g - 1) / g * g;
+}
+
+hipEvent_t take_event_locked(int dev)
+{
+    for (size_t i = g_events.size(); i-- > 0;)
+        if (g_events[i].first == dev) {
+            hipEvent_t e = g_events[i].second;
+            g_events.erase(g_events.begin() + i);
+            return e;
+        }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return e;
+}
+
+// hands cached blocks back to the device: the oldest until at most `keep`
+// bytes stay cached (the caller holds the lock)
+void trim_locked(size_t keep)
+{
+    while (g_cache_bytes > keep && !g_cache.empty()) {
+        size_t o = 0;
+        for (size_t i = 1; i < g_cache.size(); i++)
+            if (g_cache[i].age < g_cache[o].age) o = i;
+        CachedBlock b = g_cache[o];
+        g_cache.erase(g_cache.begin() + o);
+        g_cache_bytes -= b.bytes;
+        if (b.ev) (void)hipEventSynchronize(b.ev);
+        (void)hipFree(b.p);
+        if (b.ev) g_events.emplace_back(b.dev, b.ev);
+    }
+}
+}  // namespace
+
+void *irtt::dev_alloc(size_t bytes, hipStream_t stream)
+{
+    int dev = 0;
+    IRTT_HIP(hipGetDevice(&dev));
+    const size_t want = round_bytes(bytes);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        size_t best = g_cache.size();
+        for (size_t i = 0; i < g_cache.size(); i++) {
+            const CachedBlock &b = g_cache[i];
+            if (b.dev != dev || b.bytes < want || b.bytes > want + want / 4 + (32u << 20)) continue;
+            if (best == g_cache.size() || b.bytes < g_cache[best].bytes) best = i;
+        }
+        if (best < g_cache.size()) {
+            CachedBlock b = g_cache[best];
+            g_cache.erase(g_cache.begin() + best);
+            g_cache_bytes -= b.bytes;
+            if (b.ev) {
+                // the block's last use on its old stream finishes first
+                (void)hipStreamWaitEvent(stream, b.ev, 0);
+                g_events.emplace_back(dev, b.ev);
+            }
+            return b.p;
+        }
+    }
+    for (int attempt = 0; attempt < 2; attempt++) {
+        void *p = nullptr;
+        hipError_t e = stream ? hipMallocAsync(&p, want, stream) : hipMalloc(&p, want);
+        if (e == hipSuccess) return p;
+        (void)hipGetLastError();
+        if (attempt == 0) {
+            // out of memory: give the cached blocks back and try once more
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            if (g_cache.empty()) break;
+            trim_locked(0);
+            continue;
+        }
+        throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(bytes) +
+                                        " bytes failed: " + hipGetErrorString(e));
+    }
+    throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+}
+
+void irtt::dev_free(void *p, size_t bytes, hipStream_t stream)
+{
+    if (!p) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    hipEvent_t ev = take_event_locked(dev);
+    if (ev && hipEventRecord(ev, stream) != hipSuccess) {
+        (void)hipGetLastError();
+        g_events.emplace_back(dev, ev);
+        ev = nullptr;
+    }
+    if (!ev) (void)hipStreamSynchronize(stream);  // no event: wait for the last use here
+    g_cache.push_back(CachedBlock{p, round_bytes(bytes), ev, dev, ++g_cache_clock});
+    g_cache_bytes += round_bytes(bytes);
+    if (g_cache_bytes > kCacheMax) trim_locked(kCacheMax / 2);
+}
+
+size_t irtt::dev_cache_bytes()
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    return g_cache_bytes;
+}
+
+hipStream_t irtt::stream_take()
+{
+    int dev = 0;
+    IRTT_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        for (size_t i = g_streams.size(); i-- > 0;)
+            if (g_streams[i].first == dev) {
+                hipStream_t st = g_streams[i].second;
+                g_streams.erase(g_streams.begin() + i);
+                return st;
+            }
+    }
+    hipStream_t st = nullptr;
+    IRTT_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    return st;
+}
+
+void irtt::stream_give(hipStream_t st)
+{
+    if (!st) return;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    (void)hipStreamSynchronize(st);
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_streams.emplace_back(dev, st);
+}
+
+// process-wide pool of pinned host blocks (never freed: freeing pinned
+// memory synchronises the device)
+namespace {
+std::mutex g_pin_mu;
+std::vector<std::pair<void *, size_t>> g_pin_free;
+}  // namespace
+
+void *irtt::pinned_take(size_t bytes, size_t &got)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        size_t best = g_pin_free.size();
+        for (size_t i = 0; i < g_pin_free.size(); i++)
+            if (g_pin_free[i].second >= bytes &&
+                (best == g_pin_free.size() || g_pin_free[i].second < g_pin_free[best].second))
+                best = i;
+        if (best < g_pin_free.size()) {
+            void *p = g_pin_free[best].first;
+            got = g_pin_free[best].second;
+            g_pin_free.erase(g_pin_free.begin() + best);
+            return p;
+        }
+    }
+    void *p = nullptr;
+    IRTT_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    got = bytes;
+    return p;
+}
+
+void irtt::pinned_give(void *p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.emplace_back(p, bytes);
+}
+
 void irtt::set_last_error(const std::string &msg) { g_err = msg; }
 
 thread_local hipStream_t irtt::t_alloc_stream = nullptr;
@@ -340,7 +536,7 @@ int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32
         s->bin_kelvin0.assign(s->nb, 0.0);
         s->bin_fixed.assign(s->nb, 0);
         s->x.assign(s->nb, 0.0);
-        IRTT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        s->stream = stream_take();
         IRTT_HIP(hipEventCreate(&s->ev0));
         IRTT_HIP(hipEventCreate(&s->ev1));
         IRTT_HIP(hipEventCreate(&s->ev2));
@@ -353,7 +549,7 @@ int irtt_scene_create(const float *xyz, uint32_t nv, const uint32_t *tri, uint32
             s.reset();
             (void)hipStreamSynchronize(st);
             for (hipEvent_t x : e) (void)hipEventDestroy(x);
-            (void)hipStreamDestroy(st);
+            stream_give(st);
             t_alloc_stream = nullptr;
             throw;
         }
@@ -374,7 +570,7 @@ int irtt_scene_destroy(irtt_scene h)
         if (st) (void)hipStreamSynchronize(st);
         for (hipEvent_t x : e)
             if (x) (void)hipEventDestroy(x);
-        if (st) (void)hipStreamDestroy(st);
+        stream_give(st);
     });
 }
 

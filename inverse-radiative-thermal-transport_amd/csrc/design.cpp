@@ -63,9 +63,20 @@ void design_gradient(Scene &s, const irtt_solver_params &p, const double *dLdx,
                      const uint32_t *param, const float *dxdp, uint32_t n_params, double *dp,
                      double *grad_out, double *lam_out, irtt_solve_stats &st)
 {
-    const uint32_t nb = s.nb, nv = s.nv;
+    const uint32_t nb = s.nb;
     std::vector<double> x = s.x, lam(nb);
     adjoint_solve(s, p, x.data(), dLdx, lam.data(), st);
+    gradient_from_adjoint(s, p, x.data(), lam.data(), ap, nnz, vertex, param, dxdp, n_params, dp,
+                          grad_out);
+    if (lam_out) std::copy(lam.begin(), lam.end(), lam_out);
+}
+
+void gradient_from_adjoint(Scene &s, const irtt_solver_params &p, const double *x,
+                           const double *lam, const irtt_adjoint_params &ap, uint32_t nnz,
+                           const uint32_t *vertex, const uint32_t *param, const float *dxdp,
+                           uint32_t n_params, double *dp, double *grad_out)
+{
+    const uint32_t nb = s.nb, nv = s.nv;
     // history rows: (s lambda, x^4) for the operator term T x^4, and
     // (-s Ta^4, lambda) for the ambient term Ta^4 colsum(T)
     const double sc = p.steady ? 1.0 : (p.dt > 0.0 ? p.dt : 300.0);
@@ -92,7 +103,6 @@ void design_gradient(Scene &s, const irtt_solver_params &p, const double *dLdx,
         if (rc) throw Error(rc, "projection: " + std::string(irtt_last_error()));
     }
     if (grad_out) std::copy(grad.begin(), grad.end(), grad_out);
-    if (lam_out) std::copy(lam.begin(), lam.end(), lam_out);
 }
 
 }  // namespace irtt

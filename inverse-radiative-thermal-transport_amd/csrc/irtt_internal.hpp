@@ -40,6 +40,22 @@ constexpr uint32_t kMaxDepthDefault = 512;     // MAX_MAX_DEPTH, thermal_tracing
 // nullptr: plain hipMalloc / hipFree.
 extern thread_local hipStream_t t_alloc_stream;
 
+// Device memory through a process-wide cache (api.cpp): a freed block is kept
+// with an event recorded on the freeing stream, and a later allocation of a
+// similar size takes it back (its stream waits for that event on the device,
+// never on the host).  The stream-ordered pool's own allocate / free calls
+// cost milliseconds each at these sizes and stall concurrent scenes; the
+// cache makes the steady state of a request loop or a design loop free of
+// them.  Throws Error(IRTT_ERR_NOMEM) when the device is out of memory.
+void *dev_alloc(size_t bytes, hipStream_t stream);
+void dev_free(void *p, size_t bytes, hipStream_t stream);
+// bytes held by the cache for reuse (free for this process's allocations)
+size_t dev_cache_bytes();
+// streams are reused across scenes the same way (creating one can take
+// tens of milliseconds)
+hipStream_t stream_take();
+void stream_give(hipStream_t st);
+
 // Device buffer with RAII release.  alloc() keeps the allocation when the new
 // count fits in what it holds (grow-only): repeated builds and traces of one
 // scene whose sizes drift (a design loop) reuse their buffers instead of
@@ -56,10 +72,7 @@ struct DBuf {
     ~DBuf() { release(); }
     void release()
     {
-        if (p) {
-            if (st) (void)hipFreeAsync(p, st);
-            else (void)hipFree(p);
-        }
+        if (p) dev_free(p, cap * sizeof(T), st);
         p = nullptr;
         n = 0;
         cap = 0;
@@ -73,23 +86,8 @@ struct DBuf {
         }
         release();
         if (count == 0) return;
-        const size_t b = count * sizeof(T);
-        hipError_t e = hipErrorNotSupported;
-        if (t_alloc_stream) {
-            e = hipMallocAsync(reinterpret_cast<void **>(&p), b, t_alloc_stream);
-            if (e == hipSuccess) st = t_alloc_stream;
-        }
-        if (e == hipErrorNotSupported) {
-            (void)hipGetLastError();
-            e = hipMalloc(reinterpret_cast<void **>(&p), b);
-        }
-        if (e != hipSuccess) {
-            p = nullptr;
-            st = nullptr;
-            (void)hipGetLastError();
-            throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(b) +
-                                            " bytes failed: " + hipGetErrorString(e));
-        }
+        p = reinterpret_cast<T *>(dev_alloc(count * sizeof(T), t_alloc_stream));
+        st = t_alloc_stream;
         n = count;
         cap = count;
     }
@@ -220,9 +218,13 @@ struct Aca {
     }
 };
 
-// Page-locked host memory for a scene's small read-backs (hipHostMalloc),
-// grown on demand and freed with the scene.  A caller synchronizes the
-// stream that last copied into it before asking for a larger block.
+// Pinned host blocks for read-backs, taken from and returned to a
+// process-wide pool (pinned_take / pinned_give, api.cpp): freeing pinned
+// memory synchronises the whole device, which would stall every other
+// scene's stream each time a served request ends.
+void *pinned_take(size_t bytes, size_t &got);
+void pinned_give(void *p, size_t bytes);
+
 struct HostPin {
     void *p = nullptr;
     size_t n = 0;
@@ -231,16 +233,17 @@ struct HostPin {
     HostPin &operator=(const HostPin &) = delete;
     ~HostPin()
     {
-        if (p) (void)hipHostFree(p);
+        if (p) pinned_give(p, n);
     }
+    // a block of at least `bytes`; the caller has synchronised the stream
+    // that last copied into the current block before asking for a larger one
     void *get(size_t bytes)
     {
         if (bytes <= n) return p;
-        if (p) (void)hipHostFree(p);
+        if (p) pinned_give(p, n);
         p = nullptr;
         n = 0;
-        IRTT_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
-        n = bytes;
+        p = pinned_take(bytes, n);
         return p;
     }
 };
@@ -370,6 +373,12 @@ void design_gradient(Scene &s, const irtt_solver_params &p, const double *dLdx,
                      const irtt_adjoint_params &ap, uint32_t nnz, const uint32_t *vertex,
                      const uint32_t *param, const float *dxdp, uint32_t n_params, double *dp,
                      double *grad_out, double *lam_out, irtt_solve_stats &st);
+// the adjoint trace, source term and projection of design_gradient for a
+// given adjoint state lam at state x (design.cpp)
+void gradient_from_adjoint(Scene &s, const irtt_solver_params &p, const double *x,
+                           const double *lam, const irtt_adjoint_params &ap, uint32_t nnz,
+                           const uint32_t *vertex, const uint32_t *param, const float *dxdp,
+                           uint32_t n_params, double *dp, double *grad_out);
 // batch.hip
 void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double *source,
                  double *kelvin, irtt_solve_stats &st);

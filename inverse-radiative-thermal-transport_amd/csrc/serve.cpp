@@ -42,13 +42,37 @@ struct Req {
     std::vector<uint32_t> tri, vertex, param;
     std::vector<irtt_material> mat;
     std::vector<uint8_t> fixed;
-    std::vector<double> target;
+    std::vector<double> target, source;
     irtt_request r{};  // scalars and output pointers
     uint32_t nb = 0;
-    uint64_t footprint = 0;
+    uint64_t footprint = 0;   // HBM counted for this request while it runs
     int state = IRTT_REQ_QUEUED;
     Clock::time_point t_submit, t_start, t_end;
 };
+
+// at most this many requests of one group share a scene and a batched solve
+constexpr size_t kMaxGroup = 32;
+
+bool same_bytes(const void *a, const void *b, size_t n) { return n == 0 || !std::memcmp(a, b, n); }
+
+template <typename T>
+bool same_vec(const std::vector<T> &a, const std::vector<T> &b)
+{
+    return a.size() == b.size() && same_bytes(a.data(), b.data(), a.size() * sizeof(T));
+}
+
+// requests a and b may share one scene, transport build and batched solves
+bool shareable(const Req &a, const Req &b)
+{
+    const irtt_request &x = a.r, &y = b.r;
+    return x.group && x.group == y.group && x.nv == y.nv && x.nt == y.nt &&
+           x.bin_mode == y.bin_mode && x.operator_kind == y.operator_kind &&
+           same_bytes(&x.transport, &y.transport, sizeof x.transport) &&
+           same_bytes(&x.solver, &y.solver, sizeof x.solver) && x.solver.steady &&
+           x.solver.jacobian == IRTT_JAC_EXACT && same_vec(a.xyz, b.xyz) &&
+           same_vec(a.tri, b.tri) && same_vec(a.mat, b.mat) && same_vec(a.kelvin0, b.kelvin0) &&
+           same_vec(a.fixed, b.fixed) && a.target.empty() == b.target.empty();
+}
 
 // HBM needed by one request: fixed-point + FP32 operator (+ FP16 copy), or
 // for the sparse layout the peak of its build (hash table + the two key /
@@ -91,48 +115,69 @@ struct Server {
     bool stop = false;
     std::vector<std::thread> workers;
 
-    // first queued request whose footprint fits; nullptr if none
-    Req *admit_locked()
+    // first queued request whose footprint fits, with the queued requests
+    // that can share its scene (same group, identical inputs; the first
+    // one's footprint plus per-design vectors); empty if none fits
+    std::vector<Req *> admit_locked()
     {
+        std::vector<Req *> g;
         for (auto it = queue.begin(); it != queue.end(); ++it) {
-            if ((*it)->footprint <= budget - in_use) {
-                Req *q = *it;
-                queue.erase(it);
-                in_use += q->footprint;
-                q->state = IRTT_REQ_RUNNING;
-                q->t_start = Clock::now();
-                return q;
+            if ((*it)->footprint > budget - in_use) continue;
+            Req *q = *it;
+            uint64_t need = q->footprint;
+            g.push_back(q);
+            if (q->r.group) {
+                const uint64_t extra = 64ull * 8 * q->nb + (64ull << 10);
+                for (auto jt = std::next(it); jt != queue.end() && g.size() < kMaxGroup; ++jt)
+                    if (shareable(*q, **jt) && need + extra <= budget - in_use) {
+                        g.push_back(*jt);
+                        need += extra;
+                    }
             }
+            for (Req *r : g) {
+                queue.erase(std::find(queue.begin(), queue.end(), r));
+                r->state = IRTT_REQ_RUNNING;
+                r->t_start = Clock::now();
+            }
+            // the group's memory is counted on its first request
+            g[0]->footprint = need;
+            for (size_t k = 1; k < g.size(); k++) g[k]->footprint = 0;
+            in_use += need;
+            return g;
         }
-        return nullptr;
+        return g;
     }
 
     void worker()
     {
         for (;;) {
-            Req *q = nullptr;
+            std::vector<Req *> g;
             {
                 std::unique_lock<std::mutex> lk(m);
-                cv_work.wait(lk, [&] { return stop || (q = admit_locked()) != nullptr; });
-                if (!q) return;  // stop with nothing admissible left
+                cv_work.wait(lk, [&] { return stop || !(g = admit_locked()).empty(); });
+                if (g.empty()) return;  // stop with nothing admissible left
             }
             // a request is done once its results are in the caller's buffers;
             // its memory stays counted until its scene has been released
             auto publish = [&] {
                 {
                     std::lock_guard<std::mutex> lk(m);
-                    q->t_end = Clock::now();
-                    q->state = IRTT_REQ_DONE;
+                    for (Req *q : g) {
+                        q->t_end = Clock::now();
+                        q->state = IRTT_REQ_DONE;
+                    }
                 }
                 cv_done.notify_all();
             };
-            int rc = run(*q, publish);
+            int rc = g.size() == 1 ? run(*g[0], publish) : run_group(g, publish);
             {
                 std::lock_guard<std::mutex> lk(m);
-                in_use -= q->footprint;
-                if (q->state == IRTT_REQ_RUNNING) {
-                    q->t_end = Clock::now();
-                    q->state = rc == IRTT_OK ? IRTT_REQ_DONE : rc;
+                for (Req *q : g) {
+                    in_use -= q->footprint;
+                    if (q->state == IRTT_REQ_RUNNING) {
+                        q->t_end = Clock::now();
+                        q->state = rc == IRTT_OK ? IRTT_REQ_DONE : rc;
+                    }
                 }
             }
             cv_done.notify_all();
@@ -141,7 +186,54 @@ struct Server {
     }
 
     static int run(Req &q, const std::function<void()> &publish);
+    static int run_group(const std::vector<Req *> &g, const std::function<void()> &publish);
 };
+
+namespace {
+
+// scene of a request with its initial state set; nullptr and rc on failure
+irtt_scene make_scene(const Req &q, int &rc)
+{
+    const irtt_request &r = q.r;
+    irtt_scene h = nullptr;
+    rc = irtt_scene_create(q.xyz.data(), r.nv, q.tri.data(), r.nt, q.mat.data(), r.bin_mode, &h);
+    if (rc) return nullptr;
+    if (!q.kelvin0.empty() || !q.fixed.empty()) {
+        const uint32_t n = r.bin_mode == IRTT_BIN_VERTEX ? r.nv : r.nt;
+        std::vector<float> k0 = q.kelvin0.empty() ? std::vector<float>(n, 0.0f) : q.kelvin0;
+        std::vector<uint8_t> fx = q.fixed.empty() ? std::vector<uint8_t>(n, 0) : q.fixed;
+        rc = r.bin_mode == IRTT_BIN_VERTEX ? irtt_scene_set_vertex_state(h, k0.data(), fx.data())
+                                           : irtt_scene_set_bin_state(h, k0.data(), fx.data());
+        if (rc) {
+            irtt_scene_destroy(h);
+            return nullptr;
+        }
+    }
+    return h;
+}
+
+// the objective L = 0.5 w sum_i A_i (x_i - target_i)^2 and dL/dx
+// (thermal_objective.cpp:34,47-57)
+double objective(const Req &q, const double *x, const std::vector<double> &area, double *dLdx)
+{
+    double L = 0.0;
+    for (uint32_t i = 0; i < q.nb; i++) {
+        const double d = x[i] - q.target[i];
+        L += 0.5 * q.r.objective_weight * area[i] * d * d;
+        dLdx[i] = q.r.objective_weight * area[i] * d;
+    }
+    return L;
+}
+
+irtt_adjoint_params adjoint_of(const irtt_request &r)
+{
+    irtt_adjoint_params ap = r.adjoint;
+    if (!(ap.edge_ratio > 0.0f)) ap.edge_ratio = 0.05f;
+    if (!(ap.tiny_eps > 0.0f)) ap.tiny_eps = 1e-2f;
+    return ap;
+}
+
+}  // namespace
 
 // One design: scene -> transport -> solve -> adjoint solve -> adjoint trace
 // -> projection (the steps of ThermalRenderer::runForwardSim / runAdjointSim
@@ -149,23 +241,16 @@ struct Server {
 int Server::run(Req &q, const std::function<void()> &publish)
 {
     const irtt_request &r = q.r;
-    irtt_scene h = nullptr;
-    int rc = irtt_scene_create(q.xyz.data(), r.nv, q.tri.data(), r.nt, q.mat.data(), r.bin_mode, &h);
-    if (rc) return rc;
+    int rc = IRTT_OK;
+    irtt_scene h = make_scene(q, rc);
+    if (!h) return rc;
     auto done = [&](int code) {
         if (code == IRTT_OK) publish();
         irtt_scene_destroy(h);
         return code;
     };
     const uint32_t nb = q.nb;
-    if (!q.kelvin0.empty() || !q.fixed.empty()) {
-        const uint32_t n = r.bin_mode == IRTT_BIN_VERTEX ? r.nv : r.nt;
-        std::vector<float> k0 = q.kelvin0.empty() ? std::vector<float>(n, 0.0f) : q.kelvin0;
-        std::vector<uint8_t> fx = q.fixed.empty() ? std::vector<uint8_t>(n, 0) : q.fixed;
-        rc = r.bin_mode == IRTT_BIN_VERTEX ? irtt_scene_set_vertex_state(h, k0.data(), fx.data())
-                                           : irtt_scene_set_bin_state(h, k0.data(), fx.data());
-        if (rc) return done(rc);
-    }
+    if (!q.source.empty() && (rc = irtt_scene_set_source(h, q.source.data()))) return done(rc);
     irtt_transport_stats ts{};
     // an adjoint trace follows when there is an objective: keep each
     // photon's absorbing bin so it replays every photon once
@@ -181,24 +266,86 @@ int Server::run(Req &q, const std::function<void()> &publish)
     if (r.kelvin) std::memcpy(r.kelvin, x.data(), nb * sizeof(double));
     if (q.target.empty()) return done(IRTT_OK);
 
-    // objective and its gradient in x (thermal_objective.cpp:34,47-57)
     std::vector<double> area(nb), dLdx(nb);
     if ((rc = irtt_scene_get_bin_data(h, area.data(), nullptr, nullptr))) return done(rc);
-    double L = 0.0;
-    for (uint32_t i = 0; i < nb; i++) {
-        double d = x[i] - q.target[i];
-        L += 0.5 * r.objective_weight * area[i] * d * d;
-        dLdx[i] = r.objective_weight * area[i] * d;
-    }
+    const double L = objective(q, x.data(), area, dLdx.data());
     if (r.objective) *r.objective = L;
     // adjoint solve, adjoint trace, source term, projection onto the parameters
-    irtt_adjoint_params ap{0.05f, 1e-2f, 0u, 0u};
+    const irtt_adjoint_params ap = adjoint_of(r);
     const bool want_dp = r.dp && r.n_params;
     if ((rc = irtt_design_gradient(h, &r.solver, dLdx.data(), &ap, want_dp ? r.nnz : 0,
                                    q.vertex.data(), q.param.data(), q.dxdp.data(),
                                    want_dp ? r.n_params : 0, want_dp ? r.dp : nullptr, nullptr,
                                    nullptr, &ss)))
         return done(rc);
+    return done(IRTT_OK);
+}
+
+// Designs that share an operator (one group, identical inputs but source and
+// target): one scene and transport build; the steady forward solves of all
+// of them in lockstep (irtt_solve_batch: batched operator products), then
+// the adjoint solves in lockstep (irtt_adjoint_solve_batch), then each
+// design's adjoint trace, source term and projection.
+int Server::run_group(const std::vector<Req *> &g, const std::function<void()> &publish)
+{
+    const Req &q0 = *g[0];
+    const irtt_request &r0 = q0.r;
+    const uint32_t R = (uint32_t)g.size(), nb = q0.nb;
+    int rc = IRTT_OK;
+    irtt_scene h = make_scene(q0, rc);
+    if (!h) return rc;
+    auto done = [&](int code) {
+        if (code == IRTT_OK) publish();
+        irtt_scene_destroy(h);
+        return code;
+    };
+    try {
+        Scene &s = *reinterpret_cast<Scene *>(h);
+        t_alloc_stream = s.stream;
+        irtt_transport_params tp = r0.transport;
+        if (!q0.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_ABSORBERS;
+        irtt_transport_stats ts{};
+        if ((rc = irtt_build_transport(h, &tp, &ts))) return done(rc);
+        if (r0.operator_kind != IRTT_OP_DENSE && (rc = irtt_select_operator(h, r0.operator_kind)))
+            return done(rc);
+        t_alloc_stream = s.stream;
+        // per-design sources: the request's own, else the materials'
+        std::vector<double> src((size_t)R * nb), X((size_t)R * nb);
+        for (uint32_t k = 0; k < R; k++) {
+            const std::vector<double> &sk = g[k]->source.empty() ? s.bin_source : g[k]->source;
+            std::copy(sk.begin(), sk.end(), src.begin() + (size_t)k * nb);
+        }
+        irtt_solve_stats ss{};
+        solve_batch(s, r0.solver, R, src.data(), X.data(), ss);
+        for (uint32_t k = 0; k < R; k++)
+            if (g[k]->r.kelvin)
+                std::memcpy(g[k]->r.kelvin, X.data() + (size_t)k * nb, nb * sizeof(double));
+        if (q0.target.empty()) return done(IRTT_OK);
+        std::vector<double> rhs((size_t)R * nb), lam((size_t)R * nb);
+        for (uint32_t k = 0; k < R; k++) {
+            const double L = objective(*g[k], X.data() + (size_t)k * nb, s.bin_area,
+                                       rhs.data() + (size_t)k * nb);
+            if (g[k]->r.objective) *g[k]->r.objective = L;
+        }
+        adjoint_solve_batch(s, r0.solver, R, X.data(), rhs.data(), lam.data(), ss);
+        for (uint32_t k = 0; k < R; k++) {
+            const Req &q = *g[k];
+            const bool want_dp = q.r.dp && q.r.n_params;
+            // the source term follows the design's own source
+            s.source_explicit = !q.source.empty();
+            gradient_from_adjoint(s, q.r.solver, X.data() + (size_t)k * nb,
+                                  lam.data() + (size_t)k * nb, adjoint_of(q.r),
+                                  want_dp ? q.r.nnz : 0, q.vertex.data(), q.param.data(),
+                                  q.dxdp.data(), want_dp ? q.r.n_params : 0,
+                                  want_dp ? q.r.dp : nullptr, nullptr);
+        }
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return done(e.code);
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return done(IRTT_ERR_STATE);
+    }
     return done(IRTT_OK);
 }
 
@@ -294,6 +441,7 @@ int irtt_serve_submit(irtt_server h, const irtt_request *r, uint64_t *ticket)
         if (r->kelvin0) q->kelvin0.assign(r->kelvin0, r->kelvin0 + ns);
         if (r->fixed) q->fixed.assign(r->fixed, r->fixed + ns);
         if (r->target) q->target.assign(r->target, r->target + q->nb);
+        if (r->source) q->source.assign(r->source, r->source + q->nb);
         if (r->nnz) {
             q->vertex.assign(r->vertex, r->vertex + r->nnz);
             q->param.assign(r->param, r->param + r->nnz);

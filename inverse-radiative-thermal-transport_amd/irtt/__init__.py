@@ -113,6 +113,11 @@ class ServeConfig(ctypes.Structure):
                 ("memory_budget", ctypes.c_uint64)]
 
 
+class AdjointParams(ctypes.Structure):
+    _fields_ = [("edge_ratio", ctypes.c_float), ("tiny_eps", ctypes.c_float),
+                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
 class Request(ctypes.Structure):
     _fields_ = [("xyz", _vp), ("nv", ctypes.c_uint32), ("nt", ctypes.c_uint32), ("tri", _vp),
                 ("mat", _vp), ("bin_mode", ctypes.c_uint32), ("operator_kind", ctypes.c_uint32),
@@ -120,7 +125,8 @@ class Request(ctypes.Structure):
                 ("solver", SolverParams), ("target", _vp), ("objective_weight", ctypes.c_double),
                 ("nnz", ctypes.c_uint32), ("n_params", ctypes.c_uint32), ("vertex", _vp),
                 ("param", _vp), ("dxdp", _vp), ("kelvin", _vp), ("dp", _vp),
-                ("objective", _vp)]
+                ("objective", _vp), ("adjoint", AdjointParams), ("source", _vp),
+                ("group", ctypes.c_uint64)]
 
 
 class RequestStatus(ctypes.Structure):
@@ -231,11 +237,6 @@ ADJ_EDGE_GAIN = 16
 ADJ_HIT_AREA = 32
 ADJ_EDGE_PROJECTED = 64
 ADJ_COLLIMATED = 128
-
-
-class AdjointParams(ctypes.Structure):
-    _fields_ = [("edge_ratio", ctypes.c_float), ("tiny_eps", ctypes.c_float),
-                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class AdjointStats(ctypes.Structure):
@@ -671,8 +672,13 @@ class Server:
         self._keep = {}
 
     def submit(self, sd, bin_mode=BIN_VERTEX, operator=OP_DENSE, transport=None, solver=None,
-               kelvin0=None, fixed=None, target=None, objective_weight=1.0, params=None):
+               kelvin0=None, fixed=None, target=None, objective_weight=1.0, params=None,
+               adjoint_flags=0, source=None, group=0):
         """sd: SceneData.  params: (vertex, param, dxdp, n_params) for dL/dp.
+        adjoint_flags: ADJ_* terms of the trace (0: area | continuous | edge).
+        source: per-bin external power replacing the materials' flux.  group:
+        nonzero to share one scene, build and batched solves with queued
+        requests of the same group and identical inputs but source and target.
         Returns a ticket; outputs come back from wait()."""
         nb = sd.nv if bin_mode == BIN_VERTEX else sd.nt
         xyz = np.ascontiguousarray(sd.xyz, np.float32)
@@ -692,6 +698,14 @@ class Server:
             r.fixed = _ptr(fx)
         r.transport = transport if transport is not None else transport_params()
         r.solver = solver if solver is not None else solver_params()
+        r.adjoint = AdjointParams(0.0, 0.0, adjoint_flags, 0)
+        r.group = group
+        if source is not None:
+            src = np.ascontiguousarray(source, np.float64)
+            if src.shape != (nb,):
+                raise ValueError("source must have n_bins entries")
+            arrays.append(src)
+            r.source = _ptr(src)
         kelvin = np.zeros(nb)
         objective = np.zeros(1)
         dp = None
