@@ -264,6 +264,55 @@ def solve_rates(irtt, s, sd, reps=3, batch=32):
     return out
 
 
+def hmatrix_form(irtt, s, sd, sp, tol=1e-4, max_rank=32, reps=3):
+    """Hierarchical block form of a scene's sparse-layout operator at `tol`
+    and rank <= `max_rank` (FP16 factors with FP32 row scales): block
+    outcomes, stored bytes against the sparse layout, |T - H|_F / |T|_F,
+    compression time, and steady forward + adjoint solves with the H-matrix
+    Krylov products (stored-entry residuals) against the same solves on the
+    compressed rows, max |dT| between the two."""
+    import numpy as np
+    _, ea, _ = s.bin_data()
+    free = ~np.asarray(sd.fixed, bool)
+
+    def solves():
+        best = None
+        for _ in range(reps):
+            s.set_vertex_state(_state0(sd), sd.fixed)
+            t0 = time.perf_counter()
+            fst = s.solve(sp)
+            x = s.temperatures()
+            lam, ast = s.adjoint_solve(np.where(free, ea * (x - x[free].mean()), 0.0), params=sp)
+            t = time.perf_counter() - t0
+            if best is None or t < best[0]:
+                best = (t, x, fst["linear_iters"] + ast["linear_iters"])
+        return best
+
+    s.select_operator(irtt.OP_SPARSE)
+    t_sp, x_sp, it_sp = solves()
+    st = s.compress_hierarchical(tol=tol, max_rank=max_rank, precision=irtt.STORE_F16)
+    t_h, x_h, it_h = solves()
+    s.select_operator(irtt.OP_SPARSE)
+    keys = ("n_clusters", "n_leaves", "n_far", "n_near", "n_lowrank", "n_zero", "n_noconv",
+            "n_reject", "max_rank_used")
+    out = {k: int(st[k]) for k in keys}
+    out.update({
+        "tol": tol, "max_rank": max_rank, "mean_rank": round(st["mean_rank"], 2),
+        "nnz": int(st["nnz"]), "nnz_near": int(st["nnz_near"]),
+        "nnz_in_lowrank": int(st["nnz_lowrank"]),
+        "sparse_MB": round(st["bytes_sparse"] / 1e6, 1),
+        "hmatrix_MB": round(st["bytes_hmatrix"] / 1e6, 1),
+        "lowrank_MB": round(st["bytes_lowrank"] / 1e6, 2),
+        "far_field_norm_share": round(st["fro_far_rel"], 4),
+        "err_frob_rel_fp64_factors": float(f"{st['err_frob_rel']:.3e}"),
+        "err_frob_rel_stored": float(f"{st['err_frob_rel_stored']:.3e}"),
+        "compress_ms": round(st["build_ms"], 1), "aca_work_MB": round(st["build_bytes"] / 1e6, 1),
+        "solves": {"sparse_ms": round(t_sp * 1e3, 2), "sparse_linear_iters": int(it_sp),
+                   "hmatrix_ms": round(t_h * 1e3, 2), "hmatrix_linear_iters": int(it_h),
+                   "max_dT_hmatrix_vs_sparse_K": float(np.abs(x_h - x_sp)[free].max())}})
+    return out
+
+
 def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
     """The same photon transport stored in the sparse layout (device hash
     table -> compressed rows and columns, entries bit-identical to the dense
@@ -346,6 +395,8 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
                                 "occluded_edge_hits": tr[-1]["occluded"],
                                 "segments_per_s_M": round(tr[-1]["segments"] / ms / 1e3, 1),
                                 "forward_photon_kernel_ms": round(st["trace_ms"], 2)}
+        _log("sparse layout: hierarchical blocks")
+        out["hmatrix"] = hmatrix_form(irtt, s, sd, sp)
     out["unit"] = ("sparse layout of the city block operator: build, and forward steady + "
                    "adjoint solves per second (FP32 entries, FP64 sums)")
     return out
@@ -391,6 +442,8 @@ def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thr
         out.update({"objective_K": round(L0, 6), "forward": {k: (round(v, 2) if isinstance(
             v, float) else v) for k, v in fwd.items()},
             "gradient_ms": round(fz.timing["gradient_ms"], 1)})
+        _log("facade: hierarchical blocks")
+        out["hmatrix"] = hmatrix_form(irtt, fz.s, sd, sp, reps=1)
         ks = [int(k) for k in fd_params]
         _log(f"facade: objective {L0:.4f} K, gradient {fz.timing['gradient_ms']:.0f} ms; "
              f"FD on {len(ks)} depths")

@@ -217,6 +217,9 @@ int irtt_get_column_sums(irtt_scene h, double *colsum);
 #define IRTT_OP_DENSE_F16 2u  /* off-diagonal FP16 with an FP32 scale per row, FP64
                                  diagonal; Newton residuals keep the FP32 operator */
 #define IRTT_OP_SPARSE 3u     /* FP32 compressed rows / columns (IRTT_LAYOUT_SPARSE) */
+#define IRTT_OP_HMATRIX 4u    /* hierarchical blocks of irtt_compress_hierarchical (sparse
+                                 layout): Krylov products with the low-rank far blocks +
+                                 sparse near field, Newton residuals with the stored entries */
 
 /* Operator used by irtt_solve / irtt_residual / irtt_adjoint_solve (the
  * reference's transportCompressed switch, thermal_solver.cpp:126-131).
@@ -270,6 +273,59 @@ int irtt_compress_transport(irtt_scene h, const irtt_aca_params *p, irtt_aca_sta
  * NULL. */
 int irtt_get_aca(irtt_scene h, double *U, double *V, double *cdiag, int32_t *piv_rows,
                  int32_t *piv_cols);
+
+/* Hierarchical block compression of the sparse-layout operator: bins in
+ * Morton order, a cluster tree of halved index ranges, far blocks where
+ * min(diam) <= eta dist, each far block cross-approximated (partial-pivot
+ * ACA in FP64 with the reference's residual sums and pivot rule, stop at a
+ * quarter of tol, rank <= max_rank) and kept in low rank when its exact
+ * error |A - U V|_F <= tol |A|_F (and, unless allow_larger, its factors are
+ * smaller than its entries); all other entries stay in a sparse near field.
+ * Replaces the reference's single global ACA (thermal_transport.cpp:281-597,
+ * which stops at a fixed rank: its tolerance test, :439-451, is commented
+ * out) for scenes whose operator is too large for one rank-k form.  Selects
+ * IRTT_OP_HMATRIX. */
+#define IRTT_HBLOCK_ZERO 0u      /* far block without entries */
+#define IRTT_HBLOCK_NOCONV 1u    /* ACA reached max_rank above the tolerance: sparse */
+#define IRTT_HBLOCK_REJECTED 2u  /* exact error above tol, or factors not smaller: sparse */
+#define IRTT_HBLOCK_LOWRANK 3u   /* stored in low rank */
+
+typedef struct irtt_hmatrix_params {
+    double tol;             /* relative Frobenius tolerance per far block (0 -> 1e-4) */
+    double eta;             /* admissibility min(diam) <= eta dist (0 -> 1) */
+    uint32_t max_rank;      /* 1..32 (0 -> 32) */
+    uint32_t leaf_size;     /* cluster leaf size, 8..4096 (0 -> 64) */
+    uint32_t precision;     /* IRTT_STORE_F32 or IRTT_STORE_F16 (one FP32 scale per row) */
+    uint32_t allow_larger;  /* 1: keep converged blocks in low rank even when larger */
+    uint32_t keep_factors;  /* 1: keep the FP64 factors for irtt_get_hmatrix_factors */
+    uint32_t reserved;
+} irtt_hmatrix_params;
+
+typedef struct irtt_hmatrix_stats {
+    uint32_t n_clusters, n_leaves, n_far, n_near;   /* tree and block counts */
+    uint32_t n_lowrank, n_zero, n_noconv, n_reject;  /* far blocks by outcome */
+    uint32_t max_rank_used, reserved;
+    double mean_rank;                 /* over the low-rank blocks */
+    uint64_t nnz, nnz_near, nnz_lowrank;  /* entries: all, near field, inside low-rank blocks */
+    uint64_t bytes_sparse, bytes_hmatrix, bytes_lowrank, build_bytes;
+    double err_frob_rel;              /* |T - H|_F / |T|_F with the FP64 factors */
+    double err_frob_rel_stored;       /* ... with the stored (FP16 / FP32) factors */
+    double fro_far_rel;               /* |T restricted to far blocks|_F / |T|_F */
+    float build_ms, reserved2;
+} irtt_hmatrix_stats;
+
+int irtt_compress_hierarchical(irtt_scene h, const irtt_hmatrix_params *p,
+                               irtt_hmatrix_stats *st);
+/* Block structure: perm (n_bins: Morton position -> bin), *n_far, and per
+ * far block 6 words (r0 r1 c0 c1 in Morton positions, rank, IRTT_HBLOCK_*),
+ * its relative error and its pivots (64 words: 32 rows then 32 columns,
+ * block-local, -1 past the rank; only with keep_factors).  Any pointer may
+ * be NULL. */
+int irtt_get_hmatrix(irtt_scene h, uint32_t *perm, uint32_t *n_far, uint32_t *blocks,
+                     double *err_rel, int32_t *pivots);
+/* FP64 factors of far block b as built (keep_factors): U rank x m, V rank x n,
+ * rank-major (u_l at U + l m). */
+int irtt_get_hmatrix_factors(irtt_scene h, uint32_t b, double *U, double *V);
 
 /* ------------------------------------------------------------------ solve */
 #define IRTT_JAC_EXACT 0u      /* J = T diag(4x^3) + L (+ M/dt) */

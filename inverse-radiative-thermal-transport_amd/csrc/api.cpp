@@ -8,12 +8,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
 
 #include <memory>
 #include <mutex>
+#include <unordered_map>
 #include <new>
 #include <string>
 
@@ -55,9 +57,11 @@ Scene *S(irtt_scene h)
 // the scene's stream).  By default the pool hands freed memory back to the
 // driver at every synchronisation, so a request that creates, builds and
 // destroys a scene maps and unmaps all its buffers again; the pool is told to
-// keep up to kPoolKeep bytes of freed memory for reuse instead.  Once per device.
+// keep up to kPoolKeep bytes of freed memory for reuse instead (with the
+// block cache below, what a long request loop holds stays under about 160 GB
+// of the card's 288 GB).  Once per device.
 #ifndef IRTT_POOL_KEEP_GIB
-#define IRTT_POOL_KEEP_GIB 8
+#define IRTT_POOL_KEEP_GIB 64
 #endif
 constexpr uint64_t kPoolKeep = (uint64_t)IRTT_POOL_KEEP_GIB << 30;
 
@@ -283,13 +287,18 @@ struct CachedBlock {
     hipEvent_t ev;   // recorded on the freeing stream
     int dev;
     uint64_t age;
+    bool async;      // allocated stream-ordered (hipMallocAsync)
 };
 std::mutex g_cache_mu;
 std::vector<CachedBlock> g_cache;
+std::unordered_map<void *, bool> g_async;               // live block -> stream-ordered
 std::vector<std::pair<int, hipEvent_t>> g_events;      // idle events per device
 std::vector<std::pair<int, hipStream_t>> g_streams;    // idle streams per device
 size_t g_cache_bytes = 0;
 uint64_t g_cache_clock = 0;
+// counters: cache hits, fresh allocations, their bytes, nanoseconds in the
+// allocator, blocks handed back by trims, their bytes
+uint64_t g_ctr[6] = {0, 0, 0, 0, 0, 0};
 constexpr size_t kCacheMax = (size_t)IRTT_CACHE_GIB << 30;
 
 size_t round_bytes(size_t b)
@@ -314,21 +323,51 @@ hipEvent_t take_event_locked(int dev)
     return e;
 }
 
-// hands cached blocks back to the device: the oldest until at most `keep`
-// bytes stay cached (the caller holds the lock)
-void trim_locked(size_t keep)
+// takes the oldest cached blocks out of the cache until at most `keep` bytes
+// stay cached (the caller holds the lock); release_blocks hands them back
+std::vector<CachedBlock> trim_locked(size_t keep)
 {
-    while (g_cache_bytes > keep && !g_cache.empty()) {
-        size_t o = 0;
-        for (size_t i = 1; i < g_cache.size(); i++)
-            if (g_cache[i].age < g_cache[o].age) o = i;
-        CachedBlock b = g_cache[o];
-        g_cache.erase(g_cache.begin() + o);
-        g_cache_bytes -= b.bytes;
-        if (b.ev) (void)hipEventSynchronize(b.ev);
-        (void)hipFree(b.p);
-        if (b.ev) g_events.emplace_back(b.dev, b.ev);
+    std::vector<CachedBlock> out;
+    if (g_cache_bytes <= keep) return out;
+    std::vector<size_t> order(g_cache.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::sort(order.begin(), order.end(),
+              [](size_t a, size_t b) { return g_cache[a].age < g_cache[b].age; });
+    std::vector<uint8_t> take(g_cache.size(), 0);
+    for (size_t i : order) {
+        if (g_cache_bytes <= keep) break;
+        take[i] = 1;
+        g_cache_bytes -= g_cache[i].bytes;
     }
+    std::vector<CachedBlock> stay;
+    for (size_t i = 0; i < g_cache.size(); i++) (take[i] ? out : stay).push_back(g_cache[i]);
+    for (const CachedBlock &b : out) {
+        g_ctr[4]++;
+        g_ctr[5] += b.bytes;
+    }
+    g_cache.swap(stay);
+    return out;
+}
+
+// Returns blocks to the device without holding the lock.  A stream-ordered
+// block is freed on `st` after the event of its last use (hipFreeAsync: the
+// host does not wait and the device is not synchronised, so other scenes'
+// streams keep running); a plain hipMalloc block needs hipFree.
+void release_blocks(std::vector<CachedBlock> &v, hipStream_t st)
+{
+    for (CachedBlock &b : v) {
+        if (b.async && st) {
+            if (b.ev) (void)hipStreamWaitEvent(st, b.ev, 0);
+            if (hipFreeAsync(b.p, st) != hipSuccess) (void)hipGetLastError();
+        } else {
+            if (b.ev) (void)hipEventSynchronize(b.ev);
+            if (hipFree(b.p) != hipSuccess) (void)hipGetLastError();
+        }
+    }
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (CachedBlock &b : v)
+        if (b.ev) g_events.emplace_back(b.dev, b.ev);
+    v.clear();
 }
 }  // namespace
 
@@ -349,6 +388,8 @@ void *irtt::dev_alloc(size_t bytes, hipStream_t stream)
             CachedBlock b = g_cache[best];
             g_cache.erase(g_cache.begin() + best);
             g_cache_bytes -= b.bytes;
+            g_async[b.p] = b.async;
+            g_ctr[0]++;
             if (b.ev) {
                 // the block's last use on its old stream finishes first
                 (void)hipStreamWaitEvent(stream, b.ev, 0);
@@ -359,14 +400,28 @@ void *irtt::dev_alloc(size_t bytes, hipStream_t stream)
     }
     for (int attempt = 0; attempt < 2; attempt++) {
         void *p = nullptr;
+        const auto t0 = std::chrono::steady_clock::now();
         hipError_t e = stream ? hipMallocAsync(&p, want, stream) : hipMalloc(&p, want);
-        if (e == hipSuccess) return p;
+        const uint64_t ns = (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+                                std::chrono::steady_clock::now() - t0).count();
+        if (e == hipSuccess) {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            g_async[p] = stream != nullptr;
+            g_ctr[1]++;
+            g_ctr[2] += want;
+            g_ctr[3] += ns;
+            return p;
+        }
         (void)hipGetLastError();
         if (attempt == 0) {
             // out of memory: give the cached blocks back and try once more
-            std::lock_guard<std::mutex> lk(g_cache_mu);
-            if (g_cache.empty()) break;
-            trim_locked(0);
+            std::vector<CachedBlock> v;
+            {
+                std::lock_guard<std::mutex> lk(g_cache_mu);
+                v = trim_locked(0);
+            }
+            if (v.empty()) break;
+            release_blocks(v, stream);
             continue;
         }
         throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(bytes) +
@@ -383,17 +438,34 @@ void irtt::dev_free(void *p, size_t bytes, hipStream_t stream)
         (void)hipGetLastError();
         return;
     }
-    std::lock_guard<std::mutex> lk(g_cache_mu);
-    hipEvent_t ev = take_event_locked(dev);
-    if (ev && hipEventRecord(ev, stream) != hipSuccess) {
-        (void)hipGetLastError();
-        g_events.emplace_back(dev, ev);
-        ev = nullptr;
+    std::vector<CachedBlock> v;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        bool async = false;
+        auto it = g_async.find(p);
+        if (it != g_async.end()) {
+            async = it->second;
+            g_async.erase(it);
+        }
+        hipEvent_t ev = take_event_locked(dev);
+        if (ev && hipEventRecord(ev, stream) != hipSuccess) {
+            (void)hipGetLastError();
+            g_events.emplace_back(dev, ev);
+            ev = nullptr;
+        }
+        if (!ev) (void)hipStreamSynchronize(stream);  // no event: wait for the last use here
+        g_cache.push_back(CachedBlock{p, round_bytes(bytes), ev, dev, ++g_cache_clock, async});
+        g_cache_bytes += round_bytes(bytes);
+        // over the cap: the oldest blocks go back to the device, down to 3/4
+        if (g_cache_bytes > kCacheMax) v = trim_locked(kCacheMax / 4 * 3);
     }
-    if (!ev) (void)hipStreamSynchronize(stream);  // no event: wait for the last use here
-    g_cache.push_back(CachedBlock{p, round_bytes(bytes), ev, dev, ++g_cache_clock});
-    g_cache_bytes += round_bytes(bytes);
-    if (g_cache_bytes > kCacheMax) trim_locked(kCacheMax / 2);
+    if (!v.empty()) release_blocks(v, stream);
+}
+
+void irtt::dev_counters(uint64_t *out)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (int i = 0; i < 6; i++) out[i] = g_ctr[i];
 }
 
 size_t irtt::dev_cache_bytes()
@@ -820,6 +892,30 @@ int irtt_compress_transport(irtt_scene h, const irtt_aca_params *p, irtt_aca_sta
         aca_compress(*s, *p, tmp);
         if (st) *st = tmp;
     });
+}
+
+int irtt_compress_hierarchical(irtt_scene h, const irtt_hmatrix_params *p,
+                               irtt_hmatrix_stats *st)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!p) throw Error(IRTT_ERR_ARG, "params is null");
+        irtt_hmatrix_stats tmp{};
+        hm_compress(*s, *p, tmp);
+        s->op_kind = IRTT_OP_HMATRIX;
+        if (st) *st = tmp;
+    });
+}
+
+int irtt_get_hmatrix(irtt_scene h, uint32_t *perm, uint32_t *n_far, uint32_t *blocks,
+                     double *err_rel, int32_t *pivots)
+{
+    return guard([&] { hm_get(*S(h), perm, n_far, blocks, err_rel, pivots); });
+}
+
+int irtt_get_hmatrix_factors(irtt_scene h, uint32_t b, double *U, double *V)
+{
+    return guard([&] { hm_get_factors(*S(h), b, U, V); });
 }
 
 int irtt_get_aca(irtt_scene h, double *U, double *V, double *cdiag, int32_t *piv_rows,

@@ -588,7 +588,7 @@ void solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, const double
     const double f_tol = p.f_tol > 0 ? p.f_tol : 1e-6;
     const double x_tol = p.x_tol > 0 ? p.x_tol : 1e-4;
     const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
-    const bool f16 = s.op_kind == IRTT_OP_DENSE_F16;
+    const bool f16 = op_refined(s);
     const double krtol = f16 ? std::max(lin_rtol, 1e-6) : lin_rtol;
     stt = irtt_solve_stats{};
     std::vector<double> rn, rn_new;
@@ -701,7 +701,7 @@ void adjoint_solve_batch(Scene &s, const irtt_solver_params &p, uint32_t R, cons
     std::vector<double> rn(c.hsum.begin(), c.hsum.begin() + R);
     const uint32_t max_lin = p.max_linear ? p.max_linear : 200;
     const double lin_rtol = p.lin_rtol > 0 ? p.lin_rtol : 1e-10;
-    const bool f16 = s.op_kind == IRTT_OP_DENSE_F16;
+    const bool f16 = op_refined(s);
     const double krtol = f16 ? std::max(lin_rtol, 1e-6) : lin_rtol;
     stt = irtt_solve_stats{};
     std::vector<uint8_t> live(R, 1);

@@ -51,6 +51,9 @@ void *dev_alloc(size_t bytes, hipStream_t stream);
 void dev_free(void *p, size_t bytes, hipStream_t stream);
 // bytes held by the cache for reuse (free for this process's allocations)
 size_t dev_cache_bytes();
+// cache hits, fresh allocations, their bytes, nanoseconds spent allocating,
+// blocks returned by trims and their bytes (6 counters since process start)
+void dev_counters(uint64_t *out);
 // streams are reused across scenes the same way (creating one can take
 // tens of milliseconds)
 hipStream_t stream_take();
@@ -93,6 +96,20 @@ struct DBuf {
     }
     size_t bytes() const { return n * sizeof(T); }
     size_t held_bytes() const { return cap * sizeof(T); }
+    // alloc(count), returning the pointer
+    T *alloc_get(size_t count)
+    {
+        alloc(count);
+        return p;
+    }
+    // alloc(v.size()) and an asynchronous copy of v on st (v must outlive
+    // the copy: callers synchronise st before v goes out of scope)
+    void upload(const std::vector<T> &v, hipStream_t st)
+    {
+        alloc(v.size());
+        if (!v.empty())
+            IRTT_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    }
 };
 
 // Sparse matrix in CSR, FP64 (conduction Laplacian).
@@ -218,6 +235,67 @@ struct Aca {
     }
 };
 
+constexpr uint32_t kHMaxRank = 32;
+
+// Hierarchical block form of the sparse-layout operator (hmatrix.hip):
+// Morton-ordered compressed rows / columns, far blocks in low rank (U and
+// V^T row-major, FP16 with a scale per row or FP32), the rest as a sparse
+// near field.  Indices below are positions in the Morton order.
+struct Hmat {
+    bool valid = false, keep = false;
+    uint32_t precision = 0, nlr = 0, nleaf = 0, n_itn = 0, n_itt = 0;
+    uint64_t nnz_near = 0, bytes = 0;
+    std::vector<uint32_t> perm;                 // position -> bin
+    std::vector<uint4> far;                     // r0 r1 c0 c1 of every far block
+    std::vector<uint32_t> status, rank;         // per far block (IRTT_HBLOCK_*)
+    std::vector<double> err_rel;
+    std::vector<int32_t> list_pos;              // far block -> ACA list entry (-1: none)
+    std::vector<unsigned long long> list_uoff, list_voff;
+    std::vector<int32_t> piv;                   // 2 * kHMaxRank per list entry
+    DBuf<uint32_t> d_perm;
+    DBuf<uint32_t> prp, pcol, pcp, prow;        // the whole operator, Morton order
+    DBuf<float> pval, pcv;
+    DBuf<uint32_t> nrp, ncol, ncp, nrow;        // near field
+    DBuf<float> nval, ncv;
+    DBuf<double> Ub, Vb;                        // FP64 factors as built (rank-major)
+    DBuf<double> d_err;
+    // accepted blocks: rank, first row / column, factor and scale offsets
+    DBuf<uint32_t> d_rk, d_r0, d_c0;
+    DBuf<unsigned long long> d_uo, d_vo, d_suo, d_svo;
+    DBuf<uint16_t> uh, vh;                      // FP16 factors (bit patterns)
+    DBuf<float> uf, vf, su, sv;                 // FP32 factors; FP16 row scales
+    // product plan
+    DBuf<uint32_t> d_leaf_beg, d_rptr, d_rblk, d_cptr, d_cblk, d_ipn, d_ipt;
+    DBuf<uint4> d_itn, d_itt;
+    DBuf<double> tpart, tq, xp, yp;
+    void reset()
+    {
+        valid = keep = false;
+        nlr = nleaf = n_itn = n_itt = 0;
+        nnz_near = bytes = 0;
+        perm.clear();
+        far.clear();
+        status.clear();
+        rank.clear();
+        err_rel.clear();
+        list_pos.clear();
+        list_uoff.clear();
+        list_voff.clear();
+        piv.clear();
+        for (DBuf<uint32_t> *b : {&d_perm, &prp, &pcol, &pcp, &prow, &nrp, &ncol, &ncp, &nrow, &d_rk,
+                                  &d_r0, &d_c0, &d_leaf_beg, &d_rptr, &d_rblk, &d_cptr, &d_cblk,
+                                  &d_ipn, &d_ipt})
+            b->release();
+        for (DBuf<float> *b : {&pval, &pcv, &nval, &ncv, &uf, &vf, &su, &sv}) b->release();
+        for (DBuf<double> *b : {&Ub, &Vb, &d_err, &tpart, &tq, &xp, &yp}) b->release();
+        for (DBuf<unsigned long long> *b : {&d_uo, &d_vo, &d_suo, &d_svo}) b->release();
+        uh.release();
+        vh.release();
+        d_itn.release();
+        d_itt.release();
+    }
+};
+
 // Pinned host blocks for read-backs, taken from and returned to a
 // process-wide pool (pinned_take / pinned_give, api.cpp): freeing pinned
 // memory synchronises the whole device, which would stall every other
@@ -317,6 +395,7 @@ struct Scene {
     DBuf<double> d_x64;               // padded vectors of the FP64 MFMA GEMM
     DBuf<double> d_gpart64;           // its split-K partial sums
     Aca aca;
+    Hmat hm;
 
     // solver workspace
     DBuf<double> d_vec;               // 12 vectors of nb doubles
@@ -329,10 +408,18 @@ struct Scene {
     // bytes one product with the selected operator form reads
     size_t op_bytes() const
     {
+        if (op_kind == IRTT_OP_HMATRIX) return (size_t)hm.bytes;
         return op_kind == IRTT_OP_SPARSE ? (size_t)sp.nnz * 8
                                          : (size_t)nb * ld * (op_kind == IRTT_OP_DENSE_F16 ? 2 : 4);
     }
 };
+
+// operator forms whose Krylov products approximate the stored operator (FP16
+// entries, low-rank blocks): residuals and refinement use the stored one
+inline bool op_refined(const Scene &s)
+{
+    return s.op_kind == IRTT_OP_DENSE_F16 || s.op_kind == IRTT_OP_HMATRIX;
+}
 
 // api.cpp: thread-local message returned by irtt_last_error()
 void set_last_error(const std::string &msg);
@@ -394,6 +481,12 @@ void sp_download(Scene &s, uint32_t *rowptr, uint32_t *col, float *val);
 void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st);
 void aca_apply(Scene &s, const double *w, double *y, bool transposed);
 void aca_get(Scene &s, double *U, double *V, double *cdiag, int32_t *prow, int32_t *pcol);
+// hmatrix.hip
+void hm_compress(Scene &s, const irtt_hmatrix_params &p, irtt_hmatrix_stats &st);
+void hm_apply(Scene &s, const double *w, double *y, bool transposed);
+void hm_get(Scene &s, uint32_t *perm, uint32_t *n_far, uint32_t *blocks, double *err_rel,
+            int32_t *pivots);
+void hm_get_factors(Scene &s, uint32_t b, double *U, double *V);
 // trace.hip: a forward pass that only records each photon's absorbing bin
 // (absorber: one word per photon, ~0 where none); returns the ray segments
 // traced and of them the accepted ones

@@ -581,17 +581,23 @@ void op_reset(Scene &s)
     s.d_h16f.release();
     s.d_h16ft.release();
     s.aca.reset();
+    s.hm.reset();
 }
 
 void op_select(Scene &s, uint32_t kind)
 {
     if (!s.has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
-    if ((kind == IRTT_OP_SPARSE) != (s.layout == IRTT_LAYOUT_SPARSE))
-        throw Error(IRTT_ERR_STATE, s.layout == IRTT_LAYOUT_SPARSE
-                                        ? "the sparse transport layout offers IRTT_OP_SPARSE only"
-                                        : "IRTT_OP_SPARSE needs the sparse transport layout");
+    const bool sparse_kind = kind == IRTT_OP_SPARSE || kind == IRTT_OP_HMATRIX;
+    if (sparse_kind != (s.layout == IRTT_LAYOUT_SPARSE))
+        throw Error(IRTT_ERR_STATE,
+                    s.layout == IRTT_LAYOUT_SPARSE
+                        ? "the sparse transport layout offers IRTT_OP_SPARSE and IRTT_OP_HMATRIX only"
+                        : "IRTT_OP_SPARSE and IRTT_OP_HMATRIX need the sparse transport layout");
     if (kind == IRTT_OP_SPARSE) {
         // the compressed rows and columns of the build
+    } else if (kind == IRTT_OP_HMATRIX) {
+        if (!s.hm.valid)
+            throw Error(IRTT_ERR_STATE, "no H-matrix operator (irtt_compress_hierarchical)");
     } else if (kind == IRTT_OP_ACA) {
         if (!s.aca.valid) throw Error(IRTT_ERR_STATE, "no ACA operator (irtt_compress_transport)");
     } else if (kind == IRTT_OP_DENSE_F16) {
@@ -644,6 +650,11 @@ void op_apply(Scene &s, const double *w, double *y, bool transposed, bool refine
     hipStream_t st = s.stream;
     uint32_t kind = s.op_kind;
     if (kind == IRTT_OP_DENSE_F16 && refine) kind = IRTT_OP_DENSE;
+    if (kind == IRTT_OP_HMATRIX && refine) kind = IRTT_OP_SPARSE;
+    if (kind == IRTT_OP_HMATRIX) {
+        hm_apply(s, w, y, transposed);
+        return;
+    }
     if (kind == IRTT_OP_ACA) {
         aca_apply(s, w, y, transposed);
         return;
@@ -686,7 +697,7 @@ void op_apply_batch(Scene &s, uint32_t R, const double *X, double *Y, bool trans
 {
     const uint32_t nb = s.nb, ld = s.ld;
     hipStream_t st = s.stream;
-    if (s.op_kind == IRTT_OP_SPARSE) {
+    if (s.op_kind == IRTT_OP_SPARSE || (refine && s.op_kind == IRTT_OP_HMATRIX)) {
         sp_apply_batch(s, R, X, Y, transposed);
         return;
     }

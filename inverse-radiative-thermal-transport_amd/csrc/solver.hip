@@ -956,7 +956,7 @@ __global__ void k_sub_masked(Sys s, const double *__restrict__ a, const double *
 // quantisation level, the outer (FP32-residual) iteration refines instead
 static double krylov_rtol(const Scene &s, double lin_rtol)
 {
-    return s.op_kind == IRTT_OP_DENSE_F16 ? std::max(lin_rtol, 1e-6) : lin_rtol;
+    return op_refined(s) ? std::max(lin_rtol, 1e-6) : lin_rtol;
 }
 
 // Adjoint solve J(x)^T lambda = rhs (thermal_renderer.cpp:932-956): the same
@@ -985,7 +985,7 @@ void adjoint_solve(Scene &s, const irtt_solver_params &p, const double *x, const
     double *res_lam = c.v[V_DX];
     if (rn > 0.0) st.linear_iters = linsolve(c, max_lin, krylov_rtol(s, lin_rtol), rn);
     else IRTT_HIP(hipMemsetAsync(c.v[V_DX], 0, nb * sizeof(double), stm));
-    if (rn > 0.0 && s.op_kind == IRTT_OP_DENSE_F16) {
+    if (rn > 0.0 && op_refined(s)) {
         // iterative refinement: r = J^T lambda - rhs with the FP32 operator,
         // correction from the FP16 Krylov solve
         IRTT_HIP(hipMemcpyAsync(c.v[V_RHS], rhs, nb * sizeof(double), hipMemcpyHostToDevice, stm));

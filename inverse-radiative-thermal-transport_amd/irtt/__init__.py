@@ -197,12 +197,21 @@ def _declare(L):
                                            _vp, _vp, _vp, ctypes.POINTER(SolveStats)]
     L.irtt_compress_transport.argtypes = [_vp, ctypes.POINTER(AcaParams), ctypes.POINTER(AcaStats)]
     L.irtt_get_aca.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.irtt_compress_hierarchical.argtypes = [_vp, ctypes.POINTER(HmatrixParams),
+                                             ctypes.POINTER(HmatrixStats)]
+    L.irtt_get_hmatrix.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_uint32), _vp, _vp, _vp]
+    L.irtt_get_hmatrix_factors.argtypes = [_vp, ctypes.c_uint32, _vp, _vp]
 
 
 OP_DENSE = 0
 OP_ACA = 1
 OP_DENSE_F16 = 2
 OP_SPARSE = 3
+OP_HMATRIX = 4
+HBLOCK_ZERO = 0
+HBLOCK_NOCONV = 1
+HBLOCK_REJECTED = 2
+HBLOCK_LOWRANK = 3
 LAYOUT_DENSE = 0
 LAYOUT_SPARSE = 1
 TRANSPORT_KEEP_ABSORBERS = 1
@@ -227,6 +236,27 @@ class AcaStats(ctypes.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
+class HmatrixParams(ctypes.Structure):
+    _fields_ = [("tol", ctypes.c_double), ("eta", ctypes.c_double),
+                ("max_rank", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
+                ("precision", ctypes.c_uint32), ("allow_larger", ctypes.c_uint32),
+                ("keep_factors", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class HmatrixStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in
+                ("n_clusters", "n_leaves", "n_far", "n_near", "n_lowrank", "n_zero",
+                 "n_noconv", "n_reject", "max_rank_used", "reserved")] + [
+        ("mean_rank", ctypes.c_double)] + [(n, ctypes.c_uint64) for n in
+                                           ("nnz", "nnz_near", "nnz_lowrank", "bytes_sparse",
+                                            "bytes_hmatrix", "bytes_lowrank", "build_bytes")] + [
+        (n, ctypes.c_double) for n in ("err_frob_rel", "err_frob_rel_stored", "fro_far_rel")] + [
+        ("build_ms", ctypes.c_float), ("reserved2", ctypes.c_float)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if not n.startswith("reserved")}
 
 
 ADJ_AREA = 1
@@ -567,6 +597,37 @@ class Scene:
         pc = np.empty(k, np.int32)
         _check(lib().irtt_get_aca(self._h, _ptr(U), _ptr(V), _ptr(C), _ptr(pr), _ptr(pc)))
         return np.ascontiguousarray(U.T), V, C, pr, pc
+
+    def compress_hierarchical(self, tol=1e-4, eta=1.0, max_rank=32, leaf_size=64,
+                              precision=STORE_F16, allow_larger=False, keep_factors=False):
+        """H-matrix form of the sparse-layout operator; selects OP_HMATRIX
+        (irtt_compress_hierarchical)."""
+        p = HmatrixParams(tol, eta, max_rank, leaf_size, precision, int(bool(allow_larger)),
+                          int(bool(keep_factors)), 0)
+        st = HmatrixStats()
+        _check(lib().irtt_compress_hierarchical(self._h, ctypes.byref(p), ctypes.byref(st)))
+        return st.as_dict()
+
+    def hmatrix_blocks(self):
+        """(perm, blocks (n_far x 6: r0 r1 c0 c1 rank status), err_rel,
+        pivots (n_far x 64: 32 rows then 32 columns, -1 past the rank))."""
+        nf = ctypes.c_uint32(0)
+        _check(lib().irtt_get_hmatrix(self._h, None, ctypes.byref(nf), None, None, None))
+        perm = np.empty(self.n_bins, np.uint32)
+        blocks = np.empty((nf.value, 6), np.uint32)
+        err = np.empty(nf.value)
+        piv = np.empty((nf.value, 64), np.int32)
+        _check(lib().irtt_get_hmatrix(self._h, _ptr(perm), ctypes.byref(nf), _ptr(blocks),
+                                      _ptr(err), _ptr(piv)))
+        return perm, blocks, err, piv
+
+    def hmatrix_factors(self, b, blocks):
+        """FP64 factors of far block b as built: (U rank x m, V rank x n)."""
+        r0, r1, c0, c1, k, _ = (int(x) for x in blocks[b])
+        U = np.zeros((k, r1 - r0))
+        V = np.zeros((k, c1 - c0))
+        _check(lib().irtt_get_hmatrix_factors(self._h, b, _ptr(U), _ptr(V)))
+        return U, V
 
     def solve(self, params=None, **kw):
         p = params if params is not None else solver_params(**kw)

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libirtt.so")
 
 SOURCES = ["api.cpp", "compat_api.cpp", "serve.cpp", "history.cpp", "design.cpp", "lbvh.hip", "trace.hip", "operator.hip", "aca.hip",
-           "solver.hip", "batch.hip", "adjoint.hip", "sparse.hip"]
+           "solver.hip", "batch.hip", "adjoint.hip", "sparse.hip", "hmatrix.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

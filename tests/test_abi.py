@@ -48,7 +48,8 @@ def test_struct_layouts_match_the_c_header(irtt, tmp_path):
              "irtt_solve_stats": irtt.SolveStats, "irtt_adjoint_params": irtt.AdjointParams,
              "irtt_adjoint_stats": irtt.AdjointStats, "irtt_aca_params": irtt.AcaParams,
              "irtt_aca_stats": irtt.AcaStats, "irtt_serve_config": irtt.ServeConfig,
-             "irtt_request": irtt.Request, "irtt_request_status": irtt.RequestStatus}
+             "irtt_request": irtt.Request, "irtt_request_status": irtt.RequestStatus,
+             "irtt_hmatrix_params": irtt.HmatrixParams, "irtt_hmatrix_stats": irtt.HmatrixStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "irtt.h"', "int main(void){"]
     for cname, cls in pairs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
