@@ -403,7 +403,7 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
 
 
 def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thread=8,
-           step_size=0.05, fd_params=(6, 93, 162, 167, 0, 109, 240, 255)):
+           step_size=0.05, fd_params=(6, 93, 162, 167, 0, 109, 240, 255), fd_seeds=4):
     """Procedural facade (config 3): 20 buildings, 401k triangles plus the
     collimated sun quad (45 deg elevation, 135 deg azimuth, 800 W/m^2) so
     that balconies and fins shade walls and ground; 256 balcony / fin
@@ -414,7 +414,9 @@ def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thr
     transfer); central differences with common random numbers at h (the
     configured step) and h_wide on `fd_params` depths (balconies of sunlit
     +y / -x facades, of shaded -y facades, fins); then `gd_steps` Adam steps
-    on the depths (projected onto [0, 1.5] m)."""
+    on the depths (projected onto [0, 1.5] m).  The h_wide differences and
+    both gradients are repeated on `fd_seeds` independent photon sets and
+    compared as means with standard errors."""
     import numpy as np
     from irtt import params as P, scenes
     from irtt.design import FacadeDesign
@@ -459,6 +461,58 @@ def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thr
 
         def agree(g, f):
             return int(np.sum(np.sign(g[ks]) == np.sign(f[ks])))
+        # statistics over independent photon sets: at each seed the central
+        # difference (common random numbers on both sides) and both adjoint
+        # gradients; means with standard errors and the z-score of each
+        # adjoint form against the FD mean
+        base_seed = fz.tp.batch_seed
+        per = {"fd": [], "ref": [], "ext": []}
+        for i in range(fd_seeds):
+            fz.tp.batch_seed = base_seed + 7919 * i
+            if i == 0:
+                per["fd"].append(fdw[ks])
+                per["ref"].append(dp_ref[ks])
+                per["ext"].append(dp_ext[ks])
+                continue
+            _log(f"facade: FD statistics, photon set {i + 1} of {fd_seeds}")
+            fz.forward(d0)
+            per["ext"].append(fz.gradient(flags=ext_flags)[ks])
+            per["ref"].append(fz.gradient(flags=ref_flags)[ks])
+            per["fd"].append(fz.fd_gradient(d0, h_wide, dims=ks)[ks])
+        fz.tp.batch_seed = base_seed
+        stat = {}
+        for key, v in per.items():
+            v = np.array(v)
+            stat[key] = (v.mean(0), v.std(0, ddof=1) / np.sqrt(len(v)) if len(v) > 1
+                         else np.zeros(len(ks)))
+
+        def z(key):
+            # paired over the photon sets: adjoint and FD share each set's photons
+            d = np.array(per[key]) - np.array(per["fd"])
+            if len(d) < 2:
+                return np.full(len(ks), np.inf)
+            se = d.std(0, ddof=1) / np.sqrt(len(d))
+            return np.where(se > 0, d.mean(0) / np.where(se > 0, se, 1.0), np.inf)
+        zr, ze = z("ref"), z("ext")
+        seed_rows = [{"param": k, "fd_mean": round(float(stat["fd"][0][i]), 5),
+                      "fd_se": round(float(stat["fd"][1][i]), 5),
+                      "ref_mean": round(float(stat["ref"][0][i]), 5),
+                      "ref_se": round(float(stat["ref"][1][i]), 5),
+                      "ext_mean": round(float(stat["ext"][0][i]), 5),
+                      "ext_se": round(float(stat["ext"][1][i]), 5),
+                      "z_ref": round(float(zr[i]), 2), "z_ext": round(float(ze[i]), 2)}
+                     for i, k in enumerate(ks)]
+        out["fd_statistics"] = {
+            "h_m": h_wide, "photon_sets": fd_seeds, "rows": seed_rows,
+            "within_3se_reference_terms": int(np.sum(np.abs(zr) <= 3.0)),
+            "within_3se_extended": int(np.sum(np.abs(ze) <= 3.0)),
+            "sign_agreement_of_means": {"reference_terms": int(np.sum(
+                np.sign(stat["ref"][0]) == np.sign(stat["fd"][0]))), "extended": int(np.sum(
+                    np.sign(stat["ext"][0]) == np.sign(stat["fd"][0])))},
+            "note": "per photon set (seed): central difference with common random numbers "
+                    "and both adjoint gradients at the same photons; mean and standard "
+                    "error over the sets; z = mean of (adjoint - FD) over the sets / its "
+                    "standard error (paired)"}
         out["fd_check"] = {
             "h_m": h, "h_wide_m": h_wide, "rows": rows,
             "sign_agreement_reference_terms": {"fd": agree(dp_ref, fd),

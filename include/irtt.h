@@ -210,6 +210,13 @@ int irtt_get_transport_sparse(irtt_scene h, uint64_t *nnz, uint32_t *rowptr,
                               uint32_t *col, float *val);
 /* Column sums of the operator, exact from the fixed-point accumulators (FP64). */
 int irtt_get_column_sums(irtt_scene h, double *colsum);
+/* One line of the stored operator (either layout) into out (n_bins FP32):
+ * column = 1: T[:, bin], where the power emitted by `bin` is absorbed (the
+ * reference's "Emission to all" view); column = 0: T[bin, :], where the power
+ * absorbed by `bin` comes from ("Absorption from all").  Replaces the
+ * transport-buffer reads of simple_ray.rgen:413-420 (visualization modes 0
+ * and 1, thermal_gui.cpp:48). */
+int irtt_get_transport_line(irtt_scene h, uint32_t bin, uint32_t column, float *out);
 
 /* --------------------------------------------------------- operator forms */
 #define IRTT_OP_DENSE 0u      /* FP32 dense operator of irtt_build_transport */

@@ -468,8 +468,7 @@ void aca_compress(Scene &s, const irtt_aca_params &p, irtt_aca_stats &st)
         }
     } else if (p.algorithm == IRTT_ACA_FULL) {
         size_t need = (size_t)n * ld * sizeof(double);
-        size_t fr = 0, tot = 0;
-        IRTT_HIP(hipMemGetInfo(&fr, &tot));
+        const size_t fr = dev_available();
         if (need + (64u << 20) > fr)
             throw Error(IRTT_ERR_NOMEM, "full-pivot ACA needs an n_bins^2 FP64 work matrix (" +
                                             std::to_string(need) + " bytes)");

@@ -16,18 +16,24 @@
 //         non-adjacent occluded surface receives +, the path's origin vertex +,
 //         the edge vertex - of the edge-irradiance derivative.
 //
-// Three kernels instead of one, so that no kernel carries another's state:
-//   k_adj_paths  replays the paths (the forward kernel's lane scheduling and
-//                traversal) and scatters the segment terms (ii) and the
-//                hit-area extension; a hit near an edge becomes a 64-byte
-//                edge record (occlusion ray, edge-irradiance derivative,
-//                vertices) instead of a ray traced in the same loop;
-//   k_adj_occl   traces the edge records' occlusion rays and scatters (iii);
-//   k_adj_area   sums (i) per source triangle from the seeds and absorbers.
+// Four stages instead of one kernel, so that no kernel carries another's state:
+//   k_adj_paths     replays the paths (the forward kernel's lane scheduling and
+//                   traversal) and writes one 48-byte segment record per
+//                   accepted segment and one 64-byte edge record per hit near
+//                   an edge (occlusion ray, edge-irradiance derivative,
+//                   vertices), in wave-owned chunks;
+//   k_adj_segments  runs the segment terms (ii) and the hit-area extension
+//                   over the records radix-sorted by start triangle and by
+//                   hit triangle: a window of 64 records sums the terms of
+//                   equal triangles in consecutive lanes (segmented scan) and
+//                   adds each run once;
+//   k_adj_occl      traces the edge records' occlusion rays and scatters (iii);
+//   k_adj_area      sums (i) per source triangle from the seeds and absorbers.
 // Gradients are summed exactly as 128-bit fixed-point integers (a 64-bit
 // atomic add and, on carry or for large magnitudes, a second one on the high
-// word), so the result does not depend on the order of the adds: the trace is
-// bit-reproducible, alone or beside other work.
+// word), one copy per XCD, so the result does not depend on the order of the
+// adds: the trace is bit-reproducible, alone or beside other work.  When the
+// records of all photons do not fit, the photons are replayed in ranges.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -1026,12 +1032,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         1, std::min<uint64_t>((uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU_ADJ,
                               (photons + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
     const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
-    size_t free_b = 0, total_b = 0;
-    IRTT_HIP(hipMemGetInfo(&free_b, &total_b));
     // the record buffers this scene already holds count as available
-    const double budget = std::max(
-        2.0 * (1u << 30),
-        0.3 * ((double)free_b + (double)ws.record_bytes() + (double)dev_cache_bytes()));
+    const double budget =
+        std::max(2.0 * (1u << 30), 0.3 * ((double)dev_available() + (double)ws.record_bytes()));
     uint64_t seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
     uint64_t edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
     const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));

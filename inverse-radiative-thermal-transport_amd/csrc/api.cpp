@@ -462,6 +462,26 @@ void irtt::dev_free(void *p, size_t bytes, hipStream_t stream)
     if (!v.empty()) release_blocks(v, stream);
 }
 
+size_t irtt::dev_available()
+{
+    size_t fr = 0, tot = 0;
+    IRTT_HIP(hipMemGetInfo(&fr, &tot));
+    size_t pooled = 0;
+    int dev = 0;
+    hipMemPool_t pool = nullptr;
+    if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess &&
+        pool) {
+        uint64_t reserved = 0, used = 0;
+        if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReservedMemCurrent, &reserved) ==
+                hipSuccess &&
+            hipMemPoolGetAttribute(pool, hipMemPoolAttrUsedMemCurrent, &used) == hipSuccess &&
+            reserved > used)
+            pooled = (size_t)(reserved - used);
+    }
+    (void)hipGetLastError();
+    return fr + pooled + dev_cache_bytes();
+}
+
 void irtt::dev_counters(uint64_t *out)
 {
     std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -785,8 +805,7 @@ int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_tran
         if (p->layout == IRTT_LAYOUT_DENSE) {
             // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B)
             size_t need = (size_t)s->nb * s->ld * 12;
-            size_t fr = 0, tot = 0;
-            IRTT_HIP(hipMemGetInfo(&fr, &tot));
+            const size_t fr = dev_available();
             size_t held = s->d_acc.bytes() + s->d_T.bytes();
             if (need > fr + held)
                 throw Error(IRTT_ERR_LIMIT, "dense operator needs " + std::to_string(need) +
@@ -1001,6 +1020,46 @@ int irtt_get_column_sums(irtt_scene h, double *colsum)
         if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
         if (!colsum) throw Error(IRTT_ERR_ARG, "colsum is null");
         std::copy(s->colsum.begin(), s->colsum.end(), colsum);
+    });
+}
+
+int irtt_get_transport_line(irtt_scene h, uint32_t bin, uint32_t column, float *out)
+{
+    return guard([&] {
+        Scene *s = S(h);
+        if (!s->has_transport) throw Error(IRTT_ERR_STATE, "transport operator not built");
+        if (!out) throw Error(IRTT_ERR_ARG, "out is null");
+        const uint32_t nb = s->nb;
+        if (bin >= nb) throw Error(IRTT_ERR_ARG, "bin out of range");
+        hipStream_t st = s->stream;
+        if (s->layout == IRTT_LAYOUT_SPARSE) {
+            const Sparse &p = s->sp;
+            const uint32_t *ptr = column ? p.colptr.p : p.rowptr.p;
+            const uint32_t *idx = column ? p.row.p : p.col.p;
+            const float *val = column ? p.cval.p : p.val.p;
+            uint32_t be[2];
+            IRTT_HIP(hipMemcpyAsync(be, ptr + bin, 8, hipMemcpyDeviceToHost, st));
+            IRTT_HIP(hipStreamSynchronize(st));
+            const uint32_t n = be[1] - be[0];
+            std::vector<uint32_t> ix(n);
+            std::vector<float> v(n);
+            if (n) {
+                IRTT_HIP(hipMemcpyAsync(ix.data(), idx + be[0], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+                IRTT_HIP(hipMemcpyAsync(v.data(), val + be[0], (size_t)n * 4, hipMemcpyDeviceToHost, st));
+                IRTT_HIP(hipStreamSynchronize(st));
+            }
+            std::fill(out, out + nb, 0.0f);
+            for (uint32_t k = 0; k < n; k++) out[ix[k]] = v[k];
+            return;
+        }
+        if (!column) {
+            IRTT_HIP(hipMemcpyAsync(out, s->d_T.p + (size_t)bin * s->ld, (size_t)nb * 4,
+                                    hipMemcpyDeviceToHost, st));
+        } else {
+            IRTT_HIP(hipMemcpy2DAsync(out, sizeof(float), s->d_T.p + bin, (size_t)s->ld * 4,
+                                      sizeof(float), nb, hipMemcpyDeviceToHost, st));
+        }
+        IRTT_HIP(hipStreamSynchronize(st));
     });
 }
 

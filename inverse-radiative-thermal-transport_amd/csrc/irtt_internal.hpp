@@ -51,6 +51,9 @@ void *dev_alloc(size_t bytes, hipStream_t stream);
 void dev_free(void *p, size_t bytes, hipStream_t stream);
 // bytes held by the cache for reuse (free for this process's allocations)
 size_t dev_cache_bytes();
+// device memory this process can allocate: free on the device, plus what the
+// stream-ordered pool and the cache hold unused
+size_t dev_available();
 // cache hits, fresh allocations, their bytes, nanoseconds spent allocating,
 // blocks returned by trims and their bytes (6 counters since process start)
 void dev_counters(uint64_t *out);

@@ -410,9 +410,7 @@ int irtt_serve_create(const irtt_serve_config *cfg, irtt_server *out)
         if (sv->cfg.slots > 64) throw Error(IRTT_ERR_ARG, "at most 64 slots");
         sv->budget = sv->cfg.memory_budget;
         if (!sv->budget) {
-            size_t fr = 0, tot = 0;
-            IRTT_HIP(hipMemGetInfo(&fr, &tot));
-            sv->budget = (uint64_t)(0.9 * (double)fr);
+            sv->budget = (uint64_t)(0.9 * (double)irtt::dev_available());
         }
         Server *p = sv.get();
         for (uint32_t i = 0; i < sv->cfg.slots; i++) sv->workers.emplace_back([p] { p->worker(); });

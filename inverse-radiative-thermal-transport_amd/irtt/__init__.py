@@ -161,6 +161,7 @@ def _declare(L):
     L.irtt_get_transport_fixed.argtypes = [_vp, _vp, ctypes.POINTER(ctypes.c_double)]
     L.irtt_get_transport_sparse.argtypes = [_vp, ctypes.POINTER(ctypes.c_uint64), _vp, _vp, _vp]
     L.irtt_get_column_sums.argtypes = [_vp, _vp]
+    L.irtt_get_transport_line.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, _vp]
     L.irtt_solve.argtypes = [_vp, ctypes.POINTER(SolverParams), ctypes.POINTER(SolveStats)]
     L.irtt_get_temperatures.argtypes = [_vp, _vp]
     L.irtt_set_temperatures.argtypes = [_vp, _vp]
@@ -495,6 +496,13 @@ class Scene:
         _check(lib().irtt_get_transport_sparse(self._h, ctypes.byref(nnz), _ptr(rp), _ptr(col),
                                                _ptr(val)))
         return rp, col, val
+
+    def transport_line(self, b, column=True):
+        """T[:, b] (column=True: where b's emission is absorbed) or T[b, :]
+        (where the power b absorbs comes from), FP32 (irtt_get_transport_line)."""
+        out = np.empty(self.n_bins, np.float32)
+        _check(lib().irtt_get_transport_line(self._h, int(b), int(bool(column)), _ptr(out)))
+        return out
 
     def column_sums(self):
         c = np.empty(self.n_bins)
