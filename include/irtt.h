@@ -305,7 +305,9 @@ typedef struct irtt_hmatrix_params {
     uint32_t precision;     /* IRTT_STORE_F32 or IRTT_STORE_F16 (one FP32 scale per row) */
     uint32_t allow_larger;  /* 1: keep converged blocks in low rank even when larger */
     uint32_t keep_factors;  /* 1: keep the FP64 factors for irtt_get_hmatrix_factors */
-    uint32_t reserved;
+    uint32_t workspace_mib; /* FP64 work space of the cross approximations, taken in batches
+                               of blocks (0: min(8 GiB, a quarter of free HBM), at least
+                               1 GiB; all blocks at once with keep_factors) */
 } irtt_hmatrix_params;
 
 typedef struct irtt_hmatrix_stats {
@@ -314,7 +316,8 @@ typedef struct irtt_hmatrix_stats {
     uint32_t max_rank_used, reserved;
     double mean_rank;                 /* over the low-rank blocks */
     uint64_t nnz, nnz_near, nnz_lowrank;  /* entries: all, near field, inside low-rank blocks */
-    uint64_t bytes_sparse, bytes_hmatrix, bytes_lowrank, build_bytes;
+    uint64_t bytes_sparse, bytes_hmatrix, bytes_lowrank;
+    uint64_t build_bytes;             /* peak FP64 work space of the cross approximations */
     double err_frob_rel;              /* |T - H|_F / |T|_F with the FP64 factors */
     double err_frob_rel_stored;       /* ... with the stored (FP16 / FP32) factors */
     double fro_far_rel;               /* |T restricted to far blocks|_F / |T|_F */

@@ -57,11 +57,10 @@ Scene *S(irtt_scene h)
 // the scene's stream).  By default the pool hands freed memory back to the
 // driver at every synchronisation, so a request that creates, builds and
 // destroys a scene maps and unmaps all its buffers again; the pool is told to
-// keep up to kPoolKeep bytes of freed memory for reuse instead (with the
-// block cache below, what a long request loop holds stays under about 160 GB
-// of the card's 288 GB).  Once per device.
+// keep up to kPoolKeep bytes of freed memory for reuse instead.  Once per
+// device.
 #ifndef IRTT_POOL_KEEP_GIB
-#define IRTT_POOL_KEEP_GIB 64
+#define IRTT_POOL_KEEP_GIB 8
 #endif
 constexpr uint64_t kPoolKeep = (uint64_t)IRTT_POOL_KEEP_GIB << 30;
 

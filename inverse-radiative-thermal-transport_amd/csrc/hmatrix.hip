@@ -1041,18 +1041,40 @@ void hm_compress(Scene &s, const irtt_hmatrix_params &pin, irtt_hmatrix_stats &s
     });
     const uint32_t nl = (uint32_t)list.size();
     const uint32_t kmax = p.max_rank;
+    // the FP64 work space of the cross approximations (U, V: (m + n) kmax
+    // doubles per block, plus a byte per row) is taken in batches of the
+    // sorted list that fit a budget (all at once when the factors are kept);
+    // offsets are relative to the batch
     std::vector<unsigned long long> uoff(nl), voff(nl), foff(nl);
-    unsigned long long ut = 0, vt = 0, ft = 0;
-    for (uint32_t q = 0; q < nl; q++) {
-        const uint4 B = far[list[q]];
-        uoff[q] = ut;
-        voff[q] = vt;
-        foff[q] = ft;
-        ut += (unsigned long long)(B.y - B.x) * kmax;
-        vt += (unsigned long long)(B.w - B.z) * kmax;
-        ft += B.y - B.x;
+    std::vector<uint32_t> batch_beg{0};
+    unsigned long long peak_ws = 0;
+    {
+        const unsigned long long budget =
+            p.keep_factors ? ~0ull
+            : p.workspace_mib
+                ? (unsigned long long)p.workspace_mib << 20
+                : std::max<unsigned long long>(
+                      1ull << 30, std::min<unsigned long long>(8ull << 30, dev_available() / 4));
+        unsigned long long ut = 0, vt = 0, ft = 0;
+        for (uint32_t q = 0; q < nl; q++) {
+            const uint4 B = far[list[q]];
+            const unsigned long long m = B.y - B.x, nn = B.w - B.z;
+            const unsigned long long need = (m + nn) * kmax * 8 + m;
+            if (q > batch_beg.back() && (ut + vt) * 8 + ft + need > budget) {
+                batch_beg.push_back(q);
+                ut = vt = ft = 0;
+            }
+            uoff[q] = ut;
+            voff[q] = vt;
+            foff[q] = ft;
+            ut += m * kmax;
+            vt += nn * kmax;
+            ft += m;
+            peak_ws = std::max(peak_ws, (ut + vt) * 8 + ft);
+        }
+        batch_beg.push_back(nl);
     }
-    st.build_bytes = (ut + vt) * 8 + ft;
+    st.build_bytes = peak_ws;
     std::vector<uint32_t> rank(nl), conv(nl);
     h.piv.assign((size_t)nl * 2 * kR, -1);
     std::vector<double> err2(nl, 0.0);
@@ -1064,38 +1086,47 @@ void hm_compress(Scene &s, const irtt_hmatrix_params &pin, irtt_hmatrix_stats &s
     DBuf<unsigned long long> d_uoff, d_voff, d_foff;
     DBuf<int32_t> d_piv;
     DBuf<uint8_t> d_used;
-    if (nl) {
+    const double tol_aca = kAcaMargin * p.tol;
+    // ACA of the list entries [q0, q1) into the batch work space
+    auto run_aca = [&](uint32_t q0, uint32_t q1) {
+        const uint4 L = far[list[q1 - 1]];
+        unsigned long long ut = uoff[q1 - 1] + (unsigned long long)(L.y - L.x) * kmax;
+        unsigned long long vt = voff[q1 - 1] + (unsigned long long)(L.w - L.z) * kmax;
+        unsigned long long ft = foff[q1 - 1] + (L.y - L.x);
         h.Ub.alloc(ut);
         h.Vb.alloc(vt);
         d_used.alloc(ft);
-        d_list.alloc(nl);
+        k_hm_aca<<<q1 - q0, kT, 0, stm>>>(d_fb.p, d_list.p + q0, d_i0.p, h.prp.p, h.pcol.p,
+                                          h.pval.p, h.pcp.p, h.prow.p, h.pcv.p, d_uoff.p + q0,
+                                          d_voff.p + q0, d_foff.p + q0, tol_aca * tol_aca, kmax,
+                                          h.Ub.p, h.Vb.p, d_used.p, d_rank.p + q0, d_conv.p + q0,
+                                          d_piv.p + (size_t)q0 * 2 * kR);
+        IRTT_HIP(hipGetLastError());
+    };
+    if (nl) {
+        d_list.upload(list, stm);
+        d_uoff.upload(uoff, stm);
+        d_voff.upload(voff, stm);
+        d_foff.upload(foff, stm);
         d_rank.alloc(nl);
         d_conv.alloc(nl);
-        d_uoff.alloc(nl);
-        d_voff.alloc(nl);
-        d_foff.alloc(nl);
         d_piv.alloc((size_t)nl * 2 * kR);
-        IRTT_HIP(hipMemcpyAsync(d_list.p, list.data(), (size_t)nl * 4, hipMemcpyHostToDevice, stm));
-        IRTT_HIP(hipMemcpyAsync(d_uoff.p, uoff.data(), (size_t)nl * 8, hipMemcpyHostToDevice, stm));
-        IRTT_HIP(hipMemcpyAsync(d_voff.p, voff.data(), (size_t)nl * 8, hipMemcpyHostToDevice, stm));
-        IRTT_HIP(hipMemcpyAsync(d_foff.p, foff.data(), (size_t)nl * 8, hipMemcpyHostToDevice, stm));
-        const double tol_aca = kAcaMargin * p.tol;
-        k_hm_aca<<<nl, kT, 0, stm>>>(d_fb.p, d_list.p, d_i0.p, h.prp.p, h.pcol.p, h.pval.p, h.pcp.p,
-                                     h.prow.p, h.pcv.p, d_uoff.p, d_voff.p, d_foff.p,
-                                     tol_aca * tol_aca, kmax, h.Ub.p, h.Vb.p, d_used.p, d_rank.p,
-                                     d_conv.p, d_piv.p);
-        IRTT_HIP(hipGetLastError());
+        h.d_err.alloc(nl);
+        for (size_t bt = 0; bt + 1 < batch_beg.size(); bt++) {
+            const uint32_t q0 = batch_beg[bt], q1 = batch_beg[bt + 1];
+            run_aca(q0, q1);
+            // exact error of every block of the batch (blocks that did not
+            // stop get one too; they stay sparse)
+            k_hm_err<false><<<q1 - q0, kT, 0, stm>>>(
+                d_fb.p, d_list.p + q0, d_rank.p + q0, d_uoff.p + q0, d_voff.p + q0, nullptr,
+                nullptr, h.Ub.p, h.Vb.p, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                d_fro2.p, h.prp.p, h.pcol.p, h.pval.p, h.d_err.p + q0);
+            IRTT_HIP(hipGetLastError());
+        }
         IRTT_HIP(hipMemcpyAsync(rank.data(), d_rank.p, (size_t)nl * 4, hipMemcpyDeviceToHost, stm));
         IRTT_HIP(hipMemcpyAsync(conv.data(), d_conv.p, (size_t)nl * 4, hipMemcpyDeviceToHost, stm));
         IRTT_HIP(hipMemcpyAsync(h.piv.data(), d_piv.p, (size_t)nl * 2 * kR * 4,
                                 hipMemcpyDeviceToHost, stm));
-        // exact error of every block that stopped (same kernel for all: the
-        // rank buffer is the per-list-entry rank)
-        k_hm_err<false><<<nl, kT, 0, stm>>>(d_fb.p, d_list.p, d_rank.p, d_uoff.p, d_voff.p, nullptr,
-                                           nullptr, h.Ub.p, h.Vb.p, nullptr, nullptr, nullptr,
-                                           nullptr, nullptr, nullptr, d_fro2.p, h.prp.p, h.pcol.p,
-                                           h.pval.p, h.d_err.alloc_get(nl));
-        IRTT_HIP(hipGetLastError());
         IRTT_HIP(hipMemcpyAsync(err2.data(), h.d_err.p, (size_t)nl * 8, hipMemcpyDeviceToHost, stm));
         IRTT_HIP(hipStreamSynchronize(stm));
     }
@@ -1180,18 +1211,37 @@ void hm_compress(Scene &s, const irtt_hmatrix_params &pin, irtt_hmatrix_stats &s
             h.uf.alloc(fu);
             h.vf.alloc(fv);
         }
-        uint32_t mmax = 0, nmax = 0;
-        for (uint32_t a = 0; a < na; a++) {
-            mmax = std::max(mmax, a_m[a]);
-            nmax = std::max(nmax, a_n[a]);
+        // pack batch by batch: a batch's factors are rebuilt (the same
+        // deterministic steps) unless it is the one still in the work space
+        const size_t nbt = batch_beg.size() - 1;
+        size_t in_ws = nbt - 1;  // the batch whose factors the work space holds
+        size_t a0 = 0;
+        for (size_t bt = 0; bt < nbt; bt++) {
+            const uint32_t q0 = batch_beg[bt], q1 = batch_beg[bt + 1];
+            size_t a1 = a0;
+            while (a1 < na && acc[a1] < q1) a1++;
+            if (a1 == a0) continue;
+            if (bt != in_ws) {
+                run_aca(q0, q1);
+                in_ws = bt;
+            }
+            uint32_t mmax = 0, nmax = 0;
+            for (size_t a = a0; a < a1; a++) {
+                mmax = std::max(mmax, a_m[a]);
+                nmax = std::max(nmax, a_n[a]);
+            }
+            const uint32_t cnt = (uint32_t)(a1 - a0);
+            k_hm_pack<<<dim3(cdiv(mmax, kT), cnt), kT, 0, stm>>>(
+                d_lpos.p + a0, d_m.p + a0, h.d_rk.p + a0, d_uoff.p, h.d_uo.p + a0,
+                h.d_suo.p + a0, h.Ub.p, f16 ? 1u : 0u, reinterpret_cast<__half *>(h.uh.p), h.uf.p,
+                h.su.p);
+            k_hm_pack<<<dim3(cdiv(nmax, kT), cnt), kT, 0, stm>>>(
+                d_lpos.p + a0, d_n.p + a0, h.d_rk.p + a0, d_voff.p, h.d_vo.p + a0,
+                h.d_svo.p + a0, h.Vb.p, f16 ? 1u : 0u, reinterpret_cast<__half *>(h.vh.p), h.vf.p,
+                h.sv.p);
+            IRTT_HIP(hipGetLastError());
+            a0 = a1;
         }
-        k_hm_pack<<<dim3(cdiv(mmax, kT), na), kT, 0, stm>>>(
-            d_lpos.p, d_m.p, h.d_rk.p, d_uoff.p, h.d_uo.p, h.d_suo.p, h.Ub.p, f16 ? 1u : 0u,
-            reinterpret_cast<__half *>(h.uh.p), h.uf.p, h.su.p);
-        k_hm_pack<<<dim3(cdiv(nmax, kT), na), kT, 0, stm>>>(
-            d_lpos.p, d_n.p, h.d_rk.p, d_voff.p, h.d_vo.p, h.d_svo.p, h.Vb.p, f16 ? 1u : 0u,
-            reinterpret_cast<__half *>(h.vh.p), h.vf.p, h.sv.p);
-        IRTT_HIP(hipGetLastError());
         // exact error with the stored factors
         std::vector<uint32_t> alist(na);
         for (uint32_t a = 0; a < na; a++) alist[a] = list[acc[a]];

@@ -243,7 +243,7 @@ class HmatrixParams(ctypes.Structure):
     _fields_ = [("tol", ctypes.c_double), ("eta", ctypes.c_double),
                 ("max_rank", ctypes.c_uint32), ("leaf_size", ctypes.c_uint32),
                 ("precision", ctypes.c_uint32), ("allow_larger", ctypes.c_uint32),
-                ("keep_factors", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("keep_factors", ctypes.c_uint32), ("workspace_mib", ctypes.c_uint32)]
 
 
 class HmatrixStats(ctypes.Structure):
@@ -607,11 +607,12 @@ class Scene:
         return np.ascontiguousarray(U.T), V, C, pr, pc
 
     def compress_hierarchical(self, tol=1e-4, eta=1.0, max_rank=32, leaf_size=64,
-                              precision=STORE_F16, allow_larger=False, keep_factors=False):
+                              precision=STORE_F16, allow_larger=False, keep_factors=False,
+                              workspace_mib=0):
         """H-matrix form of the sparse-layout operator; selects OP_HMATRIX
         (irtt_compress_hierarchical)."""
         p = HmatrixParams(tol, eta, max_rank, leaf_size, precision, int(bool(allow_larger)),
-                          int(bool(keep_factors)), 0)
+                          int(bool(keep_factors)), int(workspace_mib))
         st = HmatrixStats()
         _check(lib().irtt_compress_hierarchical(self._h, ctypes.byref(p), ctypes.byref(st)))
         return st.as_dict()

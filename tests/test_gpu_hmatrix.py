@@ -153,6 +153,28 @@ def test_hmatrix_solves_vs_fp64(irtt):
     assert dT < 1e-7 and rel < 1e-8
 
 
+def test_hmatrix_batched_work_space_is_the_same(irtt):
+    """The cross approximations taken in batches of blocks (a 1 MiB work
+    space: many batches, kept blocks rebuilt for packing) give the same block
+    outcomes and bit-identical products as all blocks at once."""
+    sd, s = _city(irtt)
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((2, s.n_bins))
+    out = []
+    with s:
+        for kw in ({"keep_factors": True}, {"workspace_mib": 1}):
+            st = s.compress_hierarchical(tol=0.3, max_rank=16, leaf_size=32, allow_larger=True,
+                                         **kw)
+            _, blocks, err, _ = s.hmatrix_blocks()
+            out.append((st, blocks, err, s.apply_operator(X), s.apply_operator(X, True)))
+    (s1, b1, e1, y1, t1), (s2, b2, e2, y2, t2) = out
+    print(f"[hmatrix] one batch {s1['build_bytes']} B, batched {s2['build_bytes']} B", flush=True)
+    assert s2["build_bytes"] < s1["build_bytes"] and s2["build_bytes"] <= (1 << 20) + (64 << 10)
+    assert np.array_equal(b1, b2) and np.array_equal(e1, e2)
+    assert np.array_equal(y1, y2) and np.array_equal(t1, t2)
+    assert s1["n_lowrank"] > 10 and s1["err_frob_rel_stored"] == s2["err_frob_rel_stored"]
+
+
 def test_hmatrix_needs_sparse_layout(irtt):
     from irtt import scenes
     sd = scenes.city_block(seed=6, nx=2, ny=2, target_tris=1500)
