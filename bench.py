@@ -28,17 +28,18 @@ NODE_BYTES = 64      # two child boxes + two child refs
 TRI_BYTES = 40       # 3 x float3 vertices + id
 RAY_BYTES = 48       # 32 B ray in + 16 B hit out
 BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
-# Per k_photons launch on the default workload (city block seed 0, 162 x 8
-# photons per triangle, dense layout), from rocprofv3 counters; commands and
-# raw values in profiles/k_photons_city_block_2p26.md.
+# Per k_photons launch on the default workload (city block seed 0 with the
+# traced sun quad, 130 x 8 photons per triangle, dense layout), from
+# rocprofv3 counters; commands and raw values in
+# profiles/k_photons_city_block_2p26.md.
 # DRAM bytes: FETCH_SIZE (x 2.0) + WRITE_SIZE (x 1.0), the factors from a
 # 1 GiB device copy (bench.py --calibrate-copy 1024).
-TRAFFIC_BYTES_CITY = (1091398 * 2.0 + 1161716 * 1.0) * 1024
+TRAFFIC_BYTES_CITY = (2620144 * 2.0 + 1126183 * 1.0) * 1024
 # VALU wave-instructions issued (SQ_INSTS_VALU) and lane utilisation
 # (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)); the work is the same
 # every launch (fixed seeds), so the count is a property of the workload.
-VALU_INSTS_CITY = 5921752115
-VALU_LANE_UTIL_CITY = 164314725944 / (64.0 * 5985472421)
+VALU_INSTS_CITY = 5825453178
+VALU_LANE_UTIL_CITY = 156351226083 / (64.0 * 5888019129)
 # VALU issue peak: one wave64 instruction per cycle per CU (four SIMD16 each
 # taking four cycles per instruction), 256 CUs at 2.4 GHz
 VALU_PEAK_G = 256 * 2.4
