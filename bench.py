@@ -514,6 +514,27 @@ def facade(irtt, h=1e-3, h_wide=3e-2, gd_steps=50, photons=1 << 30, rays_per_thr
                     "and both adjoint gradients at the same photons; mean and standard "
                     "error over the sets; z = mean of (adjoint - FD) over the sets / its "
                     "standard error (paired)"}
+        # the reference's FD mode 2 (transport only, lambda and x held at d0)
+        # beside the adjoint trace of the same term, term by term
+        _log("facade: transport-only FD and the trace's terms")
+        tfd, _ = fz.transport_fd(d0, h_wide, dims=ks)
+        lam_t, x4_t = fz.lam.copy(), fz.x.astype(np.float64) ** 4
+        trip = fz.par.triplets(d0)
+        terms = {"area": irtt.ADJ_AREA, "continuous": irtt.ADJ_CONTINUOUS,
+                 "edge": irtt.ADJ_EDGE, "edge_projected": irtt.ADJ_EDGE | irtt.ADJ_EDGE_PROJECTED,
+                 "hit_area": irtt.ADJ_HIT_AREA, "reference_terms": ref_flags,
+                 "extended": ext_flags}
+        tterm = {}
+        for name, fl in terms.items():
+            g, _ = fz.s.adjoint_trace(lam_t, x4_t, flags=fl)
+            tterm[name] = [round(float(v), 5) for v in irtt.project_gradient(g, *trip)[ks]]
+        out["transport_fd_terms"] = {
+            "h_m": h_wide, "params": ks, "fd": [round(float(v), 5) for v in tfd[ks]],
+            "trace": tterm,
+            "note": "-lambda^T dT/dp x^4 with lambda and x held at the design (FD mode 2, "
+                    "common random numbers) and the projected adjoint trace of the same "
+                    "quantity per term set: the occluder's gain (hit_area) and the shadow's "
+                    "loss (edge_projected) are each several times the net derivative"}
         out["fd_check"] = {
             "h_m": h, "h_wide_m": h_wide, "rows": rows,
             "sign_agreement_reference_terms": {"fd": agree(dp_ref, fd),
