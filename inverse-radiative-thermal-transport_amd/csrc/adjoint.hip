@@ -605,7 +605,20 @@ struct SegArgs {
     uint64_t n;                 // slots
     uint32_t side;              // 0: start triangle's vertices, 1: hit triangle's
     uint64_t g_limit;           // records of photons at or above this are skipped
+    uint32_t *big;              // slots of records with a component too large for the
+    uint32_t *n_big;            // 64-bit window sums (n entries), and their count
 };
+
+// v in units of q, rounded once, when |v / q| < 2^55: the window sums of 64
+// such values (two terms each) stay below 2^62 and are exact in 64 bits;
+// false for larger or non-finite values, which are added on their own
+__device__ __forceinline__ bool fx_small(double v, double inv_q, long long &out)
+{
+    const double y = rint(v * inv_q);
+    if (!(fabs(y) < 36028797018963968.0)) return false;
+    out = (long long)y;
+    return true;
+}
 
 // 128-bit (lo, hi) sum
 __device__ __forceinline__ void add128(unsigned long long &lo, long long &hi, unsigned long long lo2,
@@ -616,13 +629,66 @@ __device__ __forceinline__ void add128(unsigned long long &lo, long long &hi, un
     lo = s;
 }
 
+// The terms of one segment record on one side (side 0: its start triangle's
+// vertices, side 1: its hit triangle's vertices and the hit-area extension):
+// put(e, v) receives component e = 3 corner + axis of the side's triangle.
+template <class Put>
+__device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, uint32_t side,
+                                          bool f_cont, bool f_hit, Put put, uint32_t &n_cont)
+{
+    const uint32_t pt = r.r0.x, hp = r.r0.y, t = r.r2.y;
+    const double fpath = hist_factor(a, r.r0.w, r.r0.z);
+    const uint32_t i0 = a.idx[3 * pt], i1 = a.idx[3 * pt + 1], i2 = a.idx[3 * pt + 2];
+    const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+    const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
+    const float wt = a.w[t];
+    if (f_cont) {
+        const int lo_v = side == 0 ? 0 : 3;
+        auto emit = [&](int v, V3 gv) {
+            if (v < lo_v || v >= lo_v + 3) return;
+            const V3 dv = scl3(gv, wt);
+            if (!finite3(dv)) return;
+            const int c = v - lo_v;
+            put(3 * c, fpath * (double)dv.x);
+            put(3 * c + 1, fpath * (double)dv.y);
+            put(3 * c + 2, fpath * (double)dv.z);
+        };
+        // extension: a collimated emitter's first segment carries the beam's
+        // transfer
+        const bool beam = (a.aflags & IRTT_ADJ_COLLIMATED) && r.r2.z != 0 &&
+                          !(a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION);
+        const V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
+        const float G = beam ? beam_grad(p0, p1, p2, q0, q1, q2, a.eps, emit)
+                             : geom_grad(p0, p1, p2, q0, q1, q2, r.r1.x, r.r1.y, r.r1.z, r.r1.w,
+                                         a.eps, emit);
+        if (side == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
+    }
+    if (f_hit) {
+        // extension: the hit triangle's area Jacobian (the hit point is a
+        // fixed barycentric point of it)
+        V3 ga[3];
+        area_grad(q0, q1, q2, ga);
+        const float sh = wt / a.area[hp];
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const V3 dh = scl3(ga[c], sh);
+            if (!finite3(dh)) continue;
+            put(3 * c, fpath * (double)dh.x);
+            put(3 * c + 1, fpath * (double)dh.y);
+            put(3 * c + 2, fpath * (double)dh.z);
+        }
+    }
+}
+
 // The segment terms with a sort-reduce scatter: the continuous geometric
 // term (thermal_tracing_unified.glsl:391-448) on one side of each segment --
 // side 0 its start triangle's vertices, side 1 its hit triangle's vertices
 // and the hit-area extension -- over the records sorted by that triangle.  A
 // window of 64 records sums the terms of equal triangles in consecutive
-// lanes exactly (128-bit integers, segmented scan) and adds each run once:
-// about one add per triangle and window instead of one per record.
+// lanes exactly (64-bit integers in units of q, segmented scan) and adds each
+// run once: about one add per triangle and window instead of one per record.
+// A record with a component of 2^55 q or more (rare: grazing geometry) is
+// left out of the window and listed for k_adj_seg_big.
 __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
 {
     const AdjCommon &a = sa.c;
@@ -645,62 +711,26 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             have = (uint64_t)r.r2.x < sa.g_limit;
         }
         const uint32_t key = have ? (sa.side == 0 ? r.r0.x : r.r0.y) : 0xffffffffu;
-        unsigned long long lo[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        long long hi[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        long long acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (have) {
             if (sa.side == 0) n_seg++;
-            const uint32_t pt = r.r0.x, hp = r.r0.y, t = r.r2.y;
-            const double fpath = hist_factor(a, r.r0.w, r.r0.z);
-            const uint32_t i0 = a.idx[3 * pt], i1 = a.idx[3 * pt + 1], i2 = a.idx[3 * pt + 2];
-            const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
-            const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
-            const float wt = a.w[t];
-            if (f_cont) {
-                const int lo_v = sa.side == 0 ? 0 : 3;
-                auto emit = [&](int v, V3 gv) {
-                    if (v < lo_v || v >= lo_v + 3) return;
-                    const V3 dv = scl3(gv, wt);
-                    if (!finite3(dv)) return;
-                    const int c = v - lo_v;
-                    fx_quant(fpath * (double)dv.x, a.inv_q, lo[3 * c], hi[3 * c]);
-                    fx_quant(fpath * (double)dv.y, a.inv_q, lo[3 * c + 1], hi[3 * c + 1]);
-                    fx_quant(fpath * (double)dv.z, a.inv_q, lo[3 * c + 2], hi[3 * c + 2]);
-                };
-                // extension: a collimated emitter's first segment carries the
-                // beam's transfer
-                const bool beam = (a.aflags & IRTT_ADJ_COLLIMATED) && r.r2.z != 0 &&
-                                  !(a.flags[t] & IRTT_MAT_DIFFUSE_EMISSION);
-                const V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
-                const float G = beam ? beam_grad(p0, p1, p2, q0, q1, q2, a.eps, emit)
-                                     : geom_grad(p0, p1, p2, q0, q1, q2, r.r1.x, r.r1.y, r.r1.z,
-                                                 r.r1.w, a.eps, emit);
-                if (sa.side == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
-            }
-            if (f_hit) {
-                // extension: the hit triangle's area Jacobian (the hit point is
-                // a fixed barycentric point of it)
-                V3 ga[3];
-                area_grad(q0, q1, q2, ga);
-                const float sh = wt / a.area[hp];
+            bool big = false;
+            seg_terms(a, r, sa.side, f_cont, f_hit,
+                      [&](int e, double v) {
+                          long long y;
+                          if (fx_small(v, a.inv_q, y)) acc[e] += y;
+                          else big = true;
+                      },
+                      n_cont);
+            if (big) {
 #pragma unroll
-                for (int c = 0; c < 3; c++) {
-                    const V3 dh = scl3(ga[c], sh);
-                    if (!finite3(dh)) continue;
-                    const double comp[3] = {fpath * (double)dh.x, fpath * (double)dh.y,
-                                            fpath * (double)dh.z};
-#pragma unroll
-                    for (int e = 0; e < 3; e++) {
-                        unsigned long long l2;
-                        long long h2;
-                        fx_quant(comp[e], a.inv_q, l2, h2);
-                        add128(lo[3 * c + e], hi[3 * c + e], l2, h2);
-                    }
-                }
+                for (int e = 0; e < 9; e++) acc[e] = 0;
+                sa.big[atomicAdd(sa.n_big, 1u)] = (uint32_t)slot;
             }
         }
         // segmented inclusive scan over the window: runs of equal keys (every
         // lane takes part in each shuffle: a lane outside the active set
-        // reads as zero)
+        // reads as zero); the 64-bit sums are exact
         const uint32_t key_prev = __shfl_up(key, 1, 64);
         bool head = lane == 0 || key_prev != key;
 #pragma unroll
@@ -709,9 +739,8 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             const bool take = lane >= (uint32_t)off && !head;
 #pragma unroll
             for (int e = 0; e < 9; e++) {
-                const unsigned long long l2 = __shfl_up(lo[e], off, 64);
-                const long long h2 = __shfl_up(hi[e], off, 64);
-                if (take) add128(lo[e], hi[e], l2, h2);
+                const long long y2 = __shfl_up(acc[e], off, 64);
+                if (take) acc[e] += y2;
             }
             if (lane >= (uint32_t)off) head = head || head_up;
         }
@@ -722,7 +751,8 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             const uint32_t tri = key;
 #pragma unroll
             for (int e = 0; e < 9; e++)
-                fx_add_q(gx + 2 * (3 * (size_t)a.idx[3 * tri + e / 3] + e % 3), lo[e], hi[e]);
+                fx_add_q(gx + 2 * (3 * (size_t)a.idx[3 * tri + e / 3] + e % 3),
+                         (unsigned long long)acc[e], acc[e] < 0 ? -1ll : 0ll);
         }
     }
     if (sa.side == 0) {
@@ -734,6 +764,27 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
             if (lane == 0 && v) atomicAdd(a.stats + slot_of[k], (unsigned long long)v);
         }
+    }
+}
+
+// The listed records of k_adj_segments, each component added on its own
+// (128-bit exact sums).
+__global__ __launch_bounds__(256) void k_adj_seg_big(SegArgs sa)
+{
+    const AdjCommon &a = sa.c;
+    unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
+    const bool f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
+    const bool f_hit = sa.side == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
+    const uint32_t n = *sa.n_big;
+    uint32_t unused = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const SegRec r = sa.segs[sa.big[i]];
+        const uint32_t key = sa.side == 0 ? r.r0.x : r.r0.y;
+        seg_terms(a, r, sa.side, f_cont, f_hit,
+                  [&](int e, double v) {
+                      fx_add(gx + 2 * (3 * (size_t)a.idx[3 * key + e / 3] + e % 3), v, a.inv_q);
+                  },
+                  unused);
     }
 }
 
@@ -1061,6 +1112,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     d_order.alloc(seg_chunks * kChunk);
     d_order2.alloc(seg_chunks * kChunk);
     d_ctl.alloc(3);
+    ws.n_big.alloc(1);
     uint32_t key_bits = 1;
     while (key_bits < 32 && (1ull << key_bits) <= s.nt) key_bits++;
     size_t sort_tmp_b = 0;
@@ -1117,9 +1169,15 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                     sa.n = n;
                     sa.side = side;
                     sa.g_limit = ctl[2];
+                    // the sort's other key buffer lists the records too large
+                    // for the window sums
+                    sa.big = kd.current() == d_keys.p ? d_keys2.p : d_keys.p;
+                    sa.n_big = ws.n_big.p;
+                    IRTT_HIP(hipMemsetAsync(ws.n_big.p, 0, sizeof(uint32_t), stm));
                     const uint32_t sg = (uint32_t)std::max<uint64_t>(
                         1, std::min<uint64_t>(cdiv(n / 64, 4), (uint64_t)ncu * 16));
                     k_adj_segments<<<sg, 256, 0, stm>>>(sa);
+                    k_adj_seg_big<<<ncu, 256, 0, stm>>>(sa);
                     IRTT_HIP(hipGetLastError());
                 }
             }

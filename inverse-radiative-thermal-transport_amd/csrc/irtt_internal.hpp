@@ -191,7 +191,7 @@ struct AdjWork {
     DBuf<float> w, area;
     DBuf<double> lam, x4;
     DBuf<unsigned long long> grad, st, ctl;
-    DBuf<uint32_t> abs_own, fills, keys, keys2, order, order2;
+    DBuf<uint32_t> abs_own, fills, keys, keys2, order, order2, n_big;
     DBuf<uint4> segs, edges;
     DBuf<uint8_t> sort_tmp;
     // bytes held by the record buffers (the part sized against free HBM)
