@@ -632,9 +632,9 @@ __device__ __forceinline__ void add128(unsigned long long &lo, long long &hi, un
 // The terms of one segment record on one side (side 0: its start triangle's
 // vertices, side 1: its hit triangle's vertices and the hit-area extension):
 // put(e, v) receives component e = 3 corner + axis of the side's triangle.
-template <class Put>
-__device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, uint32_t side,
-                                          bool f_cont, bool f_hit, Put put, uint32_t &n_cont)
+template <int SIDE, class Put>
+__device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, bool f_cont,
+                                          bool f_hit, Put put, uint32_t &n_cont)
 {
     const uint32_t pt = r.r0.x, hp = r.r0.y, t = r.r2.y;
     const double fpath = hist_factor(a, r.r0.w, r.r0.z);
@@ -643,7 +643,7 @@ __device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, u
     const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
     const float wt = a.w[t];
     if (f_cont) {
-        const int lo_v = side == 0 ? 0 : 3;
+        constexpr int lo_v = SIDE == 0 ? 0 : 3;
         auto emit = [&](int v, V3 gv) {
             if (v < lo_v || v >= lo_v + 3) return;
             const V3 dv = scl3(gv, wt);
@@ -661,9 +661,9 @@ __device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, u
         const float G = beam ? beam_grad(p0, p1, p2, q0, q1, q2, a.eps, emit)
                              : geom_grad(p0, p1, p2, q0, q1, q2, r.r1.x, r.r1.y, r.r1.z, r.r1.w,
                                          a.eps, emit);
-        if (side == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
+        if (SIDE == 0 && fabsf(G) > a.eps * a.eps) n_cont++;
     }
-    if (f_hit) {
+    if (SIDE == 1 && f_hit) {
         // extension: the hit triangle's area Jacobian (the hit point is a
         // fixed barycentric point of it)
         V3 ga[3];
@@ -689,12 +689,13 @@ __device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, u
 // run once: about one add per triangle and window instead of one per record.
 // A record with a component of 2^55 q or more (rare: grazing geometry) is
 // left out of the window and listed for k_adj_seg_big.
+template <int SIDE>
 __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
 {
     const AdjCommon &a = sa.c;
     unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
     const bool f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
-    const bool f_hit = sa.side == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
+    const bool f_hit = SIDE == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
     const uint32_t lane = __lane_id();
     uint32_t n_seg = 0, n_cont = 0;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64;
@@ -710,12 +711,12 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             r = sa.segs[slot];
             have = (uint64_t)r.r2.x < sa.g_limit;
         }
-        const uint32_t key = have ? (sa.side == 0 ? r.r0.x : r.r0.y) : 0xffffffffu;
+        const uint32_t key = have ? (SIDE == 0 ? r.r0.x : r.r0.y) : 0xffffffffu;
         long long acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (have) {
-            if (sa.side == 0) n_seg++;
+            if (SIDE == 0) n_seg++;
             bool big = false;
-            seg_terms(a, r, sa.side, f_cont, f_hit,
+            seg_terms<SIDE>(a, r, f_cont, f_hit,
                       [&](int e, double v) {
                           long long y;
                           if (fx_small(v, a.inv_q, y)) acc[e] += y;
@@ -755,7 +756,7 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
                          (unsigned long long)acc[e], acc[e] < 0 ? -1ll : 0ll);
         }
     }
-    if (sa.side == 0) {
+    if (SIDE == 0) {
         uint32_t cnt[2] = {n_seg, n_cont};
         const int slot_of[2] = {A_SEG, A_CONT};
 #pragma unroll
@@ -769,18 +770,19 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
 
 // The listed records of k_adj_segments, each component added on its own
 // (128-bit exact sums).
+template <int SIDE>
 __global__ __launch_bounds__(256) void k_adj_seg_big(SegArgs sa)
 {
     const AdjCommon &a = sa.c;
     unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
     const bool f_cont = a.aflags & IRTT_ADJ_CONTINUOUS;
-    const bool f_hit = sa.side == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
+    const bool f_hit = SIDE == 1 && (a.aflags & IRTT_ADJ_HIT_AREA);
     const uint32_t n = *sa.n_big;
     uint32_t unused = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const SegRec r = sa.segs[sa.big[i]];
-        const uint32_t key = sa.side == 0 ? r.r0.x : r.r0.y;
-        seg_terms(a, r, sa.side, f_cont, f_hit,
+        const uint32_t key = SIDE == 0 ? r.r0.x : r.r0.y;
+        seg_terms<SIDE>(a, r, f_cont, f_hit,
                   [&](int e, double v) {
                       fx_add(gx + 2 * (3 * (size_t)a.idx[3 * key + e / 3] + e % 3), v, a.inv_q);
                   },
@@ -1176,8 +1178,13 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                     IRTT_HIP(hipMemsetAsync(ws.n_big.p, 0, sizeof(uint32_t), stm));
                     const uint32_t sg = (uint32_t)std::max<uint64_t>(
                         1, std::min<uint64_t>(cdiv(n / 64, 4), (uint64_t)ncu * 16));
-                    k_adj_segments<<<sg, 256, 0, stm>>>(sa);
-                    k_adj_seg_big<<<ncu, 256, 0, stm>>>(sa);
+                    if (side == 0) {
+                        k_adj_segments<0><<<sg, 256, 0, stm>>>(sa);
+                        k_adj_seg_big<0><<<ncu, 256, 0, stm>>>(sa);
+                    } else {
+                        k_adj_segments<1><<<sg, 256, 0, stm>>>(sa);
+                        k_adj_seg_big<1><<<ncu, 256, 0, stm>>>(sa);
+                    }
                     IRTT_HIP(hipGetLastError());
                 }
             }
