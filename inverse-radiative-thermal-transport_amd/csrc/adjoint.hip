@@ -1068,9 +1068,32 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int S = s.bvh.max_depth + 1 <= 24 ? 24 : stack_size_for(s.bvh.max_depth);
 
+    // A second stream runs the stages that need no other stage's output
+    // beside the ones that do: the source-area terms beside the path replay,
+    // each range's occlusion rays beside its sorted segment terms.  Every
+    // gradient sum is exact, so the overlap does not change a bit.
+    struct Side {
+        hipStream_t st = nullptr;
+        hipEvent_t fork = nullptr, area_done = nullptr, occl_done = nullptr;
+        ~Side()
+        {
+            for (hipEvent_t e : {fork, area_done, occl_done})
+                if (e) (void)hipEventDestroy(e);
+            if (st) stream_give(st);
+        }
+    } side2;
+    side2.st = stream_take();
+    for (hipEvent_t *e : {&side2.fork, &side2.area_done, &side2.occl_done})
+        IRTT_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    hipStream_t stm2 = side2.st;
+    bool occl_pending = false;
+
     // source-area terms and the absorbed count
-    k_adj_area<<<s.nt, 256, 0, stm>>>(c, (c.aflags & IRTT_ADJ_AREA) ? 1u : 0u);
+    IRTT_HIP(hipEventRecord(side2.fork, stm));
+    IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
+    k_adj_area<<<s.nt, 256, 0, stm2>>>(c, (c.aflags & IRTT_ADJ_AREA) ? 1u : 0u);
     IRTT_HIP(hipGetLastError());
+    IRTT_HIP(hipEventRecord(side2.area_done, stm2));
 
     // Path replay into segment and edge records, then their terms.  The
     // records of all photons fit when the device has room (the forward
@@ -1138,6 +1161,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             pa.segs = RecBuf{segs_p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
             pa.edges = RecBuf{edges_p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
             pa.first_dropped = d_ctl.p + 2;
+            // the previous range's occlusion rays end before this range's
+            // replay rewrites the records
+            if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));
             IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
             const uint32_t grid = (uint32_t)std::max<uint64_t>(
                 1, std::min<uint64_t>(cdiv(ge - g_start, kTraceBlock), waves));
@@ -1151,6 +1177,27 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             IRTT_HIP(hipStreamSynchronize(stm));
             const uint32_t sc = (uint32_t)std::min<uint64_t>(ctl[0], seg_chunks);
             const uint32_t ec = want_edges ? (uint32_t)std::min<uint64_t>(ctl[1], edge_chunks) : 0;
+            if (ec > 0) {
+                OcclArgs oa{};
+                oa.c = c;
+                oa.edges = edges_p;
+                oa.fills = d_fills.p + seg_chunks;
+                oa.n_chunks = ec;
+                oa.g_limit = ctl[2];
+                const uint32_t og = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
+                // after the replay (complete: its counters were read above),
+                // beside the segment terms enqueued next on the scene's stream
+                IRTT_HIP(hipEventRecord(side2.fork, stm));
+                IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
+                if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm2>>>(oa);
+                else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm2>>>(oa);
+                else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm2>>>(oa);
+                else k_adj_occl<128><<<og, kTraceBlock, 0, stm2>>>(oa);
+                IRTT_HIP(hipGetLastError());
+                IRTT_HIP(hipEventRecord(side2.occl_done, stm2));
+                occl_pending = true;
+            }
             if (sc > 0) {
                 // each side over the records sorted by its triangle
                 const uint64_t n = (uint64_t)sc * kChunk;
@@ -1188,21 +1235,6 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                     IRTT_HIP(hipGetLastError());
                 }
             }
-            if (ec > 0) {
-                OcclArgs oa{};
-                oa.c = c;
-                oa.edges = edges_p;
-                oa.fills = d_fills.p + seg_chunks;
-                oa.n_chunks = ec;
-                oa.g_limit = ctl[2];
-                const uint32_t og = (uint32_t)std::max<uint64_t>(
-                    1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
-                if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm>>>(oa);
-                else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm>>>(oa);
-                else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm>>>(oa);
-                else k_adj_occl<128><<<og, kTraceBlock, 0, stm>>>(oa);
-                IRTT_HIP(hipGetLastError());
-            }
             if (ctl[2] == ~0ull) break;  // every record of the range fitted
             if (ctl[2] <= g_start)
                 throw Error(IRTT_ERR_LIMIT, "adjoint trace: the records of one photon exceed the "
@@ -1210,6 +1242,8 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             g_start = ctl[2];
         }
     }
+    IRTT_HIP(hipStreamWaitEvent(stm, side2.area_done, 0));
+    if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));
     IRTT_HIP(hipEventRecord(s.ev1, stm));
     std::vector<unsigned long long> gx(kGradCopies * 6 * (size_t)s.nv), hs(16);
     IRTT_HIP(hipMemcpyAsync(gx.data(), d_grad.p, d_grad.bytes(), hipMemcpyDeviceToHost, stm));
