@@ -166,6 +166,7 @@ static void build_laplacian(Scene &s)
 void assemble(Scene &s)
 {
     const uint32_t nt = s.nt, nv = s.nv;
+    s.assembly_gen++;
     s.rho_d.resize(nt);
     s.rho_s.resize(nt);
     s.eps.resize(nt);
@@ -567,7 +568,7 @@ uint64_t Scene::device_bytes() const
 {
     return d_xyz.bytes() + d_idx.bytes() + d_rho_d.bytes() + d_rho_s.bytes() + d_flags.bytes() +
            d_traceable.bytes() + bvh.nodes.bytes() + bvh.tris.bytes() + bvh.morton.bytes() +
-           d_q.bytes() + d_acc.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
+           d_q.bytes() + d_acc.bytes() + d_touch.bytes() + d_T.bytes() + d_diag.bytes() + d_colacc.bytes() +
            d_tdiag.bytes() + d_h16.bytes() + d_h16s.bytes() + d_h16f.bytes() + d_h16ft.bytes() +
            d_gpart.bytes() + d_gpart64.bytes() + aca.d_ut.bytes() +
            aca.d_v.bytes() + aca.d_uf.bytes() + aca.d_vf.bytes() + aca.d_uh.bytes() +
@@ -802,10 +803,10 @@ int irtt_build_transport(irtt_scene h, const irtt_transport_params *p, irtt_tran
         Scene *s = S(h);
         if (!p) throw Error(IRTT_ERR_ARG, "params is null");
         if (p->layout == IRTT_LAYOUT_DENSE) {
-            // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B)
-            size_t need = (size_t)s->nb * s->ld * 12;
+            // dense operator: nb*ld fixed-point (8 B) + FP32 (4 B) + chunk marks
+            size_t need = (size_t)s->nb * s->ld * 12 + (((size_t)s->nb * s->ld) >> kTouchShift);
             const size_t fr = dev_available();
-            size_t held = s->d_acc.bytes() + s->d_T.bytes();
+            size_t held = s->d_acc.bytes() + s->d_touch.bytes() + s->d_T.bytes();
             if (need > fr + held)
                 throw Error(IRTT_ERR_LIMIT, "dense operator needs " + std::to_string(need) +
                                                 " bytes, " + std::to_string(fr + held) +

@@ -139,8 +139,14 @@ struct alignas(16) TabSlot {
 
 // Where the photon kernel bins an absorbed photon: the dense accumulators,
 // or (tab set) the 2^lg-slot hash table of the sparse layout.
+// Dense accumulators are tracked in row chunks of 2^kTouchShift entries
+// (128 B): the photon kernel marks every chunk it adds to, so the next build
+// clears only marked chunks and the finalize pass reads only those.
+constexpr uint32_t kTouchShift = 4;
+
 struct PhotonSink {
     unsigned long long *acc = nullptr;
+    uint8_t *touch = nullptr;  // one byte per accumulator chunk (dense layout)
     TabSlot *tab = nullptr;
     uint32_t lg = 0, key_bits = 0;
     uint32_t *overflow = nullptr;
@@ -369,7 +375,13 @@ struct Scene {
     double quantum = 0.0;
     DBuf<uint64_t> d_q;               // per-triangle fixed-point weight
     std::vector<uint64_t> h_q;        // what d_q holds (uploads of the same weights are skipped)
+    uint64_t assembly_gen = 0;        // bumped by every assembly (areas, emissivities, bins)
+    uint64_t wq_gen = ~0ull, wq_total = 0;  // assembly and photon count h_q, h_diag were made for
+    uint32_t wq_per = 0;
+    double wq_quantum = 0.0;          // Q of h_q
     DBuf<unsigned long long> d_acc;   // nb*ld fixed-point accumulators
+    DBuf<uint8_t> d_touch;            // nb*ld >> kTouchShift chunk marks
+    bool acc_marked = false;          // d_acc is zero outside the chunks d_touch marks
     DBuf<float> d_T;                  // nb*ld FP32 operator, row-major
     DBuf<double> d_diag;              // sigma*eps*A per bin
     std::vector<double> h_diag;       // what d_diag holds

@@ -89,7 +89,7 @@ uint64_t footprint(const irtt_request &q, uint32_t nb)
         while (cap < keys + keys / 4) cap <<= 1;
         b = 16 * cap + 32 * keys;
     } else {
-        b = (uint64_t)nb * ld * 12;
+        b = (uint64_t)nb * ld * 12 + (((uint64_t)nb * ld) >> irtt::kTouchShift);
     }
     if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
     b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;

@@ -154,8 +154,11 @@ __device__ __forceinline__ int shade_segment(const PhotonArgs &a, const Hit &h, 
     uint32_t ab = a.bin_mode ? hp : (hc == 0 ? j0 : (hc == 1 ? j1 : j2));
     if (a.sink.tab)
         tab_add(a.sink.tab, a.sink.lg, tab_key(ab, src, a.sink.key_bits), qt, a.sink.overflow);
-    else if (a.sink.acc)
-        atomicAdd(a.sink.acc + (uint64_t)ab * a.ld + src, qt);
+    else if (a.sink.acc) {
+        const uint64_t o = (uint64_t)ab * a.ld + src;
+        atomicAdd(a.sink.acc + o, qt);
+        a.sink.touch[o >> kTouchShift] = 1;  // every writer stores the same byte
+    }
     if (a.absorber) a.absorber[g] = ab;
     return absorbed ? S_ABS : S_FORCED;
 }
@@ -250,9 +253,6 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
     flush_counts(a.stats, cnt, nvis, ntri, q_esc, q_drop);
 }
 
-// T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32; the
-// integer column sums of acc are accumulated per 256-row tile, and the stored
-// diagonal is gathered into tdiag.
 // photon kernel grid: at most this many waves per CU, lanes striding over
 // the photons.  Smaller per-wave shares even out the end of the launch
 // (city block, 2^26 photons: 24 -> 13.3 ms, 64 -> 13.0-13.1, 96 -> 12.7,
@@ -261,25 +261,162 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
 #define IRTT_PHOTON_WAVES_PER_CU 256
 #endif
 
-constexpr int kFinRows = 256;
-__global__ void k_finalize(const unsigned long long *__restrict__ acc, const double *__restrict__ diag,
-                           uint32_t nb, uint32_t ld, double Q, float *__restrict__ T,
-                           unsigned long long *colacc, float *__restrict__ tdiag)
+// T[i][j] = (i == j ? diag_i : 0) - acc[i][j] * Q in FP64, stored FP32, in two
+// passes.  k_fill_T writes every entry as if its accumulator were zero
+// (FP32(diag_i) on the diagonal, +0 elsewhere: what the expression gives for
+// acc = 0) with stores alone; k_finalize then recomputes the entries of the
+// chunks the photon kernel marked (every nonzero accumulator lies in one),
+// adds their integer column sums, and gathers the stored diagonal into tdiag.
+constexpr int kFillRows = 16;   // rows per k_fill_T block (1024 columns wide)
+__global__ __launch_bounds__(256) void k_fill_T(const double *__restrict__ diag, uint32_t nb,
+                                                uint32_t ld, float *__restrict__ T,
+                                                float *__restrict__ tdiag)
 {
-    uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.x * 1024 + threadIdx.x * 4;  // ld is a multiple of 64
     if (j >= ld) return;
-    uint32_t r0 = blockIdx.y * kFinRows, r1 = min(r0 + kFinRows, nb);
-    unsigned long long s = 0;
+    const uint32_t r0 = blockIdx.y * kFillRows, r1 = min(r0 + kFillRows, nb);
     for (uint32_t i = r0; i < r1; i++) {
-        size_t o = (size_t)i * ld + j;
-        unsigned long long v = acc[o];
-        s += v;
-        double val = (i == j ? diag[i] : 0.0) - (double)v * Q;
-        float f = j < nb ? (float)val : 0.0f;
-        T[o] = f;
-        if (i == j) tdiag[j] = f;
+        float4 f = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (i - j < 4u) {
+            const float d = (float)(diag[i] - 0.0);
+            tdiag[i] = d;
+            if (i == j) f.x = d;
+            else if (i == j + 1) f.y = d;
+            else if (i == j + 2) f.z = d;
+            else f.w = d;
+        }
+        *reinterpret_cast<float4 *>(T + (size_t)i * ld + j) = f;
     }
-    if (s && j < nb) atomicAdd(colacc + j, s);
+}
+
+constexpr int kFinRows = 256;                          // rows per tile
+constexpr int kFinCols = 1024;                         // columns per tile
+constexpr int kFinChunks = kFinCols >> kTouchShift;    // chunks per tile row (64)
+constexpr int kFinPer = 4;                             // chunk quarters per thread and round
+static_assert(kFinRows == 256 && kFinChunks == 64, "list entries are (row << 6) | chunk");
+__global__ __launch_bounds__(256) void k_finalize(const unsigned long long *__restrict__ acc,
+                                                  const uint8_t *__restrict__ touch,
+                                                  const double *__restrict__ diag, uint32_t nb,
+                                                  uint32_t ld, double Q, float *__restrict__ T,
+                                                  unsigned long long *colacc,
+                                                  float *__restrict__ tdiag)
+{
+    // the tile's marked chunks listed in LDS ((row << 6) | chunk, in any
+    // order: the entries and the integer column sums do not depend on it);
+    // then four threads per chunk, kFinPer chunks per thread in flight
+    __shared__ uint16_t list[kFinRows * kFinChunks];
+    __shared__ unsigned long long csum[kFinCols];
+    __shared__ uint32_t n_list;
+    const uint32_t r0 = blockIdx.y * kFinRows, r1 = min(r0 + kFinRows, nb);
+    const uint32_t cpr = ld >> kTouchShift;  // a multiple of 4
+    const uint32_t c0 = blockIdx.x * kFinChunks;
+    const uint32_t j0 = blockIdx.x * kFinCols;
+    for (uint32_t k = threadIdx.x; k < kFinCols; k += blockDim.x) csum[k] = 0ull;
+    if (threadIdx.x == 0) n_list = 0;
+    __syncthreads();
+    {
+        const uint32_t r = threadIdx.x, i = r0 + r;
+        const uint32_t *t4 = reinterpret_cast<const uint32_t *>(touch);
+        uint32_t mw[kFinChunks / 4];
+#pragma unroll
+        for (uint32_t w = 0; w < kFinChunks / 4; w++) {
+            const uint32_t c = c0 + 4 * w;
+            mw[w] = (i < r1 && c < cpr) ? t4[((size_t)i * cpr + c) >> 2] : 0u;
+        }
+#pragma unroll
+        for (uint32_t w = 0; w < kFinChunks / 4; w++) {
+            if (!mw[w]) continue;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; b++)
+                if ((mw[w] >> (8 * b)) & 0xffu)
+                    list[atomicAdd(&n_list, 1u)] = (uint16_t)((r << 6) | (4 * w + b));
+        }
+    }
+    __syncthreads();
+    const uint32_t n = n_list;
+    const uint32_t qc = (threadIdx.x & 3) * 4;  // this thread's four columns of a chunk
+    for (uint32_t e0 = threadIdx.x >> 2; e0 < n; e0 += (blockDim.x / 4) * kFinPer) {
+        ulonglong2 va[kFinPer], vb[kFinPer];
+        uint32_t ent[kFinPer];
+#pragma unroll
+        for (int k = 0; k < kFinPer; k++) {
+            const uint32_t e = e0 + k * (blockDim.x / 4);
+            ent[k] = e < n ? (uint32_t)list[e] : ~0u;
+            va[k] = vb[k] = make_ulonglong2(0ull, 0ull);
+            if (ent[k] != ~0u) {
+                const uint32_t i = r0 + (ent[k] >> 6), j = j0 + ((ent[k] & 63u) << kTouchShift) + qc;
+                const ulonglong2 *p = reinterpret_cast<const ulonglong2 *>(acc + (size_t)i * ld + j);
+                va[k] = p[0];
+                vb[k] = p[1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kFinPer; k++) {
+            if (ent[k] == ~0u) continue;
+            const uint32_t i = r0 + (ent[k] >> 6), jt = ((ent[k] & 63u) << kTouchShift) + qc;
+            const uint32_t j = j0 + jt;
+            const unsigned long long v[4] = {va[k].x, va[k].y, vb[k].x, vb[k].y};
+            float f[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                if (v[c]) atomicAdd(&csum[jt + c], v[c]);
+                double val = (i == j + c ? diag[i] : 0.0) - (double)v[c] * Q;
+                f[c] = j + c < nb ? (float)val : 0.0f;
+                if (i == j + c) tdiag[i] = f[c];
+            }
+            *reinterpret_cast<float4 *>(T + (size_t)i * ld + j) = make_float4(f[0], f[1], f[2], f[3]);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < kFinCols; k += blockDim.x)
+        if (csum[k] && j0 + k < nb) atomicAdd(colacc + j0 + k, csum[k]);
+}
+
+// Zeroes the accumulator chunks the last build marked, and their marks:
+// afterwards every accumulator and mark is zero.  A block takes 1024 words of
+// marks (4096 chunks) at a time, lists the marked chunks in LDS, and clears
+// each with eight lanes (128 contiguous bytes).  n4 = chunks / 4 words of
+// marks (the chunk count nb * ld / 16 is a multiple of 4 since ld is a
+// multiple of 64, and below 2^32: dense operators stay far below 2^36 entries).
+constexpr int kClearWords = 1024;  // words of marks per block and round (4 per thread)
+static_assert(kTouchShift == 4, "eight 16-byte stores clear one chunk");
+__global__ __launch_bounds__(256) void k_clear_marked(unsigned long long *__restrict__ acc,
+                                                      uint32_t *__restrict__ touch4, size_t n4)
+{
+    __shared__ uint32_t list[kClearWords * 4];
+    __shared__ uint32_t n_list;
+    for (size_t base = (size_t)blockIdx.x * kClearWords; base < n4;
+         base += (size_t)gridDim.x * kClearWords) {
+        if (threadIdx.x == 0) n_list = 0;
+        __syncthreads();
+        const size_t w0 = base + threadIdx.x * 4;
+        uint32_t mw[4];
+        if (w0 + 4 <= n4) {
+            const uint4 m = *reinterpret_cast<const uint4 *>(touch4 + w0);
+            mw[0] = m.x;
+            mw[1] = m.y;
+            mw[2] = m.z;
+            mw[3] = m.w;
+        } else {
+#pragma unroll
+            for (int w = 0; w < 4; w++) mw[w] = w0 + w < n4 ? touch4[w0 + w] : 0u;
+        }
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            if (!mw[w]) continue;
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                if ((mw[w] >> (8 * b)) & 0xffu)
+                    list[atomicAdd(&n_list, 1u)] = (uint32_t)((w0 + w) * 4 + b);
+            touch4[w0 + w] = 0u;
+        }
+        __syncthreads();
+        const uint32_t n = n_list;
+        for (uint32_t e = threadIdx.x >> 3; e < n; e += blockDim.x / 8)
+            reinterpret_cast<ulonglong2 *>(acc + ((size_t)list[e] << kTouchShift))[threadIdx.x & 7] =
+                make_ulonglong2(0ull, 0ull);
+        __syncthreads();  // the list is rebuilt next round
+    }
 }
 
 inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
@@ -290,27 +427,37 @@ inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b
 // in integers; in FP64 (W per unit emission factor): emitted
 // sum_i sigma eps_i A_i against Q times each part.  out: emitted, absorbed,
 // escaped, dropped (W), |emitted - parts| / emitted, integer identity (1/0).
-__global__ __launch_bounds__(256) void k_energy(const unsigned long long *__restrict__ q,
-                                                uint32_t nt, uint32_t per,
-                                                const unsigned long long *__restrict__ colacc,
-                                                const double *__restrict__ diag, uint32_t nb,
-                                                const unsigned long long *__restrict__ stats,
-                                                double Q, double *__restrict__ out)
+constexpr int kEnergyThreads = 1024;
+__global__ __launch_bounds__(kEnergyThreads) void k_energy(
+    const unsigned long long *__restrict__ q, uint32_t nt, uint32_t per,
+    const unsigned long long *__restrict__ colacc, const double *__restrict__ diag, uint32_t nb,
+    const unsigned long long *__restrict__ stats, double Q, double *__restrict__ out)
 {
-    __shared__ unsigned long long su[256], sa[256];
-    __shared__ double sd[256];
-    unsigned long long uq = 0, ua = 0;
-    double em = 0.0;
-    for (uint32_t t = threadIdx.x; t < nt; t += blockDim.x) uq += q[t];
-    for (uint32_t j = threadIdx.x; j < nb; j += blockDim.x) {
-        ua += colacc[j];
-        em += diag[j];
+    __shared__ unsigned long long su[kEnergyThreads], sa[kEnergyThreads];
+    __shared__ double sd[kEnergyThreads];
+    // four independent partial sums per thread (loads in flight), added in a
+    // fixed order: the FP64 total does not depend on timing
+    unsigned long long uq[4] = {0, 0, 0, 0}, ua[4] = {0, 0, 0, 0};
+    double em[4] = {0.0, 0.0, 0.0, 0.0};
+    constexpr uint32_t B = kEnergyThreads;
+    for (uint32_t t0 = threadIdx.x; t0 < nt; t0 += 4 * B) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (t0 + k * B < nt) uq[k] += q[t0 + k * B];
     }
-    su[threadIdx.x] = uq;
-    sa[threadIdx.x] = ua;
-    sd[threadIdx.x] = em;
+    for (uint32_t j0 = threadIdx.x; j0 < nb; j0 += 4 * B) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (j0 + k * B < nb) {
+                ua[k] += colacc[j0 + k * B];
+                em[k] += diag[j0 + k * B];
+            }
+    }
+    su[threadIdx.x] = (uq[0] + uq[1]) + (uq[2] + uq[3]);
+    sa[threadIdx.x] = (ua[0] + ua[1]) + (ua[2] + ua[3]);
+    sd[threadIdx.x] = (em[0] + em[1]) + (em[2] + em[3]);
     __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
+    for (int off = kEnergyThreads / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) {
             su[threadIdx.x] += su[threadIdx.x + off];
             sa[threadIdx.x] += sa[threadIdx.x + off];
@@ -560,7 +707,9 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.has_transport = false;
     if (sparse) {
         s.d_acc.release();
+        s.d_touch.release();
         s.d_T.release();
+        s.acc_marked = false;
     } else {
         s.sp.reset();
     }
@@ -573,55 +722,78 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     if (sparse) {
         sparse_table_begin(s, total, sink);
     } else {
+        // a repeated build clears only the chunks the previous one marked;
+        // fresh (or reallocated) accumulators are cleared whole
+        const void *prev_acc = s.d_acc.p, *prev_touch = s.d_touch.p;
         s.d_acc.alloc(nb2);
+        s.d_touch.alloc(nb2 >> kTouchShift);
         s.d_T.alloc(nb2);
-        IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+        const size_t n4 = s.d_touch.bytes() / 4;
+        if (s.acc_marked && s.d_acc.p == prev_acc && s.d_touch.p == prev_touch) {
+            k_clear_marked<<<std::min<uint32_t>(cdiv(n4, kClearWords), 2048), 256, 0, stm>>>(
+                s.d_acc.p, reinterpret_cast<uint32_t *>(s.d_touch.p), n4);
+            IRTT_HIP(hipGetLastError());
+        } else {
+            IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+            IRTT_HIP(hipMemsetAsync(s.d_touch.p, 0, s.d_touch.bytes(), stm));
+        }
+        // not marked-clean again until this build has completed
+        s.acc_marked = false;
         sink.acc = s.d_acc.p;
+        sink.touch = s.d_touch.p;
     }
 
     // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
-    // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s
-    std::vector<float> w(s.nt);
-    float wmax = 0.0f;
-    for (uint32_t t = 0; t < s.nt; t++) {
-        float wt = s.tri_area[t] / (float)per;
-        wt = wt * (kSigma * s.eps[t]);
-        w[t] = wt;
-        wmax = std::max(wmax, wt);
+    // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s; with the bin
+    // diagonals, derived again only when the scene's assembly or the photon
+    // counts changed since the last build (repeated builds of one scene)
+    const bool same_weights = s.wq_gen == s.assembly_gen && s.wq_per == per &&
+                              s.wq_total == total && s.h_q.size() == s.nt &&
+                              s.h_diag.size() == nb;
+    if (!same_weights) {
+        s.wq_gen = ~0ull;
+        std::vector<float> w(s.nt);
+        float wmax = 0.0f;
+        for (uint32_t t = 0; t < s.nt; t++) {
+            float wt = s.tri_area[t] / (float)per;
+            wt = wt * (kSigma * s.eps[t]);
+            w[t] = wt;
+            wmax = std::max(wmax, wt);
+        }
+        int sh = 62 - (int)ceil_log2_u64(total ? total : 1);
+        if (sh > 40) sh = 40;
+        s.wq_quantum = wmax > 0.0f ? std::ldexp((double)wmax, -sh) : 1.0;
+        s.h_q.resize(s.nt);
+        for (uint32_t t = 0; t < s.nt; t++) {
+            double r = wmax > 0.0f ? std::ldexp((double)w[t] / (double)wmax, sh) : 0.0;
+            s.h_q[t] = (uint64_t)std::rint(r > 0.0 ? r : 0.0);
+        }
+        s.h_diag.resize(nb);
+        for (uint32_t i = 0; i < nb; i++) s.h_diag[i] = kSigmaD * s.bin_eps_area[i];
     }
-    int sh = 62 - (int)ceil_log2_u64(total ? total : 1);
-    if (sh > 40) sh = 40;
-    const double Q = wmax > 0.0f ? std::ldexp((double)wmax, -sh) : 1.0;
-    std::vector<uint64_t> q(s.nt);
-    for (uint32_t t = 0; t < s.nt; t++) {
-        double r = wmax > 0.0f ? std::ldexp((double)w[t] / (double)wmax, sh) : 0.0;
-        q[t] = (uint64_t)std::rint(r > 0.0 ? r : 0.0);
-    }
+    const double Q = s.wq_quantum;
+    const std::vector<double> &diag = s.h_diag;
     s.quantum = Q;
     s.tparams = p;
 
-    // weights and diagonals are uploaded only when they differ from what the
-    // device buffers already hold (repeated builds of one scene)
+    // weights and diagonals are uploaded when they changed or their device
+    // buffers are new (the host copies stay in the scene, unchanged until the
+    // stream has been synchronised)
     const uint64_t *prev_q = s.d_q.p;
     s.d_q.alloc(s.nt);
-    if (s.d_q.p != prev_q || s.h_q != q) {
-        s.h_q.clear();
-        IRTT_HIP(hipMemcpyAsync(s.d_q.p, q.data(), s.nt * sizeof(uint64_t), hipMemcpyHostToDevice,
-                                stm));
-        s.h_q = q;
-    }
+    if (s.d_q.p != prev_q || !same_weights)
+        IRTT_HIP(hipMemcpyAsync(s.d_q.p, s.h_q.data(), s.nt * sizeof(uint64_t),
+                                hipMemcpyHostToDevice, stm));
     const double *prev_diag = s.d_diag.p;
     s.d_diag.alloc(nb);
     s.d_colacc.alloc(nb);
     s.d_stats.alloc(16);
-    std::vector<double> diag(nb);
-    for (uint32_t i = 0; i < nb; i++) diag[i] = kSigmaD * s.bin_eps_area[i];
-    if (s.d_diag.p != prev_diag || s.h_diag != diag) {
-        s.h_diag.clear();
+    if (s.d_diag.p != prev_diag || !same_weights)
         IRTT_HIP(hipMemcpyAsync(s.d_diag.p, diag.data(), nb * sizeof(double), hipMemcpyHostToDevice,
                                 stm));
-        s.h_diag = diag;
-    }
+    s.wq_gen = s.assembly_gen;
+    s.wq_per = per;
+    s.wq_total = total;
     IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
     IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
     // absorbing bin of every photon (4 B each) so the adjoint trace replays
@@ -645,16 +817,19 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     if (sparse) {
         sparse_table_finish(s, sink);
     } else {
-        dim3 fg(cdiv(ld, 256), cdiv(nb, kFinRows));
         s.d_tdiag.alloc(nb);
-        k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_diag.p, nb, ld, Q, s.d_T.p, s.d_colacc.p,
-                                        s.d_tdiag.p);
+        k_fill_T<<<dim3(cdiv(ld, 1024), cdiv(nb, kFillRows)), 256, 0, stm>>>(s.d_diag.p, nb, ld,
+                                                                              s.d_T.p, s.d_tdiag.p);
+        IRTT_HIP(hipGetLastError());
+        dim3 fg(cdiv(ld, kFinCols), cdiv(nb, kFinRows));
+        k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_touch.p, s.d_diag.p, nb, ld, Q, s.d_T.p,
+                                        s.d_colacc.p, s.d_tdiag.p);
         IRTT_HIP(hipGetLastError());
     }
     IRTT_HIP(hipEventRecord(s.ev2, stm));
     // energy conservation in FP64 on the device (after the column sums)
     s.d_energy.alloc(8);
-    k_energy<<<1, 256, 0, stm>>>(reinterpret_cast<const unsigned long long *>(s.d_q.p), s.nt, per,
+    k_energy<<<1, kEnergyThreads, 0, stm>>>(reinterpret_cast<const unsigned long long *>(s.d_q.p), s.nt, per,
                                  s.d_colacc.p, s.d_diag.p, nb, s.d_stats.p, Q, s.d_energy.p);
     IRTT_HIP(hipGetLastError());
     // read-backs into the scene's page-locked block: column sums, counters,
@@ -705,6 +880,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.layout = p.layout;
     s.op_kind = sparse ? IRTT_OP_SPARSE : IRTT_OP_DENSE;
     s.has_transport = true;
+    s.acc_marked = !sparse;
 }
 
 }  // namespace irtt
