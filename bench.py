@@ -859,6 +859,14 @@ def main():
     for _ in range(args.steps):
         stats.append(s.build_transport(p))
     t1 = time.perf_counter()
+    # a cold build for comparison: a fresh scene clears all of its accumulators
+    # and writes all of T (repeated builds of one scene reset only the chunks
+    # the previous build touched)
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s_cold:
+        s_cold.build_bvh()
+        tc = time.perf_counter()
+        s_cold.build_transport(p)
+        cold_ms = (time.perf_counter() - tc) * 1e3
     # the stored operator, for the FP64 check of the solves below
     check = {"operator": s.transport_sparse() + (s.column_sums(),)}
     ms_per_step = (t1 - t0) * 1e3 / args.steps
@@ -940,7 +948,8 @@ def main():
         "roofline": roofline,
         "breakdown_ms": {"photon_kernel": round(trace_ms, 3),
                          "finalize": round(sum(x["finalize_ms"] for x in stats) / len(stats), 3),
-                         "bvh_build": round(st["bvh_ms"], 3)},
+                         "bvh_build": round(st["bvh_ms"], 3),
+                         "cold_build": round(cold_ms, 3)},
         "steady_solve": {"ms": round(sol["solve_ms"], 1), "newton_iters": sol["newton_iters"],
                          "linear_iters": sol["linear_iters"], "converged": sol["converged"],
                          "power_in_W": round(sol["power_in"], 3),

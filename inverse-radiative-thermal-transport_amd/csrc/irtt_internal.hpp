@@ -382,6 +382,8 @@ struct Scene {
     DBuf<unsigned long long> d_acc;   // nb*ld fixed-point accumulators
     DBuf<uint8_t> d_touch;            // nb*ld >> kTouchShift chunk marks
     bool acc_marked = false;          // d_acc is zero outside the chunks d_touch marks
+    uint64_t t_fill_gen = ~0ull;      // d_T, d_tdiag outside marked chunks: zeros and the
+                                      // FP32 diagonal of this assembly
     DBuf<float> d_T;                  // nb*ld FP32 operator, row-major
     DBuf<double> d_diag;              // sigma*eps*A per bin
     std::vector<double> h_diag;       // what d_diag holds

@@ -373,18 +373,25 @@ __global__ __launch_bounds__(256) void k_finalize(const unsigned long long *__re
 }
 
 // Zeroes the accumulator chunks the last build marked, and their marks:
-// afterwards every accumulator and mark is zero.  A block takes 1024 words of
-// marks (4096 chunks) at a time, lists the marked chunks in LDS, and clears
-// each with eight lanes (128 contiguous bytes).  n4 = chunks / 4 words of
-// marks (the chunk count nb * ld / 16 is a multiple of 4 since ld is a
-// multiple of 64, and below 2^32: dense operators stay far below 2^36 entries).
+// afterwards every accumulator and mark is zero.  With T given, the same
+// chunks of T (and of the stored diagonal) are set back to what k_fill_T
+// writes, so T again holds +0 off the diagonal and FP32(diag_i) on it
+// everywhere.  A block takes 1024 words of marks (4096 chunks) at a time,
+// lists the marked chunks in LDS, and clears each with eight lanes (128
+// contiguous bytes of accumulators, 64 of T).  n4 = chunks / 4 words of marks
+// (the chunk count nb * ld / 16 is a multiple of 4 since ld is a multiple of
+// 64, and below 2^32: dense operators stay far below 2^36 entries).
 constexpr int kClearWords = 1024;  // words of marks per block and round (4 per thread)
-static_assert(kTouchShift == 4, "eight 16-byte stores clear one chunk");
+static_assert(kTouchShift == 4, "eight lanes of 16 B (accumulators) / 8 B (T) per chunk");
 __global__ __launch_bounds__(256) void k_clear_marked(unsigned long long *__restrict__ acc,
-                                                      uint32_t *__restrict__ touch4, size_t n4)
+                                                      uint32_t *__restrict__ touch4, size_t n4,
+                                                      float *__restrict__ T,
+                                                      const double *__restrict__ diag,
+                                                      float *__restrict__ tdiag, uint32_t ld)
 {
     __shared__ uint32_t list[kClearWords * 4];
     __shared__ uint32_t n_list;
+    const uint32_t cpr = ld >> kTouchShift;
     for (size_t base = (size_t)blockIdx.x * kClearWords; base < n4;
          base += (size_t)gridDim.x * kClearWords) {
         if (threadIdx.x == 0) n_list = 0;
@@ -412,9 +419,23 @@ __global__ __launch_bounds__(256) void k_clear_marked(unsigned long long *__rest
         }
         __syncthreads();
         const uint32_t n = n_list;
-        for (uint32_t e = threadIdx.x >> 3; e < n; e += blockDim.x / 8)
-            reinterpret_cast<ulonglong2 *>(acc + ((size_t)list[e] << kTouchShift))[threadIdx.x & 7] =
+        const uint32_t l = threadIdx.x & 7;
+        for (uint32_t e = threadIdx.x >> 3; e < n; e += blockDim.x / 8) {
+            const uint32_t k = list[e];
+            reinterpret_cast<ulonglong2 *>(acc + ((size_t)k << kTouchShift))[l] =
                 make_ulonglong2(0ull, 0ull);
+            if (T) {
+                const uint32_t row = k / cpr, col = ((k - row * cpr) << kTouchShift) + 2 * l;
+                float2 f = make_float2(0.0f, 0.0f);
+                if (row - col < 2u) {
+                    const float d = (float)(diag[row] - 0.0);
+                    if (row == col) f.x = d;
+                    else f.y = d;
+                    tdiag[row] = d;
+                }
+                *reinterpret_cast<float2 *>(T + (size_t)row * ld + col) = f;
+            }
+        }
         __syncthreads();  // the list is rebuilt next round
     }
 }
@@ -710,6 +731,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         s.d_touch.release();
         s.d_T.release();
         s.acc_marked = false;
+        s.t_fill_gen = ~0ull;
     } else {
         s.sp.reset();
     }
@@ -719,29 +741,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     // the accumulator (or hash table) clear runs while the host derives the
     // photon weights
     PhotonSink sink;
-    if (sparse) {
-        sparse_table_begin(s, total, sink);
-    } else {
-        // a repeated build clears only the chunks the previous one marked;
-        // fresh (or reallocated) accumulators are cleared whole
-        const void *prev_acc = s.d_acc.p, *prev_touch = s.d_touch.p;
-        s.d_acc.alloc(nb2);
-        s.d_touch.alloc(nb2 >> kTouchShift);
-        s.d_T.alloc(nb2);
-        const size_t n4 = s.d_touch.bytes() / 4;
-        if (s.acc_marked && s.d_acc.p == prev_acc && s.d_touch.p == prev_touch) {
-            k_clear_marked<<<std::min<uint32_t>(cdiv(n4, kClearWords), 2048), 256, 0, stm>>>(
-                s.d_acc.p, reinterpret_cast<uint32_t *>(s.d_touch.p), n4);
-            IRTT_HIP(hipGetLastError());
-        } else {
-            IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
-            IRTT_HIP(hipMemsetAsync(s.d_touch.p, 0, s.d_touch.bytes(), stm));
-        }
-        // not marked-clean again until this build has completed
-        s.acc_marked = false;
-        sink.acc = s.d_acc.p;
-        sink.touch = s.d_touch.p;
-    }
+    if (sparse) sparse_table_begin(s, total, sink);
 
     // photon weights, thermal_tracing_unified.glsl:287,309-310, and their
     // fixed-point form q_t = rint(w_t / Q), Q = max w * 2^-s; with the bin
@@ -794,6 +794,40 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.wq_gen = s.assembly_gen;
     s.wq_per = per;
     s.wq_total = total;
+
+    // Dense layout.  A repeated build clears only the accumulator chunks the
+    // previous one marked; fresh (or reallocated) accumulators are cleared
+    // whole.  T (and its stored diagonal) outside the marked chunks keeps
+    // what k_fill_T wrote for the scene's assembly: a repeated build of the
+    // same assembly sets back only the previous build's marked chunks of T,
+    // any other build writes the whole of T.
+    bool t_kept = false;
+    if (!sparse) {
+        const void *prev_acc = s.d_acc.p, *prev_touch = s.d_touch.p, *prev_T = s.d_T.p,
+                   *prev_tdiag = s.d_tdiag.p;
+        s.d_acc.alloc(nb2);
+        s.d_touch.alloc(nb2 >> kTouchShift);
+        s.d_T.alloc(nb2);
+        s.d_tdiag.alloc(nb);
+        const size_t n4 = s.d_touch.bytes() / 4;
+        const bool marked = s.acc_marked && s.d_acc.p == prev_acc && s.d_touch.p == prev_touch;
+        t_kept = marked && s.d_T.p == prev_T && s.d_tdiag.p == prev_tdiag &&
+                 s.t_fill_gen == s.assembly_gen;
+        if (marked) {
+            k_clear_marked<<<std::min<uint32_t>(cdiv(n4, kClearWords), 2048), 256, 0, stm>>>(
+                s.d_acc.p, reinterpret_cast<uint32_t *>(s.d_touch.p), n4,
+                t_kept ? s.d_T.p : nullptr, s.d_diag.p, s.d_tdiag.p, ld);
+            IRTT_HIP(hipGetLastError());
+        } else {
+            IRTT_HIP(hipMemsetAsync(s.d_acc.p, 0, s.d_acc.bytes(), stm));
+            IRTT_HIP(hipMemsetAsync(s.d_touch.p, 0, s.d_touch.bytes(), stm));
+        }
+        // neither is trusted again until this build has completed
+        s.acc_marked = false;
+        s.t_fill_gen = ~0ull;
+        sink.acc = s.d_acc.p;
+        sink.touch = s.d_touch.p;
+    }
     IRTT_HIP(hipMemsetAsync(s.d_colacc.p, 0, s.d_colacc.bytes(), stm));
     IRTT_HIP(hipMemsetAsync(s.d_stats.p, 0, s.d_stats.bytes(), stm));
     // absorbing bin of every photon (4 B each) so the adjoint trace replays
@@ -817,10 +851,11 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     if (sparse) {
         sparse_table_finish(s, sink);
     } else {
-        s.d_tdiag.alloc(nb);
-        k_fill_T<<<dim3(cdiv(ld, 1024), cdiv(nb, kFillRows)), 256, 0, stm>>>(s.d_diag.p, nb, ld,
-                                                                              s.d_T.p, s.d_tdiag.p);
-        IRTT_HIP(hipGetLastError());
+        if (!t_kept) {
+            k_fill_T<<<dim3(cdiv(ld, 1024), cdiv(nb, kFillRows)), 256, 0, stm>>>(
+                s.d_diag.p, nb, ld, s.d_T.p, s.d_tdiag.p);
+            IRTT_HIP(hipGetLastError());
+        }
         dim3 fg(cdiv(ld, kFinCols), cdiv(nb, kFinRows));
         k_finalize<<<fg, 256, 0, stm>>>(s.d_acc.p, s.d_touch.p, s.d_diag.p, nb, ld, Q, s.d_T.p,
                                         s.d_colacc.p, s.d_tdiag.p);
@@ -881,6 +916,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.op_kind = sparse ? IRTT_OP_SPARSE : IRTT_OP_DENSE;
     s.has_transport = true;
     s.acc_marked = !sparse;
+    s.t_fill_gen = sparse ? ~0ull : s.assembly_gen;
 }
 
 }  // namespace irtt

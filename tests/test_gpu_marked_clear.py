@@ -45,3 +45,36 @@ def test_repeated_dense_builds_bit_identical(bin_mode):
             assert q == r_q
             assert np.array_equal(acc, r_acc), f"build {k} (seed {seed}): accumulators differ"
             assert np.array_equal(s.transport(), r_T), f"build {k}: operator differs"
+
+
+def test_repeated_builds_across_geometry_updates():
+    """T outside the marked chunks is kept between builds of one assembly and
+    written whole after the geometry (hence areas and diagonals) changes."""
+    import irtt
+    from irtt import scenes
+    sd = scenes.city_block(seed=4, nx=2, ny=2, target_tris=3000)
+    xyz0 = sd.xyz.astype(np.float32)
+    xyz1 = xyz0.copy()
+    xyz1[:, 2] *= np.float32(1.05)  # taller buildings: other areas, other diagonal
+    tp = irtt.transport_params(64, 8, batch_seed=31)
+    tp2 = irtt.transport_params(64, 4, batch_seed=32)
+
+    def fresh(xyz, params):
+        with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as f:
+            f.update_vertices(xyz)
+            f.build_transport(params)
+            acc, q = f.transport_fixed()
+            return acc.copy(), q, f.transport().copy(), f.column_sums().copy()
+
+    plan = [(xyz0, tp), (xyz0, tp2), (xyz1, tp), (xyz1, tp2), (xyz0, tp2)]
+    with irtt.Scene.from_data(sd, irtt.BIN_VERTEX) as s:
+        for k, (xyz, params) in enumerate(plan):
+            if k and xyz is not plan[k - 1][0]:
+                s.update_vertices(xyz)
+            s.build_transport(params)
+            acc, q = s.transport_fixed()
+            r_acc, r_q, r_T, r_col = fresh(xyz, params)
+            assert q == r_q
+            assert np.array_equal(acc, r_acc), f"build {k}: accumulators differ"
+            assert np.array_equal(s.transport(), r_T), f"build {k}: operator differs"
+            assert np.array_equal(s.column_sums(), r_col), f"build {k}: column sums differ"
