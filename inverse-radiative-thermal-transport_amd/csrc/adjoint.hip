@@ -331,6 +331,13 @@ constexpr int adjoint_waves(int stack)
                ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
                : IRTT_ADJ_WAVES;
 }
+// the occlusion-ray kernel holds far fewer registers: up to 6 waves per SIMD
+constexpr int occl_waves(int stack)
+{
+    return (160 * 1024 / (stack * 4 * kTraceBlock)) / 4 < 6
+               ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
+               : 6;
+}
 
 // The wave's next slots for the lanes with `want`, called by all 64 lanes
 // together: chunk and fill are the wave's current chunk (the same in every
@@ -798,10 +805,74 @@ struct OcclArgs {
     uint64_t g_limit;      // records of photons at or above this are skipped
 };
 
-// The edge records' occlusion rays, chunk by chunk (the edge term of
-// thermal_tracing_unified.glsl:495-567).
+// The terms of one answered occlusion ray (the edge term of
+// thermal_tracing_unified.glsl:495-567): the edge record er and the closest
+// hit h beyond the edge.
+__device__ __forceinline__ void occl_terms(const AdjCommon &a, unsigned long long *gx,
+                                           const EdgeRec &er, const Hit &h, bool f_gain,
+                                           uint32_t &n_occl)
+{
+    const V3 od{er.r1.x, er.r1.y, er.r1.z};
+    const V3 dE{er.r2.x, er.r2.y, er.r2.z};
+    const float oht = er.r0.w;
+    const uint32_t ohp = __float_as_uint(er.r1.w), edge_v = __float_as_uint(er.r2.w);
+    const uint32_t origin_v = er.r3.x, src = er.r3.y, abs_bin = er.r3.z;
+    const bool valid = h.prim >= 0 && h.t >= a.tiny_dist;
+    bool nbr = false;
+    uint32_t shadow_v = 0;
+    if (valid) {
+        const uint32_t op = (uint32_t)h.prim;
+        const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1], c2 = a.idx[3 * op + 2];
+        const uint32_t oj0 = a.idx[3 * ohp], oj1 = a.idx[3 * ohp + 1], oj2 = a.idx[3 * ohp + 2];
+        nbr = c0 == oj0 || c0 == oj1 || c0 == oj2 || c1 == oj0 || c1 == oj1 || c1 == oj2 ||
+              c2 == oj0 || c2 == oj1 || c2 == oj2;
+        const V3 s0 = load_v(a.xyz, c0), s1 = load_v(a.xyz, c1), s2 = load_v(a.xyz, c2);
+        const V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
+        const float ob0 = 1.0f - h.b1 - h.b2;
+        const int sm = max_corner(ob0, h.b1, h.b2);
+        shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
+        if (!nbr && dot3(tn, od) < -1e-3f) {
+            n_occl++;
+            const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
+            const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
+            const float d1 = oht, d2 = h.t, tot = d1 + d2;
+            const double fs = hist_factor(a, shadow_bin, src);
+            fx_add3(gx, shadow_v, scl3(dE, sa * (d1 / tot)), fs, a.inv_q);
+            fx_add3(gx, origin_v, scl3(dE, sa * (d2 / tot)), fs, a.inv_q);
+            fx_add3(gx, edge_v, scl3(dE, -sa), fs, a.inv_q);
+        }
+    }
+    // Extension (not in the reference): the power the moving edge takes
+    // from the shadow side lands on the hit triangle and follows this
+    // photon's path, so it is credited to the path's absorbing bin with
+    // the opposite sign, also when nothing lies behind the edge.
+    if (f_gain && (h.prim < 0 || (valid && !nbr))) {
+        const double fpath = hist_factor(a, abs_bin, src);
+        const float f_or = valid ? h.t / (oht + h.t) : 1.0f;
+        fx_add3(gx, edge_v, dE, fpath, a.inv_q);
+        fx_add3(gx, origin_v, scl3(dE, f_or), -fpath, a.inv_q);
+        if (valid) fx_add3(gx, shadow_v, scl3(dE, 1.0f - f_or), -fpath, a.inv_q);
+    }
+}
+
+// Answered lanes that make a wave of k_adj_occl take new records.  Measured
+// on the city block (10.8 M occlusion rays beside the segment sort keys):
+// 64 (each window of 64 rays walked to its last answer) 3.55 ms, 48 3.4,
+// 32 3.39, 16 3.4, 8 3.5 (adjoint trace 22.13 / 21.96 / 21.92 / 22.07 /
+// 22.39 ms).
+#ifndef IRTT_OCCL_REFILL
+#define IRTT_OCCL_REFILL 32
+#endif
+
+// The edge records' occlusion rays.  Each lane walks the record slots
+// lane, lane + stride, ... (empty slots of partly filled chunks and records
+// of photons at or above g_limit skipped) and holds one ray at a time as a
+// resumable closest-hit query; the wave walks the tree until IRTT_OCCL_REFILL
+// lanes hold an answer, whose terms are then added and whose next records
+// taken while the other lanes keep their place in the tree.  The gradient
+// sums are exact integers, so the order of the rays does not matter.
 template <int STACK>
-__global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(OcclArgs oa)
+__global__ __launch_bounds__(kTraceBlock, occl_waves(STACK)) void k_adj_occl(OcclArgs oa)
 {
     const AdjCommon &a = oa.c;
     unsigned long long *const gx = a.grad + xcc_id() * a.grad_stride;
@@ -811,67 +882,34 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_occl(
     float *stt = stk_t + threadIdx.x;
     const bool f_gain = a.aflags & IRTT_ADJ_EDGE_GAIN;
     uint32_t n_edge = 0, n_occl = 0, nvis = 0, ntri = 0;
-    const uint64_t n_windows = (uint64_t)oa.n_chunks * (kChunk / 64);
-    for (uint64_t wi = blockIdx.x; wi < n_windows; wi += gridDim.x) {
-        const uint32_t c = (uint32_t)(wi / (kChunk / 64));
-        const uint32_t k = (uint32_t)(wi % (kChunk / 64)) * 64 + threadIdx.x;
-        bool have = k < oa.fills[c];
-        if (__ballot(have) == 0) continue;
-        EdgeRec er{};
-        if (have) {
-            er = oa.edges[(uint64_t)c * kChunk + k];
-            have = (uint64_t)er.r3.w < oa.g_limit;
+    const uint64_t n_slots = (uint64_t)oa.n_chunks * kChunk;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    uint64_t r = (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    EdgeRec er{};
+    Query q{};
+    bool tracing = false, answered = false;
+    for (;;) {
+        if (answered) {
+            occl_terms(a, gx, er, query_end(q, 1e12f), f_gain, n_occl);
+            answered = false;
         }
-        const V3 to{er.r0.x, er.r0.y, er.r0.z}, od{er.r1.x, er.r1.y, er.r1.z};
-        Query q{};
-        bool tracing = false;
-        if (have) {
-            n_edge++;
-            tracing = query_begin(q, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, ntri);
-        }
-        query_run(q, have && tracing, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
-        if (!have) continue;
-        const Hit h = query_end(q, 1e12f);
-        const V3 dE{er.r2.x, er.r2.y, er.r2.z};
-        const float oht = er.r0.w;
-        const uint32_t ohp = __float_as_uint(er.r1.w), edge_v = __float_as_uint(er.r2.w);
-        const uint32_t origin_v = er.r3.x, src = er.r3.y, abs_bin = er.r3.z;
-        const bool valid = h.prim >= 0 && h.t >= a.tiny_dist;
-        bool nbr = false;
-        uint32_t shadow_v = 0;
-        if (valid) {
-            const uint32_t op = (uint32_t)h.prim;
-            const uint32_t c0 = a.idx[3 * op], c1 = a.idx[3 * op + 1], c2 = a.idx[3 * op + 2];
-            const uint32_t oj0 = a.idx[3 * ohp], oj1 = a.idx[3 * ohp + 1], oj2 = a.idx[3 * ohp + 2];
-            nbr = c0 == oj0 || c0 == oj1 || c0 == oj2 || c1 == oj0 || c1 == oj1 || c1 == oj2 ||
-                  c2 == oj0 || c2 == oj1 || c2 == oj2;
-            const V3 s0 = load_v(a.xyz, c0), s1 = load_v(a.xyz, c1), s2 = load_v(a.xyz, c2);
-            const V3 tn = normalize3(cross3(s1 - s0, s2 - s0));
-            const float ob0 = 1.0f - h.b1 - h.b2;
-            const int sm = max_corner(ob0, h.b1, h.b2);
-            shadow_v = sm == 0 ? c0 : (sm == 1 ? c1 : c2);
-            if (!nbr && dot3(tn, od) < -1e-3f) {
-                n_occl++;
-                const uint32_t shadow_bin = a.bin_mode ? op : shadow_v;
-                const float sa = 1.0f - a.rho_d[op] - a.rho_s[op];
-                const float d1 = oht, d2 = h.t, tot = d1 + d2;
-                const double fs = hist_factor(a, shadow_bin, src);
-                fx_add3(gx, shadow_v, scl3(dE, sa * (d1 / tot)), fs, a.inv_q);
-                fx_add3(gx, origin_v, scl3(dE, sa * (d2 / tot)), fs, a.inv_q);
-                fx_add3(gx, edge_v, scl3(dE, -sa), fs, a.inv_q);
+        if (!tracing) {
+            while (r < n_slots) {
+                const uint64_t slot = r;
+                r += stride;
+                if ((uint32_t)(slot % kChunk) >= oa.fills[slot / kChunk]) continue;
+                er = oa.edges[slot];
+                if ((uint64_t)er.r3.w >= oa.g_limit) continue;
+                n_edge++;
+                const V3 to{er.r0.x, er.r0.y, er.r0.z}, od{er.r1.x, er.r1.y, er.r1.z};
+                tracing = query_begin(q, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, ntri);
+                answered = !tracing;
+                break;
             }
         }
-        // Extension (not in the reference): the power the moving edge takes
-        // from the shadow side lands on the hit triangle and follows this
-        // photon's path, so it is credited to the path's absorbing bin with
-        // the opposite sign, also when nothing lies behind the edge.
-        if (f_gain && (h.prim < 0 || (valid && !nbr))) {
-            const double fpath = hist_factor(a, abs_bin, src);
-            const float f_or = valid ? h.t / (oht + h.t) : 1.0f;
-            fx_add3(gx, edge_v, dE, fpath, a.inv_q);
-            fx_add3(gx, origin_v, scl3(dE, f_or), -fpath, a.inv_q);
-            if (valid) fx_add3(gx, shadow_v, scl3(dE, 1.0f - f_or), -fpath, a.inv_q);
-        }
+        if (__ballot(tracing || answered) == 0) break;
+        query_run_until(q, tracing, answered, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri,
+                        IRTT_OCCL_REFILL);
     }
     uint32_t cnt[2] = {n_edge, n_occl};
     const int slot_of[2] = {A_EDGE, A_OCCL};
@@ -1108,46 +1146,60 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         1, std::min<uint64_t>((uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU_ADJ,
                               (photons + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
     const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
-    // the record buffers this scene already holds count as available
-    const double budget =
+    // the record buffers this scene already holds count as available; when
+    // the device cannot give them (memory held elsewhere, or not in pieces
+    // this large), the budget halves and the photons run in more ranges
+    double budget =
         std::max(2.0 * (1u << 30), 0.3 * ((double)dev_available() + (double)ws.record_bytes()));
-    uint64_t seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
-    uint64_t edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
-    const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
-    uint64_t n_ranges = 1;
-    if (need > budget) {
-        const double f = budget / need;
-        seg_chunks = std::max<uint64_t>(2 * waves + 1, (uint64_t)(seg_chunks * f));
-        edge_chunks = want_edges ? std::max<uint64_t>(2 * waves + 1, (uint64_t)(edge_chunks * f))
-                                 : 1;
-        n_ranges = (uint64_t)std::ceil(1.0 / f);
-    }
     static_assert(sizeof(SegRec) == 3 * sizeof(uint4) && sizeof(EdgeRec) == 4 * sizeof(uint4),
                   "record layouts");
     DBuf<uint32_t> &d_fills = ws.fills, &d_keys = ws.keys, &d_keys2 = ws.keys2,
                    &d_order = ws.order, &d_order2 = ws.order2;
     DBuf<unsigned long long> &d_ctl = ws.ctl;  // chunks taken (segments, edges), first dropped
-    ws.segs.alloc(seg_chunks * kChunk * 3);
-    ws.edges.alloc(edge_chunks * kChunk * 4);
-    SegRec *const segs_p = reinterpret_cast<SegRec *>(ws.segs.p);
-    EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
-    d_fills.alloc(seg_chunks + edge_chunks);
-    d_keys.alloc(seg_chunks * kChunk);
-    d_keys2.alloc(seg_chunks * kChunk);
-    d_order.alloc(seg_chunks * kChunk);
-    d_order2.alloc(seg_chunks * kChunk);
-    d_ctl.alloc(3);
-    ws.n_big.alloc(1);
+    DBuf<uint8_t> &d_sort_tmp = ws.sort_tmp;
     uint32_t key_bits = 1;
     while (key_bits < 32 && (1ull << key_bits) <= s.nt) key_bits++;
+    uint64_t seg_chunks = 0, edge_chunks = 0, n_ranges = 1;
     size_t sort_tmp_b = 0;
-    {
-        rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
-        IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb,
-                                           (size_t)(seg_chunks * kChunk), 0, key_bits));
+    for (;;) {
+        seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
+        edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
+        const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
+        n_ranges = 1;
+        if (need > budget) {
+            const double f = budget / need;
+            seg_chunks = std::max<uint64_t>(2 * waves + 1, (uint64_t)(seg_chunks * f));
+            edge_chunks = want_edges ? std::max<uint64_t>(2 * waves + 1, (uint64_t)(edge_chunks * f))
+                                     : 1;
+            n_ranges = (uint64_t)std::ceil(1.0 / f);
+        }
+        try {
+            ws.segs.alloc(seg_chunks * kChunk * 3);
+            ws.edges.alloc(edge_chunks * kChunk * 4);
+            d_fills.alloc(seg_chunks + edge_chunks);
+            d_keys.alloc(seg_chunks * kChunk);
+            d_keys2.alloc(seg_chunks * kChunk);
+            d_order.alloc(seg_chunks * kChunk);
+            d_order2.alloc(seg_chunks * kChunk);
+            sort_tmp_b = 0;
+            rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+            IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb,
+                                               (size_t)(seg_chunks * kChunk), 0, key_bits));
+            d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
+            break;
+        } catch (const Error &e) {
+            if (e.code != IRTT_ERR_NOMEM || budget <= 2.0 * (1u << 30)) throw;
+            for (auto *b : {&d_fills, &d_keys, &d_keys2, &d_order, &d_order2}) b->release();
+            ws.segs.release();
+            ws.edges.release();
+            d_sort_tmp.release();
+            budget = std::max(2.0 * (1u << 30), 0.5 * budget);
+        }
     }
-    DBuf<uint8_t> &d_sort_tmp = ws.sort_tmp;
-    d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
+    d_ctl.alloc(3);
+    ws.n_big.alloc(1);
+    SegRec *const segs_p = reinterpret_cast<SegRec *>(ws.segs.p);
+    EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
     const uint64_t range = (photons + n_ranges - 1) / n_ranges;
     const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
     for (uint64_t gb = 0; gb < photons; gb += range) {

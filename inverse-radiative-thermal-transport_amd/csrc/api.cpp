@@ -398,7 +398,7 @@ void *irtt::dev_alloc(size_t bytes, hipStream_t stream)
             return b.p;
         }
     }
-    for (int attempt = 0; attempt < 2; attempt++) {
+    for (int attempt = 0; attempt < 3; attempt++) {
         void *p = nullptr;
         const auto t0 = std::chrono::steady_clock::now();
         hipError_t e = stream ? hipMallocAsync(&p, want, stream) : hipMalloc(&p, want);
@@ -420,9 +420,22 @@ void *irtt::dev_alloc(size_t bytes, hipStream_t stream)
                 std::lock_guard<std::mutex> lk(g_cache_mu);
                 v = trim_locked(0);
             }
-            if (v.empty()) break;
-            release_blocks(v, stream);
-            continue;
+            if (!v.empty()) {
+                release_blocks(v, stream);
+                continue;
+            }
+            attempt = 1;
+        }
+        if (attempt == 1) {
+            // the pool's unused reservations (returned blocks of other sizes)
+            // go back to the device once every queued free has run, so that a
+            // request larger than any of them can be reserved anew
+            hipMemPool_t pool = nullptr;
+            if (hipDeviceSynchronize() == hipSuccess &&
+                hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool &&
+                hipMemPoolTrimTo(pool, 0) == hipSuccess)
+                continue;
+            (void)hipGetLastError();
         }
         throw Error(IRTT_ERR_NOMEM, "device allocation of " + std::to_string(bytes) +
                                         " bytes failed: " + hipGetErrorString(e));

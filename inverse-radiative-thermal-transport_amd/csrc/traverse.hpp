@@ -237,6 +237,38 @@ __device__ __forceinline__ void query_run(Query &q, bool tracing, const float4 *
     }
 }
 
+// Like query_run, but returns once `want` lanes of the wave hold an answer
+// (answered: set here when a lane's query ends) or no lane is tracing, so
+// that the wave can start new rays on the answered lanes while the others
+// keep their place in the tree (their query state and LDS stack persist).
+__device__ __forceinline__ void query_run_until(Query &q, bool &tracing, bool &answered,
+                                                const float4 *__restrict__ nodes,
+                                                const float4 *__restrict__ tris, float tmin,
+                                                uint32_t *stk_n, float *stk_t, uint32_t &nvis,
+                                                uint32_t &ntri, uint32_t want)
+{
+    for (;;) {
+        for (;;) {
+            bool inner = tracing && !(q.cur & kLeafBit);
+            if (inner) {
+                tracing = query_node(q, nodes, tmin, stk_n, stk_t, nvis);
+                if (!tracing) answered = true;
+                inner = tracing && !(q.cur & kLeafBit);
+            }
+            if (__ballot(inner) == 0) break;
+            if (IRTT_LEAF_LANES < kTraceBlock &&
+                __builtin_popcountll(__ballot(tracing && !inner)) >= IRTT_LEAF_LANES)
+                break;
+        }
+        if (tracing && (q.cur & kLeafBit)) {
+            tracing = query_leaf(q, tris, tmin, stk_n, stk_t, ntri);
+            if (!tracing) answered = true;
+        }
+        if (__ballot(tracing) == 0) break;
+        if ((uint32_t)__builtin_popcountll(__ballot(answered)) >= want) break;
+    }
+}
+
 // the whole query in one call
 template <int STACK>
 __device__ __forceinline__ Hit trace_closest(const float4 *__restrict__ nodes,
