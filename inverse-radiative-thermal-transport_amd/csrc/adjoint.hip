@@ -174,13 +174,10 @@ __device__ __forceinline__ float beam_grad(V3 Ai, V3 Bi, V3 Ci, V3 Aj, V3 Bj, V3
 
 // ---- exact gradient sums ---------------------------------------------------
 // One gradient component is a signed 128-bit integer in units of q (a power
-// of two): the pair {lo, hi} of 64-bit words, value hi * 2^64 + lo.  v is
-// rounded to the nearest multiple of q once; the sums of the rounded values
-// are exact, so any order of the adds gives the same bits.  Magnitudes below
-// 2^63 q (all but rare outliers) take one 64-bit atomic add: lo wraps modulo
-// 2^64, and the carry out of lo (seen in the value the atomic returns) goes to
-// hi with the sign extension of v.
-// v in units of q, rounded once, as the 128-bit pair (lo, hi)
+// of two).  v is rounded to the nearest multiple of q once; the sums of the
+// rounded values are exact, so any order of the adds gives the same bits.
+// v in units of q, rounded once, as the 128-bit pair (lo, hi), value
+// hi * 2^64 + lo
 __device__ __forceinline__ void fx_quant(double v, double inv_q, unsigned long long &lo,
                                          long long &hi)
 {
@@ -202,28 +199,50 @@ __device__ __forceinline__ void fx_quant(double v, double inv_q, unsigned long l
     }
 }
 
-// adds the 128-bit (lo, hi) to the pair at p
-__device__ __forceinline__ void fx_add_q(unsigned long long *pair, unsigned long long lo,
-                                         long long hi)
+// A gradient component is three 64-bit words that take the 128-bit addend
+// y = lo + 2^64 hi in parts, each with an atomic add whose result is not
+// waited for.  When y fits in 64 signed bits (all but rare outliers): word 0
+// takes its low 32 bits (0 .. 2^32 - 1) and word 1 the signed rest y >> 32;
+// otherwise word 0 takes the low 32 bits of lo, word 1 its high 32 bits and
+// word 2 hi.  Every part added to words 0 and 1 is below 2^32 in magnitude,
+// so they hold 2^31 adds without overflow; the host forms word0 +
+// 2^32 word1 + 2^64 word2 (words 1 and 2 signed) modulo 2^128, the exact
+// sum in any order.
+constexpr uint32_t kCompWords = 3;
+__device__ __forceinline__ unsigned long long *fx_comp(unsigned long long *g, uint32_t vertex,
+                                                       uint32_t axis)
 {
-    if (lo == 0 && hi == 0) return;
-    const unsigned long long old = atomicAdd(pair, lo);
-    if (old + lo < old) hi += 1;
-    if (hi != 0) atomicAdd(pair + 1, (unsigned long long)hi);
+    return g + kCompWords * (3 * (size_t)vertex + axis);
 }
 
-__device__ __forceinline__ void fx_add(unsigned long long *pair, double v, double inv_q)
+// adds the 128-bit (lo, hi) to the component at c
+__device__ __forceinline__ void fx_add_q(unsigned long long *c, unsigned long long lo, long long hi)
+{
+    const unsigned long long l0 = lo & 0xffffffffull;
+    const long long sl = (long long)lo;
+    if (hi == (sl >> 63)) {
+        const long long a = sl >> 32;
+        if (l0) (void)atomicAdd(c, l0);
+        if (a) (void)atomicAdd(c + 1, (unsigned long long)a);
+    } else {
+        if (l0) (void)atomicAdd(c, l0);
+        if (lo >> 32) (void)atomicAdd(c + 1, lo >> 32);
+        (void)atomicAdd(c + 2, (unsigned long long)hi);
+    }
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long *c, double v, double inv_q)
 {
     unsigned long long lo;
     long long hi;
     fx_quant(v, inv_q, lo, hi);
-    fx_add_q(pair, lo, hi);
+    fx_add_q(c, lo, hi);
 }
 
 // The eight XCDs each sum into their own copy of the gradient (3 n_vertices
-// {lo, hi} pairs apiece), so that atomics to one component from different
-// XCDs do not queue on one address; the host adds the copies.  The copy is
-// the XCD's id (hardware register XCC_ID, bits 3:0).
+// components apiece), so that atomics to one component from different XCDs
+// do not queue on one address; the host adds the copies.  The copy is the
+// XCD's id (hardware register XCC_ID, bits 3:0).
 constexpr uint32_t kGradCopies = 8;
 __device__ __forceinline__ uint32_t xcc_id()
 {
@@ -233,10 +252,9 @@ __device__ __forceinline__ uint32_t xcc_id()
 __device__ __forceinline__ void fx_add3(unsigned long long *g, uint32_t v, V3 d, double f,
                                         double inv_q)
 {
-    unsigned long long *p = g + 6 * (size_t)v;
-    fx_add(p, f * (double)d.x, inv_q);
-    fx_add(p + 2, f * (double)d.y, inv_q);
-    fx_add(p + 4, f * (double)d.z, inv_q);
+    fx_add(fx_comp(g, v, 0), f * (double)d.x, inv_q);
+    fx_add(fx_comp(g, v, 1), f * (double)d.y, inv_q);
+    fx_add(fx_comp(g, v, 2), f * (double)d.z, inv_q);
 }
 
 // ---- shared arguments ---------------------------------------------------------
@@ -256,8 +274,8 @@ struct AdjCommon {
     const double *x4;          // n_hist x nb
     const uint32_t *absorber;  // per photon (~0: none)
     uint64_t photons;
-    unsigned long long *grad;  // kGradCopies x 3 n_vertices components, {lo, hi} each
-    uint64_t grad_stride;      // words per copy (6 n_vertices)
+    unsigned long long *grad;  // kGradCopies x 3 n_vertices components, kCompWords words each
+    uint64_t grad_stride;      // words per copy (9 n_vertices)
     double inv_q;
     unsigned long long *stats;
 };
@@ -759,8 +777,8 @@ __global__ __launch_bounds__(256) void k_adj_segments(SegArgs sa)
             const uint32_t tri = key;
 #pragma unroll
             for (int e = 0; e < 9; e++)
-                fx_add_q(gx + 2 * (3 * (size_t)a.idx[3 * tri + e / 3] + e % 3),
-                         (unsigned long long)acc[e], acc[e] < 0 ? -1ll : 0ll);
+                fx_add_q(fx_comp(gx, a.idx[3 * tri + e / 3], e % 3), (unsigned long long)acc[e],
+                         acc[e] < 0 ? -1ll : 0ll);
         }
     }
     if (SIDE == 0) {
@@ -791,7 +809,7 @@ __global__ __launch_bounds__(256) void k_adj_seg_big(SegArgs sa)
         const uint32_t key = SIDE == 0 ? r.r0.x : r.r0.y;
         seg_terms<SIDE>(a, r, f_cont, f_hit,
                   [&](int e, double v) {
-                      fx_add(gx + 2 * (3 * (size_t)a.idx[3 * key + e / 3] + e % 3), v, a.inv_q);
+                      fx_add(fx_comp(gx, a.idx[3 * key + e / 3], e % 3), v, a.inv_q);
                   },
                   unused);
     }
@@ -1046,7 +1064,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     d_area.alloc(s.nt);
     d_lam.alloc((size_t)n_hist * nb);
     d_x4.alloc((size_t)n_hist * nb);
-    d_grad.alloc(kGradCopies * 6 * (size_t)s.nv);
+    d_grad.alloc(kGradCopies * kCompWords * 3 * (size_t)s.nv);
     d_st.alloc(16);
     IRTT_HIP(hipMemcpyAsync(d_w.p, w.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
     IRTT_HIP(hipMemcpyAsync(d_area.p, s.tri_area.data(), s.nt * 4, hipMemcpyHostToDevice, stm));
@@ -1097,7 +1115,7 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     c.absorber = absorber;
     c.photons = photons;
     c.grad = d_grad.p;
-    c.grad_stride = 6 * (uint64_t)s.nv;
+    c.grad_stride = kCompWords * 3 * (uint64_t)s.nv;
     c.inv_q = inv_q;
     c.stats = d_st.p;
 
@@ -1240,6 +1258,8 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                     1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
                 // after the replay (complete: its counters were read above),
                 // beside the segment terms enqueued next on the scene's stream
+                // (measured: beside the start side's terms instead, both slow
+                // to 4.1 and 4.9 ms and the trace takes 0.3 ms longer)
                 IRTT_HIP(hipEventRecord(side2.fork, stm));
                 IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
                 if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm2>>>(oa);
@@ -1297,18 +1317,22 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     IRTT_HIP(hipStreamWaitEvent(stm, side2.area_done, 0));
     if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));
     IRTT_HIP(hipEventRecord(s.ev1, stm));
-    std::vector<unsigned long long> gx(kGradCopies * 6 * (size_t)s.nv), hs(16);
+    std::vector<unsigned long long> gx(kGradCopies * kCompWords * 3 * (size_t)s.nv), hs(16);
     IRTT_HIP(hipMemcpyAsync(gx.data(), d_grad.p, d_grad.bytes(), hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(hs.data(), d_st.p, 16 * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipStreamSynchronize(stm));
     IRTT_HIP(hipEventElapsedTime(&st.trace_ms, s.ev0, s.ev1));
     const double q = 1.0 / inv_q;
-    const size_t words = 6 * (size_t)s.nv;
+    const size_t words = kCompWords * 3 * (size_t)s.nv;
     for (size_t k = 0; k < 3 * (size_t)s.nv; k++) {
-        unsigned __int128 sum = 0;  // exact: the copies' integers added modulo 2^128
-        for (uint32_t c = 0; c < kGradCopies; c++)
-            sum += ((unsigned __int128)gx[c * words + 2 * k + 1] << 64) | gx[c * words + 2 * k];
+        unsigned __int128 sum = 0;  // exact: the copies' words added modulo 2^128
+        for (uint32_t c = 0; c < kGradCopies; c++) {
+            const unsigned long long *w = gx.data() + c * words + kCompWords * k;
+            sum += (unsigned __int128)w[0] +
+                   (unsigned __int128)((__int128)(long long)w[1] * ((__int128)1 << 32)) +
+                   ((unsigned __int128)w[2] << 64);
+        }
         grad[k] = (double)(__int128)sum * q;
     }
     st.photons = photons;
