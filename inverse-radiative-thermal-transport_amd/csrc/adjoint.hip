@@ -474,12 +474,12 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
         // slots are allocated by the whole wave together
         if (__ballot(live) == 0) break;
         Query q{};
-        bool tracing = false;
+        q.cur = kQueryDone;
         if (live) {
             ++depth;
-            tracing = query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri);
+            if (!query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri)) q.cur = kQueryDone;
         }
-        query_run(q, live && tracing, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri);
+        query_run_ref(q, a.nodes, a.tris, 1e-12f, sn, nvis, ntri);
         // the reference's reject rules: miss, too close, (near) backface
         bool seg = false;
         Hit h{-1, 0.0f, 0.0f, 0.0f};
@@ -905,13 +905,14 @@ __global__ __launch_bounds__(kTraceBlock, occl_waves(STACK)) void k_adj_occl(Occ
     uint64_t r = (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
     EdgeRec er{};
     Query q{};
-    bool tracing = false, answered = false;
+    q.cur = kQueryDone;
+    bool open = false;  // this lane holds a record whose ray has not been answered and used
     for (;;) {
-        if (answered) {
+        if (open && q.cur == kQueryDone) {
             occl_terms(a, gx, er, query_end(q, 1e12f), f_gain, n_occl);
-            answered = false;
+            open = false;
         }
-        if (!tracing) {
+        if (!open) {
             while (r < n_slots) {
                 const uint64_t slot = r;
                 r += stride;
@@ -920,14 +921,14 @@ __global__ __launch_bounds__(kTraceBlock, occl_waves(STACK)) void k_adj_occl(Occ
                 if ((uint64_t)er.r3.w >= oa.g_limit) continue;
                 n_edge++;
                 const V3 to{er.r0.x, er.r0.y, er.r0.z}, od{er.r1.x, er.r1.y, er.r1.z};
-                tracing = query_begin(q, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, ntri);
-                answered = !tracing;
+                if (!query_begin(q, a.tris, a.n_leaves, to, od, 1e-12f, 1e12f, ntri))
+                    q.cur = kQueryDone;
+                open = true;
                 break;
             }
         }
-        if (__ballot(tracing || answered) == 0) break;
-        query_run_until(q, tracing, answered, a.nodes, a.tris, 1e-12f, sn, stt, nvis, ntri,
-                        IRTT_OCCL_REFILL);
+        if (__ballot(open) == 0) break;
+        query_run_until_ref(q, open, a.nodes, a.tris, 1e-12f, sn, nvis, ntri, IRTT_OCCL_REFILL);
     }
     uint32_t cnt[2] = {n_edge, n_occl};
     const int slot_of[2] = {A_EDGE, A_OCCL};

@@ -239,8 +239,13 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
             if (!live) break;
         }
         ++depth;
-        Hit h = trace_closest<STACK>(a.nodes, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, sn, stt, nvis,
-                                     ntri);
+        Hit h;
+        {
+            Query q;
+            if (!query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri)) q.cur = kQueryDone;
+            query_run_ref(q, a.nodes, a.tris, 1e-12f, sn, nvis, ntri);
+            h = query_end(q, 1e12f);
+        }
         const int outcome = shade_segment(a, h, g, src, qt, depth, seed, o, d);
         count_segment(cnt, outcome);
         q_esc += outcome == S_MISS ? qt : 0ull;
