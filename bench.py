@@ -36,12 +36,12 @@ BIN_BYTES = 8        # one 64-bit accumulator add per absorbed photon
 # 1 GiB device copy (bench.py --calibrate-copy 1024).
 # (The chunk marks' byte stores account for about 3 GB of it: each is a
 # partial write of an 83 MB array that L2 reads and writes back whole.)
-TRAFFIC_BYTES_CITY = (3664234 * 2.0 + 1924281 * 1.0) * 1024
+TRAFFIC_BYTES_CITY = (3667984 * 2.0 + 1924083 * 1.0) * 1024
 # VALU wave-instructions issued (SQ_INSTS_VALU) and lane utilisation
 # (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)); the work is the same
 # every launch (fixed seeds), so the count is a property of the workload.
-VALU_INSTS_CITY = 5826731666
-VALU_LANE_UTIL_CITY = 156387550739 / (64.0 * 5889297617)
+VALU_INSTS_CITY = 5563105516
+VALU_LANE_UTIL_CITY = 146286162270 / (64.0 * 5625671467)
 # VALU issue peak: one wave64 instruction per cycle per CU (four SIMD16 each
 # taking four cycles per instruction), 256 CUs at 2.4 GHz
 VALU_PEAK_G = 256 * 2.4
