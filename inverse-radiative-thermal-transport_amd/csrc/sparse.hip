@@ -210,8 +210,20 @@ __global__ __launch_bounds__(256) void k_spmv(const uint32_t *__restrict__ ptr,
     const uint32_t lane = threadIdx.x & 63;
     if (r >= n) return;
     const uint32_t b = ptr[r], e = ptr[r + 1];
-    double a = 0.0;
-    for (uint32_t k = b + lane; k < e; k += 64) a += (double)val[k] * u[idx[k]];
+    // four entries per lane in flight (their index loads, then their
+    // gathers), summed in four partial sums in a fixed order
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    uint32_t k = b + lane;
+    for (; k + 192 < e; k += 256) {
+        const uint32_t i0 = idx[k], i1 = idx[k + 64], i2 = idx[k + 128], i3 = idx[k + 192];
+        const float v0 = val[k], v1 = val[k + 64], v2 = val[k + 128], v3 = val[k + 192];
+        a0 += (double)v0 * u[i0];
+        a1 += (double)v1 * u[i1];
+        a2 += (double)v2 * u[i2];
+        a3 += (double)v3 * u[i3];
+    }
+    for (; k < e; k += 64) a0 += (double)val[k] * u[idx[k]];
+    double a = (a0 + a1) + (a2 + a3);
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
     if (lane == 0) y[r] = a;
 }
@@ -242,8 +254,19 @@ __global__ __launch_bounds__(256) void k_spmm(const uint32_t *__restrict__ ptr,
     if (r >= n) return;
     const uint32_t v = lane % RP, slot = lane / RP;
     const uint32_t b = ptr[r], e = ptr[r + 1];
-    double a = 0.0;
-    for (uint32_t k = b + slot; k < e; k += S) a += (double)val[k] * Xt[(size_t)idx[k] * RP + v];
+    // four entries per lane in flight, four partial sums in a fixed order
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    uint32_t k = b + slot;
+    for (; k + 3 * S < e; k += 4 * S) {
+        const uint32_t i0 = idx[k], i1 = idx[k + S], i2 = idx[k + 2 * S], i3 = idx[k + 3 * S];
+        const float v0 = val[k], v1 = val[k + S], v2 = val[k + 2 * S], v3 = val[k + 3 * S];
+        a0 += (double)v0 * Xt[(size_t)i0 * RP + v];
+        a1 += (double)v1 * Xt[(size_t)i1 * RP + v];
+        a2 += (double)v2 * Xt[(size_t)i2 * RP + v];
+        a3 += (double)v3 * Xt[(size_t)i3 * RP + v];
+    }
+    for (; k < e; k += S) a0 += (double)val[k] * Xt[(size_t)idx[k] * RP + v];
+    double a = (a0 + a1) + (a2 + a3);
     for (int off = 32; off >= RP; off >>= 1) a += __shfl_down(a, off, 64);
     if (slot == 0 && v < rc) Y[(size_t)v * ld + r] = a;
 }
