@@ -664,8 +664,20 @@ __global__ __launch_bounds__(256) void k_hm_spmv(const uint32_t *__restrict__ pt
     const uint32_t r = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
     if (r >= n) return;
-    double a = 0.0;
-    for (uint32_t k = ptr[r] + lane; k < ptr[r + 1]; k += 64) a += (double)val[k] * u[idx[k]];
+    // four entries per lane in flight, four partial sums in a fixed order
+    const uint32_t e = ptr[r + 1];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    uint32_t k = ptr[r] + lane;
+    for (; k + 192 < e; k += 256) {
+        const uint32_t i0 = idx[k], i1 = idx[k + 64], i2 = idx[k + 128], i3 = idx[k + 192];
+        const float v0 = val[k], v1 = val[k + 64], v2 = val[k + 128], v3 = val[k + 192];
+        a0 += (double)v0 * u[i0];
+        a1 += (double)v1 * u[i1];
+        a2 += (double)v2 * u[i2];
+        a3 += (double)v3 * u[i3];
+    }
+    for (; k < e; k += 64) a0 += (double)val[k] * u[idx[k]];
+    double a = (a0 + a1) + (a2 + a3);
     for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
     if (lane == 0) y[r] = a;
 }
