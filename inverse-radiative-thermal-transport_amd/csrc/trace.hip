@@ -743,7 +743,7 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
 
     hipStream_t stm = s.stream;
     const uint64_t total = per64 * s.nt;
-    // the accumulator (or hash table) clear runs while the host derives the
+    // the hash table's clear (sparse layout) runs while the host derives the
     // photon weights
     PhotonSink sink;
     if (sparse) sparse_table_begin(s, total, sink);
