@@ -389,7 +389,7 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
         rows_x = np.stack([x ** 4, lam])
         tr = []
         for _ in range(reps):
-            _, ast = s.adjoint_trace(rows_l, rows_x)
+            g_replay, ast = s.adjoint_trace(rows_l, rows_x)
             tr.append(ast)
         ms = min(a["trace_ms"] for a in tr)
         out["adjoint_trace"] = {"ms": round(ms, 2), "photons": tr[-1]["photons"],
@@ -398,6 +398,28 @@ def sparse_layout(irtt, sd, p, x_ref, reps=3, batch=32):
                                 "occluded_edge_hits": tr[-1]["occluded"],
                                 "segments_per_s_M": round(tr[-1]["segments"] / ms / 1e3, 1),
                                 "forward_photon_kernel_ms": round(st["trace_ms"], 2)}
+        # the same with the paths kept by the build (no replay in the trace):
+        # builds (the first sizes the records at one per photon, the next
+        # ones from its count), then traces of the last build's records
+        q.flags = irtt.TRANSPORT_KEEP_PATHS
+        kb, kwall = [], []
+        for _ in range(reps + 1):
+            t0 = time.perf_counter()
+            kb.append(s.build_transport(q))
+            kwall.append(time.perf_counter() - t0)
+        ktr = []
+        for _ in range(reps):
+            g_kept, kst = s.adjoint_trace(rows_l, rows_x)
+            ktr.append(kst)
+        kms = min(a["trace_ms"] for a in ktr)
+        out["adjoint_trace"]["kept_paths"] = {
+            "trace_ms": round(kms, 2), "kept": ktr[-1]["kept_paths"],
+            "build_ms": round(min(kwall[1:]) * 1e3, 2),
+            "photon_kernel_ms": round(min(x["trace_ms"] for x in kb[1:]), 2),
+            "build_ms_without": round(min(walls) * 1e3, 2),
+            "gradient_bit_identical": bool(np.array_equal(g_kept, g_replay)),
+            "build_plus_trace_ms": round(min(kwall[1:]) * 1e3 + kms, 2),
+            "build_plus_trace_ms_replay": round(min(walls) * 1e3 + ms, 2)}
         _log("sparse layout: hierarchical blocks")
         out["hmatrix"] = hmatrix_form(irtt, s, sd, sp)
     out["unit"] = ("sparse layout of the city block operator: build, and forward steady + "

@@ -151,6 +151,16 @@ typedef struct irtt_transport_params {
  * percent; set it when an adjoint trace will follow. */
 #define IRTT_TRANSPORT_KEEP_ABSORBERS 1u
 
+/* Keep the paths as well (implies KEEP_ABSORBERS; up to 2^31 photons): the
+ * build's photon kernel writes the records irtt_adjoint_trace's path replay
+ * would write -- 48 B per accepted segment, 64 B per hit within the default
+ * edge ratio (0.05) of an edge -- so a following adjoint trace skips its
+ * replay.  A trace with another edge ratio replays; the
+ * records that do not fit the memory the build sets aside (at most 30 % of
+ * what is free) are replayed too.  No counterpart in the reference, whose
+ * adjoint shader traces every path twice (thermal_tracing_unified.glsl:292-599). */
+#define IRTT_TRANSPORT_KEEP_PATHS 2u
+
 /* Storage of the operator built by irtt_build_transport.  DENSE is the
  * reference's n_bins x n_bins FP32 matrix (thermal_transport.cpp:612-660).
  * SPARSE keeps only the entries some photon reached (plus the diagonal) in
@@ -466,7 +476,9 @@ typedef struct irtt_adjoint_params {
 typedef struct irtt_adjoint_stats {
     uint64_t photons, absorbed, segments, continuous, edge_hits, occluded, rays;
     float trace_ms;
-    uint32_t reserved;
+    uint32_t kept_paths; /* 0: every photon replayed; 1: the build's kept paths
+                            (IRTT_TRANSPORT_KEEP_PATHS) stood for the replay;
+                            2: kept paths up to a photon, the rest replayed */
 } irtt_adjoint_stats;
 
 /* Replaces ThermalTransport::computeAdjoint (thermal_transport.cpp:740-832):

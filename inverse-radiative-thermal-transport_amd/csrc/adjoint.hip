@@ -42,6 +42,7 @@
 
 #include "device_math.hpp"
 #include "irtt_internal.hpp"
+#include "path_records.hpp"
 #include "traverse.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -54,17 +55,6 @@ __device__ __forceinline__ V3 scl3(V3 a, float s) { return V3{a.x * s, a.y * s, 
 __device__ __forceinline__ bool finite3(V3 a)
 {
     return isfinite(a.x) && isfinite(a.y) && isfinite(a.z);
-}
-__device__ __forceinline__ int min_corner(float b0, float b1, float b2)
-{
-    int idx = 0;
-    float v = b0;
-    if (b1 < v) {
-        idx = 1;
-        v = b1;
-    }
-    if (b2 < v) idx = 2;
-    return idx;
 }
 __device__ __forceinline__ V3 load_v(const float *xyz, uint32_t v)
 {
@@ -296,41 +286,6 @@ __device__ __forceinline__ double hist_factor(const AdjCommon &a, uint32_t bin_l
     return f;
 }
 
-// Edge record: one hit near an edge, its occlusion ray still to trace.
-//   r0: occlusion ray origin, distance from the segment's start to the hit
-//   r1: ray direction, hit triangle
-//   r2: edge-irradiance derivative dE, edge vertex
-//   r3: origin vertex, source bin, absorbing bin, photon index
-struct EdgeRec {
-    float4 r0, r1, r2;
-    uint4 r3;
-};
-
-// Segment record: one path segment of a replayed photon, its terms still to
-// compute.
-//   r0: start triangle (the emitter on the first segment), hit triangle,
-//       source bin, absorbing bin
-//   r1: barycentric weights of corners 1 and 2 at the start and at the hit
-//   r2: photon index, source triangle, first segment (1) or not (0)
-struct SegRec {
-    uint4 r0;
-    float4 r1;
-    uint4 r2;
-};
-
-// Records are written in chunks of kChunk consecutive slots, each chunk
-// owned by one wave of the replay kernel (one atomic per chunk, not per
-// write); fills[c] is the number of records chunk c holds.
-constexpr uint32_t kChunk = 256;
-constexpr uint32_t kNoChunk = 0xffffffffu;
-
-struct RecBuf {
-    void *p;                       // SegRec or EdgeRec slots
-    uint32_t n_chunks;             // capacity in chunks
-    uint32_t *fills;               // per chunk
-    unsigned long long *next;      // chunks taken (may run past n_chunks: dropped records)
-};
-
 struct PathArgs {
     AdjCommon c;
     uint64_t g_begin, g_end;       // photons replayed
@@ -355,48 +310,6 @@ constexpr int occl_waves(int stack)
     return (160 * 1024 / (stack * 4 * kTraceBlock)) / 4 < 6
                ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
                : 6;
-}
-
-// The wave's next slots for the lanes with `want`, called by all 64 lanes
-// together: chunk and fill are the wave's current chunk (the same in every
-// lane).  The current chunk is filled to the end before a new one is taken.
-// Returns the slot; ok is false when the buffer is full.
-__device__ __forceinline__ uint64_t chunk_slot(bool want, uint32_t &chunk, uint32_t &fill,
-                                               const RecBuf &b, bool &ok)
-{
-    const unsigned long long m = __ballot(want);
-    const uint32_t k = (uint32_t)__popcll(m);
-    ok = false;
-    if (k == 0) return 0;
-    const uint32_t prefix = (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull));
-    const uint32_t rem = chunk == kNoChunk ? 0u : kChunk - fill;
-    if (k <= rem) {
-        ok = chunk < b.n_chunks;
-        const uint64_t slot = (uint64_t)chunk * kChunk + fill + prefix;
-        fill += k;
-        return slot;
-    }
-    unsigned long long nc = 0;
-    if (__lane_id() == 0) nc = atomicAdd(b.next, 1ull);
-    nc = __shfl(nc, 0, 64);
-    const uint32_t ncu = nc < (unsigned long long)(kNoChunk - 1) ? (uint32_t)nc : kNoChunk - 1;
-    uint64_t slot;
-    if (prefix < rem) {
-        ok = chunk < b.n_chunks;
-        slot = (uint64_t)chunk * kChunk + fill + prefix;
-    } else {
-        ok = ncu < b.n_chunks;
-        slot = (uint64_t)ncu * kChunk + (prefix - rem);
-    }
-    if (chunk != kNoChunk && chunk < b.n_chunks && __lane_id() == 0) b.fills[chunk] = kChunk;
-    chunk = ncu;
-    fill = k - rem;
-    return slot;
-}
-
-__device__ __forceinline__ void chunk_close(uint32_t chunk, uint32_t fill, const RecBuf &b)
-{
-    if (chunk != kNoChunk && chunk < b.n_chunks && __lane_id() == 0) b.fills[chunk] = fill;
 }
 
 // Replays the photon paths (one segment per lane per iteration, lanes of a
@@ -515,55 +428,14 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
             }
         }
         // the approximate edge term, :454-567: a hit near an edge becomes an
-        // edge record with the occlusion ray
+        // edge record, its occlusion ray derived later (k_edge_prep) by all
+        // lanes of a wave at once instead of by the few lanes here that hit
+        // near an edge
         const float hb0 = 1.0f - h.b1 - h.b2;
         bool edge = false;
-        float4 e0, e1, e2;
-        uint4 e3;
         if (seg && f_edge) {
             const int opp = min_corner(hb0, h.b1, h.b2);
-            const float hb_opp = opp == 0 ? hb0 : (opp == 1 ? h.b1 : h.b2);
-            if (hb_opp <= a.ratio) {
-                edge = true;
-                const int nr = max_corner(hb0, h.b1, h.b2);
-                int other = 0;
-                if (opp == 0 || nr == 0) {
-                    other = 1;
-                    if (opp == 1 || nr == 1) other = 2;
-                }
-                const V3 qn = nr == 0 ? q0 : (nr == 1 ? q1 : q2);
-                const V3 qo = other == 0 ? q0 : (other == 1 ? q1 : q2);
-                const V3 qp = opp == 0 ? q0 : (opp == 1 ? q1 : q2);
-                const uint32_t jn = nr == 0 ? j0 : (nr == 1 ? j1 : j2);
-                V3 edir = qn - qo;
-                const V3 oppe = qn - qp;
-                const float elen = sqrt_rn(dot3(edir, edir));
-                edir = scl3(edir, 1.0f / elen);
-                V3 n_edg = normalize3(cross3(edir, hn));
-                float th = dot3(oppe, n_edg);
-                if (th < 0.0f) {
-                    th = -th;
-                    n_edg = scl3(n_edg, -1.0f);
-                }
-                const float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
-                V3 da = normalize3(cross3(edir, d));
-                if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
-                // the reference scales by |da.n_edg| (:486); the extension
-                // divides by it: an edge moved by u sweeps (u.da)/|da.n_edg|
-                // of the occluder's plane per unit length, seen along the ray
-                const float cpl = fabsf(dot3(da, n_edg));
-                V3 dE = f_proj ? scl3(da, elen * a.w[t] / (larea * cpl))
-                               : scl3(da, cpl * elen * a.w[t] / larea);
-                if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
-                V3 to = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
-                to = add3(to, scl3(n_edg, a.eps + dot3(qn - to, n_edg)));
-                to = to - scl3(d, a.eps);
-                const uint32_t origin_v = a.idx[3 * prev_tri + prev_max];
-                e0 = make_float4(to.x, to.y, to.z, h.t);
-                e1 = make_float4(d.x, d.y, d.z, __uint_as_float(hp));
-                e2 = make_float4(dE.x, dE.y, dE.z, __uint_as_float(jn));
-                e3 = make_uint4(origin_v, src, abs_bin, (uint32_t)gp);
-            }
+            edge = (opp == 0 ? hb0 : (opp == 1 ? h.b1 : h.b2)) <= a.ratio;
         }
         if (f_edge) {
             bool ok;
@@ -571,10 +443,11 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
             if (edge) {
                 if (ok) {
                     EdgeRec *dst = static_cast<EdgeRec *>(pa.edges.p) + slot;
-                    dst->r0 = e0;
-                    dst->r1 = e1;
-                    dst->r2 = e2;
-                    dst->r3 = e3;
+                    dst->r0 = make_float4(d.x, d.y, d.z, h.t);
+                    dst->r1 = make_float4(h.b1, h.b2, __uint_as_float(hp), __uint_as_float(t));
+                    dst->r2 = make_float4(__uint_as_float(prev_tri), __uint_as_float((uint32_t)prev_max),
+                                          __uint_as_float(src), __uint_as_float(abs_bin));
+                    dst->r3 = make_uint4(0u, 0u, 0u, (uint32_t)gp);
                 } else {
                     atomicMin(pa.first_dropped, (unsigned long long)gp);
                 }
@@ -609,16 +482,92 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
     chunk_close(e_chunk, e_fill, pa.edges);
 }
 
+// Edge records, raw form (in) -> ray form (out, the same slot; in == out
+// converts in place; empty slots of partly filled chunks left alone): the
+// occlusion ray past the edge and the edge's irradiance derivative
+// (thermal_tracing_unified.glsl:454-494), from the recorded hit.  A record
+// of a photon without an absorbing bin gets photon index ~0 (skipped by
+// k_adj_occl) unless IRTT_ADJ_REFERENCE_DROPPED counts it with bin 0.
+__global__ __launch_bounds__(256) void k_edge_prep(AdjCommon a, const EdgeRec *in, EdgeRec *out,
+                                                   const uint32_t *__restrict__ fills,
+                                                   uint32_t n_chunks)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (uint64_t)n_chunks * kChunk) return;
+    if ((uint32_t)(i % kChunk) >= fills[i / kChunk]) return;
+    const bool f_proj = a.aflags & IRTT_ADJ_EDGE_PROJECTED;
+    const bool dropped_too = a.aflags & IRTT_ADJ_REFERENCE_DROPPED;
+    const EdgeRec raw = in[i];
+    const V3 d{raw.r0.x, raw.r0.y, raw.r0.z};
+    const float ht = raw.r0.w, b1 = raw.r1.x, b2 = raw.r1.y;
+    const uint32_t hp = __float_as_uint(raw.r1.z), t = __float_as_uint(raw.r1.w);
+    const uint32_t prev_tri = __float_as_uint(raw.r2.x), prev_max = __float_as_uint(raw.r2.y);
+    const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
+    const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
+    const V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
+    const float hb0 = 1.0f - b1 - b2;
+    const int opp = min_corner(hb0, b1, b2);
+    const int nr = max_corner(hb0, b1, b2);
+    int other = 0;
+    if (opp == 0 || nr == 0) {
+        other = 1;
+        if (opp == 1 || nr == 1) other = 2;
+    }
+    const V3 qn = nr == 0 ? q0 : (nr == 1 ? q1 : q2);
+    const V3 qo = other == 0 ? q0 : (other == 1 ? q1 : q2);
+    const V3 qp = opp == 0 ? q0 : (opp == 1 ? q1 : q2);
+    const uint32_t jn = nr == 0 ? j0 : (nr == 1 ? j1 : j2);
+    V3 edir = qn - qo;
+    const V3 oppe = qn - qp;
+    const float elen = sqrt_rn(dot3(edir, edir));
+    edir = scl3(edir, 1.0f / elen);
+    V3 n_edg = normalize3(cross3(edir, hn));
+    float th = dot3(oppe, n_edg);
+    if (th < 0.0f) {
+        th = -th;
+        n_edg = scl3(n_edg, -1.0f);
+    }
+    const float larea = a.ratio * th * (1.0f - 0.5f * a.ratio) * elen;
+    V3 da = normalize3(cross3(edir, d));
+    if (dot3(da, n_edg) < 0.0f) da = scl3(da, -1.0f);
+    // the reference scales by |da.n_edg| (:486); the extension divides by it:
+    // an edge moved by u sweeps (u.da)/|da.n_edg| of the occluder's plane per
+    // unit length, seen along the ray
+    const float cpl = fabsf(dot3(da, n_edg));
+    V3 dE = f_proj ? scl3(da, elen * a.w[t] / (larea * cpl)) : scl3(da, cpl * elen * a.w[t] / larea);
+    if (!finite3(dE)) dE = V3{0.0f, 0.0f, 0.0f};
+    V3 to = bary_point(q0, q1, q2, hb0, b1, b2);
+    to = add3(to, scl3(n_edg, a.eps + dot3(qn - to, n_edg)));
+    to = to - scl3(d, a.eps);
+    EdgeRec r;
+    r.r0 = make_float4(to.x, to.y, to.z, ht);
+    r.r1 = make_float4(d.x, d.y, d.z, __uint_as_float(hp));
+    r.r2 = make_float4(dE.x, dE.y, dE.z, __uint_as_float(jn));
+    uint32_t ab = __float_as_uint(raw.r2.w), gp = raw.r3.w;
+    if (ab == 0xffffffffu) {
+        ab = 0u;
+        if (!dropped_too) gp = 0xffffffffu;
+    }
+    r.r3 = make_uint4(a.idx[3 * prev_tri + prev_max], __float_as_uint(raw.r2.z), ab, gp);
+    out[i] = r;
+}
+
 // sort keys of the segment slots: the start (side 0) or hit (side 1)
-// triangle, nt for empty slots
+// triangle, nt for empty slots and, with skip_unabsorbed, for records of
+// photons without an absorbing bin (kept paths hold every photon's)
 __global__ void k_seg_keys(const SegRec *__restrict__ segs, const uint32_t *__restrict__ fills,
-                           uint32_t n_chunks, uint32_t nt, uint32_t side,
+                           uint32_t n_chunks, uint32_t nt, uint32_t side, uint32_t skip_unabsorbed,
                            uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)n_chunks * kChunk) return;
     const uint32_t c = (uint32_t)(i / kChunk), k = (uint32_t)(i % kChunk);
-    keys[i] = k < fills[c] ? (side ? segs[i].r0.y : segs[i].r0.x) : nt;
+    uint32_t key = nt;
+    if (k < fills[c]) {
+        const uint4 r0 = segs[i].r0;
+        if (!(skip_unabsorbed && r0.w == 0xffffffffu)) key = side ? r0.y : r0.x;
+    }
+    keys[i] = key;
     vals[i] = (uint32_t)i;
 }
 
@@ -662,7 +611,9 @@ __device__ __forceinline__ void seg_terms(const AdjCommon &a, const SegRec &r, b
                                           bool f_hit, Put put, uint32_t &n_cont)
 {
     const uint32_t pt = r.r0.x, hp = r.r0.y, t = r.r2.y;
-    const double fpath = hist_factor(a, r.r0.w, r.r0.z);
+    // a photon without an absorbing bin counts with bin 0 (its records are
+    // taken only under IRTT_ADJ_REFERENCE_DROPPED)
+    const double fpath = hist_factor(a, r.r0.w == 0xffffffffu ? 0u : r.r0.w, r.r0.z);
     const uint32_t i0 = a.idx[3 * pt], i1 = a.idx[3 * pt + 1], i2 = a.idx[3 * pt + 2];
     const uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
     const V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
@@ -1152,167 +1103,222 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     IRTT_HIP(hipGetLastError());
     IRTT_HIP(hipEventRecord(side2.area_done, stm2));
 
+    const bool want_edges = c.aflags & IRTT_ADJ_EDGE;
+    const bool dropped_too = c.aflags & IRTT_ADJ_REFERENCE_DROPPED;
+    uint32_t key_bits = 1;
+    while (key_bits < 32 && (1ull << key_bits) <= s.nt) key_bits++;
+    DBuf<uint32_t> &d_fills = ws.fills, &d_keys = ws.keys, &d_keys2 = ws.keys2,
+                   &d_order = ws.order, &d_order2 = ws.order2;
+    DBuf<unsigned long long> &d_ctl = ws.ctl;  // chunks taken (segments, edges), first dropped
+    DBuf<uint8_t> &d_sort_tmp = ws.sort_tmp;
+    size_t sort_tmp_b = 0;
+    ws.n_big.alloc(1);
+    static_assert(sizeof(SegRec) == 3 * sizeof(uint4) && sizeof(EdgeRec) == 4 * sizeof(uint4),
+                  "record layouts");
+
+    // The terms of one set of records: sc segment chunks and ec edge chunks
+    // (raw edge records in edges_in, their ray form into edges_out), records
+    // of photons at or above g_limit skipped.  The occlusion rays run on the
+    // side stream beside the sorted segment terms.
+    auto terms = [&](const SegRec *segs_p, const uint32_t *seg_fills, uint32_t sc,
+                     const EdgeRec *edges_in, EdgeRec *edges_out, const uint32_t *edge_fills,
+                     uint32_t ec, uint64_t g_limit, bool skip_unabsorbed) {
+        if (ec > 0) {
+            OcclArgs oa{};
+            oa.c = c;
+            oa.edges = edges_out;
+            oa.fills = edge_fills;
+            oa.n_chunks = ec;
+            oa.g_limit = g_limit;
+            const uint32_t og = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
+            // after the records are complete, beside the segment terms
+            // enqueued next on the scene's stream (measured: beside the start
+            // side's terms instead, both slow to 4.1 and 4.9 ms and the trace
+            // takes 0.3 ms longer)
+            IRTT_HIP(hipEventRecord(side2.fork, stm));
+            IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
+            k_edge_prep<<<cdiv((uint64_t)ec * kChunk, 256), 256, 0, stm2>>>(c, edges_in, edges_out,
+                                                                            edge_fills, ec);
+            IRTT_HIP(hipGetLastError());
+            if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm2>>>(oa);
+            else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm2>>>(oa);
+            else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm2>>>(oa);
+            else k_adj_occl<128><<<og, kTraceBlock, 0, stm2>>>(oa);
+            IRTT_HIP(hipGetLastError());
+            IRTT_HIP(hipEventRecord(side2.occl_done, stm2));
+            occl_pending = true;
+        }
+        if (sc == 0) return;
+        // each side over the records sorted by its triangle
+        const uint64_t n = (uint64_t)sc * kChunk;
+        for (uint32_t side = 0; side < 2; side++) {
+            k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(segs_p, seg_fills, sc, s.nt, side,
+                                                      skip_unabsorbed ? 1u : 0u, d_keys.p,
+                                                      d_order.p);
+            IRTT_HIP(hipGetLastError());
+            rocprim::double_buffer<uint32_t> kd(d_keys.p, d_keys2.p), vd(d_order.p, d_order2.p);
+            size_t tb = sort_tmp_b;
+            IRTT_HIP(rocprim::radix_sort_pairs(d_sort_tmp.p, tb, kd, vd, (size_t)n, 0, key_bits,
+                                               stm));
+            SegArgs sa{};
+            sa.c = c;
+            sa.segs = segs_p;
+            sa.keys = kd.current();
+            sa.order = vd.current();
+            sa.n = n;
+            sa.side = side;
+            sa.g_limit = g_limit;
+            // the sort's other key buffer lists the records too large for
+            // the window sums
+            sa.big = kd.current() == d_keys.p ? d_keys2.p : d_keys.p;
+            sa.n_big = ws.n_big.p;
+            IRTT_HIP(hipMemsetAsync(ws.n_big.p, 0, sizeof(uint32_t), stm));
+            const uint32_t sg = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(cdiv(n / 64, 4), (uint64_t)ncu * 16));
+            if (side == 0) {
+                k_adj_segments<0><<<sg, 256, 0, stm>>>(sa);
+                k_adj_seg_big<0><<<ncu, 256, 0, stm>>>(sa);
+            } else {
+                k_adj_segments<1><<<sg, 256, 0, stm>>>(sa);
+                k_adj_seg_big<1><<<ncu, 256, 0, stm>>>(sa);
+            }
+            IRTT_HIP(hipGetLastError());
+        }
+    };
+    // sort keys, orders and the sort's work space for n_slots segment slots
+    auto alloc_sort = [&](uint64_t n_slots) {
+        d_keys.alloc(n_slots);
+        d_keys2.alloc(n_slots);
+        d_order.alloc(n_slots);
+        d_order2.alloc(n_slots);
+        sort_tmp_b = 0;
+        rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
+        IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb, (size_t)n_slots, 0,
+                                           key_bits));
+        d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
+    };
+
+    // Paths the build kept (IRTT_TRANSPORT_KEEP_PATHS): their records stand
+    // for the replay of every photon below the first one with a record that
+    // did not fit; the photons from there on are replayed.
+    uint64_t g_first = 0;
+    PathStore &ps = s.paths;
+    const bool kept = ps.valid && ps.photons == photons && absorber == s.d_absorber.p &&
+                      c.ratio == kKeptEdgeRatio;
+    st.kept_paths = 0;
+    if (kept) {
+        const uint32_t sc = (uint32_t)ps.sc, ec = want_edges ? (uint32_t)ps.ec : 0u;
+        alloc_sort(std::max<uint64_t>((uint64_t)sc * kChunk, 1));
+        if (ec > 0) ws.edges.alloc((size_t)ec * kChunk * 4);
+        g_first = std::min<uint64_t>(ps.first_dropped, photons);
+        const EdgeRec *kept_edges = reinterpret_cast<const EdgeRec *>(ps.edges.p);
+        terms(reinterpret_cast<const SegRec *>(ps.segs.p), ps.fills.p, sc, kept_edges,
+              reinterpret_cast<EdgeRec *>(ws.edges.p), ps.fills.p + ps.seg_chunks, ec, g_first,
+              !dropped_too);
+        st.kept_paths = g_first < photons ? 2u : 1u;
+        if (g_first < photons) {
+            // the replay below reallocates the buffers these stages use
+            IRTT_HIP(hipStreamSynchronize(stm));
+            IRTT_HIP(hipStreamSynchronize(stm2));
+            occl_pending = false;
+        }
+    }
+
     // Path replay into segment and edge records, then their terms.  The
     // records of all photons fit when the device has room (the forward
     // build's accepted segments bound them; edge records are taken as at most
     // half of those); otherwise, and whenever a buffer overflows, the photons
     // are replayed in ranges, each range's terms computed up to its first
     // photon with a dropped record.
-    const bool want_edges = c.aflags & IRTT_ADJ_EDGE;
-    // replay grid: at most IRTT_PHOTON_WAVES_PER_CU_ADJ waves per CU, and at
-    // least 16 photons per lane (each wave's last chunk is partly empty)
-    const uint64_t waves = std::max<uint64_t>(
-        1, std::min<uint64_t>((uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU_ADJ,
-                              (photons + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
-    const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
-    // the record buffers this scene already holds count as available; when
-    // the device cannot give them (memory held elsewhere, or not in pieces
-    // this large), the budget halves and the photons run in more ranges
-    double budget =
-        std::max(2.0 * (1u << 30), 0.3 * ((double)dev_available() + (double)ws.record_bytes()));
-    static_assert(sizeof(SegRec) == 3 * sizeof(uint4) && sizeof(EdgeRec) == 4 * sizeof(uint4),
-                  "record layouts");
-    DBuf<uint32_t> &d_fills = ws.fills, &d_keys = ws.keys, &d_keys2 = ws.keys2,
-                   &d_order = ws.order, &d_order2 = ws.order2;
-    DBuf<unsigned long long> &d_ctl = ws.ctl;  // chunks taken (segments, edges), first dropped
-    DBuf<uint8_t> &d_sort_tmp = ws.sort_tmp;
-    uint32_t key_bits = 1;
-    while (key_bits < 32 && (1ull << key_bits) <= s.nt) key_bits++;
-    uint64_t seg_chunks = 0, edge_chunks = 0, n_ranges = 1;
-    size_t sort_tmp_b = 0;
-    for (;;) {
-        seg_chunks = (segments + kChunk - 1) / kChunk + waves + 1;
-        edge_chunks = want_edges ? (segments / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
-        const double need = (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
-        n_ranges = 1;
-        if (need > budget) {
-            const double f = budget / need;
-            seg_chunks = std::max<uint64_t>(2 * waves + 1, (uint64_t)(seg_chunks * f));
-            edge_chunks = want_edges ? std::max<uint64_t>(2 * waves + 1, (uint64_t)(edge_chunks * f))
-                                     : 1;
-            n_ranges = (uint64_t)std::ceil(1.0 / f);
-        }
-        try {
-            ws.segs.alloc(seg_chunks * kChunk * 3);
-            ws.edges.alloc(edge_chunks * kChunk * 4);
-            d_fills.alloc(seg_chunks + edge_chunks);
-            d_keys.alloc(seg_chunks * kChunk);
-            d_keys2.alloc(seg_chunks * kChunk);
-            d_order.alloc(seg_chunks * kChunk);
-            d_order2.alloc(seg_chunks * kChunk);
-            sort_tmp_b = 0;
-            rocprim::double_buffer<uint32_t> kb(nullptr, nullptr), vb(nullptr, nullptr);
-            IRTT_HIP(rocprim::radix_sort_pairs(nullptr, sort_tmp_b, kb, vb,
-                                               (size_t)(seg_chunks * kChunk), 0, key_bits));
-            d_sort_tmp.alloc(std::max<size_t>(sort_tmp_b, 1));
-            break;
-        } catch (const Error &e) {
-            if (e.code != IRTT_ERR_NOMEM || budget <= 2.0 * (1u << 30)) throw;
-            for (auto *b : {&d_fills, &d_keys, &d_keys2, &d_order, &d_order2}) b->release();
-            ws.segs.release();
-            ws.edges.release();
-            d_sort_tmp.release();
-            budget = std::max(2.0 * (1u << 30), 0.5 * budget);
-        }
-    }
-    d_ctl.alloc(3);
-    ws.n_big.alloc(1);
-    SegRec *const segs_p = reinterpret_cast<SegRec *>(ws.segs.p);
-    EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
-    const uint64_t range = (photons + n_ranges - 1) / n_ranges;
-    const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
-    for (uint64_t gb = 0; gb < photons; gb += range) {
-        const uint64_t ge = std::min<uint64_t>(photons, gb + range);
-        uint64_t g_start = gb;
+    if (g_first < photons) {
+        // replay grid: at most IRTT_PHOTON_WAVES_PER_CU_ADJ waves per CU, and
+        // at least 16 photons per lane (each wave's last chunk is partly empty)
+        const uint64_t left = photons - g_first;
+        const uint64_t waves = std::max<uint64_t>(
+            1, std::min<uint64_t>((uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU_ADJ,
+                                  (left + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
+        const uint64_t seg_bytes = sizeof(SegRec) + 4 * sizeof(uint32_t);  // + sort keys and order
+        // the record buffers this scene already holds count as available;
+        // when the device cannot give them (memory held elsewhere, or not in
+        // pieces this large), the budget halves and the photons run in more
+        // ranges
+        double budget =
+            std::max(2.0 * (1u << 30), 0.3 * ((double)dev_available() + (double)ws.record_bytes()));
+        const uint64_t seg_est =
+            (uint64_t)((double)segments * ((double)left / (double)photons)) + 1;
+        uint64_t seg_chunks = 0, edge_chunks = 0, n_ranges = 1;
         for (;;) {
-            PathArgs pa{};
-            pa.c = c;
-            pa.g_begin = g_start;
-            pa.g_end = ge;
-            pa.segs = RecBuf{segs_p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
-            pa.edges = RecBuf{edges_p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
-            pa.first_dropped = d_ctl.p + 2;
-            // the previous range's occlusion rays end before this range's
-            // replay rewrites the records
-            if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));
-            IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
-            const uint32_t grid = (uint32_t)std::max<uint64_t>(
-                1, std::min<uint64_t>(cdiv(ge - g_start, kTraceBlock), waves));
-            if (S == 24) k_adj_paths<24><<<grid, kTraceBlock, 0, stm>>>(pa);
-            else if (S == 32) k_adj_paths<32><<<grid, kTraceBlock, 0, stm>>>(pa);
-            else if (S == 64) k_adj_paths<64><<<grid, kTraceBlock, 0, stm>>>(pa);
-            else k_adj_paths<128><<<grid, kTraceBlock, 0, stm>>>(pa);
-            IRTT_HIP(hipGetLastError());
-            unsigned long long ctl[3];
-            IRTT_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stm));
-            IRTT_HIP(hipStreamSynchronize(stm));
-            const uint32_t sc = (uint32_t)std::min<uint64_t>(ctl[0], seg_chunks);
-            const uint32_t ec = want_edges ? (uint32_t)std::min<uint64_t>(ctl[1], edge_chunks) : 0;
-            if (ec > 0) {
-                OcclArgs oa{};
-                oa.c = c;
-                oa.edges = edges_p;
-                oa.fills = d_fills.p + seg_chunks;
-                oa.n_chunks = ec;
-                oa.g_limit = ctl[2];
-                const uint32_t og = (uint32_t)std::max<uint64_t>(
-                    1, std::min<uint64_t>((uint64_t)ec * (kChunk / 64), (uint64_t)ncu * 64));
-                // after the replay (complete: its counters were read above),
-                // beside the segment terms enqueued next on the scene's stream
-                // (measured: beside the start side's terms instead, both slow
-                // to 4.1 and 4.9 ms and the trace takes 0.3 ms longer)
-                IRTT_HIP(hipEventRecord(side2.fork, stm));
-                IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
-                if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm2>>>(oa);
-                else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm2>>>(oa);
-                else if (S == 64) k_adj_occl<64><<<og, kTraceBlock, 0, stm2>>>(oa);
-                else k_adj_occl<128><<<og, kTraceBlock, 0, stm2>>>(oa);
+            seg_chunks = (seg_est + kChunk - 1) / kChunk + waves + 1;
+            edge_chunks = want_edges ? (seg_est / 2 + kChunk - 1) / kChunk + waves + 1 : 1;
+            const double need =
+                (double)kChunk * (seg_chunks * seg_bytes + edge_chunks * sizeof(EdgeRec));
+            n_ranges = 1;
+            if (need > budget) {
+                const double f = budget / need;
+                seg_chunks = std::max<uint64_t>(2 * waves + 1, (uint64_t)(seg_chunks * f));
+                edge_chunks = want_edges
+                                  ? std::max<uint64_t>(2 * waves + 1, (uint64_t)(edge_chunks * f))
+                                  : 1;
+                n_ranges = (uint64_t)std::ceil(1.0 / f);
+            }
+            try {
+                ws.segs.alloc(seg_chunks * kChunk * 3);
+                ws.edges.alloc(edge_chunks * kChunk * 4);
+                d_fills.alloc(seg_chunks + edge_chunks);
+                alloc_sort(seg_chunks * kChunk);
+                break;
+            } catch (const Error &e) {
+                if (e.code != IRTT_ERR_NOMEM || budget <= 2.0 * (1u << 30)) throw;
+                for (auto *b : {&d_fills, &d_keys, &d_keys2, &d_order, &d_order2}) b->release();
+                ws.segs.release();
+                ws.edges.release();
+                d_sort_tmp.release();
+                budget = std::max(2.0 * (1u << 30), 0.5 * budget);
+            }
+        }
+        d_ctl.alloc(3);
+        SegRec *const segs_p = reinterpret_cast<SegRec *>(ws.segs.p);
+        EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
+        const uint64_t range = (left + n_ranges - 1) / n_ranges;
+        static const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
+        for (uint64_t gb = g_first; gb < photons; gb += range) {
+            const uint64_t ge = std::min<uint64_t>(photons, gb + range);
+            uint64_t g_start = gb;
+            for (;;) {
+                PathArgs pa{};
+                pa.c = c;
+                pa.g_begin = g_start;
+                pa.g_end = ge;
+                pa.segs = RecBuf{segs_p, (uint32_t)seg_chunks, d_fills.p, d_ctl.p};
+                pa.edges =
+                    RecBuf{edges_p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
+                pa.first_dropped = d_ctl.p + 2;
+                // the previous range's occlusion rays end before this range's
+                // replay rewrites the records
+                if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));
+                IRTT_HIP(hipMemcpyAsync(d_ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
+                const uint32_t grid = (uint32_t)std::max<uint64_t>(
+                    1, std::min<uint64_t>(cdiv(ge - g_start, kTraceBlock), waves));
+                if (S == 24) k_adj_paths<24><<<grid, kTraceBlock, 0, stm>>>(pa);
+                else if (S == 32) k_adj_paths<32><<<grid, kTraceBlock, 0, stm>>>(pa);
+                else if (S == 64) k_adj_paths<64><<<grid, kTraceBlock, 0, stm>>>(pa);
+                else k_adj_paths<128><<<grid, kTraceBlock, 0, stm>>>(pa);
                 IRTT_HIP(hipGetLastError());
-                IRTT_HIP(hipEventRecord(side2.occl_done, stm2));
-                occl_pending = true;
+                unsigned long long ctl[3];
+                IRTT_HIP(hipMemcpyAsync(ctl, d_ctl.p, sizeof(ctl), hipMemcpyDeviceToHost, stm));
+                IRTT_HIP(hipStreamSynchronize(stm));
+                const uint32_t sc = (uint32_t)std::min<uint64_t>(ctl[0], seg_chunks);
+                const uint32_t ec =
+                    want_edges ? (uint32_t)std::min<uint64_t>(ctl[1], edge_chunks) : 0;
+                terms(segs_p, d_fills.p, sc, edges_p, edges_p, d_fills.p + seg_chunks, ec, ctl[2],
+                      false);
+                if (ctl[2] == ~0ull) break;  // every record of the range fitted
+                if (ctl[2] <= g_start)
+                    throw Error(IRTT_ERR_LIMIT, "adjoint trace: the records of one photon exceed "
+                                                "the record buffers");
+                g_start = ctl[2];
             }
-            if (sc > 0) {
-                // each side over the records sorted by its triangle
-                const uint64_t n = (uint64_t)sc * kChunk;
-                for (uint32_t side = 0; side < 2; side++) {
-                    k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(segs_p, d_fills.p, sc, s.nt, side,
-                                                              d_keys.p, d_order.p);
-                    IRTT_HIP(hipGetLastError());
-                    rocprim::double_buffer<uint32_t> kd(d_keys.p, d_keys2.p),
-                        vd(d_order.p, d_order2.p);
-                    size_t tb = sort_tmp_b;
-                    IRTT_HIP(rocprim::radix_sort_pairs(d_sort_tmp.p, tb, kd, vd, (size_t)n, 0,
-                                                       key_bits, stm));
-                    SegArgs sa{};
-                    sa.c = c;
-                    sa.segs = segs_p;
-                    sa.keys = kd.current();
-                    sa.order = vd.current();
-                    sa.n = n;
-                    sa.side = side;
-                    sa.g_limit = ctl[2];
-                    // the sort's other key buffer lists the records too large
-                    // for the window sums
-                    sa.big = kd.current() == d_keys.p ? d_keys2.p : d_keys.p;
-                    sa.n_big = ws.n_big.p;
-                    IRTT_HIP(hipMemsetAsync(ws.n_big.p, 0, sizeof(uint32_t), stm));
-                    const uint32_t sg = (uint32_t)std::max<uint64_t>(
-                        1, std::min<uint64_t>(cdiv(n / 64, 4), (uint64_t)ncu * 16));
-                    if (side == 0) {
-                        k_adj_segments<0><<<sg, 256, 0, stm>>>(sa);
-                        k_adj_seg_big<0><<<ncu, 256, 0, stm>>>(sa);
-                    } else {
-                        k_adj_segments<1><<<sg, 256, 0, stm>>>(sa);
-                        k_adj_seg_big<1><<<ncu, 256, 0, stm>>>(sa);
-                    }
-                    IRTT_HIP(hipGetLastError());
-                }
-            }
-            if (ctl[2] == ~0ull) break;  // every record of the range fitted
-            if (ctl[2] <= g_start)
-                throw Error(IRTT_ERR_LIMIT, "adjoint trace: the records of one photon exceed the "
-                                            "record buffers");
-            g_start = ctl[2];
         }
     }
     IRTT_HIP(hipStreamWaitEvent(stm, side2.area_done, 0));

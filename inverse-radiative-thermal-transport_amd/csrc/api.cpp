@@ -588,7 +588,7 @@ uint64_t Scene::device_bytes() const
            aca.d_vh.bytes() + d_stats.bytes() + d_vec.bytes() + d_red.bytes() + d_lap_rowptr.bytes() +
            d_lap_col.bytes() + d_lap_val.bytes() + sp.rowptr.bytes() + sp.col.bytes() +
            sp.val.bytes() + sp.colptr.bytes() + sp.row.bytes() + sp.cval.bytes() +
-           sp.d_xt.bytes() + d_absorber.bytes();
+           sp.d_xt.bytes() + d_absorber.bytes() + paths.bytes();
 }
 
 extern "C" {

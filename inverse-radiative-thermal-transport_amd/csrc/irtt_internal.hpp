@@ -209,6 +209,31 @@ struct AdjWork {
     }
 };
 
+// Path records kept by the last transport build (IRTT_TRANSPORT_KEEP_PATHS):
+// segment records (3 uint4 words each) and raw edge records (4 words) of
+// every photon in wave-owned chunks, for the adjoint trace instead of its
+// path replay.  Valid from the build's end until the next build.
+struct PathStore {
+    bool valid = false;
+    DBuf<uint4> segs, edges;
+    DBuf<uint32_t> fills;             // per chunk: seg_chunks segment chunks, then edge chunks
+    DBuf<unsigned long long> ctl;     // chunks taken (segments, edges), first dropped photon
+    uint64_t seg_chunks = 0, edge_chunks = 0;  // capacity
+    uint64_t sc = 0, ec = 0;          // chunks written
+    uint64_t first_dropped = ~0ull;   // smallest photon with a record that did not fit
+    uint64_t photons = 0;
+    void release()
+    {
+        valid = false;
+        segs.release();
+        edges.release();
+        fills.release();
+        ctl.release();
+        seg_chunks = edge_chunks = sc = ec = 0;
+    }
+    size_t bytes() const { return segs.bytes() + edges.bytes() + fills.bytes() + ctl.bytes(); }
+};
+
 constexpr uint32_t kAcaMaxRank = 256;
 
 // ACA-compressed operator U V + diag(C) (aca.hip)
@@ -396,6 +421,7 @@ struct Scene {
     DBuf<uint32_t> d_absorber;        // absorbing bin of every photon (~0: none), for the adjoint
     uint64_t fwd_rays = 0;            // ray segments of the last transport build
     uint64_t fwd_segments = 0;        // ... of them accepted (not a miss, too close or backface)
+    PathStore paths;                  // path records of the last build, if kept
 
     // operator forms (operator.hip, aca.hip)
     uint32_t op_kind = IRTT_OP_DENSE;
@@ -508,6 +534,9 @@ void hm_get_factors(Scene &s, uint32_t b, double *U, double *V);
 // (absorber: one word per photon, ~0 where none); returns the ray segments
 // traced and of them the accepted ones
 void absorber_pass(Scene &s, uint32_t *absorber, uint64_t &rays, uint64_t &segments);
+// device bytes of the path records a first build with IRTT_TRANSPORT_KEEP_PATHS
+// sets aside for `photons` photons (before its 30 %-of-free cap)
+uint64_t kept_path_bytes(uint64_t photons);
 // adjoint.hip
 uint64_t adjoint_record_bytes(uint64_t photons, uint64_t segments);
 void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, const double *lambda,

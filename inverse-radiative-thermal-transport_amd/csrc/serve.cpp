@@ -94,10 +94,10 @@ uint64_t footprint(const irtt_request &q, uint32_t nb)
     if (q.operator_kind == IRTT_OP_DENSE_F16) b += (uint64_t)nb * ld * 2 + nb * 4;
     b += ld * 8 * 40 + ((nb + 511) / 512) * ld * 8;
     b += (uint64_t)q.nt * 256 + (uint64_t)q.nv * 64 + (64ull << 20);
-    if (q.target) {  // the adjoint trace's absorber record and path records
+    if (q.target) {  // the adjoint trace's absorber record, kept paths and record buffers
         const uint64_t photons = (uint64_t)q.transport.threads_per_tri *
                                  q.transport.rays_per_thread * q.nt;
-        b += adjoint_record_bytes(photons, photons);
+        b += adjoint_record_bytes(photons, photons) + kept_path_bytes(photons);
     }
     return b;
 }
@@ -252,10 +252,10 @@ int Server::run(Req &q, const std::function<void()> &publish)
     const uint32_t nb = q.nb;
     if (!q.source.empty() && (rc = irtt_scene_set_source(h, q.source.data()))) return done(rc);
     irtt_transport_stats ts{};
-    // an adjoint trace follows when there is an objective: keep each
-    // photon's absorbing bin so it replays every photon once
+    // an adjoint trace follows when there is an objective: keep the photons'
+    // paths (and absorbing bins) so it needs no replay
     irtt_transport_params tp = r.transport;
-    if (!q.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_ABSORBERS;
+    if (!q.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_PATHS;
     if ((rc = irtt_build_transport(h, &tp, &ts))) return done(rc);
     if (r.operator_kind != IRTT_OP_DENSE && (rc = irtt_select_operator(h, r.operator_kind)))
         return done(rc);
@@ -303,7 +303,7 @@ int Server::run_group(const std::vector<Req *> &g, const std::function<void()> &
         Scene &s = *reinterpret_cast<Scene *>(h);
         t_alloc_stream = s.stream;
         irtt_transport_params tp = r0.transport;
-        if (!q0.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_ABSORBERS;
+        if (!q0.target.empty()) tp.flags |= IRTT_TRANSPORT_KEEP_PATHS;
         irtt_transport_stats ts{};
         if ((rc = irtt_build_transport(h, &tp, &ts))) return done(rc);
         if (r0.operator_kind != IRTT_OP_DENSE && (rc = irtt_select_operator(h, r0.operator_kind)))

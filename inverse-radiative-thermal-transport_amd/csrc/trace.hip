@@ -21,6 +21,7 @@
 
 #include "device_math.hpp"
 #include "irtt_internal.hpp"
+#include "path_records.hpp"
 #include "sparse_table.hpp"
 #include "traverse.hpp"
 
@@ -86,7 +87,8 @@ constexpr int photon_waves(int stack)
 // seed at :291): origin, direction, RNG state after it, emitting bin and
 // triangle.  False for a degenerate (zero-area) emitter.
 __device__ __forceinline__ bool emit_photon(const PhotonArgs &a, uint64_t g, uint32_t &seed, V3 &o,
-                                            V3 &d, uint32_t &src, uint32_t &tri)
+                                            V3 &d, uint32_t &src, uint32_t &tri, float &eb1,
+                                            float &eb2, int &ecorner)
 {
     uint32_t t = (uint32_t)(g / a.per_tri);
     uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
@@ -112,6 +114,9 @@ __device__ __forceinline__ bool emit_photon(const PhotonArgs &a, uint64_t g, uin
     int sc = max_corner(b0, b1, b2);
     src = a.bin_mode ? t : (sc == 0 ? i0 : (sc == 1 ? i1 : i2));
     tri = t;
+    eb1 = b1;
+    eb2 = b2;
+    ecorner = sc;
     return true;
 }
 
@@ -119,18 +124,24 @@ __device__ __forceinline__ bool emit_photon(const PhotonArgs &a, uint64_t g, uin
 // the photon and returns the counter slot of how (S_MISS, S_CLOSE, S_BACK,
 // S_ABS, or S_FORCED at max_depth; absorptions add the photon's weight qt to
 // (absorbing bin, src)), or turns o, d, seed into its next ray (getNextRay,
-// thermal_tracing_unified.glsl:163-197) and returns -1.
+// thermal_tracing_unified.glsl:163-197) and returns -1.  With PATHS, seg
+// tells whether the adjoint's path replay takes the segment (the same
+// tests, except that a hit with a non-finite normal ends the replayed path).
+template <bool PATHS>
 __device__ __forceinline__ int shade_segment(const PhotonArgs &a, const Hit &h, uint64_t g,
                                              uint32_t src, unsigned long long qt, uint32_t depth,
-                                             uint32_t &seed, V3 &o, V3 &d)
+                                             uint32_t &seed, V3 &o, V3 &d, bool &seg)
 {
+    if constexpr (PATHS) seg = false;
     if (h.prim < 0) return S_MISS;
     if (h.t < a.tiny_dist) return S_CLOSE;
     uint32_t hp = (uint32_t)h.prim;
     uint32_t j0 = a.idx[3 * hp], j1 = a.idx[3 * hp + 1], j2 = a.idx[3 * hp + 2];
     V3 q0 = load_v(a.xyz, j0), q1 = load_v(a.xyz, j1), q2 = load_v(a.xyz, j2);
     V3 hn = normalize3(cross3(q1 - q0, q2 - q0));
-    if (dot3(hn, d) > -1e-3f) return S_BACK;
+    const float hd = dot3(hn, d);
+    if (hd > -1e-3f) return S_BACK;
+    if constexpr (PATHS) seg = hd <= -1e-3f;
     float hb0 = 1.0f - h.b1 - h.b2;
     V3 hpos = bary_point(q0, q1, q2, hb0, h.b1, h.b2);
     o = offset_ray(hpos, hn);
@@ -227,7 +238,9 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
             // this lane's next photon; degenerate emitters are skipped
             while (g < a.photon_end) {
                 uint32_t tri;
-                if (emit_photon(a, g, seed, o, d, src, tri)) {
+                float eb1, eb2;
+                int ec;
+                if (emit_photon(a, g, seed, o, d, src, tri, eb1, eb2, ec)) {
                     qt = a.q[tri];
                     depth = 0;
                     live = true;
@@ -246,7 +259,8 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
             query_run_ref(q, a.nodes, a.tris, 1e-12f, sn, nvis, ntri);
             h = query_end(q, 1e12f);
         }
-        const int outcome = shade_segment(a, h, g, src, qt, depth, seed, o, d);
+        bool unused;
+        const int outcome = shade_segment<false>(a, h, g, src, qt, depth, seed, o, d, unused);
         count_segment(cnt, outcome);
         q_esc += outcome == S_MISS ? qt : 0ull;
         q_drop += (outcome == S_CLOSE || outcome == S_BACK) ? qt : 0ull;
@@ -256,6 +270,173 @@ __global__ __launch_bounds__(kTraceBlock, photon_waves(STACK)) void k_photons(Ph
         }
     }
     flush_counts(a.stats, cnt, nvis, ntri, q_esc, q_drop);
+}
+
+// Where a transport build keeps its photons' paths (IRTT_TRANSPORT_KEEP_PATHS).
+struct PathRecs {
+    RecBuf segs, edges;
+    unsigned long long *first_dropped;  // smallest photon with a record that did not fit
+};
+
+// waves per SIMD of the path-keeping photon kernel: what its LDS stack
+// admits, at most 5 (its records' state takes about 90 VGPRs)
+constexpr int path_waves(int stack)
+{
+    return (160 * 1024 / (stack * 4 * kTraceBlock)) / 4 < 5
+               ? (160 * 1024 / (stack * 4 * kTraceBlock)) / 4
+               : 5;
+}
+
+// k_photons that also keeps the paths: per accepted segment the segment
+// record, per hit within kKeptEdgeRatio of an edge the raw edge record, as
+// the adjoint trace's path replay writes them for the photons it replays
+// (the absorbing bin is filled in afterwards by k_path_absorbers).  Every
+// lane stays in the loop until its wave is done, since the record slots are
+// taken by the whole wave together.
+template <int STACK>
+__global__ __launch_bounds__(kTraceBlock, path_waves(STACK)) void k_photons_paths(PhotonArgs a,
+                                                                                  PathRecs pr)
+{
+    __shared__ uint32_t stk_n[STACK * kTraceBlock];
+    __shared__ float stk_t[stack_t_entries(STACK) * kTraceBlock];
+    uint32_t *sn = stk_n + threadIdx.x;
+    float *stt = stk_t + threadIdx.x;
+    (void)stt;
+    uint32_t cnt[S_COUNT] = {0, 0, 0, 0, 0, 0, 0};
+    unsigned long long q_esc = 0, q_drop = 0;
+    uint32_t nvis = 0, ntri = 0;
+    uint32_t s_chunk = kNoChunk, s_fill = 0, e_chunk = kNoChunk, e_fill = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
+    uint64_t g = a.photon_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x;
+    bool live = false, rec = false;
+    uint32_t seed = 0, src = 0, depth = 0, t = 0, prev_tri = 0;
+    int prev_max = 0;
+    float pb1 = 0.0f, pb2 = 0.0f;
+    unsigned long long qt = 0;
+    V3 o{0.0f, 0.0f, 0.0f}, d{0.0f, 0.0f, 1.0f};
+    for (;;) {
+        if (!live) {
+            while (g < a.photon_end) {
+                uint32_t tri;
+                float eb1, eb2;
+                int ec;
+                if (emit_photon(a, g, seed, o, d, src, tri, eb1, eb2, ec)) {
+                    qt = a.q[tri];
+                    depth = 0;
+                    live = true;
+                    rec = true;
+                    t = tri;
+                    prev_tri = tri;
+                    pb1 = eb1;
+                    pb2 = eb2;
+                    prev_max = ec;
+                    break;
+                }
+                cnt[S_DEG]++;
+                g += stride;
+            }
+        }
+        if (__ballot(live) == 0) break;
+        Query q{};
+        q.cur = kQueryDone;
+        if (live) {
+            ++depth;
+            if (!query_begin(q, a.tris, a.n_leaves, o, d, 1e-12f, 1e12f, ntri)) q.cur = kQueryDone;
+        }
+        query_run_ref(q, a.nodes, a.tris, 1e-12f, sn, nvis, ntri);
+        Hit h{-1, 0.0f, 0.0f, 0.0f};
+        bool seg = false;
+        const V3 din = d;
+        const uint64_t gp = g;
+        if (live) {
+            h = query_end(q, 1e12f);
+            bool replayed;
+            const int outcome = shade_segment<true>(a, h, g, src, qt, depth, seed, o, d, replayed);
+            count_segment(cnt, outcome);
+            q_esc += outcome == S_MISS ? qt : 0ull;
+            q_drop += (outcome == S_CLOSE || outcome == S_BACK) ? qt : 0ull;
+            seg = rec && replayed;
+            if (!replayed) rec = false;
+            if (outcome >= 0) {
+                live = false;
+                g += stride;
+            }
+        }
+        const uint32_t hp = (uint32_t)h.prim;
+        {
+            bool ok;
+            const uint64_t slot = chunk_slot(seg, s_chunk, s_fill, pr.segs, ok);
+            if (seg) {
+                if (ok) {
+                    SegRec *dst = static_cast<SegRec *>(pr.segs.p) + slot;
+                    dst->r0 = make_uint4(prev_tri, hp, src, 0xffffffffu);
+                    dst->r1 = make_float4(pb1, pb2, h.b1, h.b2);
+                    dst->r2 = make_uint4((uint32_t)gp, t, depth == 1 ? 1u : 0u, 0u);
+                } else {
+                    atomicMin(pr.first_dropped, (unsigned long long)gp);
+                }
+            }
+        }
+        const float hb0 = 1.0f - h.b1 - h.b2;
+        bool edge = false;
+        if (seg) {
+            const int opp = min_corner(hb0, h.b1, h.b2);
+            edge = (opp == 0 ? hb0 : (opp == 1 ? h.b1 : h.b2)) <= kKeptEdgeRatio;
+        }
+        {
+            bool ok;
+            const uint64_t slot = chunk_slot(edge, e_chunk, e_fill, pr.edges, ok);
+            if (edge) {
+                if (ok) {
+                    EdgeRec *dst = static_cast<EdgeRec *>(pr.edges.p) + slot;
+                    dst->r0 = make_float4(din.x, din.y, din.z, h.t);
+                    dst->r1 = make_float4(h.b1, h.b2, __uint_as_float(hp), __uint_as_float(t));
+                    dst->r2 = make_float4(__uint_as_float(prev_tri),
+                                          __uint_as_float((uint32_t)prev_max), __uint_as_float(src),
+                                          __uint_as_float(0xffffffffu));
+                    dst->r3 = make_uint4(0u, 0u, 0u, (uint32_t)gp);
+                } else {
+                    atomicMin(pr.first_dropped, (unsigned long long)gp);
+                }
+            }
+        }
+        if (seg) {
+            prev_tri = hp;
+            pb1 = h.b1;
+            pb2 = h.b2;
+            prev_max = max_corner(hb0, h.b1, h.b2);
+        }
+    }
+    chunk_close(s_chunk, s_fill, pr.segs);
+    chunk_close(e_chunk, e_fill, pr.edges);
+    flush_counts(a.stats, cnt, nvis, ntri, q_esc, q_drop);
+}
+
+// The absorbing bin of each kept record's photon (~0: none) into the record:
+// slots of the seg_chunks segment chunks, then of the edge chunks; chunks not
+// taken (ctl[0], ctl[1]) and empty slots left alone.
+__global__ __launch_bounds__(256) void k_path_absorbers(uint4 *__restrict__ segs,
+                                                        uint4 *__restrict__ edges,
+                                                        const uint32_t *__restrict__ fills,
+                                                        uint64_t seg_chunks, uint64_t edge_chunks,
+                                                        const unsigned long long *__restrict__ ctl,
+                                                        const uint32_t *__restrict__ absorber)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t ns = seg_chunks * kChunk;
+    if (i < ns) {
+        const uint64_t c = i / kChunk;
+        if (c >= ctl[0] || (uint32_t)(i % kChunk) >= fills[c]) return;
+        uint4 *r = segs + 3 * i;
+        r[0].w = absorber[r[2].x];
+    } else {
+        const uint64_t j = i - ns;
+        if (j >= edge_chunks * kChunk) return;
+        const uint64_t c = j / kChunk;
+        if (c >= ctl[1] || (uint32_t)(j % kChunk) >= fills[seg_chunks + c]) return;
+        uint4 *r = edges + 4 * j;
+        r[2].w = absorber[r[3].w];
+    }
 }
 
 // photon kernel grid: at most this many waves per CU, lanes striding over
@@ -632,12 +813,22 @@ static uint32_t ceil_log2_u64(uint64_t x)
     return l;
 }
 
+// Waves of the path-keeping photon kernel: at most 128 per CU, at least 16
+// photons per lane.
+static uint64_t path_waves_total(int ncu, uint64_t total)
+{
+    return std::max<uint64_t>(
+        1, std::min<uint64_t>((uint64_t)ncu * 128, (total + 16 * kTraceBlock - 1) / (16 * kTraceBlock)));
+}
+
 // Launches the photon kernel over every photon of the scene on the scene's
 // stream (timed: between events ev0 and ev1): bins into sink (none: absorbers
-// only), records absorbing bins into absorber (or not), counts into stats.
+// only), records absorbing bins into absorber (or not), counts into stats,
+// writes path records into pr (or not).
 static void launch_photons(Scene &s, const irtt_transport_params &p, uint32_t per,
                            uint32_t max_depth, uint64_t total, const PhotonSink &sink,
-                           uint32_t *absorber, unsigned long long *stats, bool timed)
+                           uint32_t *absorber, unsigned long long *stats, bool timed,
+                           const PathRecs *pr = nullptr)
 {
     hipStream_t stm = s.stream;
     PhotonArgs a{};
@@ -681,12 +872,28 @@ static void launch_photons(Scene &s, const irtt_transport_params &p, uint32_t pe
     uint32_t grid = (uint32_t)std::min<uint64_t>(want, (uint64_t)ncu * IRTT_PHOTON_WAVES_PER_CU);
     if (grid == 0) grid = 1;
     if (timed) IRTT_HIP(hipEventRecord(s.ev0, stm));
-    if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
+    if (pr) {
+        // the path replay's grid (each wave's last record chunks are partly
+        // empty: fewer, longer waves)
+        grid = (uint32_t)path_waves_total(ncu, total);
+        if (S == 24) k_photons_paths<24><<<grid, kTraceBlock, 0, stm>>>(a, *pr);
+        else if (S == 32) k_photons_paths<32><<<grid, kTraceBlock, 0, stm>>>(a, *pr);
+        else if (S == 64) k_photons_paths<64><<<grid, kTraceBlock, 0, stm>>>(a, *pr);
+        else k_photons_paths<128><<<grid, kTraceBlock, 0, stm>>>(a, *pr);
+    } else if (S == 24) k_photons<24><<<grid, kTraceBlock, 0, stm>>>(a);
     else if (S == 32) k_photons<32><<<grid, kTraceBlock, 0, stm>>>(a);
     else if (S == 64) k_photons<64><<<grid, kTraceBlock, 0, stm>>>(a);
     else k_photons<128><<<grid, kTraceBlock, 0, stm>>>(a);
     IRTT_HIP(hipGetLastError());
     if (timed) IRTT_HIP(hipEventRecord(s.ev1, stm));
+}
+
+uint64_t kept_path_bytes(uint64_t photons)
+{
+    const uint64_t waves = path_waves_total(256, photons);
+    const uint64_t sch = (photons + kChunk - 1) / kChunk + waves + 1;
+    const uint64_t ech = (photons / 2 + kChunk - 1) / kChunk + waves + 1;
+    return sch * kChunk * sizeof(SegRec) + ech * kChunk * sizeof(EdgeRec) + (sch + ech) * 4 + 24;
 }
 
 // A forward pass that records each photon's absorbing bin and bins nothing:
@@ -838,7 +1045,10 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     // absorbing bin of every photon (4 B each) so the adjoint trace replays
     // each photon once instead of twice; not kept above 2^31 photons or
     // when HBM is short (the adjoint then finds absorbers with a pass of its own)
-    if (!(p.flags & IRTT_TRANSPORT_KEEP_ABSORBERS) || total > (1ull << 31)) {
+    PathStore &ps = s.paths;
+    ps.valid = false;
+    if (!(p.flags & (IRTT_TRANSPORT_KEEP_ABSORBERS | IRTT_TRANSPORT_KEEP_PATHS)) ||
+        total > (1ull << 31)) {
         s.d_absorber.release();
     } else {
         try {
@@ -850,7 +1060,65 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
         }
     }
 
-    launch_photons(s, p, per, max_depth, total, sink, s.d_absorber.p, s.d_stats.p, true);
+    // the paths (with the absorbers): record buffers sized for the accepted
+    // segments of the last build of the same photon count (else one per
+    // photon) and half as many edge records, plus each wave's partial chunks,
+    // within 30 % of the free device memory; records that do not fit are
+    // replayed by the adjoint trace
+    PathRecs pr{};
+    const bool keep_paths = (p.flags & IRTT_TRANSPORT_KEEP_PATHS) && s.d_absorber.p;
+    if (keep_paths) {
+        int dev = 0, ncu = 256;
+        IRTT_HIP(hipGetDevice(&dev));
+        IRTT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const uint64_t waves = path_waves_total(ncu, total);
+        const uint64_t seg_est =
+            (ps.photons == total && s.fwd_segments > 0) ? s.fwd_segments + s.fwd_segments / 32
+                                                        : total;
+        double budget = std::max(1.0 * (1u << 30), 0.3 * ((double)dev_available() + (double)ps.bytes()));
+        uint64_t sch = 0, ech = 0;
+        for (;;) {
+            sch = (seg_est + kChunk - 1) / kChunk + waves + 1;
+            ech = (seg_est / 2 + kChunk - 1) / kChunk + waves + 1;
+            const double need = (double)kChunk * (sch * sizeof(SegRec) + ech * sizeof(EdgeRec));
+            if (need > budget) {
+                const double f = budget / need;
+                sch = std::max<uint64_t>(2 * waves + 1, (uint64_t)(sch * f));
+                ech = std::max<uint64_t>(2 * waves + 1, (uint64_t)(ech * f));
+            }
+            try {
+                ps.segs.alloc(sch * kChunk * 3);
+                ps.edges.alloc(ech * kChunk * 4);
+                ps.fills.alloc(sch + ech);
+                ps.ctl.alloc(3);
+                break;
+            } catch (const Error &e) {
+                if (e.code != IRTT_ERR_NOMEM) throw;
+                ps.release();
+                if (budget <= 1.0 * (1u << 30)) break;
+                budget = std::max(1.0 * (1u << 30), 0.5 * budget);
+            }
+        }
+        if (ps.ctl.p) {
+            static const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
+            IRTT_HIP(hipMemcpyAsync(ps.ctl.p, ctl0, sizeof(ctl0), hipMemcpyHostToDevice, stm));
+            ps.seg_chunks = sch;
+            ps.edge_chunks = ech;
+            pr.segs = RecBuf{ps.segs.p, (uint32_t)sch, ps.fills.p, ps.ctl.p};
+            pr.edges = RecBuf{ps.edges.p, (uint32_t)ech, ps.fills.p + sch, ps.ctl.p + 1};
+            pr.first_dropped = ps.ctl.p + 2;
+        }
+    } else {
+        ps.release();
+    }
+    launch_photons(s, p, per, max_depth, total, sink, s.d_absorber.p, s.d_stats.p, true,
+                   pr.first_dropped ? &pr : nullptr);
+    if (pr.first_dropped) {
+        k_path_absorbers<<<cdiv((ps.seg_chunks + ps.edge_chunks) * kChunk, 256), 256, 0, stm>>>(
+            ps.segs.p, ps.edges.p, ps.fills.p, ps.seg_chunks, ps.edge_chunks, ps.ctl.p,
+            s.d_absorber.p);
+        IRTT_HIP(hipGetLastError());
+    }
 
     // finalize timed from ev1 to ev2; both times are read after the last sync
     if (sparse) {
@@ -876,10 +1144,15 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     // stored diagonal, energy figures
     auto *colacc = static_cast<unsigned long long *>(
         s.h_pin.get((size_t)nb * sizeof(unsigned long long) + 16 * sizeof(unsigned long long) +
-                    8 * sizeof(double) + (size_t)nb * sizeof(float)));
+                    8 * sizeof(double) + 4 * sizeof(unsigned long long) +
+                    (size_t)nb * sizeof(float)));
     unsigned long long *hstats = colacc + nb;
     double *henergy = reinterpret_cast<double *>(hstats + 16);
-    float *dcol = reinterpret_cast<float *>(henergy + 8);
+    unsigned long long *hctl = reinterpret_cast<unsigned long long *>(henergy + 8);
+    float *dcol = reinterpret_cast<float *>(hctl + 4);
+    if (pr.first_dropped)
+        IRTT_HIP(hipMemcpyAsync(hctl, ps.ctl.p, 3 * sizeof(unsigned long long),
+                                hipMemcpyDeviceToHost, stm));
     IRTT_HIP(hipMemcpyAsync(henergy, s.d_energy.p, 6 * sizeof(double), hipMemcpyDeviceToHost,
                             stm));
     IRTT_HIP(hipMemcpyAsync(colacc, s.d_colacc.p, nb * sizeof(unsigned long long),
@@ -897,12 +1170,12 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     s.Tdiag.assign(dcol, dcol + nb);
     st.photons = total;
     st.rays = hstats[S_RAYS];
-    s.fwd_rays = st.rays;
-    s.fwd_segments = st.rays - st.miss - st.too_close - st.backface;
     st.absorbed = hstats[S_ABS];
     st.miss = hstats[S_MISS];
     st.too_close = hstats[S_CLOSE];
     st.backface = hstats[S_BACK];
+    s.fwd_rays = st.rays;
+    s.fwd_segments = st.rays - st.miss - st.too_close - st.backface;
     st.degenerate = hstats[S_DEG];
     st.forced_absorb = hstats[S_FORCED];
     st.node_visits = hstats[S_COUNT];
@@ -919,6 +1192,13 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     st.nnz = sparse ? s.sp.nnz : (uint64_t)nb * nb;
     s.layout = p.layout;
     s.op_kind = sparse ? IRTT_OP_SPARSE : IRTT_OP_DENSE;
+    if (pr.first_dropped) {
+        ps.sc = std::min<uint64_t>(hctl[0], ps.seg_chunks);
+        ps.ec = std::min<uint64_t>(hctl[1], ps.edge_chunks);
+        ps.first_dropped = hctl[2];
+        ps.photons = total;
+        ps.valid = true;
+    }
     s.has_transport = true;
     s.acc_marked = !sparse;
     s.t_fill_gen = sparse ? ~0ull : s.assembly_gen;

@@ -216,6 +216,7 @@ HBLOCK_LOWRANK = 3
 LAYOUT_DENSE = 0
 LAYOUT_SPARSE = 1
 TRANSPORT_KEEP_ABSORBERS = 1
+TRANSPORT_KEEP_PATHS = 2
 ACA_FULL = 0
 ACA_PARTIAL = 1
 ACA_PLUS = 2
@@ -273,7 +274,7 @@ ADJ_COLLIMATED = 128
 class AdjointStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("photons", "absorbed", "segments", "continuous", "edge_hits", "occluded",
-                 "rays")] + [("trace_ms", ctypes.c_float), ("reserved", ctypes.c_uint32)]
+                 "rays")] + [("trace_ms", ctypes.c_float), ("kept_paths", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
