@@ -490,7 +490,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
 // k_adj_occl) unless IRTT_ADJ_REFERENCE_DROPPED counts it with bin 0.
 __global__ __launch_bounds__(256) void k_edge_prep(AdjCommon a, const EdgeRec *in, EdgeRec *out,
                                                    const uint32_t *__restrict__ fills,
-                                                   uint32_t n_chunks)
+                                                   uint32_t n_chunks, uint32_t absorber_from_photon)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)n_chunks * kChunk) return;
@@ -543,7 +543,8 @@ __global__ __launch_bounds__(256) void k_edge_prep(AdjCommon a, const EdgeRec *i
     r.r0 = make_float4(to.x, to.y, to.z, ht);
     r.r1 = make_float4(d.x, d.y, d.z, __uint_as_float(hp));
     r.r2 = make_float4(dE.x, dE.y, dE.z, __uint_as_float(jn));
-    uint32_t ab = __float_as_uint(raw.r2.w), gp = raw.r3.w;
+    // kept paths: the absorbing bin from the per-photon record
+    uint32_t gp = raw.r3.w, ab = absorber_from_photon ? a.absorber[gp] : __float_as_uint(raw.r2.w);
     if (ab == 0xffffffffu) {
         ab = 0u;
         if (!dropped_too) gp = 0xffffffffu;
@@ -554,17 +555,24 @@ __global__ __launch_bounds__(256) void k_edge_prep(AdjCommon a, const EdgeRec *i
 
 // sort keys of the segment slots: the start (side 0) or hit (side 1)
 // triangle, nt for empty slots and, with skip_unabsorbed, for records of
-// photons without an absorbing bin (kept paths hold every photon's)
-__global__ void k_seg_keys(const SegRec *__restrict__ segs, const uint32_t *__restrict__ fills,
+// photons without an absorbing bin (kept paths hold every photon's).  With
+// absorber (kept paths, side 0), each record's absorbing bin is first taken
+// from the per-photon record into the segment record.
+__global__ void k_seg_keys(SegRec *__restrict__ segs, const uint32_t *__restrict__ fills,
                            uint32_t n_chunks, uint32_t nt, uint32_t side, uint32_t skip_unabsorbed,
-                           uint32_t *__restrict__ keys, uint32_t *__restrict__ vals)
+                           const uint32_t *__restrict__ absorber, uint32_t *__restrict__ keys,
+                           uint32_t *__restrict__ vals)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (uint64_t)n_chunks * kChunk) return;
     const uint32_t c = (uint32_t)(i / kChunk), k = (uint32_t)(i % kChunk);
     uint32_t key = nt;
     if (k < fills[c]) {
-        const uint4 r0 = segs[i].r0;
+        uint4 r0 = segs[i].r0;
+        if (absorber) {
+            r0.w = absorber[segs[i].r2.x];
+            segs[i].r0.w = r0.w;
+        }
         if (!(skip_unabsorbed && r0.w == 0xffffffffu)) key = side ? r0.y : r0.x;
     }
     keys[i] = key;
@@ -1120,9 +1128,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
     // (raw edge records in edges_in, their ray form into edges_out), records
     // of photons at or above g_limit skipped.  The occlusion rays run on the
     // side stream beside the sorted segment terms.
-    auto terms = [&](const SegRec *segs_p, const uint32_t *seg_fills, uint32_t sc,
+    auto terms = [&](SegRec *segs_p, const uint32_t *seg_fills, uint32_t sc,
                      const EdgeRec *edges_in, EdgeRec *edges_out, const uint32_t *edge_fills,
-                     uint32_t ec, uint64_t g_limit, bool skip_unabsorbed) {
+                     uint32_t ec, uint64_t g_limit, bool kept_records) {
         if (ec > 0) {
             OcclArgs oa{};
             oa.c = c;
@@ -1138,8 +1146,8 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
             // takes 0.3 ms longer)
             IRTT_HIP(hipEventRecord(side2.fork, stm));
             IRTT_HIP(hipStreamWaitEvent(stm2, side2.fork, 0));
-            k_edge_prep<<<cdiv((uint64_t)ec * kChunk, 256), 256, 0, stm2>>>(c, edges_in, edges_out,
-                                                                            edge_fills, ec);
+            k_edge_prep<<<cdiv((uint64_t)ec * kChunk, 256), 256, 0, stm2>>>(
+                c, edges_in, edges_out, edge_fills, ec, kept_records ? 1u : 0u);
             IRTT_HIP(hipGetLastError());
             if (S == 24) k_adj_occl<24><<<og, kTraceBlock, 0, stm2>>>(oa);
             else if (S == 32) k_adj_occl<32><<<og, kTraceBlock, 0, stm2>>>(oa);
@@ -1153,9 +1161,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         // each side over the records sorted by its triangle
         const uint64_t n = (uint64_t)sc * kChunk;
         for (uint32_t side = 0; side < 2; side++) {
-            k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(segs_p, seg_fills, sc, s.nt, side,
-                                                      skip_unabsorbed ? 1u : 0u, d_keys.p,
-                                                      d_order.p);
+            k_seg_keys<<<cdiv(n, 256), 256, 0, stm>>>(
+                segs_p, seg_fills, sc, s.nt, side, kept_records && !dropped_too ? 1u : 0u,
+                kept_records && side == 0 ? absorber : nullptr, d_keys.p, d_order.p);
             IRTT_HIP(hipGetLastError());
             rocprim::double_buffer<uint32_t> kd(d_keys.p, d_keys2.p), vd(d_order.p, d_order2.p);
             size_t tb = sort_tmp_b;
@@ -1213,9 +1221,9 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         if (ec > 0) ws.edges.alloc((size_t)ec * kChunk * 4);
         g_first = std::min<uint64_t>(ps.first_dropped, photons);
         const EdgeRec *kept_edges = reinterpret_cast<const EdgeRec *>(ps.edges.p);
-        terms(reinterpret_cast<const SegRec *>(ps.segs.p), ps.fills.p, sc, kept_edges,
+        terms(reinterpret_cast<SegRec *>(ps.segs.p), ps.fills.p, sc, kept_edges,
               reinterpret_cast<EdgeRec *>(ws.edges.p), ps.fills.p + ps.seg_chunks, ec, g_first,
-              !dropped_too);
+              true);
         st.kept_paths = g_first < photons ? 2u : 1u;
         if (g_first < photons) {
             // the replay below reallocates the buffers these stages use

@@ -290,9 +290,10 @@ constexpr int path_waves(int stack)
 // k_photons that also keeps the paths: per accepted segment the segment
 // record, per hit within kKeptEdgeRatio of an edge the raw edge record, as
 // the adjoint trace's path replay writes them for the photons it replays
-// (the absorbing bin is filled in afterwards by k_path_absorbers).  Every
-// lane stays in the loop until its wave is done, since the record slots are
-// taken by the whole wave together.
+// (the absorbing bin, known only at the photon's end, is taken from the
+// per-photon record by the adjoint trace).  Every lane stays in the loop
+// until its wave is done, since the record slots are taken by the whole wave
+// together.
 template <int STACK>
 __global__ __launch_bounds__(kTraceBlock, path_waves(STACK)) void k_photons_paths(PhotonArgs a,
                                                                                   PathRecs pr)
@@ -410,33 +411,6 @@ __global__ __launch_bounds__(kTraceBlock, path_waves(STACK)) void k_photons_path
     chunk_close(s_chunk, s_fill, pr.segs);
     chunk_close(e_chunk, e_fill, pr.edges);
     flush_counts(a.stats, cnt, nvis, ntri, q_esc, q_drop);
-}
-
-// The absorbing bin of each kept record's photon (~0: none) into the record:
-// slots of the seg_chunks segment chunks, then of the edge chunks; chunks not
-// taken (ctl[0], ctl[1]) and empty slots left alone.
-__global__ __launch_bounds__(256) void k_path_absorbers(uint4 *__restrict__ segs,
-                                                        uint4 *__restrict__ edges,
-                                                        const uint32_t *__restrict__ fills,
-                                                        uint64_t seg_chunks, uint64_t edge_chunks,
-                                                        const unsigned long long *__restrict__ ctl,
-                                                        const uint32_t *__restrict__ absorber)
-{
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t ns = seg_chunks * kChunk;
-    if (i < ns) {
-        const uint64_t c = i / kChunk;
-        if (c >= ctl[0] || (uint32_t)(i % kChunk) >= fills[c]) return;
-        uint4 *r = segs + 3 * i;
-        r[0].w = absorber[r[2].x];
-    } else {
-        const uint64_t j = i - ns;
-        if (j >= edge_chunks * kChunk) return;
-        const uint64_t c = j / kChunk;
-        if (c >= ctl[1] || (uint32_t)(j % kChunk) >= fills[seg_chunks + c]) return;
-        uint4 *r = edges + 4 * j;
-        r[2].w = absorber[r[3].w];
-    }
 }
 
 // photon kernel grid: at most this many waves per CU, lanes striding over
@@ -1113,12 +1087,6 @@ void transport_build(Scene &s, const irtt_transport_params &p, irtt_transport_st
     }
     launch_photons(s, p, per, max_depth, total, sink, s.d_absorber.p, s.d_stats.p, true,
                    pr.first_dropped ? &pr : nullptr);
-    if (pr.first_dropped) {
-        k_path_absorbers<<<cdiv((ps.seg_chunks + ps.edge_chunks) * kChunk, 256), 256, 0, stm>>>(
-            ps.segs.p, ps.edges.p, ps.fills.p, ps.seg_chunks, ps.edge_chunks, ps.ctl.p,
-            s.d_absorber.p);
-        IRTT_HIP(hipGetLastError());
-    }
 
     // finalize timed from ev1 to ev2; both times are read after the last sync
     if (sparse) {
