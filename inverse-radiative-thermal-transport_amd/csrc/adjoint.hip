@@ -16,12 +16,15 @@
 //         non-adjacent occluded surface receives +, the path's origin vertex +,
 //         the edge vertex - of the edge-irradiance derivative.
 //
-// Four stages instead of one kernel, so that no kernel carries another's state:
+// Stages instead of one kernel, so that no kernel carries another's state:
 //   k_adj_paths     replays the paths (the forward kernel's lane scheduling and
 //                   traversal) and writes one 48-byte segment record per
 //                   accepted segment and one 64-byte edge record per hit near
-//                   an edge (occlusion ray, edge-irradiance derivative,
-//                   vertices), in wave-owned chunks;
+//                   an edge (the hit as found), in wave-owned chunks -- or,
+//                   when the build kept its paths (IRTT_TRANSPORT_KEEP_PATHS),
+//                   the build's photon kernel has written them already;
+//   k_edge_prep     turns each edge record into its occlusion ray and
+//                   edge-irradiance derivative;
 //   k_adj_segments  runs the segment terms (ii) and the hit-area extension
 //                   over the records radix-sorted by start triangle and by
 //                   hit triangle: a window of 64 records sums the terms of

@@ -43,7 +43,7 @@ struct EdgeRec {
 // Segment record: one accepted path segment of a photon, its terms still to
 // compute.
 //   r0: start triangle (the emitter on the first segment), hit triangle,
-//       source bin, absorbing bin (~0: none)
+//       source bin, absorbing bin (~0: none, or not yet known)
 //   r1: barycentric weights of corners 1 and 2 at the start and at the hit
 //   r2: photon index, source triangle, first segment (1) or not (0)
 struct SegRec {
