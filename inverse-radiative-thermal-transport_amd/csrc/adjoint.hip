@@ -49,6 +49,7 @@
 #include "traverse.hpp"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 namespace irtt {
 namespace {
@@ -292,6 +293,8 @@ __device__ __forceinline__ double hist_factor(const AdjCommon &a, uint32_t bin_l
 struct PathArgs {
     AdjCommon c;
     uint64_t g_begin, g_end;       // photons replayed
+    const uint32_t *live;          // or, if set: the photons listed in live[0, *n_live)
+    const uint32_t *n_live;
     RecBuf segs, edges;
     unsigned long long *first_dropped;  // smallest photon with a record that did not fit
 };
@@ -334,8 +337,11 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
     uint32_t nvis = 0, ntri = 0;
     uint32_t s_chunk = kNoChunk, s_fill = 0, e_chunk = kNoChunk, e_fill = 0;
     const uint64_t stride = (uint64_t)gridDim.x * kTraceBlock;
-    // g: this lane's next photon to look at; gp: the photon being replayed
-    uint64_t g = pa.g_begin + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x, gp = 0;
+    // g: this lane's next photon (or list position) to look at; gp: the
+    // photon being replayed
+    const uint64_t g_stop = pa.live ? (uint64_t)*pa.n_live : pa.g_end;
+    uint64_t g = (pa.live ? 0 : pa.g_begin) + (uint64_t)blockIdx.x * kTraceBlock + threadIdx.x,
+             gp = 0;
     uint32_t t = 0, src = 0, abs_bin = 0, seed = 0, depth = 0, prev_tri = 0;
     int prev_max = 0;
     float pb1 = 0.0f, pb2 = 0.0f;
@@ -344,16 +350,17 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
     for (;;) {
         if (!live) {
             // this lane's next photon with a path to replay
-            for (; g < pa.g_end; g += stride) {
-                const uint32_t r = a.absorber[g];
+            for (; g < g_stop; g += stride) {
+                const uint64_t ph = pa.live ? (uint64_t)pa.live[g] : g;
+                const uint32_t r = a.absorber[ph];
                 if (r == 0xffffffffu && !dropped_too) continue;
-                t = (uint32_t)(g / a.per_tri);
+                t = (uint32_t)(ph / a.per_tri);
                 // the emission ray (getEmissionRay, thermal_tracing_unified.glsl:125-161)
                 const uint32_t i0 = a.idx[3 * t], i1 = a.idx[3 * t + 1], i2 = a.idx[3 * t + 2];
                 const V3 p0 = load_v(a.xyz, i0), p1 = load_v(a.xyz, i1), p2 = load_v(a.xyz, i2);
                 {
                     // thermal_tracing_unified.glsl:291
-                    const uint32_t k = (uint32_t)(g - (uint64_t)t * a.per_tri);
+                    const uint32_t k = (uint32_t)(ph - (uint64_t)t * a.per_tri);
                     const uint32_t xx = k / a.rays_per_thread, nn = k - xx * a.rays_per_thread;
                     seed = tea16(a.nt * xx + t, a.batch_seed + nn);
                 }
@@ -380,7 +387,7 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
                 pb2 = b2;
                 prev_max = sc;
                 depth = 0;
-                gp = g;
+                gp = ph;
                 g += stride;
                 live = true;
                 break;
@@ -483,6 +490,38 @@ __global__ __launch_bounds__(kTraceBlock, adjoint_waves(STACK)) void k_adj_paths
     }
     chunk_close(s_chunk, s_fill, pa.segs);
     chunk_close(e_chunk, e_fill, pa.edges);
+}
+
+// The photons with an absorbing bin, in ascending order, so that the
+// replay's lanes take only photons with a path instead of skipping the others
+// one by one: per 256-photon block its count, then (after an exclusive scan
+// of the counts) its photons at its offset.
+__global__ __launch_bounds__(256) void k_live_count(const uint32_t *__restrict__ absorber,
+                                                    uint64_t photons, uint32_t *__restrict__ counts)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool has = g < photons && absorber[g] != 0xffffffffu;
+    const int n = __syncthreads_count(has);
+    if (threadIdx.x == 0) counts[blockIdx.x] = (uint32_t)n;
+}
+
+__global__ __launch_bounds__(256) void k_live_fill(const uint32_t *__restrict__ absorber,
+                                                   uint64_t photons,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   uint32_t n_blocks, const uint32_t *__restrict__ counts,
+                                                   uint32_t *__restrict__ live, uint32_t *__restrict__ n_live)
+{
+    __shared__ uint32_t wsum[4];
+    const uint64_t g = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    const bool has = g < photons && absorber[g] != 0xffffffffu;
+    const unsigned long long m = __ballot(has);
+    const uint32_t wave = threadIdx.x / 64;
+    if (__lane_id() == 0) wsum[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = offsets[blockIdx.x];
+    for (uint32_t w = 0; w < wave; w++) base += wsum[w];
+    if (has) live[base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1ull))] = (uint32_t)g;
+    if (blockIdx.x == n_blocks - 1 && threadIdx.x == 0) *n_live = offsets[blockIdx.x] + counts[blockIdx.x];
 }
 
 // Edge records, raw form (in) -> ray form (out, the same slot; in == out
@@ -1293,6 +1332,28 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
         EdgeRec *const edges_p = reinterpret_cast<EdgeRec *>(ws.edges.p);
         const uint64_t range = (left + n_ranges - 1) / n_ranges;
         static const unsigned long long ctl0[3] = {0ull, 0ull, ~0ull};
+        // one range from the first photon, photons without an absorbing bin
+        // not replayed: the replay's first launch takes its photons from the
+        // list of those with one
+        const uint32_t *live_p = nullptr;
+        if (n_ranges == 1 && g_first == 0 && !dropped_too) {
+            const uint32_t nblk = cdiv(photons, 256);
+            ws.live.alloc(photons);
+            ws.live_counts.alloc(2 * (size_t)nblk + 1);
+            uint32_t *counts = ws.live_counts.p, *offsets = counts + nblk, *n_live = offsets + nblk;
+            size_t tb = 0;
+            IRTT_HIP(rocprim::exclusive_scan(nullptr, tb, counts, offsets, 0u, (size_t)nblk,
+                                             rocprim::plus<uint32_t>(), stm));
+            ws.scan_tmp.alloc(std::max<size_t>(tb, 1));
+            k_live_count<<<nblk, 256, 0, stm>>>(absorber, photons, counts);
+            IRTT_HIP(hipGetLastError());
+            IRTT_HIP(rocprim::exclusive_scan(ws.scan_tmp.p, tb, counts, offsets, 0u, (size_t)nblk,
+                                             rocprim::plus<uint32_t>(), stm));
+            k_live_fill<<<nblk, 256, 0, stm>>>(absorber, photons, offsets, nblk, counts,
+                                              ws.live.p, n_live);
+            IRTT_HIP(hipGetLastError());
+            live_p = ws.live.p;
+        }
         for (uint64_t gb = g_first; gb < photons; gb += range) {
             const uint64_t ge = std::min<uint64_t>(photons, gb + range);
             uint64_t g_start = gb;
@@ -1305,6 +1366,10 @@ void adjoint_trace(Scene &s, const irtt_adjoint_params &ap, uint32_t n_hist, con
                 pa.edges =
                     RecBuf{edges_p, (uint32_t)edge_chunks, d_fills.p + seg_chunks, d_ctl.p + 1};
                 pa.first_dropped = d_ctl.p + 2;
+                if (live_p && g_start == 0) {
+                    pa.live = live_p;
+                    pa.n_live = ws.live_counts.p + 2 * (size_t)cdiv(photons, 256);
+                }
                 // the previous range's occlusion rays end before this range's
                 // replay rewrites the records
                 if (occl_pending) IRTT_HIP(hipStreamWaitEvent(stm, side2.occl_done, 0));

@@ -198,6 +198,9 @@ struct AdjWork {
     DBuf<double> lam, x4;
     DBuf<unsigned long long> grad, st, ctl;
     DBuf<uint32_t> abs_own, fills, keys, keys2, order, order2, n_big;
+    DBuf<uint32_t> live, live_counts;  // photons with an absorbing bin; per-block counts,
+                                       // offsets and their number
+    DBuf<uint8_t> scan_tmp;
     DBuf<uint4> segs, edges;
     DBuf<uint8_t> sort_tmp;
     // bytes held by the record buffers (the part sized against free HBM)
