@@ -155,7 +155,7 @@ typedef struct irtt_transport_params {
  * build's photon kernel writes the records irtt_adjoint_trace's path replay
  * would write -- 48 B per accepted segment, 64 B per hit within the default
  * edge ratio (0.05) of an edge -- so a following adjoint trace skips its
- * replay (city block, 2^26 photons: build 14.4 -> 15.9 ms, trace 21.2 ->
+ * replay (city block, 2^26 photons: build 14.4 -> 15.9 ms, trace 19.2 ->
  * 10.7 ms, same gradient bits).  A trace with another edge ratio replays; the
  * records that do not fit the memory the build sets aside (at most 30 % of
  * what is free) are replayed too.  No counterpart in the reference, whose
